@@ -1,0 +1,192 @@
+// Device arithmetic for the 2D SPH hot path on gfx950.
+//
+// Every helper here evaluates the same IEEE-754 single-precision operations, in
+// the same order, as the reference CPU code it cites, so that the GPU results
+// are bit-identical to the reference's x86-64 SSE build. This relies on the
+// build flags in emerald_amd/build.py:
+//   -ffp-contract=off   (no a*b+c fusion into FMA)
+//   no fast-math; HIP's default correctly-rounded f32 divide and sqrt
+//   f32 denormals preserved (no flush-to-zero)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace emsph {
+
+// Imath Vec2<float> stand-in: component-wise operators, dot = x*x' + y*y'.
+struct __align__(8) V2 {
+    float x;
+    float y;
+};
+
+__device__ __forceinline__ V2 v2(float x, float y) { return V2{x, y}; }
+__device__ __forceinline__ V2 add(V2 a, V2 b) { return V2{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ V2 sub(V2 a, V2 b) { return V2{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ V2 neg(V2 a) { return V2{-a.x, -a.y}; }
+// scalar * vector and vector * scalar are both (s*x, s*y) in IEEE arithmetic.
+__device__ __forceinline__ V2 scale(float s, V2 a) { return V2{s * a.x, s * a.y}; }
+__device__ __forceinline__ V2 divs(V2 a, float s) { return V2{a.x / s, a.y / s}; }
+__device__ __forceinline__ float dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+
+// Imath Vec2<T>::length with its lengthTiny branch for |v|^2 < 2*FLT_MIN.
+__device__ __forceinline__ float imath_length(V2 v) {
+    float const l2 = v.x * v.x + v.y * v.y;
+    if (l2 < 2.0f * FLT_MIN) {
+        float ax = (v.x >= 0.0f) ? v.x : -v.x;
+        float ay = (v.y >= 0.0f) ? v.y : -v.y;
+        float mx = ax;
+        if (mx < ay) { mx = ay; }
+        if (mx == 0.0f) { return 0.0f; }
+        ax /= mx;
+        ay /= mx;
+        return mx * __builtin_sqrtf(ax * ax + ay * ay);
+    }
+    return __builtin_sqrtf(l2);
+}
+
+// util/safe_divide.h:14-99 is_safe_divide for float and Vec2<float>.
+__device__ __forceinline__ bool is_safe_div(float n, float d) {
+    if (d == 0.0f) { return false; }
+    float const ad = __builtin_fabsf(d);
+    if (ad >= 1.0f) { return true; }
+    float const mr = ad / FLT_MIN;  // exact: FLT_MIN is a power of two
+    return mr > __builtin_fabsf(n);
+}
+__device__ __forceinline__ bool is_safe_div(V2 n, float d) {
+    if (d == 0.0f) { return false; }
+    float const ad = __builtin_fabsf(d);
+    if (ad >= 1.0f) { return true; }
+    float const mr = ad / FLT_MIN;
+    return (mr > __builtin_fabsf(n.x)) && (mr > __builtin_fabsf(n.y));
+}
+
+// Constants of the cubic-spline kernel for one support h, evaluated on the
+// host in the same float operation order as sph_common/kernels.h:25-55.
+struct KernelConsts {
+    float h;       // support (the reference's params.support)
+    float k;       // h * h * h * pi
+    float hh;      // h * h
+    float hhh;     // h * h * h
+    float four_h;  // 4 * h
+    float w0;      // W(0, h)
+};
+
+// sph_common/kernels.h:16-35 W(r, h)
+__device__ __forceinline__ float kernel_W(float r, KernelConsts const& c) {
+    float const q = r / c.h;
+    if (q <= 1.0f) {
+        float const wa = 1.0f + q * q * (q * 0.75f - 1.5f);
+        return wa / c.k;
+    }
+    if (q <= 2.0f) {
+        float const t = 2.0f - q;
+        float const wb = (t * t * t) / 4.0f;
+        return wb / c.k;
+    }
+    return 0.0f;
+}
+
+// sph_common/kernels.h:37-55 dW(r, h)
+__device__ __forceinline__ float kernel_dW(float r, KernelConsts const& c) {
+    float const q = r / c.h;
+    if (q <= 1.0f) {
+        float const dwa = -3.0f * r / c.hh + 2.25f * r * r / c.hhh;
+        return dwa / c.k;
+    }
+    if (q <= 2.0f) {
+        float const t = 2.0f - (r / c.h);
+        float const dwb = -3.0f * (t * t) / c.four_h;
+        return dwb / c.k;
+    }
+    return 0.0f;
+}
+
+// sph_common/kernels.h:57-64 GradW(dP, h)
+__device__ __forceinline__ V2 kernel_GradW(V2 dP, KernelConsts const& c) {
+    float const r = imath_length(dP);
+    if (is_safe_div(dP, r)) {
+        float const dw = kernel_dW(r, c);
+        V2 const u = divs(dP, r);
+        return V2{u.x * dw, u.y * dw};
+    }
+    return V2{0.0f, 0.0f};
+}
+
+// Same, with r = |dP| already known (r == imath_length(dP) bit for bit).
+__device__ __forceinline__ V2 kernel_GradW_r(V2 dP, float r, KernelConsts const& c) {
+    if (is_safe_div(dP, r)) {
+        float const dw = kernel_dW(r, c);
+        V2 const u = divs(dP, r);
+        return V2{u.x * dw, u.y * dw};
+    }
+    return V2{0.0f, 0.0f};
+}
+
+// sph_common/common.cpp:43-67 grid coordinate: floor(p / cell) per axis with
+// the bounds minimum at the origin.
+__device__ __forceinline__ int2 grid_coord(V2 p, float cell) {
+    float const qx = (p.x - 0.0f) / cell;
+    float const qy = (p.y - 0.0f) / cell;
+    return make_int2(static_cast<int32_t>(__builtin_floorf(qx)),
+                     static_cast<int32_t>(__builtin_floorf(qy)));
+}
+
+// z_index/z_index.cpp:17-48: full 64-bit interleave of two 32-bit axes (x on
+// even bits, y on odd bits), written as bit spreading instead of a LUT.
+__device__ __host__ __forceinline__ uint64_t spread_bits_2(uint32_t v) {
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+__device__ __host__ __forceinline__ uint64_t morton2(int32_t gx, int32_t gy) {
+    return spread_bits_2(static_cast<uint32_t>(gx)) |
+           (spread_bits_2(static_cast<uint32_t>(gy)) << 1);
+}
+
+// Open-addressed cell table: cell key -> cell ordinal. A slot is empty while
+// its value is EMPTY_CELL; keys are written by the thread that claims the slot.
+constexpr uint32_t EMPTY_CELL = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t cell_hash(uint64_t key, uint32_t mask) {
+    key ^= key >> 33;
+    key *= 0xff51afd7ed558ccdull;
+    key ^= key >> 33;
+    key *= 0xc4ceb9fe1a85ec53ull;
+    key ^= key >> 33;
+    return static_cast<uint32_t>(key) & mask;
+}
+
+__device__ __forceinline__ uint32_t cell_lookup(uint64_t key, uint64_t const* __restrict__ keys,
+                                                uint32_t const* __restrict__ vals,
+                                                uint32_t mask) {
+    uint32_t s = cell_hash(key, mask);
+    for (;;) {
+        uint32_t const v = vals[s];
+        if (v == EMPTY_CELL) { return EMPTY_CELL; }
+        if (keys[s] == key) { return v; }
+        s = (s + 1) & mask;
+    }
+}
+
+// Workgroup -> tile remap so that each XCD (workgroups are dealt round-robin
+// over the 8 XCDs) walks one contiguous run of Morton-sorted tiles and keeps
+// its neighbour gathers in its own L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t nblocks) {
+    constexpr uint32_t XCDS = 8;
+    uint32_t const xcd = bid % XCDS;
+    uint32_t const idx = bid / XCDS;
+    // Blocks with bid >= nblocks - (nblocks % XCDS) would map past the end when
+    // nblocks is not a multiple of 8; fall back to identity for the tail.
+    uint32_t const full = nblocks - (nblocks % XCDS);
+    if (bid >= full) { return bid; }
+    return xcd * (full / XCDS) + idx;
+}
+
+}  // namespace emsph

@@ -1,0 +1,303 @@
+// Stable LSD radix sort of (64-bit Morton key, 32-bit particle index) pairs and
+// the device-wide exclusive scan it is built on, written for gfx950 wave64.
+//
+// The reference sorts (key, index) pairs with tbb::parallel_sort
+// (sph_common/neighborhood.cpp:18-26). Pairs are unique and the indices enter
+// in increasing order, so a stable sort by key yields the same sequence.
+//
+// One pass sorts one 8-bit digit:
+//   rs_hist    per-tile digit counts (wave-aggregated LDS histograms)
+//   scan       exclusive scan of the digit-major [256][tiles] count matrix
+//   rs_scatter stable per-tile ranking (wave match via 8 ballots, per-wave
+//              counts in LDS) and scatter to the scanned offsets
+// Only the digits below the highest bit in which min(key) and max(key) differ
+// are sorted: every key shares the bits above it.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace emsph {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    uint32_t const lane = __lane_id();
+    return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Lanes of this wave holding the same 8-bit digit (and valid).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        bool const bit = (d >> b) & 1u;
+        uint64_t const m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_hist(uint64_t const* __restrict__ keys,
+                                                      uint32_t n, uint32_t shift,
+                                                      uint32_t* __restrict__ counts,
+                                                      uint32_t tiles) {
+    __shared__ uint32_t h[RS_WAVES][256];
+    uint32_t const tid = threadIdx.x;
+    uint32_t const wave = tid / 64;
+    for (int w = 0; w < RS_WAVES; ++w) { h[w][tid] = 0; }
+    __syncthreads();
+    uint32_t const base = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        uint32_t const i = base + r * RS_THREADS + tid;
+        bool const valid = i < n;
+        uint32_t const d = valid ? static_cast<uint32_t>((keys[i] >> shift) & 255u) : 0u;
+        uint64_t const peers = match_digit(d, valid);
+        if (valid && (peers & lanemask_lt()) == 0) {
+            atomicAdd(&h[wave][d], static_cast<uint32_t>(__popcll(peers)));
+        }
+    }
+    __syncthreads();
+    uint32_t s = 0;
+    for (int w = 0; w < RS_WAVES; ++w) { s += h[w][tid]; }
+    counts[tid * tiles + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_scatter(
+  uint64_t const* __restrict__ kin, uint32_t const* __restrict__ vin,
+  uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t shift,
+  uint32_t const* __restrict__ offsets, uint32_t tiles) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    uint32_t const tid = threadIdx.x;
+    uint32_t const wave = tid / 64;
+    run[tid] = offsets[tid * tiles + blockIdx.x];
+    for (int w = 0; w < RS_WAVES; ++w) { wcnt[w][tid] = 0; }
+    __syncthreads();
+    uint32_t const base = blockIdx.x * RS_TILE;
+    uint64_t const lt = lanemask_lt();
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        uint32_t const i = base + r * RS_THREADS + tid;
+        bool const valid = i < n;
+        uint64_t const k = valid ? kin[i] : 0ull;
+        uint32_t const v = valid ? vin[i] : 0u;
+        uint32_t const d = static_cast<uint32_t>((k >> shift) & 255u);
+        uint64_t const peers = match_digit(d, valid);
+        bool const leader = valid && (peers & lt) == 0;
+        uint32_t const cnt = static_cast<uint32_t>(__popcll(peers));
+        if (leader) { wcnt[wave][d] = cnt; }
+        __syncthreads();
+        uint32_t pos = 0;
+        if (valid) {
+            pos = run[d] + static_cast<uint32_t>(__popcll(peers & lt));
+            for (uint32_t w = 0; w < wave; ++w) { pos += wcnt[w][d]; }
+        }
+        __syncthreads();
+        if (leader) {
+            atomicAdd(&run[d], cnt);
+            wcnt[wave][d] = 0;
+        }
+        __syncthreads();
+        if (valid) {
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+    }
+}
+
+// Min and max of the keys (for the significant-bit range).
+__global__ __launch_bounds__(256) void key_minmax(uint64_t const* __restrict__ keys, uint32_t n,
+                                                 unsigned long long* __restrict__ out) {
+    unsigned long long mn = ~0ull;
+    unsigned long long mx = 0ull;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        unsigned long long const k = keys[i];
+        mn = k < mn ? k : mn;
+        mx = k > mx ? k : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long const a = __shfl_xor(mn, off);
+        unsigned long long const b = __shfl_xor(mx, off);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if (__lane_id() == 0) {
+        atomicMin(&out[0], mn);
+        atomicMax(&out[1], mx);
+    }
+}
+
+//------------------------------------------------------------------------------
+// Exclusive scan of uint32 (reduce -> top scan -> downsweep).
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    uint32_t const lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t const y = __shfl_up(x, off);
+        if (lane >= static_cast<uint32_t>(off)) { x += y; }
+    }
+    return x;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t& total) {
+    __shared__ uint32_t wsum[THREADS / 64];
+    uint32_t const lane = __lane_id();
+    uint32_t const wave = threadIdx.x / 64;
+    uint32_t const inc = wave_inclusive_scan(x);
+    if (lane == 63) { wsum[wave] = inc; }
+    __syncthreads();
+    uint32_t prefix = 0;
+    uint32_t tot = 0;
+    for (int w = 0; w < THREADS / 64; ++w) {
+        uint32_t const s = wsum[w];
+        if (static_cast<uint32_t>(w) < wave) { prefix += s; }
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return prefix + inc - x;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void scan_reduce(uint32_t const* __restrict__ in,
+                                                          uint32_t n,
+                                                          uint32_t* __restrict__ bsum) {
+    uint32_t const base = blockIdx.x * SC_TILE;
+    uint32_t s = 0;
+    for (int r = 0; r < SC_ITEMS; ++r) {
+        uint32_t const i = base + r * SC_THREADS + threadIdx.x;
+        if (i < n) { s += in[i]; }
+    }
+    for (int off = 32; off > 0; off >>= 1) { s += __shfl_xor(s, off); }
+    __shared__ uint32_t ws[SC_THREADS / 64];
+    if (__lane_id() == 0) { ws[threadIdx.x / 64] = s; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < SC_THREADS / 64; ++w) { t += ws[w]; }
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// Single workgroup: exclusive scan of nb block sums in place; writes the grand
+// total to *total when total is not null.
+__global__ __launch_bounds__(1024) void scan_top(uint32_t* __restrict__ bsum, uint32_t nb,
+                                                 uint32_t* __restrict__ total) {
+    uint32_t const per = (nb + 1023) / 1024;
+    uint32_t const lo = threadIdx.x * per;
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < per; ++i) {
+        uint32_t const j = lo + i;
+        if (j < nb) { s += bsum[j]; }
+    }
+    uint32_t tot = 0;
+    uint32_t run = block_exclusive_scan<1024>(s, tot);
+    for (uint32_t i = 0; i < per; ++i) {
+        uint32_t const j = lo + i;
+        if (j < nb) {
+            uint32_t const v = bsum[j];
+            bsum[j] = run;
+            run += v;
+        }
+    }
+    if (total && threadIdx.x == 0) { *total = tot; }
+}
+
+__global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t const* __restrict__ in,
+                                                        uint32_t* __restrict__ out, uint32_t n,
+                                                        uint32_t const* __restrict__ bsum) {
+    __shared__ uint32_t tile[SC_TILE];
+    uint32_t const base = blockIdx.x * SC_TILE;
+    for (int r = 0; r < SC_ITEMS; ++r) {
+        uint32_t const i = base + r * SC_THREADS + threadIdx.x;
+        tile[r * SC_THREADS + threadIdx.x] = (i < n) ? in[i] : 0u;
+    }
+    __syncthreads();
+    // Each thread scans SC_ITEMS consecutive elements.
+    uint32_t const t0 = threadIdx.x * SC_ITEMS;
+    uint32_t s = 0;
+    for (int r = 0; r < SC_ITEMS; ++r) { s += tile[t0 + r]; }
+    uint32_t tot = 0;
+    uint32_t run = block_exclusive_scan<SC_THREADS>(s, tot) + bsum[blockIdx.x];
+    for (int r = 0; r < SC_ITEMS; ++r) {
+        uint32_t const v = tile[t0 + r];
+        tile[t0 + r] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (int r = 0; r < SC_ITEMS; ++r) {
+        uint32_t const i = base + r * SC_THREADS + threadIdx.x;
+        if (i < n) { out[i] = tile[r * SC_THREADS + threadIdx.x]; }
+    }
+}
+
+//------------------------------------------------------------------------------
+// Host launchers.
+
+size_t scan_scratch_words(size_t n) { return (n + SC_TILE - 1) / SC_TILE + 1; }
+
+void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32_t* scratch,
+                           uint32_t* total, hipStream_t s) {
+    if (n == 0) { return; }
+    uint32_t const nb = (n + SC_TILE - 1) / SC_TILE;
+    hipLaunchKernelGGL(scan_reduce, dim3(nb), dim3(SC_THREADS), 0, s, in, n, scratch);
+    hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, s, scratch, nb, total);
+    hipLaunchKernelGGL(scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, scratch);
+}
+
+size_t radix_counts_words(size_t n) {
+    size_t const tiles = (n + RS_TILE - 1) / RS_TILE;
+    return 256 * tiles;
+}
+
+void launch_key_minmax(uint64_t const* keys, uint32_t n, unsigned long long* out,
+                       hipStream_t s) {
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > 2048) { blocks = 2048; }
+    if (blocks == 0) { blocks = 1; }
+    hipLaunchKernelGGL(key_minmax, dim3(blocks), dim3(256), 0, s, keys, n, out);
+}
+
+int radix_pass_count(uint64_t kmin, uint64_t kmax) {
+    uint64_t const diff = kmin ^ kmax;
+    if (diff == 0) { return 0; }
+    int const bits = 64 - __builtin_clzll(diff);
+    return (bits + 7) / 8;
+}
+
+int launch_radix_sort(uint64_t* keys0, uint32_t* vals0, uint64_t* keys1, uint32_t* vals1,
+                      uint32_t n, int passes, uint32_t* counts, uint32_t* scan_scratch,
+                      hipStream_t s) {
+    uint32_t const tiles = (n + RS_TILE - 1) / RS_TILE;
+    uint64_t* kin = keys0;
+    uint32_t* vin = vals0;
+    uint64_t* kout = keys1;
+    uint32_t* vout = vals1;
+    for (int p = 0; p < passes; ++p) {
+        uint32_t const shift = 8u * static_cast<uint32_t>(p);
+        hipLaunchKernelGGL(rs_hist, dim3(tiles), dim3(RS_THREADS), 0, s, kin, n, shift, counts,
+                           tiles);
+        launch_exclusive_scan(counts, counts, 256u * tiles, scan_scratch, nullptr, s);
+        hipLaunchKernelGGL(rs_scatter, dim3(tiles), dim3(RS_THREADS), 0, s, kin, vin, kout, vout,
+                           n, shift, counts, tiles);
+        uint64_t* tk = kin;
+        kin = kout;
+        kout = tk;
+        uint32_t* tv = vin;
+        vin = vout;
+        vout = tv;
+    }
+    // Result buffer index: 0 if in keys0/vals0, 1 otherwise.
+    return passes & 1;
+}
+
+}  // namespace emsph

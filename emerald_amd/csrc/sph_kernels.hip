@@ -1,0 +1,894 @@
+// HIP kernels of the 2D SPH sub-step for gfx950.
+//
+// Layout: particle fields live in Morton-sorted order ("slots") for the
+// duration of a sub-step; neighbour lists are slot-major ([k * stride + t]) so
+// that the thread of slot t reads entry k of 64 consecutive slots with one
+// coalesced load. Each kernel evaluates one particle per thread, summing its
+// neighbours in the reference's list order (distance, then original index), so
+// every float result matches the reference CPU path bit for bit.
+//
+// The reference runs separate fluid and solid passes for each sum
+// (e.g. density.cpp:84-109); both passes only write the particle's own element
+// and read others' inputs that the pass does not modify, so one thread can run
+// the fluid pass and then the solid pass of its particle without changing any
+// result.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_math.hpp"
+#include "kernels.hpp"
+
+namespace emsph {
+
+constexpr int BUILD_THREADS = 64;
+constexpr int SWEEP_THREADS = 256;
+
+__device__ __forceinline__ V2 ld(V2 const* p, uint32_t i) { return p[i]; }
+
+// Quantised residual accumulation over a workgroup followed by one atomic per
+// workgroup. Every thread of the block must call it.
+template <int THREADS>
+__device__ __forceinline__ void block_reduce_residual(unsigned long long v, unsigned int m,
+                                                      Reduction* r) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+        unsigned int const o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    __shared__ unsigned long long sv[THREADS / 64];
+    __shared__ unsigned int sm[THREADS / 64];
+    if (__lane_id() == 0) {
+        sv[threadIdx.x / 64] = v;
+        sm[threadIdx.x / 64] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tv = 0;
+        unsigned int tm = 0;
+        for (int w = 0; w < THREADS / 64; ++w) {
+            tv += sv[w];
+            tm = sm[w] > tm ? sm[w] : tm;
+        }
+        if (tv) { atomicAdd(&r->isum, tv); }
+        if (tm) { atomicMax(&r->max_bits, tm); }
+    }
+}
+
+template <int THREADS>
+__device__ __forceinline__ void block_reduce_max(unsigned int m, unsigned int* out) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned int const o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    __shared__ unsigned int sm[THREADS / 64];
+    if (__lane_id() == 0) { sm[threadIdx.x / 64] = m; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int tm = 0;
+        for (int w = 0; w < THREADS / 64; ++w) { tm = sm[w] > tm ? sm[w] : tm; }
+        if (tm) { atomicMax(out, tm); }
+    }
+}
+
+//------------------------------------------------------------------------------
+// Keys: sph_common/common.cpp:43-76 (grid coordinate, z key) and
+// neighborhood.cpp:10-16 (index pairs).
+__global__ __launch_bounds__(256) void k_keys(KeysArgs a) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0) {
+        if (a.red_reset) {
+            for (uint32_t k = threadIdx.x; k < MAX_RED_SLOTS; k += blockDim.x) {
+                a.red_reset[k].isum = 0;
+                a.red_reset[k].max_bits = 0;
+                a.red_reset[k].pad = 0;
+            }
+        }
+        if (a.vmax2_reset && threadIdx.x == 0) { *a.vmax2_reset = 0; }
+    }
+    if (i >= a.n) { return; }
+    int2 const g = grid_coord(a.pos[i], a.cell);
+    a.keys[i] = morton2(g.x, g.y);
+    a.vals[i] = i;
+    if (a.grid) { a.grid[i] = g; }
+}
+
+void launch_keys(KeysArgs const& a, hipStream_t s) {
+    uint32_t const blocks = (a.n + 255) / 256;
+    hipLaunchKernelGGL(k_keys, dim3(blocks ? blocks : 1), dim3(256), 0, s, a);
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        p[i] = v;
+    }
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+    if (!n) { return; }
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) { blocks = 4096; }
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
+}
+
+//------------------------------------------------------------------------------
+// Cells: neighborhood.cpp:33-124 (block indices, block ranges, key map).
+__global__ __launch_bounds__(256) void k_cell_heads(CellsArgs a) {
+    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) { return; }
+    a.heads[t] = (t == 0 || a.sorted_keys[t] != a.sorted_keys[t - 1]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_cells_finalize(CellsArgs a) {
+    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) { return; }
+    uint64_t const key = a.sorted_keys[t];
+    uint32_t const head = a.heads[t];
+    uint32_t const b = a.head_scan[t] + head - 1u;
+    a.block_index[t] = b;
+    if (head) {
+        a.cell_begin[b] = t;
+        uint32_t s = cell_hash(key, a.mask);
+        for (;;) {
+            uint32_t const prev = atomicCAS(&a.table_vals[s], EMPTY_CELL, b);
+            if (prev == EMPTY_CELL) {
+                a.table_keys[s] = key;
+                break;
+            }
+            s = (s + 1) & a.mask;
+        }
+    }
+    if (t + 1 == a.n || a.sorted_keys[t + 1] != key) { a.cell_end[b] = t + 1; }
+    uint32_t const o = a.sorted_vals[t];
+    a.orig[t] = o;
+    a.rank[o] = t;
+    if (a.pos_out) { a.pos_out[t] = a.pos_in[o]; }
+    if (a.vel_out) { a.vel_out[t] = a.vel_in[o]; }
+    if (a.kv_out) { a.kv_out[t] = a.kv_in[o]; }
+    if (a.kd_out) { a.kd_out[t] = a.kd_in[o]; }
+}
+
+void launch_cell_heads(CellsArgs const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_heads, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_cells_finalize(CellsArgs const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_cells_finalize, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+
+//------------------------------------------------------------------------------
+// Neighbour build: sph_common/neighborhood.h:198-282.
+//
+// Candidates are scanned in the reference's order (cells j-outer, i-inner; in a
+// cell, sorted slots, i.e. increasing original index) and accepted while
+// distance < cut-off, up to 64 in scan order. Each thread keeps its accepted
+// list sorted by (distance bits, original index) in LDS by insertion, which is
+// the reference's neighbor_order (neighborhood.h:172-186): distances are
+// non-negative, so their bit patterns order like the floats.
+__device__ __forceinline__ int collect_candidates(uint64_t (*ent)[BUILD_THREADS], uint32_t lane,
+                                                  V2 p, int2 g, uint32_t self, bool same,
+                                                  CellTable const& tb, V2 const* __restrict__ opos,
+                                                  uint32_t const* __restrict__ oorig, float maxd) {
+    int cnt = 0;
+    for (int32_t j = g.y - 1; cnt < MAXN && j <= g.y + 1; ++j) {
+        for (int32_t i = g.x - 1; cnt < MAXN && i <= g.x + 1; ++i) {
+            uint32_t const ord = cell_lookup(morton2(i, j), tb.keys, tb.vals, tb.mask);
+            if (ord == EMPTY_CELL) { continue; }
+            uint32_t const e = tb.end[ord];
+            for (uint32_t s = tb.begin[ord]; cnt < MAXN && s != e; ++s) {
+                if (same && s == self) { continue; }
+                V2 const v = sub(opos[s], p);
+                float const d = imath_length(v);
+                if (d >= maxd) { continue; }
+                uint64_t const key =
+                  (static_cast<uint64_t>(__float_as_uint(d)) << 32) | oorig[s];
+                int k = cnt;
+                while (k > 0) {
+                    uint64_t const prev = ent[k - 1][lane];
+                    if (prev <= key) { break; }
+                    ent[k][lane] = prev;
+                    --k;
+                }
+                ent[k][lane] = key;
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BUILD_THREADS) void k_build(BuildArgs a) {
+    __shared__ uint64_t ent[MAXN][BUILD_THREADS];
+    uint32_t const lane = threadIdx.x;
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * BUILD_THREADS + lane;
+    if (t >= a.n) { return; }
+    StepConsts const& c = a.c;
+    V2 const p = a.pos[t];
+    int2 const g = grid_coord(p, c.cell);
+    float const mf = c.rho0 * c.fluid_volume;  // rho0 * V_j (uniform fluid volume)
+
+    // Fluid neighbours.
+    int const fc = collect_candidates(ent, lane, p, g, t, true, a.table, a.pos, a.orig, c.cell);
+    a.fl.count[t] = static_cast<uint8_t>(fc);
+    float dens = c.fluid_volume * c.rho0 * c.kc.w0;  // density.cpp:57-59
+    V2 msum_f{0.0f, 0.0f};
+    V2 sum_mg{0.0f, 0.0f};    // DFSPH: sum of m grad W over fluid and solid
+    float sum_mg2 = 0.0f;     // DFSPH: sum of |m grad W|^2 over fluid
+    for (int k = 0; k < fc; ++k) {
+        uint64_t const e = ent[k][lane];
+        uint32_t const s = a.rank[static_cast<uint32_t>(e)];
+        float const d = __uint_as_float(static_cast<uint32_t>(e >> 32));
+        V2 const vec = sub(a.pos[s], p);
+        size_t const o = static_cast<size_t>(k) * a.stride + t;
+        a.fl.idx[o] = s;
+        if (a.store_geometry) {
+            a.fl.dist[o] = d;
+            a.fl.vec[o] = vec;
+        }
+        if (MODE != BUILD_LISTS) {
+            V2 const gw = kernel_GradW_r(neg(vec), d, c.kc);
+            a.fl.gw[o] = gw;
+            float const w = kernel_W(d, c.kc);
+            dens += w * mf;                                  // density.cpp:69-74
+            if (MODE == BUILD_IISPH) { msum_f = add(msum_f, scale(mf, gw)); }  // iisph_ops.cpp:42-47
+            if (MODE == BUILD_DFSPH) {                        // dfsph_p_ops.cpp:47-57
+                V2 const mg = scale(mf, gw);
+                sum_mg = add(sum_mg, mg);
+                sum_mg2 += dot(mg, mg);
+            }
+        }
+    }
+
+    // Solid neighbours (no self exclusion: different arrays).
+    SolidView const& sv = a.solids;
+    int sc = 0;
+    if (sv.n) {
+        sc = collect_candidates(ent, lane, p, g, t, false, sv.table, sv.pos, sv.orig, c.cell);
+    }
+    a.sl.count[t] = static_cast<uint8_t>(sc);
+    float dens_s = 0.0f;
+    V2 msum_s{0.0f, 0.0f};
+    for (int k = 0; k < sc; ++k) {
+        uint64_t const e = ent[k][lane];
+        uint32_t const b = sv.rank[static_cast<uint32_t>(e)];
+        float const d = __uint_as_float(static_cast<uint32_t>(e >> 32));
+        V2 const vec = sub(sv.pos[b], p);
+        size_t const o = static_cast<size_t>(k) * a.stride + t;
+        a.sl.idx[o] = b;
+        if (a.store_geometry) {
+            a.sl.dist[o] = d;
+            a.sl.vec[o] = vec;
+        }
+        if (MODE != BUILD_LISTS) {
+            V2 const gw = kernel_GradW_r(neg(vec), d, c.kc);
+            a.sl.gw[o] = gw;
+            float const ms = c.rho0 * sv.vol[b];
+            dens_s += kernel_W(d, c.kc) * ms;
+            if (MODE == BUILD_IISPH) { msum_s = add(msum_s, scale(ms, gw)); }
+            if (MODE == BUILD_DFSPH) { sum_mg = add(sum_mg, scale(ms, gw)); }
+        }
+    }
+    if (MODE == BUILD_LISTS) { return; }
+
+    float rho = dens;
+    if (sc) { rho += dens_s; }  // density.cpp:76-80
+    a.rho[t] = rho;
+
+    if (MODE == BUILD_IISPH) {
+        // iisph_ops.cpp:49-57 d_ii, fluid pass then solid pass.
+        float const neg_dt2 = -(c.dt * c.dt);
+        float const denom = rho * rho;
+        V2 dii{0.0f, 0.0f};
+        if (fc) {
+            V2 const numer = scale(neg_dt2, msum_f);
+            dii = is_safe_div(numer, denom) ? divs(numer, denom) : V2{0.0f, 0.0f};
+        }
+        if (sc) {
+            V2 const numer = scale(neg_dt2, msum_s);
+            V2 const ds = is_safe_div(numer, denom) ? divs(numer, denom) : V2{0.0f, 0.0f};
+            dii = add(dii, ds);
+        }
+        a.dii[t] = dii;
+        // forces.cpp:11-83 (zeroed force, y -= m g) and dynamics.cpp:24-37.
+        V2 const f{0.0f, 0.0f - c.mass * c.gravity};
+        float const vden = c.rho0 * c.fluid_volume;
+        V2 const numer = scale(c.dt, f);
+        if (is_safe_div(numer, vden)) { a.vel[t] = add(a.vel[t], divs(numer, vden)); }
+    }
+    if (MODE == BUILD_DFSPH) {
+        // dfsph_p_ops.cpp:78-82
+        float alpha = 0.0f;
+        if (fc + sc) {
+            float const numer = rho * rho;
+            float const denom = dot(sum_mg, sum_mg) + sum_mg2;
+            alpha = is_safe_div(numer, denom) ? numer / denom : 0.0f;
+        }
+        a.alpha[t] = alpha;
+    }
+}
+
+void launch_build(int mode, BuildArgs const& a, hipStream_t s) {
+    uint32_t const blocks = (a.n + BUILD_THREADS - 1) / BUILD_THREADS;
+    if (!blocks) { return; }
+    switch (mode) {
+    case BUILD_IISPH:
+        hipLaunchKernelGGL(k_build<BUILD_IISPH>, dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+        break;
+    case BUILD_DFSPH:
+        hipLaunchKernelGGL(k_build<BUILD_DFSPH>, dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+        break;
+    default:
+        hipLaunchKernelGGL(k_build<BUILD_LISTS>, dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+        break;
+    }
+}
+
+// solids.cpp:52-74 compute_volumes: 1.7 / (W0 + sum W) over solid-solid lists.
+__global__ __launch_bounds__(256) void k_solid_volumes(KernelConsts kc, uint32_t n,
+                                                       uint32_t stride, Lists l, float* vol) {
+    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) { return; }
+    int const cnt = l.count[t];
+    if (!cnt) {
+        vol[t] = 1.0f / kc.w0;
+        return;
+    }
+    float m = kc.w0;
+    for (int k = 0; k < cnt; ++k) { m += kernel_W(l.dist[static_cast<size_t>(k) * stride + t], kc); }
+    vol[t] = 1.7f / m;
+}
+
+void launch_solid_volumes(KernelConsts const& kc, uint32_t n, uint32_t stride, Lists const& l,
+                          float* vol_sorted, hipStream_t s) {
+    if (!n) { return; }
+    hipLaunchKernelGGL(k_solid_volumes, dim3((n + 255) / 256), dim3(256), 0, s, kc, n, stride, l,
+                       vol_sorted);
+}
+
+//------------------------------------------------------------------------------
+// IISPH sweeps: sph2d_box/iisph_ops.cpp.
+
+// :94-221 a_ii and density stars; also zeroes the pressures (iisph.cpp:83).
+__global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_aii(SweepArgs a) {
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+    if (t >= a.n) { return; }
+    StepConsts const& c = a.c;
+    float const mf = c.fluid_volume * c.rho0;
+    float const neg_dt2 = -(c.dt * c.dt);
+    float const kf = neg_dt2 * mf;
+    float const dtm = c.dt * mf;
+    int const fc = a.fl.count[t];
+    V2 const d = a.dii[t];
+    V2 const vi = a.vel[t];
+    float aii;
+    float rs;
+    if (!fc) {
+        rs = a.rho[t];
+        aii = 0.0f;
+    } else {
+        float acc = 0.0f;
+        float dd = 0.0f;
+        for (int k = 0; k < fc; ++k) {
+            size_t const o = static_cast<size_t>(k) * a.stride + t;
+            uint32_t const s = a.fl.idx[o];
+            V2 const gw = a.fl.gw[o];
+            V2 const numer = scale(kf, gw);
+            float const rj = a.rho[s];
+            float const denom = rj * rj;
+            if (is_safe_div(numer, denom)) {
+                V2 const dij = divs(numer, denom);
+                V2 const dji = neg(dij);
+                acc += mf * dot(sub(d, dji), gw);
+            }
+            dd += dtm * dot(sub(vi, a.vel[s]), gw);
+        }
+        aii = acc;
+        rs = a.rho[t] + dd;
+    }
+    int const sc = a.sl.count[t];
+    if (sc) {
+        float acc = 0.0f;
+        float dd = 0.0f;
+        for (int k = 0; k < sc; ++k) {
+            size_t const o = static_cast<size_t>(k) * a.stride + t;
+            uint32_t const b = a.sl.idx[o];
+            V2 const gw = a.sl.gw[o];
+            float const ms = a.solids.vol[b] * c.rho0;
+            acc += ms * dot(d, gw);
+            dd += c.dt * ms * dot(sub(vi, a.solids.vel[b]), gw);
+        }
+        aii += acc;
+        rs += dd;
+    }
+    a.aii[t] = aii;
+    a.rho_star[t] = rs;
+    a.p_out[t] = 0.0f;
+}
+
+// :225-258 sum_j d_ij p_j (fluid only).
+__global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_sdp(SweepArgs a) {
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+    if (t >= a.n) { return; }
+    StepConsts const& c = a.c;
+    float const kf = -(c.dt * c.dt) * (c.fluid_volume * c.rho0);
+    int const fc = a.fl.count[t];
+    V2 s{0.0f, 0.0f};
+    for (int k = 0; k < fc; ++k) {
+        size_t const o = static_cast<size_t>(k) * a.stride + t;
+        uint32_t const j = a.fl.idx[o];
+        V2 const gw = a.fl.gw[o];
+        float const rj = a.rho[j];
+        V2 const numer = scale(kf * a.p_in[j], gw);
+        float const denom = rj * rj;
+        if (is_safe_div(numer, denom)) { s = add(s, divs(numer, denom)); }
+    }
+    a.sdp[t] = s;
+}
+
+// :261-353 beta (into the new-pressure buffer) fused with :357-425 new
+// pressure, quantised error sum and error max.
+__global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_beta_p(SweepArgs a, int slot) {
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+    unsigned long long qsum = 0;
+    unsigned int emax = 0;
+    if (t < a.n) {
+        StepConsts const& c = a.c;
+        float const mf = c.rho0 * c.fluid_volume;
+        float const kf = -(c.dt * c.dt) * mf;
+        float const pi = a.p_in[t];
+        V2 const si = a.sdp[t];
+        int const fc = a.fl.count[t];
+        float beta = 0.0f;
+        for (int k = 0; k < fc; ++k) {
+            size_t const o = static_cast<size_t>(k) * a.stride + t;
+            uint32_t const j = a.fl.idx[o];
+            V2 const gw = a.fl.gw[o];
+            float const rj = a.rho[j];
+            V2 const numer = scale(kf, gw);
+            float const denom = rj * rj;
+            V2 const dij = is_safe_div(numer, denom) ? divs(numer, denom) : V2{0.0f, 0.0f};
+            V2 const dji = neg(dij);
+            V2 const sjk = sub(a.sdp[j], scale(pi, dji));
+            V2 const djj_pj = scale(a.p_in[j], a.dii[j]);
+            beta += mf * dot(sub(sub(si, djj_pj), sjk), gw);
+        }
+        int const sc = a.sl.count[t];
+        if (sc) {
+            float bs = 0.0f;
+            for (int k = 0; k < sc; ++k) {
+                size_t const o = static_cast<size_t>(k) * a.stride + t;
+                uint32_t const b = a.sl.idx[o];
+                float const ms = c.rho0 * a.solids.vol[b];
+                bs += ms * dot(si, a.sl.gw[o]);
+            }
+            beta += bs;
+        }
+        // iisph_compute_new_pressures
+        float const alpha = a.aii[t];
+        float pn = pi;
+        if (alpha != 0.0f) {
+            float const numer = c.rho0 - a.rho_star[t] - beta;
+            if (is_safe_div(numer, alpha)) {
+                float err = -(numer - (alpha * pi)) / c.rho0;
+                err = (0.0f < err) ? err : 0.0f;  // std::max(0.0f, err)
+                emax = __float_as_uint(err);
+                qsum = static_cast<unsigned long long>(err * 1000.0f);
+                float const np = numer / alpha;
+                float const mixed = (pi * (1.0f - c.omega)) + (np * c.omega);
+                pn = (0.0f < mixed) ? mixed : 0.0f;
+            }
+        }
+        a.p_out[t] = pn;
+    }
+    block_reduce_residual<SWEEP_THREADS>(qsum, emax, &a.red[slot]);
+}
+
+//------------------------------------------------------------------------------
+// pressure.cpp:9-80 apply_pressures_in_place (fluid pass, then solid pass).
+template <int GRAVITY, int FINAL>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_apply(SweepArgs a) {
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+    unsigned int vm = 0;
+    if (t < a.n) {
+        StepConsts const& c = a.c;
+        V2 v = a.vel[t];
+        if (GRAVITY) {
+            V2 const f{0.0f, 0.0f - c.mass * c.gravity};
+            float const vden = c.rho0 * c.fluid_volume;
+            V2 const numer = scale(c.dt, f);
+            if (is_safe_div(numer, vden)) { v = add(v, divs(numer, vden)); }
+        }
+        int const fc = a.fl.count[t];
+        int const sc = a.sl.count[t];
+        if (fc || sc) {
+            float const ri = a.rho[t];
+            float const ri2 = ri * ri;
+            float const pi = a.p_in[t];
+            float const pori = is_safe_div(pi, ri2) ? pi / ri2 : 0.0f;
+            if (fc) {
+                float const mf = c.rho0 * c.fluid_volume;
+                V2 acc{0.0f, 0.0f};
+                for (int k = 0; k < fc; ++k) {
+                    size_t const o = static_cast<size_t>(k) * a.stride + t;
+                    uint32_t const j = a.fl.idx[o];
+                    float const rj = a.rho[j];
+                    float const rj2 = rj * rj;
+                    float const pj = a.p_in[j];
+                    float const porj = is_safe_div(pj, rj2) ? pj / rj2 : 0.0f;
+                    acc = add(acc, scale(-mf * (pori + porj), a.fl.gw[o]));
+                }
+                v = add(v, scale(c.dt, acc));
+            }
+            if (sc) {
+                V2 acc{0.0f, 0.0f};
+                for (int k = 0; k < sc; ++k) {
+                    size_t const o = static_cast<size_t>(k) * a.stride + t;
+                    uint32_t const b = a.sl.idx[o];
+                    float const ms = c.rho0 * a.solids.vol[b];
+                    acc = add(acc, scale(-ms * pori, a.sl.gw[o]));
+                }
+                v = add(v, scale(c.dt, acc));
+            }
+        }
+        if (FINAL) {
+            // dynamics.cpp:46-51: x += dt * v, written back in original order.
+            V2 const x = add(a.pos[t], scale(c.dt, v));
+            uint32_t const o = a.orig[t];
+            a.pos_orig[o] = x;
+            a.vel_orig[o] = v;
+            vm = __float_as_uint(dot(v, v));
+        } else {
+            a.vel[t] = v;
+        }
+    }
+    if (FINAL) { block_reduce_max<SWEEP_THREADS>(vm, a.vmax2_bits); }
+}
+
+// x += dt v and scatter the sorted state back to original order
+// (dfsph_p.cpp:368-369), including the DFSPH warm-start stiffnesses.
+template <int DFSPH>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_finalize(SweepArgs a) {
+    uint32_t const t = blockIdx.x * SWEEP_THREADS + threadIdx.x;
+    unsigned int vm = 0;
+    if (t < a.n) {
+        V2 const v = a.vel[t];
+        V2 const x = add(a.pos[t], scale(a.c.dt, v));
+        uint32_t const o = a.orig[t];
+        a.pos_orig[o] = x;
+        a.vel_orig[o] = v;
+        if (DFSPH) {
+            a.kappa_v_orig[o] = a.kappa_v_sorted[t];
+            a.kappa_d_orig[o] = a.kappa_d_sorted[t];
+        }
+        vm = __float_as_uint(dot(v, v));
+    }
+    block_reduce_max<SWEEP_THREADS>(vm, a.vmax2_bits);
+}
+
+//------------------------------------------------------------------------------
+// DFSPH_P sweeps: divergence.cpp:5-74 and dfsph_p_ops.cpp:87-126, with the
+// residual of common.h:209-232 fused in.
+__device__ __forceinline__ float divergence_of(SweepArgs const& a, uint32_t t, V2 vi) {
+    StepConsts const& c = a.c;
+    float const mf = c.rho0 * c.fluid_volume;
+    int const fc = a.fl.count[t];
+    float div = 0.0f;
+    for (int k = 0; k < fc; ++k) {
+        size_t const o = static_cast<size_t>(k) * a.stride + t;
+        uint32_t const j = a.fl.idx[o];
+        div += mf * dot(sub(vi, a.vel[j]), a.fl.gw[o]);
+    }
+    int const sc = a.sl.count[t];
+    if (sc) {
+        float ds = 0.0f;
+        for (int k = 0; k < sc; ++k) {
+            size_t const o = static_cast<size_t>(k) * a.stride + t;
+            uint32_t const b = a.sl.idx[o];
+            float const ms = c.rho0 * a.solids.vol[b];
+            ds += ms * dot(sub(vi, a.solids.vel[b]), a.sl.gw[o]);
+        }
+        div += ds;
+    }
+    return div;
+}
+
+// One divergence-solve iteration: divergence, kappa_v, residual of
+// max(0, dt * div / rho0) (dfsph_p.cpp:157-197).
+__global__ __launch_bounds__(SWEEP_THREADS) void k_dfsph_div(SweepArgs a, int slot) {
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+    unsigned long long qsum = 0;
+    unsigned int emax = 0;
+    if (t < a.n) {
+        StepConsts const& c = a.c;
+        float const div = divergence_of(a, t, a.vel[t]);
+        a.rho_star[t] = div;
+        float kv = 0.0f;
+        if (!(div <= 0.0f)) {
+            float const num = div * a.alpha[t];
+            kv = is_safe_div(num, c.dt) ? num / c.dt : 0.0f;
+        }
+        a.kappa[t] = kv;
+        float const val = (div <= 0.0f) ? 0.0f : (c.dt * div / c.rho0);
+        emax = __float_as_uint(val);
+        qsum = static_cast<unsigned long long>(static_cast<long long>(val * 1000.0f));
+    }
+    block_reduce_residual<SWEEP_THREADS>(qsum, emax, &a.red[slot]);
+}
+
+// One density-solve iteration: density star (density.cpp:111-145), kappa_rho,
+// residual of max(0, (rho* - rho0) / rho0) (dfsph_p.cpp:82-122).
+__global__ __launch_bounds__(SWEEP_THREADS) void k_dfsph_dens(SweepArgs a, int slot) {
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+    unsigned long long qsum = 0;
+    unsigned int emax = 0;
+    if (t < a.n) {
+        StepConsts const& c = a.c;
+        float const div = divergence_of(a, t, a.vel[t]);
+        float rs = a.rho[t] + (c.dt * div);
+        rs = (0.0f < rs) ? rs : 0.0f;  // std::max(0.0f, ...)
+        a.rho_star[t] = rs;
+        float const dd = rs - c.rho0;
+        float kd = 0.0f;
+        if (!(dd <= 0.0f)) {
+            float const num = dd * a.alpha[t];
+            float const den = c.dt * c.dt;
+            kd = is_safe_div(num, den) ? num / den : 0.0f;
+        }
+        a.kappa[t] = kd;
+        float const err = rs - c.rho0;
+        float const val = (err <= 0.0f) ? 0.0f : (err / c.rho0);
+        emax = __float_as_uint(val);
+        qsum = static_cast<unsigned long long>(static_cast<long long>(val * 1000.0f));
+    }
+    block_reduce_residual<SWEEP_THREADS>(qsum, emax, &a.red[slot]);
+}
+
+//------------------------------------------------------------------------------
+// PCISPH: sph2d_box/pcisph.cpp:36-164 and pcisph_ops.cpp.
+
+// :54-88 first predictor: v_ext = v + (dt/m) F_ext; v* = v_ext; x* = x + dt v*;
+// p = 0.
+__global__ __launch_bounds__(SWEEP_THREADS) void k_pci_init(SweepArgs a) {
+    uint32_t const t = blockIdx.x * SWEEP_THREADS + threadIdx.x;
+    if (t >= a.n) { return; }
+    StepConsts const& c = a.c;
+    V2 const f{0.0f, 0.0f - c.mass * c.gravity};
+    float const dtm = c.dt / c.mass;
+    V2 const vext = add(a.vel[t], scale(dtm, f));
+    a.vext[t] = vext;
+    a.vstar_out[t] = vext;
+    a.xstar[t] = add(a.pos[t], scale(c.dt, vext));
+    a.p_out[t] = 0.0f;
+}
+
+// :90-115 geometry at x* (neighborhoods_from_state.cpp:103-128), densities
+// (pcisph_ops.cpp:18-63), pressure update (:65-76) and the max density error
+// (common.cpp:78-85).
+__global__ __launch_bounds__(SWEEP_THREADS) void k_pci_density(SweepArgs a, int slot) {
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+    unsigned int emax = 0;
+    if (t < a.n) {
+        StepConsts const& c = a.c;
+        V2 const xs = a.xstar[t];
+        int const fc = a.fl.count[t];
+        float w = c.kc.w0;
+        for (int k = 0; k < fc; ++k) {
+            size_t const o = static_cast<size_t>(k) * a.stride + t;
+            uint32_t const j = a.fl.idx[o];
+            V2 const vec = sub(a.xstar[j], xs);
+            float const d = imath_length(vec);
+            a.fl.vec[o] = vec;
+            a.fl.dist[o] = d;
+            a.fl.gw[o] = kernel_GradW_r(neg(vec), d, c.kc);
+            w += kernel_W(d, c.kc);
+        }
+        float rho = c.mass * w;
+        int const sc = a.sl.count[t];
+        if (sc) {
+            float vf = 0.0f;
+            for (int k = 0; k < sc; ++k) {
+                size_t const o = static_cast<size_t>(k) * a.stride + t;
+                uint32_t const b = a.sl.idx[o];
+                V2 const vec = sub(a.solids.pos[b], xs);
+                float const d = imath_length(vec);
+                a.sl.gw[o] = kernel_GradW_r(neg(vec), d, c.kc);
+                vf += a.solids.vol[b] * kernel_W(d, c.kc);
+            }
+            rho += c.rho0 * vf;
+        }
+        a.rho_star[t] = rho;  // PCISPH densities live in the rho_star buffer
+        float e = rho - c.rho0;
+        e = (e < 0.0f) ? 0.0f : e;  // std::max(rho - rho0, 0.0f)
+        a.p_out[t] = a.p_out[t] + e / c.pci_denom;
+        emax = __float_as_uint(e);
+    }
+    block_reduce_max<SWEEP_THREADS>(emax, &a.red[slot].max_bits);
+}
+
+// :117-142 pressure + artificial viscosity forces, solid forces, then the next
+// predictor (:73-88) from the new force.
+__global__ __launch_bounds__(SWEEP_THREADS) void k_pci_force(SweepArgs a) {
+    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+    if (t >= a.n) { return; }
+    StepConsts const& c = a.c;
+    float const H = c.kc.h;
+    float const M = c.mass;
+    float const N2 = 0.01f * H * H;
+    float const* rho = a.rho_star;
+    float const* p = a.p_out;
+    V2 F{0.0f, 0.0f};
+    float const density = rho[t];
+    float const por = p[t] / (density * density);
+    int const fc = a.fl.count[t];
+    if (fc) {
+        V2 const v = a.vstar_in[t];
+        V2 f{0.0f, 0.0f};
+        for (int k = 0; k < fc; ++k) {
+            size_t const o = static_cast<size_t>(k) * a.stride + t;
+            uint32_t const j = a.fl.idx[o];
+            V2 const r = a.fl.vec[o];
+            V2 const gw = a.fl.gw[o];
+            float const od = rho[j];
+            f = add(f, scale(por + (p[j] / (od * od)), gw));
+            V2 const dv = sub(a.vstar_in[j], v);
+            float const dvr = dot(r, dv);
+            if (dvr < 0.0f) {
+                float const dist = a.fl.dist[o];
+                float const mu = H * dvr / (dist + N2);
+                float const den_avg = (density + od) / 2.0f;
+                float const pi_ab = ((-1.0f * c.viscosity * mu) + (2.0f * (mu * mu))) / den_avg;
+                f = add(f, scale(pi_ab, gw));
+            }
+        }
+        F = add(F, scale(-(M * M), f));
+    }
+    int const sc = a.sl.count[t];
+    if (sc) {
+        V2 g{0.0f, 0.0f};
+        for (int k = 0; k < sc; ++k) {
+            size_t const o = static_cast<size_t>(k) * a.stride + t;
+            uint32_t const b = a.sl.idx[o];
+            g = add(g, scale(a.solids.vol[b], a.sl.gw[o]));
+        }
+        F = add(F, scale(-M * c.rho0 * por, g));
+    }
+    a.force[t] = F;
+    float const dtm = c.dt / c.mass;
+    V2 const vn = add(a.vext[t], scale(dtm, F));
+    a.vstar_out[t] = vn;
+    a.xstar[t] = add(a.pos[t], scale(c.dt, vn));
+}
+
+// :151-163 final update: v = v_ext + (dt/m) F; x += dt v. Both equal the last
+// predictor written by k_pci_force.
+__global__ __launch_bounds__(SWEEP_THREADS) void k_pci_finalize(SweepArgs a) {
+    uint32_t const t = blockIdx.x * SWEEP_THREADS + threadIdx.x;
+    unsigned int vm = 0;
+    if (t < a.n) {
+        V2 const v = a.vstar_in[t];
+        uint32_t const o = a.orig[t];
+        a.pos_orig[o] = a.xstar[t];
+        a.vel_orig[o] = v;
+        vm = __float_as_uint(dot(v, v));
+    }
+    block_reduce_max<SWEEP_THREADS>(vm, a.vmax2_bits);
+}
+
+//------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_vmax2(V2 const* vel, uint32_t n, unsigned int* out) {
+    uint32_t const t = blockIdx.x * 256 + threadIdx.x;
+    unsigned int vm = 0;
+    if (t < n) { vm = __float_as_uint(dot(vel[t], vel[t])); }
+    block_reduce_max<256>(vm, out);
+}
+
+__global__ __launch_bounds__(256) void k_scatter_f32(float const* src, uint32_t const* orig,
+                                                     float* dst, uint32_t n) {
+    uint32_t const t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n) { dst[orig[t]] = src[t]; }
+}
+__global__ __launch_bounds__(256) void k_scatter_v2(V2 const* src, uint32_t const* orig, V2* dst,
+                                                    uint32_t n) {
+    uint32_t const t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n) { dst[orig[t]] = src[t]; }
+}
+
+__global__ __launch_bounds__(256) void k_export_lists(Lists l, uint32_t n, uint32_t stride,
+                                                      uint32_t const* orig_self,
+                                                      uint32_t const* orig_other,
+                                                      uint8_t* counts, uint32_t* idx,
+                                                      float* dist, V2* vec) {
+    uint32_t const t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) { return; }
+    uint32_t const o = orig_self[t];
+    int const cnt = l.count[t];
+    counts[o] = static_cast<uint8_t>(cnt);
+    for (int k = 0; k < MAXN; ++k) {
+        size_t const src = static_cast<size_t>(k) * stride + t;
+        size_t const dst = static_cast<size_t>(o) * MAXN + k;
+        if (k < cnt) {
+            idx[dst] = orig_other[l.idx[src]];
+            if (dist) { dist[dst] = l.dist[src]; }
+            if (vec) { vec[dst] = l.vec[src]; }
+        } else {
+            idx[dst] = 0;
+            if (dist) { dist[dst] = 0.0f; }
+            if (vec) { vec[dst] = V2{0.0f, 0.0f}; }
+        }
+    }
+}
+
+//------------------------------------------------------------------------------
+static inline dim3 sweep_grid(uint32_t n) { return dim3((n + SWEEP_THREADS - 1) / SWEEP_THREADS); }
+
+void launch_iisph_aii(SweepArgs const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_iisph_aii, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+}
+void launch_iisph_sdp(SweepArgs const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_iisph_sdp, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+}
+void launch_iisph_beta_p(SweepArgs const& a, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_iisph_beta_p, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a, slot);
+}
+void launch_apply(SweepArgs const& a, int with_gravity, int finalize, hipStream_t s) {
+    if (with_gravity && finalize) {
+        hipLaunchKernelGGL((k_apply<1, 1>), sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+    } else if (with_gravity) {
+        hipLaunchKernelGGL((k_apply<1, 0>), sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+    } else if (finalize) {
+        hipLaunchKernelGGL((k_apply<0, 1>), sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_apply<0, 0>), sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+    }
+}
+void launch_finalize(SweepArgs const& a, int dfsph, hipStream_t s) {
+    if (dfsph) {
+        hipLaunchKernelGGL(k_finalize<1>, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_finalize<0>, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+    }
+}
+void launch_dfsph_div(SweepArgs const& a, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_dfsph_div, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a, slot);
+}
+void launch_dfsph_dens(SweepArgs const& a, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_dfsph_dens, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a, slot);
+}
+void launch_pci_init(SweepArgs const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_init, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+}
+void launch_pci_density(SweepArgs const& a, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_density, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a, slot);
+}
+void launch_pci_force(SweepArgs const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_force, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+}
+void launch_pci_finalize(SweepArgs const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_finalize, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+}
+void launch_vmax2(V2 const* vel, uint32_t n, unsigned int* out, hipStream_t s) {
+    if (!n) { return; }
+    hipLaunchKernelGGL(k_vmax2, dim3((n + 255) / 256), dim3(256), 0, s, vel, n, out);
+}
+void launch_scatter_f32(float const* src, uint32_t const* orig, float* dst, uint32_t n,
+                        hipStream_t s) {
+    if (!n) { return; }
+    hipLaunchKernelGGL(k_scatter_f32, dim3((n + 255) / 256), dim3(256), 0, s, src, orig, dst, n);
+}
+void launch_scatter_v2(V2 const* src, uint32_t const* orig, V2* dst, uint32_t n,
+                       hipStream_t s) {
+    if (!n) { return; }
+    hipLaunchKernelGGL(k_scatter_v2, dim3((n + 255) / 256), dim3(256), 0, s, src, orig, dst, n);
+}
+void launch_export_lists(Lists const& l, uint32_t n, uint32_t stride, uint32_t const* orig_self,
+                         uint32_t const* orig_other, uint8_t* counts, uint32_t* idx, float* dist,
+                         V2* vec, hipStream_t s) {
+    if (!n) { return; }
+    hipLaunchKernelGGL(k_export_lists, dim3((n + 255) / 256), dim3(256), 0, s, l, n, stride,
+                       orig_self, orig_other, counts, idx, dist, vec);
+}
+
+}  // namespace emsph

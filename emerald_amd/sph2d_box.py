@@ -1,0 +1,357 @@
+"""Python mirror of the reference's sph2d_box step API over the gfx950 C ABI.
+
+Names, argument meaning and error behaviour follow the reference:
+  Parameters                 sph2d_box/parameters.h:12-43 (parse_parameters: parameters.cpp:10-44)
+  Simulation_config          sph2d_box/config.h:7-18, config.cpp:16-132
+  State / Solid_state        sph2d_box/state.h:13-35
+  world_walls_initial_solid_state   solids.cpp:139-213
+  dam_break_initial_state    initial_state.cpp:14-97
+  <method>_simulation_step   iisph.cpp:168-188, pcisph.cpp:166-204, dfsph_p.cpp:376-396
+  EZ_EXAMPLE_SIM             simulation.h:24-44, simulation.cpp:14-93
+
+The per-sub-step work runs on the GPU through libemerald_sph.so; this module
+only moves host arrays in and out and holds the frame clock (flicks).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+
+FLICKS_PER_SECOND = 705_600_000
+K_FLICKS_ONE_NINETIETH_OF_SECOND = FLICKS_PER_SECOND // 90
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclasses.dataclass
+class IisphParameters:
+    max_pressure_iterations: int = 30
+    error_average_threshold: float = 0.005
+    error_max_threshold: float = 0.05
+    omega: float = 0.5
+
+
+@dataclasses.dataclass
+class DfsphParameters:
+    max_correction_iterations: int = 15
+    density_error_average_threshold: float = 0.0035
+    density_error_max_threshold: float = 0.03
+    divergence_error_average_threshold: float = 0.0035
+    divergence_error_max_threshold: float = 0.03
+
+
+@dataclasses.dataclass
+class Parameters:
+    """sph2d_box/parameters.h:12-43 (same field names and defaults)."""
+    seed: int = 1
+    time_per_step: int = 3 * K_FLICKS_ONE_NINETIETH_OF_SECOND  # flicks
+    sub_steps: int = 30
+    length: float = 1.0
+    support: float = 0.025
+    gravity: float = 9.81
+    viscosity: float = 0.01
+    target_density: float = 1000.0
+    iisph: IisphParameters = dataclasses.field(default_factory=IisphParameters)
+    dfsph: DfsphParameters = dataclasses.field(default_factory=DfsphParameters)
+    num_batch_steps: int = 100
+    method: str = "iisph_pseudo_ap"
+
+    def to_c(self) -> _capi.Parameters:
+        p = _capi.Parameters()
+        p.seed = self.seed
+        p.time_per_step = self.time_per_step
+        p.sub_steps = self.sub_steps
+        p.length = self.length
+        p.support = self.support
+        p.gravity = self.gravity
+        p.viscosity = self.viscosity
+        p.target_density = self.target_density
+        p.iisph_max_pressure_iterations = self.iisph.max_pressure_iterations
+        p.iisph_error_average_threshold = self.iisph.error_average_threshold
+        p.iisph_error_max_threshold = self.iisph.error_max_threshold
+        p.iisph_omega = self.iisph.omega
+        p.dfsph_max_correction_iterations = self.dfsph.max_correction_iterations
+        p.dfsph_density_error_average_threshold = self.dfsph.density_error_average_threshold
+        p.dfsph_density_error_max_threshold = self.dfsph.density_error_max_threshold
+        p.dfsph_divergence_error_average_threshold = self.dfsph.divergence_error_average_threshold
+        p.dfsph_divergence_error_max_threshold = self.dfsph.divergence_error_max_threshold
+        p.num_batch_steps = self.num_batch_steps
+        p.method = _capi.METHODS.get(self.method, -1)
+        return p
+
+
+def parse_parameters(argv) -> Parameters:
+    """parameters.cpp:10-44: --seed --sub_steps --length --support --gravity
+    --viscosity --density -n/--num_steps --method."""
+    ap = argparse.ArgumentParser(prog="sph2d_box")
+    d = Parameters()
+    ap.add_argument("--seed", type=int, default=d.seed)
+    ap.add_argument("--sub_steps", type=int, default=d.sub_steps)
+    ap.add_argument("--length", type=float, default=d.length)
+    ap.add_argument("--support", type=float, default=d.support)
+    ap.add_argument("--gravity", type=float, default=d.gravity)
+    ap.add_argument("--viscosity", type=float, default=d.viscosity)
+    ap.add_argument("--density", type=float, default=d.target_density)
+    ap.add_argument("-n", "--num_steps", type=int, default=d.num_batch_steps)
+    ap.add_argument("--method", type=str, default=d.method)
+    a = ap.parse_args(argv)
+    return Parameters(seed=a.seed, sub_steps=a.sub_steps, length=a.length, support=a.support,
+                      gravity=a.gravity, viscosity=a.viscosity, target_density=a.density,
+                      num_batch_steps=a.num_steps, method=a.method)
+
+
+@dataclasses.dataclass
+class Simulation_config:  # noqa: N801 - the reference's type name
+    """config.cpp:16-132. Optional overrides replace the calibrated values."""
+    params: Parameters
+    seconds_per_sub_step: float = 0.0
+    mass_per_particle: float = 0.0
+    draw_radius: float = 0.0
+    pressure_correction_denom: float = 0.0
+
+    @staticmethod
+    def from_parameters(params: Parameters, **overrides) -> "Simulation_config":
+        c = _capi.Config()
+        cp = params.to_c()
+        check(lib().emsph_config_from_parameters(ctypes.byref(cp), ctypes.byref(c)),
+              "emsph_config_from_parameters")
+        cfg = Simulation_config(params, float(c.seconds_per_sub_step),
+                                float(c.mass_per_particle), float(c.draw_radius),
+                                float(c.pressure_correction_denom))
+        for k, v in overrides.items():
+            if not hasattr(cfg, k):
+                raise AttributeError(k)
+            setattr(cfg, k, v)
+        return cfg
+
+    def to_c(self) -> _capi.Config:
+        c = _capi.Config()
+        c.params = self.params.to_c()
+        c.seconds_per_sub_step = self.seconds_per_sub_step
+        c.mass_per_particle = self.mass_per_particle
+        c.draw_radius = self.draw_radius
+        c.pressure_correction_denom = self.pressure_correction_denom
+        return c
+
+
+@dataclasses.dataclass
+class State:
+    """state.h:13-17 (colors are display-only and not carried)."""
+    positions: np.ndarray   # [N, 2] float32
+    velocities: np.ndarray  # [N, 2] float32
+
+
+@dataclasses.dataclass
+class Solid_state:  # noqa: N801
+    """state.h:19-35: positions (static); volumes are computed on the device."""
+    positions: np.ndarray
+    velocities: Optional[np.ndarray] = None
+
+
+def emit_solid_box(box, R: float) -> np.ndarray:
+    """solids.cpp:13-50."""
+    (x0, y0), (x1, y1) = box
+    n = lib().emsph_estimate_solid_box_emission_count(x0, y0, x1, y1, R)
+    out = np.empty((max(n, 1), 2), np.float32)
+    m = lib().emsph_emit_solid_box(x0, y0, x1, y1, R, n, _ptr(out))
+    return out[:m].copy()
+
+
+def world_walls_initial_solid_state(params: Parameters, random_obstacles: bool = True) -> Solid_state:
+    """solids.cpp:139-213 (walls 3h thick; optional 16 random obstacle blocks)."""
+    p = params.to_c()
+    n = lib().emsph_world_walls_solid_positions(ctypes.byref(p), 1 if random_obstacles else 0,
+                                                None, 0)
+    out = np.empty((n, 2), np.float32)
+    lib().emsph_world_walls_solid_positions(ctypes.byref(p), 1 if random_obstacles else 0,
+                                            _ptr(out), n)
+    return Solid_state(out, np.zeros_like(out))
+
+
+def dam_break_initial_state(params: Parameters, solid_state: Solid_state) -> State:
+    """initial_state.cpp:14-97: hex block [0, L/2] x [0, 3.25 L/4], then drop
+    fluid particles closer than 2R to any solid."""
+    H = np.float32(params.support)
+    R = np.float32(0.99) * (H * np.float32(0.5))
+    L = np.float32(params.length)
+    pos = emit_solid_box(((0.0, 0.0), (float(L / np.float32(2.0)),
+                                       float(np.float32(3.25) * L / np.float32(4.0)))), float(R))
+    if len(solid_state.positions):
+        from scipy.spatial import cKDTree  # scene generation only (host side)
+        tree = cKDTree(solid_state.positions.astype(np.float64))
+        d, _ = tree.query(pos.astype(np.float64), k=1)
+        pos = pos[~(d < 2.0 * float(R))]
+    return State(np.ascontiguousarray(pos, np.float32), np.zeros_like(pos, dtype=np.float32))
+
+
+class Context:
+    """Owns one device-resident scene (emsph_context)."""
+
+    def __init__(self, config: Simulation_config, solid_state: Solid_state, capacity: int):
+        self.config = config
+        c = config.to_c()
+        self._cfg = c
+        sp = np.ascontiguousarray(solid_state.positions, np.float32).reshape(-1, 2)
+        h = ctypes.c_void_p()
+        check(lib().emsph_create(ctypes.byref(c), _ptr(sp), len(sp), int(capacity),
+                                 ctypes.byref(h)), "emsph_create")
+        self._h = h
+        self.capacity = int(capacity)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().emsph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def n(self) -> int:
+        return int(lib().emsph_fluid_count(self._h))
+
+    def set_store_geometry(self, enable: bool) -> None:
+        check(lib().emsph_set_store_geometry(self._h, 1 if enable else 0), "set_store_geometry")
+
+    def set_state(self, state: State) -> None:
+        pos = np.ascontiguousarray(state.positions, np.float32).reshape(-1, 2)
+        vel = None if state.velocities is None else np.ascontiguousarray(
+            state.velocities, np.float32).reshape(-1, 2)
+        check(lib().emsph_set_state(self._h, _ptr(pos), _ptr(vel), len(pos)), "set_state")
+
+    def get_state(self) -> State:
+        n = self.n
+        pos = np.empty((n, 2), np.float32)
+        vel = np.empty((n, 2), np.float32)
+        check(lib().emsph_get_state(self._h, _ptr(pos), _ptr(vel)), "get_state")
+        return State(pos, vel)
+
+    def field(self, name: str) -> np.ndarray:
+        fid, comps = _capi.FIELDS[name]
+        out = np.empty((self.n, comps), np.float32)
+        check(lib().emsph_get_field(self._h, fid, _ptr(out)), f"get_field({name})")
+        return out[:, 0].copy() if comps == 1 else out
+
+    def solid_volumes(self) -> np.ndarray:
+        n = int(lib().emsph_solid_count(self._h))
+        out = np.empty(n, np.float32)
+        check(lib().emsph_get_solid_volumes(self._h, _ptr(out)), "get_solid_volumes")
+        return out
+
+    def neighborhood(self, which: int = 0, geometry: bool = False):
+        n = self.n
+        counts = np.empty(n, np.uint8)
+        idx = np.empty((n, 64), np.uint32)
+        dist = np.empty((n, 64), np.float32) if geometry else None
+        vec = np.empty((n, 64, 2), np.float32) if geometry else None
+        check(lib().emsph_get_neighborhood(self._h, which, _ptr(counts), _ptr(idx), _ptr(dist),
+                                           _ptr(vec)), "get_neighborhood")
+        return counts, idx, dist, vec
+
+    def frame_init(self, method: str) -> None:
+        check(lib().emsph_frame_init(self._h, _capi.METHODS[method]), "frame_init")
+
+    def sub_step(self, method: str, t: float, dt: float) -> _capi.StepStats:
+        st = _capi.StepStats()
+        check(lib().emsph_sub_step(self._h, _capi.METHODS[method], t, dt, ctypes.byref(st)),
+              "sub_step")
+        return st
+
+    def frame_step(self, method: str, global_time_flicks: int) -> _capi.FrameStats:
+        fs = _capi.FrameStats()
+        check(lib().emsph_frame_step(self._h, _capi.METHODS[method], int(global_time_flicks),
+                                     ctypes.byref(fs)), "frame_step")
+        return fs
+
+    def run_sub_steps(self, method: str, dt: float, n: int):
+        ms = ctypes.c_float()
+        st = _capi.StepStats()
+        check(lib().emsph_run_sub_steps(self._h, _capi.METHODS[method], dt, int(n),
+                                        ctypes.byref(ms), ctypes.byref(st)), "run_sub_steps")
+        return float(ms.value), st
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        check(lib().emsph_set_kernel_timing(self._h, 1 if enable else 0), "set_kernel_timing")
+
+    def kernel_timing(self) -> dict:
+        m = 64
+        names = (ctypes.c_char_p * m)()
+        ms = (ctypes.c_double * m)()
+        calls = (ctypes.c_int64 * m)()
+        k = lib().emsph_get_kernel_timing(self._h, names, ms, calls, m)
+        return {names[i].decode(): (ms[i], calls[i]) for i in range(k)}
+
+
+def _simulation_step(method: str, global_time: int, config: Simulation_config, state: State,
+                     solid_state: Solid_state, ctx: Optional[Context] = None) -> State:
+    own = ctx is None
+    if own:
+        ctx = Context(config, solid_state, len(state.positions))
+    try:
+        ctx.set_state(state)
+        ctx.frame_step(method, global_time)
+        return ctx.get_state()
+    finally:
+        if own:
+            ctx.close()
+
+
+def iisph_simulation_step(global_time, config, state, solid_state, ctx=None) -> State:
+    """iisph.cpp:168-188 (adaptive 60/6, CFL 0.2)."""
+    return _simulation_step("iisph", global_time, config, state, solid_state, ctx)
+
+
+def pcisph_simulation_step(global_time, config, state, solid_state, ctx=None) -> State:
+    """pcisph.cpp:166-204 (fixed sub-steps of config.seconds_per_sub_step)."""
+    return _simulation_step("pcisph", global_time, config, state, solid_state, ctx)
+
+
+def dfsph_p_simulation_step(global_time, config, state, solid_state, ctx=None) -> State:
+    """dfsph_p.cpp:376-396 (adaptive 60/4, CFL 0.4)."""
+    return _simulation_step("dfsph_p", global_time, config, state, solid_state, ctx)
+
+
+class EZ_EXAMPLE_SIM:  # noqa: N801 - the reference's type name
+    """simulation.h:24-44: parameters -> solids -> fluid -> step(method)."""
+
+    def __init__(self, params: Parameters, solid_initial_state=world_walls_initial_solid_state,
+                 fluid_initial_state=dam_break_initial_state, config: Simulation_config = None):
+        if params.method not in ("pcisph", "iisph", "iisph_ap", "iisph_pseudo_ap", "dfsph_p"):
+            # simulation.cpp:37 EMLD_FAIL
+            raise ValueError("Unknown simulation method: " + params.method)
+        self.params = params
+        self.config = config or Simulation_config.from_parameters(params)
+        self.solid_state = solid_initial_state(params)
+        self.state = fluid_initial_state(params, self.solid_state)
+        self.method = params.method
+        self.time = 0  # flicks
+        self.ctx = Context(self.config, self.solid_state, max(len(self.state.positions), 1))
+        self.ctx.set_state(self.state)
+
+    def step(self, override_method: Optional[str] = None) -> _capi.FrameStats:
+        fs = self.ctx.frame_step(override_method or self.method, self.time)
+        self.time += self.params.time_per_step
+        return fs
+
+    def fetch_state(self) -> State:
+        self.state = self.ctx.get_state()
+        return self.state

@@ -1,0 +1,221 @@
+/* emerald_sph.h - C ABI of the MI355X (gfx950) 2D SPH step.
+ *
+ * Drop-in for the CPU step of the reference's sph2d_box (paths relative to
+ * the reference root). The reference has no FFI layer; its boundary is the
+ * in-process C++ API below, which this ABI replaces entry by entry:
+ *
+ *   Parameters / parse defaults   emerald/sph2d_box/parameters.h:12-43
+ *   Simulation_config ctor        emerald/sph2d_box/config.cpp:16-132
+ *   emit_solid_box                emerald/sph2d_box/solids.cpp:13-50
+ *   world_walls_initial_solid_state + compute_neighbor_data_and_volumes
+ *                                 emerald/sph2d_box/solids.cpp:76-213
+ *   <method>_sub_step             iisph.cpp:23-151, pcisph.cpp:36-164,
+ *                                 dfsph_p.cpp:265-374
+ *   <method>_simulation_step      iisph.cpp:168-188, pcisph.cpp:166-204,
+ *                                 dfsph_p.cpp:376-396 via
+ *                                 adaptive_time_step.h:35-122
+ *   EZ_EXAMPLE_SIM::Method        emerald/sph2d_box/simulation.h:24-44
+ *   neighbour search              emerald/sph_common/neighborhood.h:198-282,
+ *                                 neighborhoods_from_state.cpp:9-67
+ *
+ * Conventions: plain C types; positions/velocities are interleaved xy floats
+ * (the reference's Imath V2f arrays); every call returns an emsph_status;
+ * no exception crosses the ABI. A context owns device-resident state and one
+ * HIP stream; distinct contexts may be driven from distinct threads, one
+ * context is not re-entrant. Host buffers are copied in or out only by the
+ * set/get entry points.
+ */
+#ifndef EMERALD_SPH_H
+#define EMERALD_SPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum emsph_status {
+    EMSPH_OK = 0,
+    EMSPH_ERR_INVALID_ARGUMENT = 1,
+    EMSPH_ERR_OUT_OF_MEMORY = 2,
+    EMSPH_ERR_HIP = 3,
+    EMSPH_ERR_UNKNOWN_METHOD = 4, /* simulation.cpp:26-38 aborts here */
+    EMSPH_ERR_CAPACITY = 5,
+    EMSPH_ERR_NO_DEVICE = 6
+} emsph_status;
+
+/* EZ_EXAMPLE_SIM::Method (simulation.h:26-32). IISPH_AP and IISPH_PSEUDO_AP
+ * are not implemented and return EMSPH_ERR_UNKNOWN_METHOD. */
+typedef enum emsph_method {
+    EMSPH_PCISPH = 0,
+    EMSPH_IISPH = 1,
+    EMSPH_IISPH_AP = 2,
+    EMSPH_IISPH_PSEUDO_AP = 3,
+    EMSPH_DFSPH_P = 4
+} emsph_method;
+
+/* Parameters (parameters.h:12-43); time_per_step in flicks (1/705600000 s). */
+typedef struct emsph_parameters {
+    uint64_t seed;
+    int64_t time_per_step;
+    int32_t sub_steps;
+    float length;
+    float support;
+    float gravity;
+    float viscosity;
+    float target_density;
+    int32_t iisph_max_pressure_iterations;
+    float iisph_error_average_threshold;
+    float iisph_error_max_threshold;
+    float iisph_omega;
+    int32_t dfsph_max_correction_iterations;
+    float dfsph_density_error_average_threshold;
+    float dfsph_density_error_max_threshold;
+    float dfsph_divergence_error_average_threshold;
+    float dfsph_divergence_error_max_threshold;
+    int32_t num_batch_steps;
+    int32_t method; /* emsph_method */
+} emsph_parameters;
+
+/* Simulation_config (config.h:7-18). */
+typedef struct emsph_config {
+    emsph_parameters params;
+    float seconds_per_sub_step;
+    float mass_per_particle;
+    float draw_radius;
+    float pressure_correction_denom;
+} emsph_config;
+
+/* Per-sub-step solver statistics (the reference prints none of these; its
+ * commented-out prints are at iisph.cpp:121-136). */
+typedef struct emsph_step_stats {
+    int32_t iterations;            /* IISPH/PCISPH pressure or DFSPH density iterations */
+    int32_t divergence_iterations; /* DFSPH divergence iterations */
+    float error_average;           /* last residual average (quantised, as the reference) */
+    float error_max;               /* last residual max */
+    float divergence_error_average;
+    float divergence_error_max;
+    float max_velocity_sq;         /* max |v|^2 after the sub-step */
+    uint32_t cell_count;           /* occupied grid cells */
+    uint32_t sort_passes;          /* 8-bit radix passes of the key sort */
+    uint32_t max_neighbors;        /* reserved (0) */
+} emsph_step_stats;
+
+typedef struct emsph_frame_stats {
+    int32_t sub_steps;
+    int32_t total_iterations;
+    int32_t total_divergence_iterations;
+    float last_dt;
+    emsph_step_stats last;
+} emsph_frame_stats;
+
+typedef enum emsph_field {
+    EMSPH_FIELD_DENSITY = 0,
+    EMSPH_FIELD_DENSITY_STAR = 1,
+    EMSPH_FIELD_PRESSURE = 2,
+    EMSPH_FIELD_AII = 3,
+    EMSPH_FIELD_DII = 4,          /* 2 floats per particle */
+    EMSPH_FIELD_SUM_DIJ_PJ = 5,   /* 2 floats per particle */
+    EMSPH_FIELD_ALPHA = 6,
+    EMSPH_FIELD_KAPPA_DENSITY = 7,
+    EMSPH_FIELD_KAPPA_DIVERGENCE = 8,
+    EMSPH_FIELD_PRESSURE_FORCE = 9 /* 2 floats per particle */
+} emsph_field;
+
+typedef struct emsph_context emsph_context;
+
+/* Last error message of the calling thread (never NULL). */
+const char* emsph_last_error(void);
+/* Library build identification (gfx target, flags). */
+const char* emsph_build_info(void);
+/* Number of visible HIP devices (0 when none; never fails). */
+int emsph_device_count(void);
+
+/* Parameters defaults (parameters.h:12-43). */
+emsph_status emsph_default_parameters(emsph_parameters* out);
+/* Simulation_config ctor: hex calibration block -> mass per particle, seconds
+ * per sub-step, draw radius and the PCISPH correction denominator
+ * (config.cpp:16-132). */
+emsph_status emsph_config_from_parameters(const emsph_parameters* params, emsph_config* out);
+
+/* solids.cpp:13-20 / :22-50. Returns the count. */
+size_t emsph_estimate_solid_box_emission_count(float min_x, float min_y, float max_x,
+                                               float max_y, float R);
+size_t emsph_emit_solid_box(float min_x, float min_y, float max_x, float max_y, float R,
+                            size_t max_count, float* out_xy);
+/* world_walls_initial_solid_state positions (solids.cpp:139-206): the four
+ * walls, plus the 16 random obstacle blocks when random_obstacles != 0.
+ * out_xy may be NULL to query the count. Returns the count. */
+size_t emsph_world_walls_solid_positions(const emsph_parameters* params, int random_obstacles,
+                                         float* out_xy, size_t capacity);
+
+/* Create a context on the current HIP device with static solids (their
+ * neighbourhoods and volumes are computed on the device, solids.cpp:76-137)
+ * and room for up to fluid_capacity fluid particles. */
+emsph_status emsph_create(const emsph_config* config, const float* solid_xy, size_t solid_count,
+                          size_t fluid_capacity, emsph_context** out);
+emsph_status emsph_destroy(emsph_context* ctx);
+/* Store neighbour distances/vectors for emsph_get_neighborhood (off by default). */
+emsph_status emsph_set_store_geometry(emsph_context* ctx, int enable);
+
+/* Upload/download the fluid State (state.h:13-17); vel may be NULL (zero). */
+emsph_status emsph_set_state(emsph_context* ctx, const float* pos_xy, const float* vel_xy,
+                             size_t count);
+emsph_status emsph_get_state(emsph_context* ctx, float* pos_xy, float* vel_xy);
+size_t emsph_fluid_count(const emsph_context* ctx);
+size_t emsph_solid_count(const emsph_context* ctx);
+/* Solid volumes in original solid order (solids.cpp:52-74). */
+emsph_status emsph_get_solid_volumes(emsph_context* ctx, float* out);
+/* A per-particle field of the last sub-step, original particle order. */
+emsph_status emsph_get_field(emsph_context* ctx, int field, float* out);
+/* Neighbourhood of the last sub-step, original order, [count][64] layout:
+ * which = 0 fluid, 1 solid. dist/vec need emsph_set_store_geometry(1) before
+ * the sub-step; pass NULL to skip them. */
+emsph_status emsph_get_neighborhood(emsph_context* ctx, int which, uint8_t* counts,
+                                    uint32_t* indices, float* distances, float* vectors_xy);
+
+/* Frame-start initialisation of the reference's std_adaptive_time_step and
+ * DFSPH temps (constant volumes, zeroed DFSPH stiffness warm starts). */
+emsph_status emsph_frame_init(emsph_context* ctx, int method);
+/* One sub-step of fixed dt: the drop-in for <method>_sub_step. */
+emsph_status emsph_sub_step(emsph_context* ctx, int method, float time_seconds, float dt,
+                            emsph_step_stats* stats);
+/* One frame: the drop-in for <method>_simulation_step (adaptive CFL sub-steps
+ * in 1/1800 s quanta for IISPH/DFSPH; fixed sub-steps for PCISPH). */
+emsph_status emsph_frame_step(emsph_context* ctx, int method, int64_t global_time_flicks,
+                              emsph_frame_stats* stats);
+/* Run n_sub_steps fixed-dt sub-steps back to back and report the elapsed
+ * device time measured with HIP events on the context's stream. */
+emsph_status emsph_run_sub_steps(emsph_context* ctx, int method, float dt, int n_sub_steps,
+                                 float* elapsed_ms, emsph_step_stats* last);
+/* Per-kernel-class device timing (HIP events around each launch; adds
+ * synchronisation, for measurement only). names/ms/calls arrays of size max. */
+emsph_status emsph_set_kernel_timing(emsph_context* ctx, int enable);
+int emsph_get_kernel_timing(emsph_context* ctx, const char** names, double* total_ms,
+                            int64_t* calls, int max);
+
+/* Stand-alone z-index sort and neighbour search on one point set (no solids,
+ * no dynamics): the reference's compute_all_neighbhorhoods fluid part. */
+typedef struct emsph_search emsph_search;
+emsph_status emsph_search_create(size_t capacity, emsph_search** out);
+emsph_status emsph_search_destroy(emsph_search* s);
+emsph_status emsph_search_set_points(emsph_search* s, const float* xy, size_t count);
+/* Keys, sort, cells and 64-capped neighbour lists on the device. */
+emsph_status emsph_search_run(emsph_search* s, float cell_size, int store_geometry,
+                              float* elapsed_ms);
+/* Results in the reference's layouts: grid coords [n][2], z keys [n], sorted
+ * (key, index) pairs split in two arrays, block indices [n], block ranges
+ * [blocks], neighbour counts [n] and indices/distances/vectors [n][64]. Any
+ * output may be NULL. Returns the block count through block_count. */
+emsph_status emsph_search_get(emsph_search* s, int32_t* grid_xy, uint64_t* keys,
+                              uint64_t* sorted_keys, uint32_t* sorted_idx,
+                              uint32_t* block_indices, uint32_t* block_begin,
+                              uint32_t* block_end, size_t* block_count, uint8_t* counts,
+                              uint32_t* indices, float* distances, float* vectors_xy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMERALD_SPH_H */

@@ -1,0 +1,102 @@
+"""GPU z-index keys, sort, cell ranges and neighbour lists vs the oracle, bit for bit.
+
+Reference behaviour: sph_common/common.cpp:43-76 (grid coords, keys),
+neighborhood.cpp:10-124 (pairs, sort, block indices, blocks, map),
+neighborhood.h:198-282 (64-capped lists in (distance, index) order).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from emerald_amd import scenes
+from emerald_amd.search import NeighborSearch
+from helpers import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(xy, cell, geometry=True):
+    xy = np.ascontiguousarray(xy, np.float32)
+    n = len(xy)
+    print(f"[search parity] n={n} cell={cell}", flush=True)
+    s = NeighborSearch(max(n, 1))
+    s.set_points(xy)
+    s.run(cell, store_geometry=geometry)
+    g = s.results(geometry=geometry)
+    o = oracle.spatial_index(xy, cell)
+    assert g["block_count"] == o["block_count"]
+    for k in ("grid", "keys", "sorted_keys", "sorted_idx", "block_indices", "block_begin",
+              "block_end"):
+        assert np.array_equal(g[k], o[k]), k
+    oc, oi, od, ov = oracle.neighborhoods(xy, xy, True, cell)
+    assert np.array_equal(g["counts"], oc), "counts"
+    mask = np.arange(64)[None, :] < oc[:, None]
+    assert np.array_equal(np.where(mask, g["indices"], 0), np.where(mask, oi, 0)), "indices"
+    if geometry:
+        assert_bits_equal(np.where(mask, g["distances"], 0), np.where(mask, od, 0), "distances")
+        m2 = mask[:, :, None]
+        assert_bits_equal(np.where(m2, g["vectors"], 0), np.where(m2, ov, 0), "vectors")
+    s.close()
+    return g
+
+
+def test_reference_fixture_7927(gpu):
+    # sph_common/tests/test_neighborhood.cpp:18-31 (mt19937_64{17272}, U(-100,100)^2,
+    # cell 2.1): negative cells give keys with the high bits set.
+    xy = oracle.fixture_points(17272, 7927, -100.0, 100.0)
+    g = _compare(xy, 2.1)
+    assert (g["keys"] > np.uint64(1 << 62)).any()
+
+
+def test_clustered_points_cap_64(gpu):
+    # Many candidates per particle: the 64-cap in scan order is live.
+    rng = np.random.default_rng(7)
+    core = (0.5 + 0.002 * rng.standard_normal((6000, 2))).astype(np.float32)
+    bg = rng.random((20000, 2)).astype(np.float32)
+    xy = np.concatenate([bg, core])
+    g = _compare(xy, 1.0 / 256.0)
+    assert g["counts"].max() == 64
+
+
+def test_lattice_distance_ties(gpu):
+    # An unjittered lattice has exact distance ties, broken by original index.
+    i, j = np.meshgrid(np.arange(60), np.arange(50), indexing="ij")
+    xy = np.stack([(i.ravel() + 0.5) * 0.005, (j.ravel() + 0.5) * 0.005], 1).astype(np.float32)
+    _compare(xy, 0.02)
+
+
+def test_search_points_sample(gpu):
+    xy = scenes.search_points(200_000, seed=3)
+    _compare(xy, 1.0 / 2048.0, geometry=False)
+
+
+def test_single_and_empty(gpu):
+    _compare(np.array([[0.25, 0.75]], np.float32), 0.1)
+    s = NeighborSearch(16)
+    s.set_points(np.zeros((0, 2), np.float32))
+    s.run(0.1)
+    assert s.results()["block_count"] == 0
+
+
+@pytest.mark.parametrize("n", [4_194_304])
+def test_full_size_search_sortedness(gpu, n):
+    """4,194,304 points, h = 1/4096, cell 1/2048: sorted order and lists against
+    size-independent properties (the oracle check runs on a sample)."""
+    xy = scenes.search_points(n, seed=3)
+    print(f"[search full] n={n}", flush=True)
+    s = NeighborSearch(n)
+    s.set_points(xy)
+    s.run(1.0 / 2048.0)
+    r = s.results()
+    sk = r["sorted_keys"]
+    si = r["sorted_idx"].astype(np.int64)
+    assert np.all(sk[1:] >= sk[:-1])
+    same = sk[1:] == sk[:-1]
+    assert np.all(si[1:][same] > si[:-1][same])       # ties keep index order
+    assert np.array_equal(np.sort(si), np.arange(n))  # a permutation
+    # Lists on a sample of particles against the oracle's full-array lists.
+    oc, oi, _, _ = oracle.neighborhoods(xy, xy, True, 1.0 / 2048.0)
+    assert np.array_equal(r["counts"], oc)
+    mask = np.arange(64)[None, :] < oc[:, None]
+    assert np.array_equal(np.where(mask, r["indices"], 0), np.where(mask, oi, 0))
