@@ -114,6 +114,7 @@ SIGNATURES = {
     "emsph_estimate_solid_box_emission_count": (_SZ, [_F, _F, _F, _F, _F]),
     "emsph_emit_solid_box": (_SZ, [_F, _F, _F, _F, _F, _SZ, _P]),
     "emsph_world_walls_solid_positions": (_SZ, [_P, _I, _P, _SZ]),
+    "emsph_dam_break_positions": (_SZ, [_P, _P, _SZ, _P, _SZ]),
     "emsph_create": (_ST, [_P, _P, _SZ, _SZ, _P]),
     "emsph_destroy": (_ST, [_P]),
     "emsph_set_store_geometry": (_ST, [_P, _I]),

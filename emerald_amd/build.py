@@ -66,7 +66,23 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_headless(verbose)
     return LIB
+
+
+HEADLESS = os.path.join(HERE, "sph2d_box_headless")
+
+
+def build_headless(verbose: bool = False) -> str:
+    """The C++ headless driver, linked against the C ABI only (plain g++)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", HEADLESS + ".tmp",
+           os.path.join(CSRC, "headless_main.cpp"), "-L" + HERE, "-lemerald_sph",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(HEADLESS + ".tmp", HEADLESS)
+    return HEADLESS
 
 
 def _wait(item) -> None:

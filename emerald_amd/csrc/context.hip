@@ -144,7 +144,8 @@ struct KernelTimer {
 
 // Small pinned read-back block.
 struct Pinned {
-    unsigned long long minmax[2];
+    KeyState keys_reset;   // source of the per-sort KeyState reset
+    KeyState keys;         // read back
     Reduction red[MAX_RED_SLOTS];
     unsigned int vmax2;
     unsigned int cells;
@@ -188,10 +189,14 @@ struct emsph_context {
 
     // Reductions.
     DBuf<Reduction> red;
-    DBuf<unsigned long long> minmax;
+    DBuf<KeyState> keystate;
     DBuf<unsigned int> vmax2;
     DBuf<unsigned int> cells_total;
     Pinned* host = nullptr;
+    // Radix passes of the previous sort (-1: read the key range before sorting).
+    int passes_hint = -1;
+    bool keys_checked = false;  // this sub-step's KeyState has been read back
+    bool sort_short = false;    // ... and showed the hint was too small
 
     KernelTimer timer;
 
@@ -217,7 +222,7 @@ struct emsph_search {
     DBuf<uint32_t> lidx;
     DBuf<float> dist;
     DBuf<V2> vec;
-    DBuf<unsigned long long> minmax;
+    DBuf<KeyState> keystate;
     DBuf<unsigned int> cells_total;
     Pinned* host = nullptr;
     ~emsph_search() {
@@ -248,9 +253,14 @@ hipError_t timed(emsph_context* c, char const* name, F&& launch) {
 }
 
 // Keys -> sort -> cells of n points at pos (original order), plus gathers.
-// Returns the number of radix passes through *passes.
+//
+// passes_hint < 0 reads the key range back before sorting (one host sync) and
+// sorts exactly the significant digits. passes_hint >= 0 sorts that many
+// digits without waiting; k_cell_heads then flags (KeyState::sort_short) a key
+// range the digits do not cover, and the caller must redo the work with the
+// exact count before anything it produced is used.
 hipError_t index_points(SpatialIndex& si, V2 const* pos, uint32_t n, float cell,
-                        hipStream_t s, Pinned* host, unsigned long long* d_minmax,
+                        hipStream_t s, Pinned* host, KeyState* d_keys, int passes_hint,
                         unsigned int* d_cells, Reduction* red_reset, unsigned int* vmax2_reset,
                         CellsArgs gathers, emsph_context* timer_ctx) {
     auto run = [&](char const* name, auto&& f) -> hipError_t {
@@ -266,19 +276,28 @@ hipError_t index_points(SpatialIndex& si, V2 const* pos, uint32_t n, float cell,
     ka.keys = si.keys0.p;
     ka.vals = si.vals0.p;
     ka.grid = si.grid.p;
-    ka.minmax = d_minmax;
+    ka.keystate = d_keys;
     ka.red_reset = red_reset;
     ka.vmax2_reset = vmax2_reset;
-    host->minmax[0] = ~0ull;
-    host->minmax[1] = 0ull;
-    if ((e = hipMemcpyAsync(d_minmax, host->minmax, 16, hipMemcpyHostToDevice, s))) { return e; }
-    if ((e = run("keys", [&] { launch_keys(ka, s); }))) { return e; }
-    if ((e = run("key_minmax", [&] { launch_key_minmax(si.keys0.p, n, d_minmax, s); }))) {
+    host->keys_reset.kmin = ~0ull;
+    host->keys_reset.kmax = 0ull;
+    host->keys_reset.sort_short = 0u;
+    host->keys_reset.pad = 0u;
+    if ((e = hipMemcpyAsync(d_keys, &host->keys_reset, sizeof(KeyState), hipMemcpyHostToDevice,
+                            s))) {
         return e;
     }
-    if ((e = hipMemcpyAsync(host->minmax, d_minmax, 16, hipMemcpyDeviceToHost, s))) { return e; }
-    if ((e = hipStreamSynchronize(s))) { return e; }
-    si.passes = radix_pass_count(host->minmax[0], host->minmax[1]);
+    if ((e = run("keys", [&] { launch_keys(ka, s); }))) { return e; }
+    if (passes_hint < 0) {
+        if ((e = hipMemcpyAsync(&host->keys, d_keys, sizeof(KeyState), hipMemcpyDeviceToHost,
+                                s))) {
+            return e;
+        }
+        if ((e = hipStreamSynchronize(s))) { return e; }
+        si.passes = radix_pass_count(host->keys.kmin, host->keys.kmax);
+    } else {
+        si.passes = passes_hint;
+    }
     int which = 0;
     if ((e = run("radix_sort", [&] {
              which = launch_radix_sort(si.keys0.p, si.vals0.p, si.keys1.p, si.vals1.p, n,
@@ -303,6 +322,8 @@ hipError_t index_points(SpatialIndex& si, V2 const* pos, uint32_t n, float cell,
     ca.mask = si.table_cap - 1;
     ca.orig = si.orig.p;
     ca.rank = si.rank.p;
+    ca.keystate = (passes_hint >= 0) ? d_keys : nullptr;
+    ca.passes = si.passes;
     if ((e = run("cells", [&] {
              launch_cell_heads(ca, s);
              launch_exclusive_scan(si.heads.p, si.head_scan.p, n, si.scan_scratch.p, d_cells, s);
@@ -374,9 +395,12 @@ emsph_status prepare(emsph_context* c, int build_mode, bool dfsph, float dt,
         g.kv_out = c->kvs.p;
         g.kd_out = c->kds.p;
     }
+    c->keys_checked = c->passes_hint < 0;  // an exact sort needs no check
+    c->sort_short = false;
     HIP_TRY(index_points(c->fidx, c->pos.p, n, c->cfg.params.support * 2.0f, c->stream, c->host,
-                         c->minmax.p, c->cells_total.p, c->red.p, c->vmax2.p, g,
-                         c->timer.enabled ? c : nullptr));
+                         c->keystate.p, c->passes_hint, c->cells_total.p, c->red.p, c->vmax2.p,
+                         g, c->timer.enabled ? c : nullptr));
+    if (c->passes_hint < 0) { c->passes_hint = c->fidx.passes; }
     BuildArgs b{};
     b.c = step_consts(c, dt);
     b.n = n;
@@ -432,11 +456,38 @@ SweepArgs sweep_args(emsph_context* c, float dt) {
 
 // Read one residual slot: {average, max} as the reference computes them
 // (iisph_ops.cpp:422-424, common.h:228-231).
+// The first read-back of a sub-step also fetches its KeyState: when the sort's
+// pass hint was too small, c->sort_short is set and the caller abandons the
+// sub-step (nothing in original order has been written yet) to redo it.
+hipError_t fetch_keystate_async(emsph_context* c) {
+    if (c->keys_checked) { return hipSuccess; }
+    return hipMemcpyAsync(&c->host->keys, c->keystate.p, sizeof(KeyState),
+                          hipMemcpyDeviceToHost, c->stream);
+}
+void note_keystate(emsph_context* c) {
+    if (c->keys_checked) { return; }
+    c->keys_checked = true;
+    int const exact = radix_pass_count(c->host->keys.kmin, c->host->keys.kmax);
+    c->sort_short = c->host->keys.sort_short != 0u;
+    // Next sub-step: the exact count of this one (keys move little per step).
+    c->passes_hint = exact;
+}
+hipError_t check_keys(emsph_context* c) {
+    if (c->keys_checked) { return hipSuccess; }
+    hipError_t e = fetch_keystate_async(c);
+    if (e) { return e; }
+    if ((e = hipStreamSynchronize(c->stream))) { return e; }
+    note_keystate(c);
+    return hipSuccess;
+}
+
 hipError_t read_residual(emsph_context* c, int slot, float* avg, float* mx) {
     hipError_t e = hipMemcpyAsync(&c->host->red[slot], &c->red.p[slot], sizeof(Reduction),
                                   hipMemcpyDeviceToHost, c->stream);
     if (e) { return e; }
+    if ((e = fetch_keystate_async(c))) { return e; }
     if ((e = hipStreamSynchronize(c->stream))) { return e; }
+    note_keystate(c);
     double const num = static_cast<double>(c->host->red[slot].isum);
     double const den = static_cast<double>(1000ull * c->n);
     *avg = static_cast<float>(num / den);
@@ -473,13 +524,20 @@ emsph_status iisph_sub_step(emsph_context* c, float dt, emsph_step_stats* st) {
         HIP_TRY(timed(c, "iisph_sdp", [&] { launch_iisph_sdp(a, c->stream); }));
         a.p_out = pb;
         HIP_TRY(timed(c, "iisph_beta_p", [&] { launch_iisph_beta_p(a, iter, c->stream); }));
-        HIP_TRY(read_residual(c, iter, &avg, &mx));
         std::swap(pa, pb);
+        // iisph.cpp:126: the exit test only runs from the third iteration, so
+        // the first two never wait for their residuals.
+        if (iter > 1 || iter + 1 == max_it) {
+            HIP_TRY(read_residual(c, iter, &avg, &mx));
+            if (c->sort_short) { return EMSPH_OK; }
+        }
         if (iter > 1 && avg <= P.iisph_error_average_threshold &&
             mx <= P.iisph_error_max_threshold) {
             break;
         }
     }
+    HIP_TRY(check_keys(c));
+    if (c->sort_short) { return EMSPH_OK; }
     a.p_in = pa;
     HIP_TRY(timed(c, "apply_integrate", [&] { launch_apply(a, 0, 1, c->stream); }));
     c->cur_p = pa;
@@ -510,6 +568,7 @@ emsph_status dfsph_sub_step(emsph_context* c, float dt, emsph_step_stats* st) {
         HIP_TRY(timed(c, "apply", [&] { launch_apply(a, 0, 0, c->stream); }));
         if (it > 0) {
             HIP_TRY(read_residual(c, it, &avg, &mx));
+            if (c->sort_short) { return EMSPH_OK; }
             if (avg <= P.dfsph_divergence_error_average_threshold &&
                 mx <= P.dfsph_divergence_error_max_threshold) {
                 break;
@@ -534,6 +593,7 @@ emsph_status dfsph_sub_step(emsph_context* c, float dt, emsph_step_stats* st) {
         HIP_TRY(timed(c, "apply", [&] { launch_apply(a, 0, 0, c->stream); }));
         if (it > 1) {
             HIP_TRY(read_residual(c, base + it, &avg, &mx));
+            if (c->sort_short) { return EMSPH_OK; }
             if (avg <= P.dfsph_density_error_average_threshold &&
                 mx <= P.dfsph_density_error_max_threshold) {
                 break;
@@ -545,6 +605,8 @@ emsph_status dfsph_sub_step(emsph_context* c, float dt, emsph_step_stats* st) {
         st->error_average = avg;
         st->error_max = mx;
     }
+    HIP_TRY(check_keys(c));
+    if (c->sort_short) { return EMSPH_OK; }
     HIP_TRY(timed(c, "finalize", [&] { launch_finalize(a, 1, c->stream); }));
     c->cur_p = c->kds.p;
     return EMSPH_OK;
@@ -575,10 +637,13 @@ emsph_status pcisph_sub_step(emsph_context* c, float dt, emsph_step_stats* st) {
             float avg_unused = 0.0f;
             float me = 0.0f;
             HIP_TRY(read_residual(c, it, &avg_unused, &me));
+            if (c->sort_short) { return EMSPH_OK; }
             err = me / c->cfg.params.target_density;
             if (err <= 0.01f) { break; }
         }
     }
+    HIP_TRY(check_keys(c));
+    if (c->sort_short) { return EMSPH_OK; }
     a.vstar_in = vin;
     HIP_TRY(timed(c, "pci_finalize", [&] { launch_pci_finalize(a, c->stream); }));
     c->cur_p = c->p0.p;
@@ -604,12 +669,19 @@ emsph_status sub_step_impl(emsph_context* c, int method, float dt, emsph_step_st
     if (s) { return s; }
     if (c->n == 0) { return EMSPH_OK; }  // the reference reads block_indices.back() on N=0
     HIP_TRY(hipSetDevice(c->device));
-    switch (method) {
-    case EMSPH_IISPH: s = iisph_sub_step(c, dt, st); break;
-    case EMSPH_DFSPH_P: s = dfsph_sub_step(c, dt, st); break;
-    default: s = pcisph_sub_step(c, dt, st); break;
+    // A sort run with the previous sub-step's pass count is checked at the
+    // first read-back; when it was short the sub-step is redone with the exact
+    // count (the second attempt sorts exactly and cannot be short).
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (attempt == 1) { c->passes_hint = -1; }
+        switch (method) {
+        case EMSPH_IISPH: s = iisph_sub_step(c, dt, st); break;
+        case EMSPH_DFSPH_P: s = dfsph_sub_step(c, dt, st); break;
+        default: s = pcisph_sub_step(c, dt, st); break;
+        }
+        if (s) { return s; }
+        if (!c->sort_short) { break; }
     }
-    if (s) { return s; }
     c->last_method = method;
     if (st) {
         HIP_TRY(hipMemcpyAsync(&c->host->cells, c->cells_total.p, 4, hipMemcpyDeviceToHost,
@@ -672,7 +744,7 @@ emsph_status emsph_create(const emsph_config* config, const float* solid_xy, siz
         (e = c->kds.alloc(cap)) || (e = c->fcount.alloc(c->stride)) ||
         (e = c->scount.alloc(c->stride)) || (e = c->fl_idx.alloc(L)) ||
         (e = c->sl_idx.alloc(L)) || (e = c->fl_gw.alloc(L)) || (e = c->sl_gw.alloc(L)) ||
-        (e = c->red.alloc(MAX_RED_SLOTS)) || (e = c->minmax.alloc(2)) ||
+        (e = c->red.alloc(MAX_RED_SLOTS)) || (e = c->keystate.alloc(1)) ||
         (e = c->vmax2.alloc(1)) || (e = c->cells_total.alloc(1))) {
         delete c;
         return fail(e == hipErrorOutOfMemory ? EMSPH_ERR_OUT_OF_MEMORY : EMSPH_ERR_HIP,
@@ -707,7 +779,7 @@ emsph_status emsph_create(const emsph_config* config, const float* solid_xy, siz
         g.pos_in = c->s_pos_orig.p;
         g.pos_out = c->s_pos.p;
         HIP_TRY(index_points(c->sidx, c->s_pos_orig.p, ns, config->params.support * 2.0f,
-                             c->stream, c->host, c->minmax.p, c->cells_total.p, nullptr, nullptr,
+                             c->stream, c->host, c->keystate.p, -1, c->cells_total.p, nullptr, nullptr,
                              g, nullptr));
         BuildArgs b{};
         b.c = step_consts(c, 0.0f);
@@ -751,6 +823,7 @@ emsph_status emsph_set_state(emsph_context* c, const float* pos_xy, const float*
     if (count > c->cap) { return fail(EMSPH_ERR_CAPACITY, "fluid count above capacity"); }
     c->n = static_cast<uint32_t>(count);
     c->have_lists = false;
+    c->passes_hint = -1;
     if (count) {
         HIP_TRY(hipMemcpyAsync(c->pos.p, pos_xy, count * sizeof(V2), hipMemcpyHostToDevice,
                                c->stream));
@@ -1018,7 +1091,7 @@ emsph_status emsph_search_create(size_t capacity, emsph_search** out) {
     hipError_t e;
     if ((e = s->pos.alloc(cap)) || (e = s->spos.alloc(cap)) || (e = s->idx.alloc(cap, true)) ||
         (e = s->count.alloc(s->stride)) || (e = s->lidx.alloc(L)) ||
-        (e = s->minmax.alloc(2)) || (e = s->cells_total.alloc(1))) {
+        (e = s->keystate.alloc(1)) || (e = s->cells_total.alloc(1))) {
         delete s;
         return fail(EMSPH_ERR_OUT_OF_MEMORY, std::string("search allocation: ") +
                                                hipGetErrorString(e));
@@ -1066,7 +1139,7 @@ emsph_status emsph_search_run(emsph_search* s, float cell_size, int store_geomet
         CellsArgs g{};
         g.pos_in = s->pos.p;
         g.pos_out = s->spos.p;
-        HIP_TRY(index_points(s->idx, s->pos.p, s->n, cell_size, s->stream, s->host, s->minmax.p,
+        HIP_TRY(index_points(s->idx, s->pos.p, s->n, cell_size, s->stream, s->host, s->keystate.p, -1,
                              s->cells_total.p, nullptr, nullptr, g, nullptr));
         BuildArgs b{};
         b.c.cell = cell_size;

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <random>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/emerald_sph.h"
@@ -237,6 +238,65 @@ size_t emsph_world_walls_solid_positions(const emsph_parameters* params, int ran
         std::memcpy(out_xy, xy.data(), 2 * m * sizeof(float));
     }
     return at;
+}
+
+// initial_state.cpp:14-97: a hex block [0, L/2] x [0, 3.25 L / 4] of radius R,
+// minus every fluid particle within 2R of a solid (searched in the 3x3 cells
+// of size 2H around it, as the reference does).
+size_t emsph_dam_break_positions(const emsph_parameters* params, const float* solid_xy,
+                                 size_t solid_count, float* out_xy, size_t capacity) {
+    if (!params) { return 0; }
+    float const H = params->support;
+    float const R = 0.99f * (H * 0.5f);
+    float const L = params->length;
+    float const x1 = L / 2.0f;
+    float const y1 = 3.25f * L / 4.0f;
+    size_t const cap = 100 + emsph_estimate_solid_box_emission_count(0.0f, 0.0f, x1, y1, R);
+    std::vector<float> xy(2 * cap);
+    size_t const n = emsph_emit_solid_box(0.0f, 0.0f, x1, y1, R, cap, xy.data());
+    float const cell = 2.0f * H;
+    auto cell_of = [cell](float x, float y) {
+        int32_t const gx = static_cast<int32_t>(std::floor(x / cell));
+        int32_t const gy = static_cast<int32_t>(std::floor(y / cell));
+        return std::pair<int32_t, int32_t>{gx, gy};
+    };
+    auto pack = [](int32_t gx, int32_t gy) {
+        return (static_cast<uint64_t>(static_cast<uint32_t>(gx)) << 32) |
+               static_cast<uint32_t>(gy);
+    };
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    for (size_t b = 0; b < solid_count; ++b) {
+        auto const g = cell_of(solid_xy[2 * b], solid_xy[2 * b + 1]);
+        grid[pack(g.first, g.second)].push_back(static_cast<uint32_t>(b));
+    }
+    size_t kept = 0;
+    for (size_t i = 0; i < n; ++i) {
+        float const px = xy[2 * i];
+        float const py = xy[2 * i + 1];
+        auto const g = cell_of(px, py);
+        bool kill = false;
+        for (int32_t j = g.second - 1; !kill && j <= g.second + 1; ++j) {
+            for (int32_t k = g.first - 1; !kill && k <= g.first + 1; ++k) {
+                auto const it = grid.find(pack(k, j));
+                if (it == grid.end()) { continue; }
+                for (uint32_t b : it->second) {
+                    float const d = imath_len(solid_xy[2 * b] - px, solid_xy[2 * b + 1] - py);
+                    if (d < 2 * R) {
+                        kill = true;
+                        break;
+                    }
+                }
+            }
+        }
+        if (!kill) {
+            if (out_xy && kept < capacity) {
+                out_xy[2 * kept] = px;
+                out_xy[2 * kept + 1] = py;
+            }
+            ++kept;
+        }
+    }
+    return kept;
 }
 
 }  // extern "C"

@@ -126,13 +126,21 @@ size_t scan_scratch_words(size_t n);
 size_t radix_counts_words(size_t n);
 void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32_t* scratch,
                            uint32_t* total, hipStream_t s);
-void launch_key_minmax(uint64_t const* keys, uint32_t n, unsigned long long* out, hipStream_t s);
 int radix_pass_count(uint64_t kmin, uint64_t kmax);
 int launch_radix_sort(uint64_t* keys0, uint32_t* vals0, uint64_t* keys1, uint32_t* vals1,
                       uint32_t n, int passes, uint32_t* counts, uint32_t* scan_scratch,
                       hipStream_t s);
 
 // sph_kernels.hip
+// Key range of one sort and the flag raised when the digits that were sorted
+// do not cover it (the sub-step is then redone with the exact pass count).
+struct KeyState {
+    unsigned long long kmin;
+    unsigned long long kmax;
+    unsigned int sort_short;
+    unsigned int pad;
+};
+
 struct KeysArgs {
     V2 const* pos;
     uint32_t n;
@@ -140,13 +148,15 @@ struct KeysArgs {
     uint64_t* keys;
     uint32_t* vals;
     int2* grid;                   // optional
-    unsigned long long* minmax;   // [2], reset by the launcher's first block
+    KeyState* keystate;           // min/max accumulated here (reset before the launch)
     Reduction* red_reset;         // optional: zeroed (MAX_RED_SLOTS entries)
     unsigned int* vmax2_reset;    // optional: zeroed
 };
 void launch_keys(KeysArgs const& a, hipStream_t s);
 
 struct CellsArgs {
+    KeyState* keystate;           // checked against `passes` by the first thread
+    int passes;
     uint64_t const* sorted_keys;
     uint32_t const* sorted_vals;
     uint32_t n;

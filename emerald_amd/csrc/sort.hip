@@ -113,28 +113,6 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
     }
 }
 
-// Min and max of the keys (for the significant-bit range).
-__global__ __launch_bounds__(256) void key_minmax(uint64_t const* __restrict__ keys, uint32_t n,
-                                                 unsigned long long* __restrict__ out) {
-    unsigned long long mn = ~0ull;
-    unsigned long long mx = 0ull;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        unsigned long long const k = keys[i];
-        mn = k < mn ? k : mn;
-        mx = k > mx ? k : mx;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long const a = __shfl_xor(mn, off);
-        unsigned long long const b = __shfl_xor(mx, off);
-        mn = a < mn ? a : mn;
-        mx = b > mx ? b : mx;
-    }
-    if (__lane_id() == 0) {
-        atomicMin(&out[0], mn);
-        atomicMax(&out[1], mx);
-    }
-}
-
 //------------------------------------------------------------------------------
 // Exclusive scan of uint32 (reduce -> top scan -> downsweep).
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
@@ -257,14 +235,6 @@ void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32
 size_t radix_counts_words(size_t n) {
     size_t const tiles = (n + RS_TILE - 1) / RS_TILE;
     return 256 * tiles;
-}
-
-void launch_key_minmax(uint64_t const* keys, uint32_t n, unsigned long long* out,
-                       hipStream_t s) {
-    uint32_t blocks = (n + 255) / 256;
-    if (blocks > 2048) { blocks = 2048; }
-    if (blocks == 0) { blocks = 1; }
-    hipLaunchKernelGGL(key_minmax, dim3(blocks), dim3(256), 0, s, keys, n, out);
 }
 
 int radix_pass_count(uint64_t kmin, uint64_t kmax) {

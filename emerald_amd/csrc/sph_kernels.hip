@@ -88,11 +88,28 @@ __global__ __launch_bounds__(256) void k_keys(KeysArgs a) {
         }
         if (a.vmax2_reset && threadIdx.x == 0) { *a.vmax2_reset = 0; }
     }
-    if (i >= a.n) { return; }
-    int2 const g = grid_coord(a.pos[i], a.cell);
-    a.keys[i] = morton2(g.x, g.y);
-    a.vals[i] = i;
-    if (a.grid) { a.grid[i] = g; }
+    unsigned long long key_min = ~0ull;
+    unsigned long long key_max = 0ull;
+    if (i < a.n) {
+        int2 const g = grid_coord(a.pos[i], a.cell);
+        uint64_t const key = morton2(g.x, g.y);
+        a.keys[i] = key;
+        a.vals[i] = i;
+        if (a.grid) { a.grid[i] = g; }
+        key_min = key;
+        key_max = key;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long const lo = __shfl_xor(key_min, off);
+        unsigned long long const hi = __shfl_xor(key_max, off);
+        key_min = lo < key_min ? lo : key_min;
+        key_max = hi > key_max ? hi : key_max;
+    }
+    if (__lane_id() == 0 && key_min <= key_max) {
+        atomicMin(&a.keystate->kmin, key_min);
+        atomicMax(&a.keystate->kmax, key_max);
+    }
 }
 
 void launch_keys(KeysArgs const& a, hipStream_t s) {
@@ -118,6 +135,13 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 // Cells: neighborhood.cpp:33-124 (block indices, block ranges, key map).
 __global__ __launch_bounds__(256) void k_cell_heads(CellsArgs a) {
     uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0 && a.keystate) {
+        // The LSD sort ordered only the low 8*passes bits; every key must share
+        // the bits above them.
+        unsigned long long const diff = a.keystate->kmin ^ a.keystate->kmax;
+        bool const covered = (a.passes >= 8) || ((diff >> (8 * a.passes)) == 0ull);
+        if (!covered) { a.keystate->sort_short = 1u; }
+    }
     if (t >= a.n) { return; }
     a.heads[t] = (t == 0 || a.sorted_keys[t] != a.sorted_keys[t - 1]) ? 1u : 0u;
 }

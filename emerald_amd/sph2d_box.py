@@ -181,17 +181,12 @@ def world_walls_initial_solid_state(params: Parameters, random_obstacles: bool =
 def dam_break_initial_state(params: Parameters, solid_state: Solid_state) -> State:
     """initial_state.cpp:14-97: hex block [0, L/2] x [0, 3.25 L/4], then drop
     fluid particles closer than 2R to any solid."""
-    H = np.float32(params.support)
-    R = np.float32(0.99) * (H * np.float32(0.5))
-    L = np.float32(params.length)
-    pos = emit_solid_box(((0.0, 0.0), (float(L / np.float32(2.0)),
-                                       float(np.float32(3.25) * L / np.float32(4.0)))), float(R))
-    if len(solid_state.positions):
-        from scipy.spatial import cKDTree  # scene generation only (host side)
-        tree = cKDTree(solid_state.positions.astype(np.float64))
-        d, _ = tree.query(pos.astype(np.float64), k=1)
-        pos = pos[~(d < 2.0 * float(R))]
-    return State(np.ascontiguousarray(pos, np.float32), np.zeros_like(pos, dtype=np.float32))
+    p = params.to_c()
+    sp = np.ascontiguousarray(solid_state.positions, np.float32).reshape(-1, 2)
+    n = lib().emsph_dam_break_positions(ctypes.byref(p), _ptr(sp), len(sp), None, 0)
+    pos = np.empty((n, 2), np.float32)
+    lib().emsph_dam_break_positions(ctypes.byref(p), _ptr(sp), len(sp), _ptr(pos), n)
+    return State(pos, np.zeros_like(pos))
 
 
 class Context:

@@ -149,6 +149,11 @@ size_t emsph_emit_solid_box(float min_x, float min_y, float max_x, float max_y, 
  * out_xy may be NULL to query the count. Returns the count. */
 size_t emsph_world_walls_solid_positions(const emsph_parameters* params, int random_obstacles,
                                          float* out_xy, size_t capacity);
+/* dam_break_initial_state positions (initial_state.cpp:14-97): hex block
+ * [0, L/2] x [0, 3.25 L/4] minus fluid within 2R of any solid. out_xy may be
+ * NULL to query the count. Returns the count. */
+size_t emsph_dam_break_positions(const emsph_parameters* params, const float* solid_xy,
+                                 size_t solid_count, float* out_xy, size_t capacity);
 
 /* Create a context on the current HIP device with static solids (their
  * neighbourhoods and volumes are computed on the device, solids.cpp:76-137)
