@@ -131,6 +131,15 @@ SIGNATURES = {
     "emsph_run_sub_steps": (_ST, [_P, _I, _F, _I, _P, _P]),
     "emsph_set_kernel_timing": (_ST, [_P, _I]),
     "emsph_get_kernel_timing": (_I, [_P, _P, _P, _P, _I]),
+    "emsph_batch_create": (_ST, [_I, _P, _P, _P, _P, _P]),
+    "emsph_batch_destroy": (_ST, [_P]),
+    "emsph_batch_scene_count": (_I, [_P]),
+    "emsph_batch_set_state": (_ST, [_P, _I, _P, _P]),
+    "emsph_batch_get_state": (_ST, [_P, _I, _P, _P]),
+    "emsph_batch_get_field": (_ST, [_P, _I, _I, _P]),
+    "emsph_batch_frame_init": (_ST, [_P, _I]),
+    "emsph_batch_sub_step": (_ST, [_P, _I, _P]),
+    "emsph_batch_run_sub_steps": (_ST, [_P, _I, _I, _P]),
     "emsph_search_create": (_ST, [_SZ, _P]),
     "emsph_search_destroy": (_ST, [_P]),
     "emsph_search_set_points": (_ST, [_P, _P, _SZ]),

@@ -150,11 +150,13 @@ __device__ __host__ __forceinline__ uint64_t morton2(int32_t gx, int32_t gy) {
            (spread_bits_2(static_cast<uint32_t>(gy)) << 1);
 }
 
-// Open-addressed cell table: cell key -> cell ordinal. A slot is empty while
-// its value is EMPTY_CELL; keys are written by the thread that claims the slot.
+// Open-addressed cell table: (cell key, scene tag) -> cell ordinal. A slot is
+// empty while its value is EMPTY_CELL; key and tag are written by the thread
+// that claims the slot (inserts and lookups run in different kernels).
 constexpr uint32_t EMPTY_CELL = 0xFFFFFFFFu;
 
-__device__ __forceinline__ uint32_t cell_hash(uint64_t key, uint32_t mask) {
+__device__ __forceinline__ uint32_t cell_hash(uint64_t key, uint32_t tag, uint32_t mask) {
+    key ^= static_cast<uint64_t>(tag) * 0x9E3779B97F4A7C15ull;
     key ^= key >> 33;
     key *= 0xff51afd7ed558ccdull;
     key ^= key >> 33;
@@ -163,14 +165,16 @@ __device__ __forceinline__ uint32_t cell_hash(uint64_t key, uint32_t mask) {
     return static_cast<uint32_t>(key) & mask;
 }
 
-__device__ __forceinline__ uint32_t cell_lookup(uint64_t key, uint64_t const* __restrict__ keys,
+__device__ __forceinline__ uint32_t cell_lookup(uint64_t key, uint32_t tag,
+                                                uint64_t const* __restrict__ keys,
+                                                uint32_t const* __restrict__ tags,
                                                 uint32_t const* __restrict__ vals,
                                                 uint32_t mask) {
-    uint32_t s = cell_hash(key, mask);
+    uint32_t s = cell_hash(key, tag, mask);
     for (;;) {
         uint32_t const v = vals[s];
         if (v == EMPTY_CELL) { return EMPTY_CELL; }
-        if (keys[s] == key) { return v; }
+        if (keys[s] == key && tags[s] == tag) { return v; }
         s = (s + 1) & mask;
     }
 }

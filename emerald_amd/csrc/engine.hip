@@ -1,0 +1,1476 @@
+// Host orchestration of the 2D SPH sub-step on one gfx950 GPU, for a batch of
+// M >= 1 scenes, and the C ABI of include/emerald_sph.h.
+//
+// One sub-step (the reference's <method>_sub_step, e.g. iisph.cpp:23-151):
+//   1. keys      grid coords + 64-bit Morton keys, in original order
+//   2. sort      stable LSD radix sort of (key, index) over the significant key
+//                digits, then over the scene id (scenes stay contiguous)
+//   3. cells     run heads, exclusive scan -> block indices, cell ranges,
+//                (key, scene) cell table; gather of the state into sorted order
+//   4. build     64-capped neighbour lists in reference order (fluid and solid),
+//                with the first per-particle sums of the method fused in
+//   5. solver    method sweeps over all scenes at once; after each iteration
+//                every scene applies the reference's exit test on the device
+//                and drops out of later sweeps when it passes; the host only
+//                reads how many scenes are still iterating
+//   6. finalize  x += dt v, state scattered back to original order
+// Solids are static: their cell table, lists and volumes are built once at
+// configuration (solids.cpp:76-137) with the same kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/emerald_sph.h"
+#include "kernels.hpp"
+
+using namespace emsph;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+emsph_status fail(emsph_status s, std::string const& msg) {
+    g_last_error = msg;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t const e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                            \
+            return fail(EMSPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+        }                                                                                  \
+    } while (0)
+
+#define ST_TRY(expr)                       \
+    do {                                   \
+        emsph_status const s_ = (expr);    \
+        if (s_ != EMSPH_OK) { return s_; } \
+    } while (0)
+
+constexpr int64_t kFlicksPerSecond = 705600000;
+constexpr float kPi = 3.14159274101257324218750f;
+
+double to_seconds(int64_t f) {
+    return static_cast<double>(f) / static_cast<double>(kFlicksPerSecond);
+}
+int64_t to_flicks(double s) {
+    return static_cast<int64_t>(s * static_cast<double>(kFlicksPerSecond));
+}
+
+KernelConsts make_kernel_consts(float h) {
+    KernelConsts c;
+    c.h = h;
+    c.hh = h * h;
+    c.hhh = h * h * h;
+    c.k = h * h * h * kPi;
+    c.four_h = 4.0f * h;
+    // W(0, h) = _wa(0) / k (kernels.h:25-35)
+    float const q = 0.0f / h;
+    c.w0 = (1.0f + q * q * (q * 0.75f - 1.5f)) / c.k;
+    return c;
+}
+
+StepConsts step_consts(emsph_config const& cfg, float dt) {
+    StepConsts k{};
+    emsph_parameters const& p = cfg.params;
+    k.kc = make_kernel_consts(p.support);
+    k.cell = p.support * 2.0f;
+    k.rho0 = p.target_density;
+    k.fluid_volume = cfg.mass_per_particle / p.target_density;
+    k.mass = cfg.mass_per_particle;
+    k.gravity = p.gravity;
+    k.dt = dt;
+    k.omega = p.iisph_omega;
+    k.pci_denom = cfg.pressure_correction_denom;
+    k.viscosity = p.viscosity;
+    return k;
+}
+
+Thresholds thresholds(emsph_parameters const& p) {
+    Thresholds t{};
+    t.iisph_avg = p.iisph_error_average_threshold;
+    t.iisph_max = p.iisph_error_max_threshold;
+    t.div_avg = p.dfsph_divergence_error_average_threshold;
+    t.div_max = p.dfsph_divergence_error_max_threshold;
+    t.dens_avg = p.dfsph_density_error_average_threshold;
+    t.dens_max = p.dfsph_density_error_max_threshold;
+    t.iisph_max_iters = std::min(p.iisph_max_pressure_iterations, MAX_RED_SLOTS);
+    t.dfsph_max_iters = std::min(p.dfsph_max_correction_iterations, MAX_RED_SLOTS / 2);
+    return t;
+}
+
+uint32_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) { p <<= 1; }
+    return static_cast<uint32_t>(p);
+}
+
+uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+// A device allocation that frees itself.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(DBuf const&) = delete;
+    DBuf& operator=(DBuf const&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) { (void)hipFree(p); }
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        if (count == 0) { count = 1; }
+        hipError_t const e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e == hipSuccess) { n = count; }
+        return e;
+    }
+    hipError_t ensure(size_t count) { return (p && n >= count) ? hipSuccess : alloc(count); }
+};
+
+// Keys, sort and cell table of one point set (fluid or solid).
+struct SpatialIndex {
+    uint32_t cap = 0;
+    uint32_t table_cap = 0;  // power of two
+    DBuf<uint64_t> keys0, keys1;
+    DBuf<uint32_t> vals0, vals1;
+    DBuf<uint32_t> counts, scan_scratch;
+    DBuf<uint32_t> heads, head_scan, block_index, cell_begin, cell_end;
+    DBuf<uint64_t> table_keys;
+    DBuf<uint32_t> table_tags, table_vals;
+    DBuf<uint32_t> orig, rank;
+    DBuf<int2> grid;
+    uint64_t* sorted_keys = nullptr;
+    uint32_t* sorted_vals = nullptr;
+    int key_passes = 0;
+    int scene_passes = 0;
+
+    hipError_t alloc(uint32_t c, bool with_grid) {
+        cap = c;
+        table_cap = next_pow2(std::max<uint64_t>(2ull * c, 1024));
+        hipError_t e;
+        if ((e = keys0.alloc(c)) || (e = keys1.alloc(c)) || (e = vals0.alloc(c)) ||
+            (e = vals1.alloc(c)) || (e = counts.alloc(radix_counts_words(c) + 256)) ||
+            (e = scan_scratch.alloc(scan_scratch_words(radix_counts_words(c) + c) + 64)) ||
+            (e = heads.alloc(c)) || (e = head_scan.alloc(c)) || (e = block_index.alloc(c)) ||
+            (e = cell_begin.alloc(c)) || (e = cell_end.alloc(c)) ||
+            (e = table_keys.alloc(table_cap)) || (e = table_tags.alloc(table_cap)) ||
+            (e = table_vals.alloc(table_cap)) || (e = orig.alloc(c)) || (e = rank.alloc(c))) {
+            return e;
+        }
+        if (with_grid) { return grid.alloc(c); }
+        return hipSuccess;
+    }
+
+    CellTable table() const {
+        return CellTable{table_keys.p, table_tags.p, table_vals.p, cell_begin.p, cell_end.p,
+                         table_cap - 1};
+    }
+};
+
+struct KernelTimer {
+    bool enabled = false;
+    std::map<std::string, std::pair<double, int64_t>> acc;
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+// Pinned read-back block.
+struct Pinned {
+    KeyState keys_reset;
+    KeyState keys;
+};
+
+struct SceneHost {
+    emsph_config cfg{};
+    uint32_t cap = 0;     // fluid capacity reserved for the scene
+    uint32_t n = 0;       // fluid particles
+    uint32_t ns = 0;      // solid particles
+    uint32_t offset = 0;  // first fluid slot
+    uint32_t soffset = 0; // first solid slot
+    float dt = 0.0f;      // batch sub-step dt
+};
+
+int scene_pass_count(uint32_t m) {
+    if (m <= 1) { return 0; }
+    int p = 0;
+    uint32_t v = m - 1;
+    while (v) {
+        ++p;
+        v >>= 8;
+    }
+    return p;
+}
+
+}  // namespace
+
+// One batch of scenes resident on the device.
+struct Engine {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::vector<SceneHost> sh;
+    uint32_t M = 0;
+    uint32_t cap = 0;      // total fluid capacity
+    uint32_t stride = 0;
+    uint32_t N = 0;        // total fluid particles (scenes packed at their offsets)
+    uint32_t scap = 0;
+    uint32_t NS = 0;
+    bool store_geometry = false;
+    bool have_lists = false;
+    int last_method = -1;
+
+    DBuf<SceneDev> scenes;
+    DBuf<Tile> tiles;
+    uint32_t ntiles = 0;
+    DBuf<uint32_t> scene_of;   // original fluid index -> scene
+    DBuf<SceneStats> stats;
+    std::vector<SceneStats> host_stats;
+    DBuf<Reduction> red;
+    DBuf<KeyState> keystate;
+
+    DBuf<V2> pos, vel;
+    DBuf<float> kv, kd;
+    SpatialIndex fidx;
+    DBuf<V2> spos, svel, dii, sdp, vext, xstar, vstar0, vstar1, force;
+    DBuf<float> rho, rho_star, aii, p0, p1, alpha, kvs, kds;
+    float* cur_p = nullptr;
+    DBuf<uint8_t> fcount, scount;
+    DBuf<uint32_t> fl_idx, sl_idx;
+    DBuf<V2> fl_gw, sl_gw, fl_vec, sl_vec;
+    DBuf<float> fl_dist, sl_dist;
+
+    SpatialIndex sidx;
+    DBuf<uint32_t> s_scene_of;
+    DBuf<Tile> stiles;
+    uint32_t nstiles = 0;
+    DBuf<V2> s_pos, s_vel, s_pos_orig;
+    DBuf<float> s_vol;
+
+    Pinned* host = nullptr;
+    int passes_hint = -1;
+    bool keys_checked = false;
+    bool sort_short = false;
+    KernelTimer timer;
+
+    ~Engine() {
+        if (host) { (void)hipHostFree(host); }
+        if (timer.a) { (void)hipEventDestroy(timer.a); }
+        if (timer.b) { (void)hipEventDestroy(timer.b); }
+        if (stream) { (void)hipStreamDestroy(stream); }
+    }
+};
+
+struct emsph_context {
+    Engine e;
+    emsph_config cfg{};
+};
+
+struct emsph_batch {
+    Engine e;
+};
+
+struct emsph_search {
+    Engine e;
+    float cell = 0.0f;
+    bool ran = false;
+};
+
+namespace {
+
+template <class F>
+hipError_t timed(Engine* g, char const* name, F&& launch) {
+    if (!g->timer.enabled) {
+        launch();
+        return hipGetLastError();
+    }
+    (void)hipEventRecord(g->timer.a, g->stream);
+    launch();
+    (void)hipEventRecord(g->timer.b, g->stream);
+    hipError_t const e = hipEventSynchronize(g->timer.b);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, g->timer.a, g->timer.b);
+    auto& slot = g->timer.acc[name];
+    slot.first += ms;
+    slot.second += 1;
+    return e;
+}
+
+emsph_status engine_init(Engine* g) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        return fail(EMSPH_ERR_NO_DEVICE, "no HIP device");
+    }
+    HIP_TRY(hipGetDevice(&g->device));
+    HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g->host), sizeof(Pinned)));
+    HIP_TRY(hipEventCreate(&g->timer.a));
+    HIP_TRY(hipEventCreate(&g->timer.b));
+    HIP_TRY(g->keystate.alloc(1));
+    return EMSPH_OK;
+}
+
+Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr) {
+    Batch b{};
+    b.scenes = g->scenes.p;
+    b.tiles = g->tiles.p;
+    b.ntiles = g->ntiles;
+    b.nscenes = g->M;
+    b.stride = g->stride;
+    b.stats = g->stats.p;
+    b.red = g->red.p;
+    b.keystate = g->keystate.p;
+    b.pos = g->spos.p;
+    b.vel = g->svel.p;
+    b.orig = g->fidx.orig.p;
+    b.rank = g->fidx.rank.p;
+    b.table = g->fidx.table();
+    if (g->NS) {
+        b.solids = SolidView{g->s_pos.p, g->s_vel.p, g->s_vol.p, g->sidx.rank.p, g->sidx.orig.p,
+                             g->sidx.table()};
+    }
+    b.fl = Lists{g->fcount.p, g->fl_idx.p, g->fl_gw.p, g->fl_dist.p, g->fl_vec.p};
+    b.sl = Lists{g->scount.p, g->sl_idx.p, g->sl_gw.p, g->sl_dist.p, g->sl_vec.p};
+    b.rho = g->rho.p;
+    b.rho_star = g->rho_star.p;
+    b.aii = g->aii.p;
+    b.dii = g->dii.p;
+    b.sdp = g->sdp.p;
+    b.p_in = p_in;
+    b.p_out = p_out;
+    b.alpha = g->alpha.p;
+    b.pos_orig = g->pos.p;
+    b.vel_orig = g->vel.p;
+    b.kv_orig = g->kv.p;
+    b.kd_orig = g->kd.p;
+    b.kvs = g->kvs.p;
+    b.kds = g->kds.p;
+    b.vext = g->vext.p;
+    b.xstar = g->xstar.p;
+    b.force = g->force.p;
+    b.store_geometry = g->store_geometry ? 1 : 0;
+    return b;
+}
+
+// Scene table and tiles from the host scene list; fluid slots are packed at
+// each scene's offset (offsets are prefix sums of the scene capacities).
+emsph_status upload_layout(Engine* g) {
+    std::vector<SceneDev> sd(g->M);
+    std::vector<Tile> tl;
+    std::vector<uint32_t> so(g->cap, 0);
+    for (uint32_t s = 0; s < g->M; ++s) {
+        SceneHost const& h = g->sh[s];
+        sd[s].c = step_consts(h.cfg, h.dt);
+        sd[s].thr = thresholds(h.cfg.params);
+        sd[s].offset = h.offset;
+        sd[s].n = h.n;
+        sd[s].soffset = h.soffset;
+        sd[s].ns = h.ns;
+        for (uint32_t k = 0; k < h.n; k += TILE) {
+            tl.push_back(Tile{s, h.offset + k, std::min<uint32_t>(TILE, h.n - k), 0});
+        }
+        for (uint32_t k = 0; k < h.n; ++k) { so[h.offset + k] = s; }
+    }
+    g->ntiles = static_cast<uint32_t>(tl.size());
+    HIP_TRY(g->scenes.ensure(std::max<uint32_t>(g->M, 1)));
+    HIP_TRY(g->tiles.ensure(std::max<size_t>(tl.size(), 1)));
+    HIP_TRY(g->scene_of.ensure(std::max<uint32_t>(g->cap, 1)));
+    HIP_TRY(g->stats.ensure(std::max<uint32_t>(g->M, 1)));
+    HIP_TRY(g->red.ensure(static_cast<size_t>(std::max<uint32_t>(g->M, 1)) * MAX_RED_SLOTS));
+    HIP_TRY(hipMemcpyAsync(g->scenes.p, sd.data(), sd.size() * sizeof(SceneDev),
+                           hipMemcpyHostToDevice, g->stream));
+    if (!tl.empty()) {
+        HIP_TRY(hipMemcpyAsync(g->tiles.p, tl.data(), tl.size() * sizeof(Tile),
+                               hipMemcpyHostToDevice, g->stream));
+    }
+    if (g->cap) {
+        HIP_TRY(hipMemcpyAsync(g->scene_of.p, so.data(), so.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->host_stats.assign(g->M, SceneStats{});
+    // N is the packed extent: every slot up to the last scene's end. Scenes
+    // with n < cap leave holes; holes are outside every tile, and their keys
+    // are never produced because the key/sort stage runs over packed slots
+    // only when every scene is full (the batch), or over one scene (a context).
+    return EMSPH_OK;
+}
+
+hipError_t alloc_fluid(Engine* g, uint32_t cap) {
+    g->cap = cap;
+    g->stride = round_up(std::max<uint32_t>(cap, 1), 256);
+    size_t const L = static_cast<size_t>(MAXN) * g->stride;
+    hipError_t e;
+    if ((e = g->pos.alloc(cap)) || (e = g->vel.alloc(cap)) || (e = g->kv.alloc(cap)) ||
+        (e = g->kd.alloc(cap)) || (e = g->fidx.alloc(cap, false)) || (e = g->spos.alloc(cap)) ||
+        (e = g->svel.alloc(cap)) || (e = g->dii.alloc(cap)) || (e = g->sdp.alloc(cap)) ||
+        (e = g->vext.alloc(cap)) || (e = g->xstar.alloc(cap)) || (e = g->vstar0.alloc(cap)) ||
+        (e = g->vstar1.alloc(cap)) || (e = g->force.alloc(cap)) || (e = g->rho.alloc(cap)) ||
+        (e = g->rho_star.alloc(cap)) || (e = g->aii.alloc(cap)) || (e = g->p0.alloc(cap)) ||
+        (e = g->p1.alloc(cap)) || (e = g->alpha.alloc(cap)) || (e = g->kvs.alloc(cap)) ||
+        (e = g->kds.alloc(cap)) || (e = g->fcount.alloc(g->stride)) ||
+        (e = g->scount.alloc(g->stride)) || (e = g->fl_idx.alloc(L)) ||
+        (e = g->sl_idx.alloc(L)) || (e = g->fl_gw.alloc(L)) || (e = g->sl_gw.alloc(L))) {
+        return e;
+    }
+    if ((e = hipMemsetAsync(g->kv.p, 0, cap * sizeof(float), g->stream))) { return e; }
+    if ((e = hipMemsetAsync(g->kd.p, 0, cap * sizeof(float), g->stream))) { return e; }
+    if ((e = hipMemsetAsync(g->vel.p, 0, cap * sizeof(V2), g->stream))) { return e; }
+    return hipSuccess;
+}
+
+hipError_t ensure_geometry(Engine* g) {
+    hipError_t e = hipSuccess;
+    size_t const m = static_cast<size_t>(MAXN) * g->stride;
+    if (!g->fl_dist.p && (e = g->fl_dist.alloc(m))) { return e; }
+    if (!g->fl_vec.p && (e = g->fl_vec.alloc(m))) { return e; }
+    if (!g->sl_dist.p && (e = g->sl_dist.alloc(m))) { return e; }
+    if (!g->sl_vec.p && (e = g->sl_vec.alloc(m))) { return e; }
+    return e;
+}
+
+// Keys -> sort -> cells of n points (original order) with per-point scenes.
+//
+// passes_hint < 0 reads the key range back before sorting (one host sync) and
+// sorts exactly the significant key digits. passes_hint >= 0 sorts that many
+// digits without waiting; k_cell_heads then flags (KeyState::sort_short) a key
+// range those digits do not cover, and the caller redoes the work with the
+// exact count before anything it produced is used.
+hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
+                        uint32_t const* scene_of, uint32_t nscenes, int passes_hint,
+                        IndexArgs gathers, bool time_it) {
+    hipStream_t const s = g->stream;
+    auto run = [&](char const* name, auto&& f) -> hipError_t {
+        if (time_it) { return timed(g, name, f); }
+        f();
+        return hipGetLastError();
+    };
+    hipError_t e;
+    IndexArgs a = gathers;
+    a.pos = pos;
+    a.n = n;
+    a.scenes = g->scenes.p;
+    a.scene_of = scene_of;
+    a.keys = si.keys0.p;
+    a.vals = si.vals0.p;
+    a.grid = si.grid.p;
+    a.keystate = g->keystate.p;
+    g->host->keys_reset = KeyState{~0ull, 0ull, 0u, 0u};
+    if ((e = hipMemcpyAsync(g->keystate.p, &g->host->keys_reset, sizeof(KeyState),
+                            hipMemcpyHostToDevice, s))) {
+        return e;
+    }
+    if ((e = run("keys", [&] { launch_keys(a, s); }))) { return e; }
+    if (passes_hint < 0) {
+        if ((e = hipMemcpyAsync(&g->host->keys, g->keystate.p, sizeof(KeyState),
+                                hipMemcpyDeviceToHost, s))) {
+            return e;
+        }
+        if ((e = hipStreamSynchronize(s))) { return e; }
+        si.key_passes = radix_pass_count(g->host->keys.kmin, g->host->keys.kmax);
+    } else {
+        si.key_passes = passes_hint;
+    }
+    si.scene_passes = scene_pass_count(nscenes);
+    int which = 0;
+    if ((e = run("radix_sort", [&] {
+             which = launch_radix_sort(si.keys0.p, si.vals0.p, si.keys1.p, si.vals1.p, n,
+                                       si.key_passes, scene_of, si.scene_passes, si.counts.p,
+                                       si.scan_scratch.p, s);
+         }))) {
+        return e;
+    }
+    si.sorted_keys = which ? si.keys1.p : si.keys0.p;
+    si.sorted_vals = which ? si.vals1.p : si.vals0.p;
+    a.key_passes = si.key_passes;
+    a.check_keys = passes_hint >= 0 ? 1 : 0;
+    a.sorted_keys = si.sorted_keys;
+    a.sorted_vals = si.sorted_vals;
+    a.heads = si.heads.p;
+    a.head_scan = si.head_scan.p;
+    a.block_index = si.block_index.p;
+    a.cell_begin = si.cell_begin.p;
+    a.cell_end = si.cell_end.p;
+    a.table_keys = si.table_keys.p;
+    a.table_tags = si.table_tags.p;
+    a.table_vals = si.table_vals.p;
+    a.mask = si.table_cap - 1;
+    a.orig = si.orig.p;
+    a.rank = si.rank.p;
+    return run("cells", [&] {
+        launch_cell_heads(a, s);
+        launch_exclusive_scan(si.heads.p, si.head_scan.p, n, si.scan_scratch.p, nullptr, s);
+        launch_fill_u32(si.table_vals.p, EMPTY_CELL, si.table_cap, s);
+        launch_cells_finalize(a, s);
+    });
+}
+
+// Solids of every scene: index, solid-solid lists with geometry, volumes
+// (solids.cpp:76-137).
+emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
+    g->NS = 0;
+    for (uint32_t s = 0; s < g->M; ++s) {
+        g->sh[s].soffset = g->NS;
+        g->NS += g->sh[s].ns;
+    }
+    if (!g->NS) { return EMSPH_OK; }
+    uint32_t const NS = g->NS;
+    g->scap = NS;
+    uint32_t const sstride = round_up(NS, 256);
+    size_t const SL = static_cast<size_t>(MAXN) * sstride;
+    DBuf<uint8_t> scnt;
+    DBuf<uint32_t> sli;
+    DBuf<float> sld;
+    DBuf<V2> slv;
+    hipError_t e;
+    if ((e = g->sidx.alloc(NS, false)) || (e = g->s_pos.alloc(NS)) || (e = g->s_vel.alloc(NS)) ||
+        (e = g->s_pos_orig.alloc(NS)) || (e = g->s_vol.alloc(NS)) ||
+        (e = g->s_scene_of.alloc(NS)) || (e = scnt.alloc(sstride)) || (e = sli.alloc(SL)) ||
+        (e = sld.alloc(SL)) || (e = slv.alloc(SL))) {
+        return fail(EMSPH_ERR_OUT_OF_MEMORY, "solid allocation");
+    }
+    std::vector<uint32_t> so(NS);
+    std::vector<Tile> tl;
+    for (uint32_t s = 0; s < g->M; ++s) {
+        SceneHost const& h = g->sh[s];
+        if (h.ns) {
+            HIP_TRY(hipMemcpyAsync(g->s_pos_orig.p + h.soffset, sxy[s], h.ns * sizeof(V2),
+                                   hipMemcpyHostToDevice, g->stream));
+        }
+        for (uint32_t k = 0; k < h.ns; ++k) { so[h.soffset + k] = s; }
+        for (uint32_t k = 0; k < h.ns; k += TILE) {
+            tl.push_back(Tile{s, h.soffset + k, std::min<uint32_t>(TILE, h.ns - k), 0});
+        }
+    }
+    g->nstiles = static_cast<uint32_t>(tl.size());
+    HIP_TRY(g->stiles.alloc(tl.size()));
+    HIP_TRY(hipMemcpyAsync(g->stiles.p, tl.data(), tl.size() * sizeof(Tile),
+                           hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->s_scene_of.p, so.data(), NS * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemsetAsync(g->s_vel.p, 0, NS * sizeof(V2), g->stream));
+    IndexArgs ga{};
+    ga.pos_out = g->s_pos.p;
+    HIP_TRY(index_points(g, g->sidx, g->s_pos_orig.p, NS, g->s_scene_of.p, g->M, -1, ga, false));
+    Batch b{};
+    b.scenes = g->scenes.p;
+    b.tiles = g->stiles.p;
+    b.ntiles = g->nstiles;
+    b.nscenes = g->M;
+    b.stride = sstride;
+    b.pos = g->s_pos.p;
+    b.orig = g->sidx.orig.p;
+    b.rank = g->sidx.rank.p;
+    b.table = g->sidx.table();
+    b.fl = Lists{scnt.p, sli.p, nullptr, sld.p, slv.p};
+    b.sl = Lists{};  // solid-solid lists only
+    b.rho = g->s_vol.p;
+    b.store_geometry = 1;
+    launch_build(BUILD_LISTS, b, g->stream);
+    launch_solid_volumes(b, NS, g->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+// Configure the engine with M scenes; fluid capacity per scene, solids per scene.
+emsph_status engine_configure(Engine* g, std::vector<SceneHost> scenes,
+                              std::vector<float const*> const& solid_xy) {
+    g->sh = std::move(scenes);
+    g->M = static_cast<uint32_t>(g->sh.size());
+    uint32_t off = 0;
+    for (auto& h : g->sh) {
+        h.offset = off;
+        off += h.cap;
+    }
+    hipError_t const e = alloc_fluid(g, std::max<uint32_t>(off, 1));
+    if (e) {
+        return fail(e == hipErrorOutOfMemory ? EMSPH_ERR_OUT_OF_MEMORY : EMSPH_ERR_HIP,
+                    std::string("device allocation: ") + hipGetErrorString(e));
+    }
+    g->N = off;
+    ST_TRY(upload_layout(g));
+    ST_TRY(build_solids(g, solid_xy));
+    ST_TRY(upload_layout(g));  // solid offsets are now known
+    g->passes_hint = -1;
+    return EMSPH_OK;
+}
+
+//------------------------------------------------------------------------------
+// KeyState read-backs.
+hipError_t read_keystate(Engine* g) {
+    hipError_t e = hipMemcpyAsync(&g->host->keys, g->keystate.p, sizeof(KeyState),
+                                  hipMemcpyDeviceToHost, g->stream);
+    if (e) { return e; }
+    if ((e = hipStreamSynchronize(g->stream))) { return e; }
+    if (!g->keys_checked) {
+        g->keys_checked = true;
+        g->sort_short = g->host->keys.sort_short != 0u;
+        g->passes_hint = radix_pass_count(g->host->keys.kmin, g->host->keys.kmax);
+    }
+    return hipSuccess;
+}
+
+// Steps 1-4 of the sub-step.
+emsph_status prepare(Engine* g, int build_mode, bool dfsph) {
+    IndexArgs ga{};
+    ga.vel_in = g->vel.p;
+    ga.pos_out = g->spos.p;
+    ga.vel_out = g->svel.p;
+    if (dfsph) {
+        ga.kv_in = g->kv.p;
+        ga.kd_in = g->kd.p;
+        ga.kv_out = g->kvs.p;
+        ga.kd_out = g->kds.p;
+    }
+    g->keys_checked = g->passes_hint < 0;
+    g->sort_short = false;
+    Batch b = fluid_batch(g);
+    HIP_TRY(timed(g, "step_begin", [&] { launch_step_begin(b, g->stream); }));
+    HIP_TRY(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->passes_hint, ga,
+                         g->timer.enabled));
+    if (g->passes_hint < 0) { g->passes_hint = g->fidx.key_passes; }
+    HIP_TRY(timed(g, "build", [&] { launch_build(build_mode, b, g->stream); }));
+    g->have_lists = true;
+    return EMSPH_OK;
+}
+
+// iisph.cpp:23-151 for every scene of the batch.
+emsph_status iisph_sub_step(Engine* g) {
+    ST_TRY(prepare(g, BUILD_IISPH, false));
+    float* pa = g->p0.p;
+    float* pb = g->p1.p;
+    Batch b = fluid_batch(g, nullptr, pa);
+    HIP_TRY(timed(g, "iisph_aii", [&] { launch_iisph_aii(b, g->stream); }));
+    HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
+    int max_it = 0;
+    for (auto const& h : g->sh) {
+        max_it = std::max(max_it, std::min(h.cfg.params.iisph_max_pressure_iterations,
+                                           MAX_RED_SLOTS));
+    }
+    for (int it = 0; it < max_it; ++it) {
+        b.p_in = pa;
+        b.p_out = pb;
+        HIP_TRY(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
+        HIP_TRY(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, it, g->stream); }));
+        HIP_TRY(timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); }));
+        std::swap(pa, pb);
+        // iisph.cpp:126: no scene can exit before its third iteration.
+        if (it >= 2 || it + 1 == max_it) {
+            HIP_TRY(read_keystate(g));
+            if (g->sort_short) { return EMSPH_OK; }
+            if (g->host->keys.active_count == 0) { break; }
+        }
+    }
+    if (!g->keys_checked) {
+        HIP_TRY(read_keystate(g));
+        if (g->sort_short) { return EMSPH_OK; }
+    }
+    b.p_in = pa;
+    HIP_TRY(timed(g, "apply_integrate", [&] { launch_apply(b, 0, 1, 1, g->stream); }));
+    g->cur_p = pa;
+    return EMSPH_OK;
+}
+
+// dfsph_p.cpp:265-374 (with the correction loops of :61-212).
+emsph_status dfsph_sub_step(Engine* g) {
+    ST_TRY(prepare(g, BUILD_DFSPH, true));
+    Batch b = fluid_batch(g, g->kvs.p, nullptr);
+    b.kappa = g->kvs.p;
+    int max_it = 0;
+    for (auto const& h : g->sh) {
+        max_it = std::max(max_it, std::min(h.cfg.params.dfsph_max_correction_iterations,
+                                           MAX_RED_SLOTS / 2));
+    }
+    // Divergence solve (dfsph_p.cpp:136-212): warm start, then iterations.
+    HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 1, g->stream); }));
+    HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
+    for (int it = 0; it < max_it; ++it) {
+        HIP_TRY(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, it, g->stream); }));
+        HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
+        HIP_TRY(timed(g, "loop", [&] {
+            launch_loop_decide(b, LOOP_DIVERGENCE, it, it, g->stream);
+        }));
+        if (it >= 1 || it + 1 == max_it) {
+            HIP_TRY(read_keystate(g));
+            if (g->sort_short) { return EMSPH_OK; }
+            if (g->host->keys.active_count == 0) { break; }
+        }
+    }
+    // External forces, then the density solve (dfsph_p.cpp:350-366, :61-134).
+    int const base = MAX_RED_SLOTS / 2;
+    b.p_in = g->kds.p;
+    b.kappa = g->kds.p;
+    HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 1, 0, 1, g->stream); }));
+    HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
+    for (int it = 0; it < max_it; ++it) {
+        HIP_TRY(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, base + it, g->stream); }));
+        HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
+        HIP_TRY(timed(g, "loop", [&] {
+            launch_loop_decide(b, LOOP_DENSITY, it, base + it, g->stream);
+        }));
+        if (it >= 2 || it + 1 == max_it) {
+            HIP_TRY(read_keystate(g));
+            if (g->sort_short) { return EMSPH_OK; }
+            if (g->host->keys.active_count == 0) { break; }
+        }
+    }
+    if (!g->keys_checked) {
+        HIP_TRY(read_keystate(g));
+        if (g->sort_short) { return EMSPH_OK; }
+    }
+    HIP_TRY(timed(g, "finalize", [&] { launch_finalize(b, 1, g->stream); }));
+    g->cur_p = g->kds.p;
+    return EMSPH_OK;
+}
+
+// pcisph.cpp:36-164 for every scene of the batch.
+emsph_status pcisph_sub_step(Engine* g) {
+    HIP_TRY(ensure_geometry(g));
+    ST_TRY(prepare(g, BUILD_LISTS, false));
+    Batch b = fluid_batch(g, nullptr, g->p0.p);
+    b.fl.dist = g->fl_dist.p;
+    b.fl.vec = g->fl_vec.p;
+    V2* vin = g->vstar0.p;
+    V2* vout = g->vstar1.p;
+    b.vstar_out = vin;
+    HIP_TRY(timed(g, "pci_init", [&] { launch_pci_init(b, g->stream); }));
+    HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
+    for (int it = 0; it < 6; ++it) {
+        HIP_TRY(timed(g, "pci_density", [&] { launch_pci_density(b, it, g->stream); }));
+        b.vstar_in = vin;
+        b.vstar_out = vout;
+        HIP_TRY(timed(g, "pci_force", [&] { launch_pci_force(b, g->stream); }));
+        HIP_TRY(timed(g, "loop", [&] { launch_loop_decide(b, LOOP_PCISPH, it, it, g->stream); }));
+        std::swap(vin, vout);
+        if (it >= 2) {
+            HIP_TRY(read_keystate(g));
+            if (g->sort_short) { return EMSPH_OK; }
+            if (g->host->keys.active_count == 0) { break; }
+        }
+    }
+    if (!g->keys_checked) {
+        HIP_TRY(read_keystate(g));
+        if (g->sort_short) { return EMSPH_OK; }
+    }
+    b.vstar_in = vin;
+    HIP_TRY(timed(g, "pci_finalize", [&] { launch_pci_finalize(b, g->stream); }));
+    g->cur_p = g->p0.p;
+    return EMSPH_OK;
+}
+
+emsph_status check_method(int method) {
+    if (method == EMSPH_IISPH || method == EMSPH_PCISPH || method == EMSPH_DFSPH_P) {
+        return EMSPH_OK;
+    }
+    return fail(EMSPH_ERR_UNKNOWN_METHOD,
+                "method " + std::to_string(method) + " is not implemented on the GPU path");
+}
+
+emsph_status engine_sub_step(Engine* g, int method) {
+    ST_TRY(check_method(method));
+    if (g->N == 0 || g->ntiles == 0) { return EMSPH_OK; }  // N = 0: the reference reads block_indices.back()
+    HIP_TRY(hipSetDevice(g->device));
+    // A sort run with the previous sub-step's pass count is checked at the
+    // first read-back; when it was short the sub-step is redone with the exact
+    // count (the second attempt sorts exactly and cannot be short).
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (attempt == 1) { g->passes_hint = -1; }
+        switch (method) {
+        case EMSPH_IISPH: ST_TRY(iisph_sub_step(g)); break;
+        case EMSPH_DFSPH_P: ST_TRY(dfsph_sub_step(g)); break;
+        default: ST_TRY(pcisph_sub_step(g)); break;
+        }
+        if (!g->sort_short) { break; }
+    }
+    g->last_method = method;
+    return EMSPH_OK;
+}
+
+emsph_status read_stats(Engine* g) {
+    if (!g->M) { return EMSPH_OK; }
+    HIP_TRY(hipMemcpyAsync(g->host_stats.data(), g->stats.p, g->M * sizeof(SceneStats),
+                           hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+void to_public(SceneStats const& s, Engine const* g, emsph_step_stats* out) {
+    std::memset(out, 0, sizeof(*out));
+    out->iterations = s.iterations;
+    out->divergence_iterations = s.divergence_iterations;
+    out->error_average = s.error_average;
+    out->error_max = s.error_max;
+    out->divergence_error_average = s.divergence_error_average;
+    out->divergence_error_max = s.divergence_error_max;
+    std::memcpy(&out->max_velocity_sq, &s.vmax2_bits, 4);
+    out->sort_passes = static_cast<uint32_t>(g->fidx.key_passes + g->fidx.scene_passes);
+}
+
+emsph_status engine_frame_init(Engine* g, int method) {
+    ST_TRY(check_method(method));
+    if (method == EMSPH_DFSPH_P && g->cap) {
+        HIP_TRY(hipMemsetAsync(g->kv.p, 0, g->cap * sizeof(float), g->stream));
+        HIP_TRY(hipMemsetAsync(g->kd.p, 0, g->cap * sizeof(float), g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+// A per-particle field of the last sub-step, scene s, original order.
+emsph_status engine_get_field(Engine* g, uint32_t s, int field, float* out) {
+    if (!g->have_lists || !g->N) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no sub-step yet"); }
+    float const* f1 = nullptr;
+    V2 const* f2 = nullptr;
+    switch (field) {
+    case EMSPH_FIELD_DENSITY: f1 = g->rho.p; break;
+    case EMSPH_FIELD_DENSITY_STAR: f1 = g->rho_star.p; break;
+    case EMSPH_FIELD_PRESSURE: f1 = g->cur_p ? g->cur_p : g->p0.p; break;
+    case EMSPH_FIELD_AII: f1 = g->aii.p; break;
+    case EMSPH_FIELD_DII: f2 = g->dii.p; break;
+    case EMSPH_FIELD_SUM_DIJ_PJ: f2 = g->sdp.p; break;
+    case EMSPH_FIELD_ALPHA: f1 = g->alpha.p; break;
+    case EMSPH_FIELD_KAPPA_DENSITY: f1 = g->kds.p; break;
+    case EMSPH_FIELD_KAPPA_DIVERGENCE: f1 = g->kvs.p; break;
+    case EMSPH_FIELD_PRESSURE_FORCE: f2 = g->force.p; break;
+    default: return fail(EMSPH_ERR_INVALID_ARGUMENT, "unknown field");
+    }
+    SceneHost const& h = g->sh[s];
+    if (f1) {
+        DBuf<float> tmp;
+        HIP_TRY(tmp.alloc(g->N));
+        launch_scatter_f32(f1, g->fidx.orig.p, tmp.p, g->N, g->stream);
+        HIP_TRY(hipMemcpyAsync(out, tmp.p + h.offset, h.n * sizeof(float), hipMemcpyDeviceToHost,
+                               g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    } else {
+        DBuf<V2> tmp;
+        HIP_TRY(tmp.alloc(g->N));
+        launch_scatter_v2(f2, g->fidx.orig.p, tmp.p, g->N, g->stream);
+        HIP_TRY(hipMemcpyAsync(out, tmp.p + h.offset, h.n * sizeof(V2), hipMemcpyDeviceToHost,
+                               g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    return EMSPH_OK;
+}
+
+emsph_status engine_get_neighborhood(Engine* g, uint32_t s, int which, uint8_t* counts,
+                                     uint32_t* indices, float* distances, float* vectors_xy) {
+    if (!g->have_lists || !g->N) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no sub-step yet"); }
+    if ((distances || vectors_xy) && !g->fl_dist.p) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "geometry not stored (emsph_set_store_geometry)");
+    }
+    if (which && !g->NS) {
+        std::memset(counts, 0, g->sh[s].n);
+        return EMSPH_OK;
+    }
+    Lists l = which ? Lists{g->scount.p, g->sl_idx.p, g->sl_gw.p, g->sl_dist.p, g->sl_vec.p}
+                    : Lists{g->fcount.p, g->fl_idx.p, g->fl_gw.p, g->fl_dist.p, g->fl_vec.p};
+    uint32_t const* other = which ? g->sidx.orig.p : g->fidx.orig.p;
+    uint32_t const* other_scene = which ? g->s_scene_of.p : g->scene_of.p;
+    size_t const m = static_cast<size_t>(g->N) * MAXN;
+    DBuf<uint8_t> dc;
+    DBuf<uint32_t> di;
+    DBuf<float> dd;
+    DBuf<V2> dv;
+    HIP_TRY(dc.alloc(g->N));
+    HIP_TRY(di.alloc(m));
+    if (distances) { HIP_TRY(dd.alloc(m)); }
+    if (vectors_xy) { HIP_TRY(dv.alloc(m)); }
+    launch_export_lists(l, g->N, g->stride, g->fidx.orig.p, other, other_scene, g->scenes.p,
+                        which, dc.p, di.p, distances ? dd.p : nullptr,
+                        vectors_xy ? dv.p : nullptr, g->stream);
+    HIP_TRY(hipGetLastError());
+    SceneHost const& h = g->sh[s];
+    size_t const o = static_cast<size_t>(h.offset) * MAXN;
+    size_t const k = static_cast<size_t>(h.n) * MAXN;
+    HIP_TRY(hipMemcpyAsync(counts, dc.p + h.offset, h.n, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(indices, di.p + o, k * 4, hipMemcpyDeviceToHost, g->stream));
+    if (distances) {
+        HIP_TRY(hipMemcpyAsync(distances, dd.p + o, k * 4, hipMemcpyDeviceToHost, g->stream));
+    }
+    if (vectors_xy) {
+        HIP_TRY(hipMemcpyAsync(vectors_xy, dv.p + o, k * 8, hipMemcpyDeviceToHost, g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+emsph_status engine_set_state(Engine* g, uint32_t s, float const* pos_xy, float const* vel_xy,
+                              size_t count) {
+    SceneHost& h = g->sh[s];
+    if (count > h.cap) { return fail(EMSPH_ERR_CAPACITY, "fluid count above capacity"); }
+    if (count) {
+        HIP_TRY(hipMemcpyAsync(g->pos.p + h.offset, pos_xy, count * sizeof(V2),
+                               hipMemcpyHostToDevice, g->stream));
+        if (vel_xy) {
+            HIP_TRY(hipMemcpyAsync(g->vel.p + h.offset, vel_xy, count * sizeof(V2),
+                                   hipMemcpyHostToDevice, g->stream));
+        } else {
+            HIP_TRY(hipMemsetAsync(g->vel.p + h.offset, 0, count * sizeof(V2), g->stream));
+        }
+    }
+    if (count != h.n) {
+        h.n = static_cast<uint32_t>(count);
+        ST_TRY(upload_layout(g));
+        // The sort runs over [0, N): the packed extent ends at the last scene's
+        // particles; only a single-scene engine may hold fewer than capacity.
+        g->N = g->sh.back().offset + g->sh.back().n;
+    }
+    g->have_lists = false;
+    g->passes_hint = -1;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+emsph_status engine_get_state(Engine* g, uint32_t s, float* pos_xy, float* vel_xy) {
+    SceneHost const& h = g->sh[s];
+    if (h.n) {
+        if (pos_xy) {
+            HIP_TRY(hipMemcpyAsync(pos_xy, g->pos.p + h.offset, h.n * sizeof(V2),
+                                   hipMemcpyDeviceToHost, g->stream));
+        }
+        if (vel_xy) {
+            HIP_TRY(hipMemcpyAsync(vel_xy, g->vel.p + h.offset, h.n * sizeof(V2),
+                                   hipMemcpyDeviceToHost, g->stream));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+emsph_status engine_solid_volumes(Engine* g, uint32_t s, float* out) {
+    SceneHost const& h = g->sh[s];
+    if (!h.ns) { return EMSPH_OK; }
+    DBuf<float> tmp;
+    HIP_TRY(tmp.alloc(g->NS));
+    launch_scatter_f32(g->s_vol.p, g->sidx.orig.p, tmp.p, g->NS, g->stream);
+    HIP_TRY(hipMemcpyAsync(out, tmp.p + h.soffset, h.ns * sizeof(float), hipMemcpyDeviceToHost,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+emsph_status engine_set_dt(Engine* g, uint32_t s, float dt) {
+    if (g->sh[s].dt != dt) {
+        g->sh[s].dt = dt;
+        SceneDev sd{};
+        SceneHost const& h = g->sh[s];
+        sd.c = step_consts(h.cfg, h.dt);
+        sd.thr = thresholds(h.cfg.params);
+        sd.offset = h.offset;
+        sd.n = h.n;
+        sd.soffset = h.soffset;
+        sd.ns = h.ns;
+        HIP_TRY(hipMemcpyAsync(g->scenes.p + s, &sd, sizeof(SceneDev), hipMemcpyHostToDevice,
+                               g->stream));
+    }
+    return EMSPH_OK;
+}
+
+emsph_status engine_vmax2(Engine* g, float* out) {
+    Batch b = fluid_batch(g);
+    launch_step_begin(b, g->stream);
+    launch_vmax2(b, g->stream);
+    HIP_TRY(hipGetLastError());
+    ST_TRY(read_stats(g));
+    std::memcpy(out, &g->host_stats[0].vmax2_bits, 4);
+    return EMSPH_OK;
+}
+
+}  // namespace
+
+//==============================================================================
+extern "C" {
+
+const char* emsph_last_error(void) { return g_last_error.c_str(); }
+
+int emsph_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { return 0; }
+    return n;
+}
+
+const char* emsph_build_info(void) {
+    return "emerald_sph gfx950; f32 IEEE (contract=off, correctly rounded div/sqrt, "
+           "denormals kept); multi-scene tiles, sorted-slot neighbour lists, LSD radix sort";
+}
+
+//------------------------------------------------------------------------------
+// Single scene: a batch of one.
+emsph_status emsph_create(const emsph_config* config, const float* solid_xy, size_t solid_count,
+                          size_t fluid_capacity, emsph_context** out) {
+    if (!config || !out || (solid_count && !solid_xy)) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    if (fluid_capacity > (1ull << 30) || solid_count > (1ull << 30)) {
+        return fail(EMSPH_ERR_CAPACITY, "capacity above 2^30 particles");
+    }
+    auto* c = new emsph_context();
+    emsph_status s = engine_init(&c->e);
+    if (s) {
+        delete c;
+        return s;
+    }
+    c->cfg = *config;
+    SceneHost h;
+    h.cfg = *config;
+    h.cap = static_cast<uint32_t>(std::max<size_t>(fluid_capacity, 1));
+    h.n = 0;
+    h.ns = static_cast<uint32_t>(solid_count);
+    h.dt = config->seconds_per_sub_step;
+    s = engine_configure(&c->e, {h}, {solid_xy});
+    if (s) {
+        delete c;
+        return s;
+    }
+    *out = c;
+    return EMSPH_OK;
+}
+
+emsph_status emsph_destroy(emsph_context* ctx) {
+    if (!ctx) { return EMSPH_OK; }
+    (void)hipStreamSynchronize(ctx->e.stream);
+    delete ctx;
+    return EMSPH_OK;
+}
+
+emsph_status emsph_set_store_geometry(emsph_context* c, int enable) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    c->e.store_geometry = enable != 0;
+    if (c->e.store_geometry) { HIP_TRY(ensure_geometry(&c->e)); }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_set_state(emsph_context* c, const float* pos_xy, const float* vel_xy,
+                             size_t count) {
+    if (!c || (count && !pos_xy)) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    return engine_set_state(&c->e, 0, pos_xy, vel_xy, count);
+}
+
+emsph_status emsph_get_state(emsph_context* c, float* pos_xy, float* vel_xy) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    return engine_get_state(&c->e, 0, pos_xy, vel_xy);
+}
+
+size_t emsph_fluid_count(const emsph_context* c) { return c ? c->e.sh[0].n : 0; }
+size_t emsph_solid_count(const emsph_context* c) { return c ? c->e.sh[0].ns : 0; }
+
+emsph_status emsph_get_solid_volumes(emsph_context* c, float* out) {
+    if (!c || !out) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    return engine_solid_volumes(&c->e, 0, out);
+}
+
+emsph_status emsph_get_field(emsph_context* c, int field, float* out) {
+    if (!c || !out) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    return engine_get_field(&c->e, 0, field, out);
+}
+
+emsph_status emsph_get_neighborhood(emsph_context* c, int which, uint8_t* counts,
+                                    uint32_t* indices, float* distances, float* vectors_xy) {
+    if (!c || !counts || !indices) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    return engine_get_neighborhood(&c->e, 0, which, counts, indices, distances, vectors_xy);
+}
+
+emsph_status emsph_frame_init(emsph_context* c, int method) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    return engine_frame_init(&c->e, method);
+}
+
+emsph_status emsph_sub_step(emsph_context* c, int method, float time_seconds, float dt,
+                            emsph_step_stats* stats) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    (void)time_seconds;  // the built-in gravity force model does not depend on time
+    if (stats) { std::memset(stats, 0, sizeof(*stats)); }
+    ST_TRY(check_method(method));
+    ST_TRY(engine_set_dt(&c->e, 0, dt));
+    ST_TRY(engine_sub_step(&c->e, method));
+    if (stats) {
+        ST_TRY(read_stats(&c->e));
+        to_public(c->e.host_stats[0], &c->e, stats);
+    }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_frame_step(emsph_context* c, int method, int64_t global_time_flicks,
+                              emsph_frame_stats* fs) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    ST_TRY(emsph_frame_init(c, method));
+    emsph_frame_stats local{};
+    emsph_step_stats st{};
+    emsph_parameters const& P = c->cfg.params;
+    if (method == EMSPH_PCISPH) {
+        // pcisph.cpp:166-204: fixed sub-steps of the configured dt.
+        int64_t const per = P.time_per_step / P.sub_steps;
+        int64_t time = global_time_flicks;
+        for (int k = 0; k < P.sub_steps; ++k) {
+            ST_TRY(emsph_sub_step(c, method, static_cast<float>(to_seconds(time)),
+                                  c->cfg.seconds_per_sub_step, &st));
+            time += per;
+            local.sub_steps += 1;
+            local.total_iterations += st.iterations;
+            local.last_dt = c->cfg.seconds_per_sub_step;
+        }
+    } else {
+        // adaptive_time_step.h:35-122 with (60, 6, 0.2) for IISPH
+        // (iisph.cpp:175-181) and (60, 4, 0.4) for DFSPH (dfsph_p.cpp:383-389).
+        int const min_den = 60;
+        int const max_den = (method == EMSPH_IISPH) ? 6 : 4;
+        float const cfl = (method == EMSPH_IISPH) ? 0.2f : 0.4f;
+        int64_t const per = P.time_per_step / min_den;
+        int const divisor = min_den / max_den;
+        int remaining = min_den;
+        int64_t time = global_time_flicks;
+        float vmax2 = 0.0f;
+        if (c->e.N) { ST_TRY(engine_vmax2(&c->e, &vmax2)); }
+        while (remaining > 0) {
+            // cfl.cpp:12-21
+            double const numer = cfl * P.support;
+            double const denom = std::sqrt(static_cast<double>(vmax2));
+            bool safe = false;
+            if (denom != 0.0) {
+                safe = std::fabs(denom) >= 1.0 ||
+                       std::fabs(denom) / std::numeric_limits<double>::min() > std::fabs(numer);
+            }
+            int64_t const cfl_step = to_flicks(safe ? numer / denom : 1.0);
+            int ns = static_cast<int>(std::ceil(to_seconds(cfl_step) / to_seconds(per)));
+            ns = std::clamp(ns, 1, divisor);
+            ns = std::min(ns, remaining);
+            int64_t const sub = ns * per;
+            float const dt = static_cast<float>(to_seconds(sub));
+            ST_TRY(emsph_sub_step(c, method, static_cast<float>(to_seconds(time)), dt, &st));
+            vmax2 = st.max_velocity_sq;
+            remaining -= ns;
+            local.sub_steps += 1;
+            local.total_iterations += st.iterations;
+            local.total_divergence_iterations += st.divergence_iterations;
+            local.last_dt = dt;
+            time += sub;
+        }
+    }
+    local.last = st;
+    if (fs) { *fs = local; }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_run_sub_steps(emsph_context* c, int method, float dt, int n_sub_steps,
+                                 float* elapsed_ms, emsph_step_stats* last) {
+    if (!c || n_sub_steps < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
+    ST_TRY(check_method(method));
+    ST_TRY(engine_set_dt(&c->e, 0, dt));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->e.stream));
+    for (int k = 0; k < n_sub_steps; ++k) {
+        emsph_status const s = engine_sub_step(&c->e, method);
+        if (s) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return s;
+        }
+    }
+    HIP_TRY(hipEventRecord(e1, c->e.stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) { *elapsed_ms = ms; }
+    if (last) {
+        ST_TRY(read_stats(&c->e));
+        to_public(c->e.host_stats[0], &c->e, last);
+    }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_set_kernel_timing(emsph_context* c, int enable) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    c->e.timer.enabled = enable != 0;
+    c->e.timer.acc.clear();
+    return EMSPH_OK;
+}
+
+int emsph_get_kernel_timing(emsph_context* c, const char** names, double* total_ms,
+                            int64_t* calls, int max) {
+    if (!c) { return 0; }
+    int k = 0;
+    for (auto const& kv : c->e.timer.acc) {
+        if (k >= max) { break; }
+        if (names) { names[k] = kv.first.c_str(); }
+        if (total_ms) { total_ms[k] = kv.second.first; }
+        if (calls) { calls[k] = kv.second.second; }
+        ++k;
+    }
+    return k;
+}
+
+//------------------------------------------------------------------------------
+// Batches of scenes stepped together.
+emsph_status emsph_batch_create(int n_scenes, const emsph_config* configs,
+                                const float* const* solid_xy, const size_t* solid_counts,
+                                const size_t* fluid_counts, emsph_batch** out) {
+    if (n_scenes <= 0 || !configs || !fluid_counts || !out) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument");
+    }
+    size_t total = 0;
+    for (int s = 0; s < n_scenes; ++s) { total += fluid_counts[s]; }
+    if (total > (1ull << 30)) { return fail(EMSPH_ERR_CAPACITY, "batch above 2^30 particles"); }
+    auto* b = new emsph_batch();
+    emsph_status st = engine_init(&b->e);
+    if (st) {
+        delete b;
+        return st;
+    }
+    std::vector<SceneHost> hs(n_scenes);
+    std::vector<float const*> sx(n_scenes, nullptr);
+    for (int s = 0; s < n_scenes; ++s) {
+        hs[s].cfg = configs[s];
+        hs[s].cap = static_cast<uint32_t>(fluid_counts[s]);
+        hs[s].n = static_cast<uint32_t>(fluid_counts[s]);
+        hs[s].ns = solid_counts ? static_cast<uint32_t>(solid_counts[s]) : 0u;
+        hs[s].dt = configs[s].seconds_per_sub_step;
+        sx[s] = solid_xy ? solid_xy[s] : nullptr;
+        if (hs[s].ns && !sx[s]) {
+            delete b;
+            return fail(EMSPH_ERR_INVALID_ARGUMENT, "solid positions missing");
+        }
+    }
+    st = engine_configure(&b->e, std::move(hs), sx);
+    if (st) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return EMSPH_OK;
+}
+
+emsph_status emsph_batch_destroy(emsph_batch* b) {
+    if (b) {
+        (void)hipStreamSynchronize(b->e.stream);
+        delete b;
+    }
+    return EMSPH_OK;
+}
+
+int emsph_batch_scene_count(const emsph_batch* b) { return b ? static_cast<int>(b->e.M) : 0; }
+
+emsph_status emsph_batch_set_state(emsph_batch* b, int scene, const float* pos_xy,
+                                   const float* vel_xy) {
+    if (!b || scene < 0 || scene >= static_cast<int>(b->e.M) || !pos_xy) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument");
+    }
+    return engine_set_state(&b->e, scene, pos_xy, vel_xy, b->e.sh[scene].cap);
+}
+
+emsph_status emsph_batch_get_state(emsph_batch* b, int scene, float* pos_xy, float* vel_xy) {
+    if (!b || scene < 0 || scene >= static_cast<int>(b->e.M)) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument");
+    }
+    return engine_get_state(&b->e, scene, pos_xy, vel_xy);
+}
+
+emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* out) {
+    if (!b || scene < 0 || scene >= static_cast<int>(b->e.M) || !out) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument");
+    }
+    return engine_get_field(&b->e, scene, field, out);
+}
+
+emsph_status emsph_batch_frame_init(emsph_batch* b, int method) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    return engine_frame_init(&b->e, method);
+}
+
+emsph_status emsph_batch_sub_step(emsph_batch* b, int method, emsph_step_stats* stats) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    ST_TRY(engine_sub_step(&b->e, method));
+    if (stats) {
+        ST_TRY(read_stats(&b->e));
+        for (uint32_t s = 0; s < b->e.M; ++s) { to_public(b->e.host_stats[s], &b->e, &stats[s]); }
+    }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_batch_run_sub_steps(emsph_batch* b, int method, int n_sub_steps,
+                                       float* elapsed_ms) {
+    if (!b || n_sub_steps < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, b->e.stream));
+    for (int k = 0; k < n_sub_steps; ++k) {
+        emsph_status const s = engine_sub_step(&b->e, method);
+        if (s) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return s;
+        }
+    }
+    HIP_TRY(hipEventRecord(e1, b->e.stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) { *elapsed_ms = ms; }
+    return EMSPH_OK;
+}
+
+//------------------------------------------------------------------------------
+// Stand-alone search: one scene, no solids, cut-off = cell size.
+emsph_status emsph_search_create(size_t capacity, emsph_search** out) {
+    if (!out) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    if (capacity > (1ull << 30)) { return fail(EMSPH_ERR_CAPACITY, "capacity above 2^30"); }
+    auto* s = new emsph_search();
+    emsph_status st = engine_init(&s->e);
+    if (st) {
+        delete s;
+        return st;
+    }
+    SceneHost h;
+    emsph_default_parameters(&h.cfg.params);
+    h.cfg.mass_per_particle = 1.0f;
+    h.cap = static_cast<uint32_t>(std::max<size_t>(capacity, 1));
+    h.n = 0;
+    h.ns = 0;
+    st = engine_configure(&s->e, {h}, {nullptr});
+    if (st) {
+        delete s;
+        return st;
+    }
+    *out = s;
+    return EMSPH_OK;
+}
+
+emsph_status emsph_search_destroy(emsph_search* s) {
+    if (s) {
+        (void)hipStreamSynchronize(s->e.stream);
+        delete s;
+    }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_search_set_points(emsph_search* s, const float* xy, size_t count) {
+    if (!s || (count && !xy)) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    s->ran = false;
+    return engine_set_state(&s->e, 0, xy, nullptr, count);
+}
+
+emsph_status emsph_search_run(emsph_search* s, float cell_size, int store_geometry,
+                              float* elapsed_ms) {
+    if (!s || !(cell_size > 0.0f)) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
+    Engine* g = &s->e;
+    g->store_geometry = store_geometry != 0;
+    if (g->store_geometry) { HIP_TRY(ensure_geometry(g)); }
+    if (s->cell != cell_size) {
+        // The scene's cut-off (2h) is the requested cell size.
+        g->sh[0].cfg.params.support = cell_size * 0.5f;
+        ST_TRY(upload_layout(g));
+        g->N = g->sh[0].n;
+        s->cell = cell_size;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, g->stream));
+    if (g->N) {
+        IndexArgs ga{};
+        ga.pos_out = g->spos.p;
+        HIP_TRY(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, 1, -1, ga, false));
+        Batch b = fluid_batch(g);
+        b.fl.gw = nullptr;
+        launch_build(BUILD_LISTS, b, g->stream);
+        HIP_TRY(hipGetLastError());
+        g->have_lists = true;
+    }
+    HIP_TRY(hipEventRecord(e1, g->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) { *elapsed_ms = ms; }
+    s->ran = true;
+    return EMSPH_OK;
+}
+
+emsph_status emsph_search_get(emsph_search* s, int32_t* grid_xy, uint64_t* keys,
+                              uint64_t* sorted_keys, uint32_t* sorted_idx,
+                              uint32_t* block_indices, uint32_t* block_begin,
+                              uint32_t* block_end, size_t* block_count, uint8_t* counts,
+                              uint32_t* indices, float* distances, float* vectors_xy) {
+    if (!s || !s->ran) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "search not run"); }
+    Engine* g = &s->e;
+    uint32_t const n = g->N;
+    if (n == 0) {
+        if (block_count) { *block_count = 0; }
+        return EMSPH_OK;
+    }
+    hipStream_t const st = g->stream;
+    // Block count = last block index + 1 (neighborhoods_from_state.cpp:33).
+    uint32_t last = 0;
+    HIP_TRY(hipMemcpyAsync(&last, g->fidx.block_index.p + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    uint32_t const nb = last + 1;
+    if (block_count) { *block_count = nb; }
+    if (grid_xy || keys) {
+        // Recompute the keys in original order (the sort's buffers were reused).
+        IndexArgs ka{};
+        ka.pos = g->pos.p;
+        ka.n = n;
+        ka.scenes = g->scenes.p;
+        ka.scene_of = g->scene_of.p;
+        DBuf<uint64_t> k2;
+        DBuf<uint32_t> v2;
+        DBuf<int2> g2;
+        HIP_TRY(k2.alloc(n));
+        HIP_TRY(v2.alloc(n));
+        HIP_TRY(g2.alloc(n));
+        ka.keys = k2.p;
+        ka.vals = v2.p;
+        ka.grid = g2.p;
+        ka.keystate = g->keystate.p;
+        launch_keys(ka, st);
+        HIP_TRY(hipGetLastError());
+        if (keys) { HIP_TRY(hipMemcpyAsync(keys, k2.p, n * 8, hipMemcpyDeviceToHost, st)); }
+        if (grid_xy) { HIP_TRY(hipMemcpyAsync(grid_xy, g2.p, n * 8, hipMemcpyDeviceToHost, st)); }
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    if (sorted_keys) {
+        HIP_TRY(hipMemcpyAsync(sorted_keys, g->fidx.sorted_keys, n * 8, hipMemcpyDeviceToHost, st));
+    }
+    if (sorted_idx) {
+        HIP_TRY(hipMemcpyAsync(sorted_idx, g->fidx.orig.p, n * 4, hipMemcpyDeviceToHost, st));
+    }
+    if (block_indices) {
+        HIP_TRY(hipMemcpyAsync(block_indices, g->fidx.block_index.p, n * 4, hipMemcpyDeviceToHost,
+                               st));
+    }
+    if (block_begin) {
+        HIP_TRY(hipMemcpyAsync(block_begin, g->fidx.cell_begin.p, nb * 4, hipMemcpyDeviceToHost,
+                               st));
+    }
+    if (block_end) {
+        HIP_TRY(hipMemcpyAsync(block_end, g->fidx.cell_end.p, nb * 4, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    if (counts && indices) {
+        if ((distances || vectors_xy) && !g->store_geometry) {
+            return fail(EMSPH_ERR_INVALID_ARGUMENT, "geometry not stored by emsph_search_run");
+        }
+        ST_TRY(engine_get_neighborhood(g, 0, 0, counts, indices, distances, vectors_xy));
+    }
+    return EMSPH_OK;
+}
+
+}  // extern "C"

@@ -46,8 +46,18 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
     return peers;
 }
 
+// Digit of element i: bits [shift, shift+8) of its key, or of scene_of[val]
+// for the scene passes.
+__device__ __forceinline__ uint32_t sort_digit(uint64_t key, uint32_t val, uint32_t shift,
+                                               uint32_t const* __restrict__ scene_of) {
+    if (scene_of) { return (scene_of[val] >> shift) & 255u; }
+    return static_cast<uint32_t>((key >> shift) & 255u);
+}
+
 __global__ __launch_bounds__(RS_THREADS) void rs_hist(uint64_t const* __restrict__ keys,
+                                                      uint32_t const* __restrict__ vals,
                                                       uint32_t n, uint32_t shift,
+                                                      uint32_t const* __restrict__ scene_of,
                                                       uint32_t* __restrict__ counts,
                                                       uint32_t tiles) {
     __shared__ uint32_t h[RS_WAVES][256];
@@ -59,7 +69,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist(uint64_t const* __restrict
     for (int r = 0; r < RS_ITEMS; ++r) {
         uint32_t const i = base + r * RS_THREADS + tid;
         bool const valid = i < n;
-        uint32_t const d = valid ? static_cast<uint32_t>((keys[i] >> shift) & 255u) : 0u;
+        uint32_t const d =
+          valid ? sort_digit(scene_of ? 0ull : keys[i], scene_of ? vals[i] : 0u, shift, scene_of)
+                : 0u;
         uint64_t const peers = match_digit(d, valid);
         if (valid && (peers & lanemask_lt()) == 0) {
             atomicAdd(&h[wave][d], static_cast<uint32_t>(__popcll(peers)));
@@ -74,7 +86,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist(uint64_t const* __restrict
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
   uint64_t const* __restrict__ kin, uint32_t const* __restrict__ vin,
   uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t shift,
-  uint32_t const* __restrict__ offsets, uint32_t tiles) {
+  uint32_t const* __restrict__ scene_of, uint32_t const* __restrict__ offsets, uint32_t tiles) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_WAVES][256];
     uint32_t const tid = threadIdx.x;
@@ -89,7 +101,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
         bool const valid = i < n;
         uint64_t const k = valid ? kin[i] : 0ull;
         uint32_t const v = valid ? vin[i] : 0u;
-        uint32_t const d = static_cast<uint32_t>((k >> shift) & 255u);
+        uint32_t const d = valid ? sort_digit(k, v, shift, scene_of) : 0u;
         uint64_t const peers = match_digit(d, valid);
         bool const leader = valid && (peers & lt) == 0;
         uint32_t const cnt = static_cast<uint32_t>(__popcll(peers));
@@ -245,20 +257,23 @@ int radix_pass_count(uint64_t kmin, uint64_t kmax) {
 }
 
 int launch_radix_sort(uint64_t* keys0, uint32_t* vals0, uint64_t* keys1, uint32_t* vals1,
-                      uint32_t n, int passes, uint32_t* counts, uint32_t* scan_scratch,
-                      hipStream_t s) {
+                      uint32_t n, int key_passes, uint32_t const* scene_of, int scene_passes,
+                      uint32_t* counts, uint32_t* scan_scratch, hipStream_t s) {
     uint32_t const tiles = (n + RS_TILE - 1) / RS_TILE;
     uint64_t* kin = keys0;
     uint32_t* vin = vals0;
     uint64_t* kout = keys1;
     uint32_t* vout = vals1;
+    int const passes = key_passes + (scene_of ? scene_passes : 0);
     for (int p = 0; p < passes; ++p) {
-        uint32_t const shift = 8u * static_cast<uint32_t>(p);
-        hipLaunchKernelGGL(rs_hist, dim3(tiles), dim3(RS_THREADS), 0, s, kin, n, shift, counts,
-                           tiles);
+        bool const scene_pass = p >= key_passes;
+        uint32_t const shift = 8u * static_cast<uint32_t>(scene_pass ? p - key_passes : p);
+        uint32_t const* so = scene_pass ? scene_of : nullptr;
+        hipLaunchKernelGGL(rs_hist, dim3(tiles), dim3(RS_THREADS), 0, s, kin, vin, n, shift, so,
+                           counts, tiles);
         launch_exclusive_scan(counts, counts, 256u * tiles, scan_scratch, nullptr, s);
         hipLaunchKernelGGL(rs_scatter, dim3(tiles), dim3(RS_THREADS), 0, s, kin, vin, kout, vout,
-                           n, shift, counts, tiles);
+                           n, shift, so, counts, tiles);
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;

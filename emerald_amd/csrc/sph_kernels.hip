@@ -1,9 +1,10 @@
-// HIP kernels of the 2D SPH sub-step for gfx950.
+// HIP kernels of the 2D SPH sub-step for gfx950, over a batch of scenes.
 //
-// Layout: particle fields live in Morton-sorted order ("slots") for the
-// duration of a sub-step; neighbour lists are slot-major ([k * stride + t]) so
-// that the thread of slot t reads entry k of 64 consecutive slots with one
-// coalesced load. Each kernel evaluates one particle per thread, summing its
+// Layout: particle fields live in sorted order ("slots"): scenes one after
+// another, each in Morton order, for the duration of a sub-step; neighbour
+// lists are slot-major ([k * stride + t]) so that the thread of slot t reads
+// entry k of consecutive slots with one coalesced load. Fluid kernels run one
+// workgroup per scene-aligned tile and one thread per particle, summing its
 // neighbours in the reference's list order (distance, then original index), so
 // every float result matches the reference CPU path bit for bit.
 //
@@ -22,9 +23,24 @@
 namespace emsph {
 
 constexpr int BUILD_THREADS = 64;
-constexpr int SWEEP_THREADS = 256;
 
-__device__ __forceinline__ V2 ld(V2 const* p, uint32_t i) { return p[i]; }
+// The tile of this workgroup (XCD-aware order: each XCD walks one contiguous
+// run of tiles, i.e. of Morton-ordered space, and keeps its gathers in its L2).
+struct TileRef {
+    uint32_t scene;
+    uint32_t t;       // this thread's slot
+    bool valid;
+};
+__device__ __forceinline__ TileRef tile_ref(Batch const& b, uint32_t threads) {
+    uint32_t const k = xcd_tile(blockIdx.x, gridDim.x);
+    Tile const tl = b.tiles[k];
+    (void)threads;
+    TileRef r;
+    r.scene = tl.scene;
+    r.t = tl.start + threadIdx.x;
+    r.valid = threadIdx.x < tl.count;
+    return r;
+}
 
 // Quantised residual accumulation over a workgroup followed by one atomic per
 // workgroup. Every thread of the block must call it.
@@ -75,23 +91,14 @@ __device__ __forceinline__ void block_reduce_max(unsigned int m, unsigned int* o
 
 //------------------------------------------------------------------------------
 // Keys: sph_common/common.cpp:43-76 (grid coordinate, z key) and
-// neighborhood.cpp:10-16 (index pairs).
-__global__ __launch_bounds__(256) void k_keys(KeysArgs a) {
+// neighborhood.cpp:10-16 (index pairs), with the key range reduced on the fly.
+__global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0) {
-        if (a.red_reset) {
-            for (uint32_t k = threadIdx.x; k < MAX_RED_SLOTS; k += blockDim.x) {
-                a.red_reset[k].isum = 0;
-                a.red_reset[k].max_bits = 0;
-                a.red_reset[k].pad = 0;
-            }
-        }
-        if (a.vmax2_reset && threadIdx.x == 0) { *a.vmax2_reset = 0; }
-    }
     unsigned long long key_min = ~0ull;
     unsigned long long key_max = 0ull;
     if (i < a.n) {
-        int2 const g = grid_coord(a.pos[i], a.cell);
+        float const cell = a.scenes[a.scene_of[i]].c.cell;
+        int2 const g = grid_coord(a.pos[i], cell);
         uint64_t const key = morton2(g.x, g.y);
         a.keys[i] = key;
         a.vals[i] = i;
@@ -112,7 +119,7 @@ __global__ __launch_bounds__(256) void k_keys(KeysArgs a) {
     }
 }
 
-void launch_keys(KeysArgs const& a, hipStream_t s) {
+void launch_keys(IndexArgs const& a, hipStream_t s) {
     uint32_t const blocks = (a.n + 255) / 256;
     hipLaunchKernelGGL(k_keys, dim3(blocks ? blocks : 1), dim3(256), 0, s, a);
 }
@@ -132,54 +139,174 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 }
 
 //------------------------------------------------------------------------------
-// Cells: neighborhood.cpp:33-124 (block indices, block ranges, key map).
-__global__ __launch_bounds__(256) void k_cell_heads(CellsArgs a) {
+// Cells: neighborhood.cpp:33-124 (block indices, block ranges, key map). A
+// block starts where the (scene, key) pair changes.
+__global__ __launch_bounds__(256) void k_cell_heads(IndexArgs a) {
     uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0 && a.keystate) {
-        // The LSD sort ordered only the low 8*passes bits; every key must share
-        // the bits above them.
+    if (t == 0 && a.check_keys) {
+        // The LSD sort ordered only the low 8*passes key bits; every key must
+        // share the bits above them.
         unsigned long long const diff = a.keystate->kmin ^ a.keystate->kmax;
-        bool const covered = (a.passes >= 8) || ((diff >> (8 * a.passes)) == 0ull);
+        bool const covered = (a.key_passes >= 8) || ((diff >> (8 * a.key_passes)) == 0ull);
         if (!covered) { a.keystate->sort_short = 1u; }
     }
     if (t >= a.n) { return; }
-    a.heads[t] = (t == 0 || a.sorted_keys[t] != a.sorted_keys[t - 1]) ? 1u : 0u;
+    bool head = (t == 0);
+    if (!head) {
+        head = a.sorted_keys[t] != a.sorted_keys[t - 1] ||
+               a.scene_of[a.sorted_vals[t]] != a.scene_of[a.sorted_vals[t - 1]];
+    }
+    a.heads[t] = head ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_cells_finalize(CellsArgs a) {
+__global__ __launch_bounds__(256) void k_cells_finalize(IndexArgs a) {
     uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) { return; }
     uint64_t const key = a.sorted_keys[t];
+    uint32_t const o = a.sorted_vals[t];
+    uint32_t const tag = a.scene_of[o];
     uint32_t const head = a.heads[t];
     uint32_t const b = a.head_scan[t] + head - 1u;
     a.block_index[t] = b;
     if (head) {
         a.cell_begin[b] = t;
-        uint32_t s = cell_hash(key, a.mask);
+        uint32_t s = cell_hash(key, tag, a.mask);
         for (;;) {
             uint32_t const prev = atomicCAS(&a.table_vals[s], EMPTY_CELL, b);
             if (prev == EMPTY_CELL) {
                 a.table_keys[s] = key;
+                a.table_tags[s] = tag;
                 break;
             }
             s = (s + 1) & a.mask;
         }
     }
-    if (t + 1 == a.n || a.sorted_keys[t + 1] != key) { a.cell_end[b] = t + 1; }
-    uint32_t const o = a.sorted_vals[t];
+    bool const tail = (t + 1 == a.n) || a.sorted_keys[t + 1] != key ||
+                      a.scene_of[a.sorted_vals[t + 1]] != tag;
+    if (tail) { a.cell_end[b] = t + 1; }
     a.orig[t] = o;
     a.rank[o] = t;
-    if (a.pos_out) { a.pos_out[t] = a.pos_in[o]; }
+    if (a.pos_out) { a.pos_out[t] = a.pos[o]; }
     if (a.vel_out) { a.vel_out[t] = a.vel_in[o]; }
     if (a.kv_out) { a.kv_out[t] = a.kv_in[o]; }
     if (a.kd_out) { a.kd_out[t] = a.kd_in[o]; }
 }
 
-void launch_cell_heads(CellsArgs const& a, hipStream_t s) {
+void launch_cell_heads(IndexArgs const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_cell_heads, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
-void launch_cells_finalize(CellsArgs const& a, hipStream_t s) {
+void launch_cells_finalize(IndexArgs const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_cells_finalize, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+
+//------------------------------------------------------------------------------
+// Per-sub-step and per-loop bookkeeping.
+__global__ __launch_bounds__(256) void k_step_begin(Batch b) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < b.nscenes;
+         s += gridDim.x * blockDim.x) {
+        SceneStats st{};
+        b.stats[s] = st;
+        for (int k = 0; k < MAX_RED_SLOTS; ++k) {
+            b.red[s * MAX_RED_SLOTS + k].isum = 0;
+            b.red[s * MAX_RED_SLOTS + k].max_bits = 0;
+            b.red[s * MAX_RED_SLOTS + k].pad = 0;
+        }
+    }
+}
+
+void launch_step_begin(Batch const& b, hipStream_t s) {
+    uint32_t blocks = (b.nscenes + 255) / 256;
+    hipLaunchKernelGGL(k_step_begin, dim3(blocks ? blocks : 1), dim3(256), 0, s, b);
+}
+
+__global__ __launch_bounds__(1024) void k_loop_begin(Batch b) {
+    uint32_t cnt = 0;
+    for (uint32_t s = threadIdx.x; s < b.nscenes; s += blockDim.x) {
+        unsigned int const act = b.scenes[s].n > 0 ? 1u : 0u;
+        b.stats[s].active = act;
+        cnt += act;
+    }
+    for (int off = 32; off > 0; off >>= 1) { cnt += __shfl_xor(cnt, off); }
+    __shared__ uint32_t ws[16];
+    if (__lane_id() == 0) { ws[threadIdx.x / 64] = cnt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < 16; ++w) { tot += ws[w]; }
+        b.keystate->active_count = tot;
+    }
+}
+
+void launch_loop_begin(Batch const& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_loop_begin, dim3(1), dim3(1024), 0, s, b);
+}
+
+// Exit tests of the reference's solver loops, evaluated per scene on the
+// device: IISPH iisph.cpp:126-134 (from the third iteration), DFSPH divergence
+// dfsph_p.cpp:186-210 (from the second), DFSPH density dfsph_p.cpp:112-132
+// (from the third), PCISPH pcisph.cpp:144-148 (from the third). The average is
+// the reference's (double) isum / (1000 n) narrowed to float.
+__global__ __launch_bounds__(256) void k_loop_decide(Batch b, int kind, int it, int slot) {
+    uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.nscenes) { return; }
+    SceneStats& st = b.stats[s];
+    if (!st.active) { return; }
+    SceneDev const& sc = b.scenes[s];
+    Reduction const r = b.red[s * MAX_RED_SLOTS + slot];
+    double const num = static_cast<double>(r.isum);
+    double const den = static_cast<double>(1000ull * sc.n);
+    float const avg = static_cast<float>(num / den);
+    float const mx = __uint_as_float(r.max_bits);
+    bool exit = false;
+    int max_iters = 0;
+    switch (kind) {
+    case LOOP_IISPH:
+        max_iters = sc.thr.iisph_max_iters;
+        exit = it > 1 && avg <= sc.thr.iisph_avg && mx <= sc.thr.iisph_max;
+        st.error_average = avg;
+        st.error_max = mx;
+        break;
+    case LOOP_DIVERGENCE:
+        max_iters = sc.thr.dfsph_max_iters;
+        if (it > 0) {
+            exit = avg <= sc.thr.div_avg && mx <= sc.thr.div_max;
+            st.divergence_error_average = avg;
+            st.divergence_error_max = mx;
+        }
+        break;
+    case LOOP_DENSITY:
+        max_iters = sc.thr.dfsph_max_iters;
+        if (it > 1) {
+            exit = avg <= sc.thr.dens_avg && mx <= sc.thr.dens_max;
+            st.error_average = avg;
+            st.error_max = mx;
+        }
+        break;
+    default: {  // LOOP_PCISPH
+        max_iters = 6;
+        if (it > 1) {
+            float const err = mx / sc.c.rho0;
+            exit = err <= 0.01f;
+            st.error_average = 0.0f;
+            st.error_max = err;
+        }
+        break;
+    }
+    }
+    if (exit || it + 1 >= max_iters) {
+        st.active = 0u;
+        if (kind == LOOP_DIVERGENCE) {
+            st.divergence_iterations = it + 1;
+        } else {
+            st.iterations = it + 1;
+        }
+        atomicSub(&b.keystate->active_count, 1u);
+    }
+}
+
+void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_loop_decide, dim3((b.nscenes + 255) / 256), dim3(256), 0, s, b, kind,
+                       it, slot);
 }
 
 //------------------------------------------------------------------------------
@@ -193,12 +320,14 @@ void launch_cells_finalize(CellsArgs const& a, hipStream_t s) {
 // non-negative, so their bit patterns order like the floats.
 __device__ __forceinline__ int collect_candidates(uint64_t (*ent)[BUILD_THREADS], uint32_t lane,
                                                   V2 p, int2 g, uint32_t self, bool same,
-                                                  CellTable const& tb, V2 const* __restrict__ opos,
+                                                  uint32_t tag, CellTable const& tb,
+                                                  V2 const* __restrict__ opos,
                                                   uint32_t const* __restrict__ oorig, float maxd) {
     int cnt = 0;
     for (int32_t j = g.y - 1; cnt < MAXN && j <= g.y + 1; ++j) {
         for (int32_t i = g.x - 1; cnt < MAXN && i <= g.x + 1; ++i) {
-            uint32_t const ord = cell_lookup(morton2(i, j), tb.keys, tb.vals, tb.mask);
+            uint32_t const ord =
+              cell_lookup(morton2(i, j), tag, tb.keys, tb.tags, tb.vals, tb.mask);
             if (ord == EMPTY_CELL) { continue; }
             uint32_t const e = tb.end[ord];
             for (uint32_t s = tb.begin[ord]; cnt < MAXN && s != e; ++s) {
@@ -224,29 +353,35 @@ __device__ __forceinline__ int collect_candidates(uint64_t (*ent)[BUILD_THREADS]
 }
 
 template <int MODE>
-__global__ __launch_bounds__(BUILD_THREADS) void k_build(BuildArgs a) {
+__global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
     __shared__ uint64_t ent[MAXN][BUILD_THREADS];
     uint32_t const lane = threadIdx.x;
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * BUILD_THREADS + lane;
-    if (t >= a.n) { return; }
-    StepConsts const& c = a.c;
+    // Build tiles are the fluid tiles split in quarters (64 threads).
+    uint32_t const k = xcd_tile(blockIdx.x, gridDim.x);
+    Tile const tl = a.tiles[k >> 2];
+    uint32_t const sub_off = (k & 3u) * BUILD_THREADS + lane;
+    if (sub_off >= tl.count) { return; }
+    uint32_t const t = tl.start + sub_off;
+    uint32_t const scene = tl.scene;
+    StepConsts const c = a.scenes[scene].c;
     V2 const p = a.pos[t];
     int2 const g = grid_coord(p, c.cell);
     float const mf = c.rho0 * c.fluid_volume;  // rho0 * V_j (uniform fluid volume)
 
     // Fluid neighbours.
-    int const fc = collect_candidates(ent, lane, p, g, t, true, a.table, a.pos, a.orig, c.cell);
+    int const fc =
+      collect_candidates(ent, lane, p, g, t, true, scene, a.table, a.pos, a.orig, c.cell);
     a.fl.count[t] = static_cast<uint8_t>(fc);
     float dens = c.fluid_volume * c.rho0 * c.kc.w0;  // density.cpp:57-59
     V2 msum_f{0.0f, 0.0f};
     V2 sum_mg{0.0f, 0.0f};    // DFSPH: sum of m grad W over fluid and solid
     float sum_mg2 = 0.0f;     // DFSPH: sum of |m grad W|^2 over fluid
-    for (int k = 0; k < fc; ++k) {
-        uint64_t const e = ent[k][lane];
+    for (int k2 = 0; k2 < fc; ++k2) {
+        uint64_t const e = ent[k2][lane];
         uint32_t const s = a.rank[static_cast<uint32_t>(e)];
         float const d = __uint_as_float(static_cast<uint32_t>(e >> 32));
         V2 const vec = sub(a.pos[s], p);
-        size_t const o = static_cast<size_t>(k) * a.stride + t;
+        size_t const o = static_cast<size_t>(k2) * a.stride + t;
         a.fl.idx[o] = s;
         if (a.store_geometry) {
             a.fl.dist[o] = d;
@@ -267,20 +402,23 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build(BuildArgs a) {
     }
 
     // Solid neighbours (no self exclusion: different arrays).
+    // A batch without solid lists (the solid-solid build) skips this part.
+    if (!a.sl.count) { return; }
     SolidView const& sv = a.solids;
     int sc = 0;
-    if (sv.n) {
-        sc = collect_candidates(ent, lane, p, g, t, false, sv.table, sv.pos, sv.orig, c.cell);
+    if (sv.table.keys) {
+        sc = collect_candidates(ent, lane, p, g, t, false, scene, sv.table, sv.pos, sv.orig,
+                                c.cell);
     }
     a.sl.count[t] = static_cast<uint8_t>(sc);
     float dens_s = 0.0f;
     V2 msum_s{0.0f, 0.0f};
-    for (int k = 0; k < sc; ++k) {
-        uint64_t const e = ent[k][lane];
+    for (int k2 = 0; k2 < sc; ++k2) {
+        uint64_t const e = ent[k2][lane];
         uint32_t const b = sv.rank[static_cast<uint32_t>(e)];
         float const d = __uint_as_float(static_cast<uint32_t>(e >> 32));
         V2 const vec = sub(sv.pos[b], p);
-        size_t const o = static_cast<size_t>(k) * a.stride + t;
+        size_t const o = static_cast<size_t>(k2) * a.stride + t;
         a.sl.idx[o] = b;
         if (a.store_geometry) {
             a.sl.dist[o] = d;
@@ -334,8 +472,8 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build(BuildArgs a) {
     }
 }
 
-void launch_build(int mode, BuildArgs const& a, hipStream_t s) {
-    uint32_t const blocks = (a.n + BUILD_THREADS - 1) / BUILD_THREADS;
+void launch_build(int mode, Batch const& a, hipStream_t s) {
+    uint32_t const blocks = a.ntiles * 4;
     if (!blocks) { return; }
     switch (mode) {
     case BUILD_IISPH:
@@ -350,36 +488,37 @@ void launch_build(int mode, BuildArgs const& a, hipStream_t s) {
     }
 }
 
-// solids.cpp:52-74 compute_volumes: 1.7 / (W0 + sum W) over solid-solid lists.
-__global__ __launch_bounds__(256) void k_solid_volumes(KernelConsts kc, uint32_t n,
-                                                       uint32_t stride, Lists l, float* vol) {
-    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) { return; }
-    int const cnt = l.count[t];
+// solids.cpp:52-74 compute_volumes: 1.7 / (W0 + sum W) over solid-solid lists
+// (the batch's "fluid" arrays are the solid arrays here).
+__global__ __launch_bounds__(TILE) void k_solid_volumes(Batch b) {
+    TileRef const r = tile_ref(b, TILE);
+    if (!r.valid) { return; }
+    KernelConsts const kc = b.scenes[r.scene].c.kc;
+    uint32_t const t = r.t;
+    int const cnt = b.fl.count[t];
     if (!cnt) {
-        vol[t] = 1.0f / kc.w0;
+        b.rho[t] = 1.0f / kc.w0;
         return;
     }
     float m = kc.w0;
-    for (int k = 0; k < cnt; ++k) { m += kernel_W(l.dist[static_cast<size_t>(k) * stride + t], kc); }
-    vol[t] = 1.7f / m;
+    for (int k = 0; k < cnt; ++k) { m += kernel_W(b.fl.dist[static_cast<size_t>(k) * b.stride + t], kc); }
+    b.rho[t] = 1.7f / m;
 }
 
-void launch_solid_volumes(KernelConsts const& kc, uint32_t n, uint32_t stride, Lists const& l,
-                          float* vol_sorted, hipStream_t s) {
-    if (!n) { return; }
-    hipLaunchKernelGGL(k_solid_volumes, dim3((n + 255) / 256), dim3(256), 0, s, kc, n, stride, l,
-                       vol_sorted);
+void launch_solid_volumes(Batch const& b, uint32_t n, hipStream_t s) {
+    if (!n || !b.ntiles) { return; }
+    hipLaunchKernelGGL(k_solid_volumes, dim3(b.ntiles), dim3(TILE), 0, s, b);
 }
 
 //------------------------------------------------------------------------------
 // IISPH sweeps: sph2d_box/iisph_ops.cpp.
 
 // :94-221 a_ii and density stars; also zeroes the pressures (iisph.cpp:83).
-__global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_aii(SweepArgs a) {
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
-    if (t >= a.n) { return; }
-    StepConsts const& c = a.c;
+__global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!r.valid) { return; }
+    uint32_t const t = r.t;
+    StepConsts const c = a.scenes[r.scene].c;
     float const mf = c.fluid_volume * c.rho0;
     float const neg_dt2 = -(c.dt * c.dt);
     float const kf = neg_dt2 * mf;
@@ -432,11 +571,12 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_aii(SweepArgs a) {
     a.p_out[t] = 0.0f;
 }
 
-// :225-258 sum_j d_ij p_j (fluid only).
-__global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_sdp(SweepArgs a) {
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
-    if (t >= a.n) { return; }
-    StepConsts const& c = a.c;
+// :225-258 sum_j d_ij p_j (fluid only). Converged scenes skip the sweep.
+__global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!r.valid || !a.stats[r.scene].active) { return; }
+    uint32_t const t = r.t;
+    StepConsts const c = a.scenes[r.scene].c;
     float const kf = -(c.dt * c.dt) * (c.fluid_volume * c.rho0);
     int const fc = a.fl.count[t];
     V2 s{0.0f, 0.0f};
@@ -453,13 +593,20 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_sdp(SweepArgs a) {
 }
 
 // :261-353 beta (into the new-pressure buffer) fused with :357-425 new
-// pressure, quantised error sum and error max.
-__global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_beta_p(SweepArgs a, int slot) {
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+// pressure, quantised error sum and error max. A converged scene copies its
+// pressures forward so that the last buffer holds every scene's result.
+__global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, int slot) {
+    TileRef const r = tile_ref(a, TILE);
+    bool const active = a.stats[r.scene].active != 0u;
+    if (!active) {
+        if (r.valid) { a.p_out[r.t] = a.p_in[r.t]; }
+        return;  // the whole workgroup takes this branch
+    }
     unsigned long long qsum = 0;
     unsigned int emax = 0;
-    if (t < a.n) {
-        StepConsts const& c = a.c;
+    if (r.valid) {
+        uint32_t const t = r.t;
+        StepConsts const c = a.scenes[r.scene].c;
         float const mf = c.rho0 * c.fluid_volume;
         float const kf = -(c.dt * c.dt) * mf;
         float const pi = a.p_in[t];
@@ -507,17 +654,19 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_iisph_beta_p(SweepArgs a, int
         }
         a.p_out[t] = pn;
     }
-    block_reduce_residual<SWEEP_THREADS>(qsum, emax, &a.red[slot]);
+    block_reduce_residual<TILE>(qsum, emax, &a.red[r.scene * MAX_RED_SLOTS + slot]);
 }
 
 //------------------------------------------------------------------------------
 // pressure.cpp:9-80 apply_pressures_in_place (fluid pass, then solid pass).
 template <int GRAVITY, int FINAL>
-__global__ __launch_bounds__(SWEEP_THREADS) void k_apply(SweepArgs a) {
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+__global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!all_scenes && !a.stats[r.scene].active) { return; }  // whole workgroup
     unsigned int vm = 0;
-    if (t < a.n) {
-        StepConsts const& c = a.c;
+    if (r.valid) {
+        uint32_t const t = r.t;
+        StepConsts const c = a.scenes[r.scene].c;
         V2 v = a.vel[t];
         if (GRAVITY) {
             V2 const f{0.0f, 0.0f - c.mass * c.gravity};
@@ -568,35 +717,36 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_apply(SweepArgs a) {
             a.vel[t] = v;
         }
     }
-    if (FINAL) { block_reduce_max<SWEEP_THREADS>(vm, a.vmax2_bits); }
+    if (FINAL) { block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits); }
 }
 
 // x += dt v and scatter the sorted state back to original order
 // (dfsph_p.cpp:368-369), including the DFSPH warm-start stiffnesses.
 template <int DFSPH>
-__global__ __launch_bounds__(SWEEP_THREADS) void k_finalize(SweepArgs a) {
-    uint32_t const t = blockIdx.x * SWEEP_THREADS + threadIdx.x;
+__global__ __launch_bounds__(TILE) void k_finalize(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
     unsigned int vm = 0;
-    if (t < a.n) {
+    if (r.valid) {
+        uint32_t const t = r.t;
         V2 const v = a.vel[t];
-        V2 const x = add(a.pos[t], scale(a.c.dt, v));
+        V2 const x = add(a.pos[t], scale(a.scenes[r.scene].c.dt, v));
         uint32_t const o = a.orig[t];
         a.pos_orig[o] = x;
         a.vel_orig[o] = v;
         if (DFSPH) {
-            a.kappa_v_orig[o] = a.kappa_v_sorted[t];
-            a.kappa_d_orig[o] = a.kappa_d_sorted[t];
+            a.kv_orig[o] = a.kvs[t];
+            a.kd_orig[o] = a.kds[t];
         }
         vm = __float_as_uint(dot(v, v));
     }
-    block_reduce_max<SWEEP_THREADS>(vm, a.vmax2_bits);
+    block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
 }
 
 //------------------------------------------------------------------------------
 // DFSPH_P sweeps: divergence.cpp:5-74 and dfsph_p_ops.cpp:87-126, with the
 // residual of common.h:209-232 fused in.
-__device__ __forceinline__ float divergence_of(SweepArgs const& a, uint32_t t, V2 vi) {
-    StepConsts const& c = a.c;
+__device__ __forceinline__ float divergence_of(Batch const& a, StepConsts const& c, uint32_t t,
+                                               V2 vi) {
     float const mf = c.rho0 * c.fluid_volume;
     int const fc = a.fl.count[t];
     float div = 0.0f;
@@ -621,13 +771,15 @@ __device__ __forceinline__ float divergence_of(SweepArgs const& a, uint32_t t, V
 
 // One divergence-solve iteration: divergence, kappa_v, residual of
 // max(0, dt * div / rho0) (dfsph_p.cpp:157-197).
-__global__ __launch_bounds__(SWEEP_THREADS) void k_dfsph_div(SweepArgs a, int slot) {
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+__global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a, int slot) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!a.stats[r.scene].active) { return; }
     unsigned long long qsum = 0;
     unsigned int emax = 0;
-    if (t < a.n) {
-        StepConsts const& c = a.c;
-        float const div = divergence_of(a, t, a.vel[t]);
+    if (r.valid) {
+        uint32_t const t = r.t;
+        StepConsts const c = a.scenes[r.scene].c;
+        float const div = divergence_of(a, c, t, a.vel[t]);
         a.rho_star[t] = div;
         float kv = 0.0f;
         if (!(div <= 0.0f)) {
@@ -639,18 +791,20 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_dfsph_div(SweepArgs a, int sl
         emax = __float_as_uint(val);
         qsum = static_cast<unsigned long long>(static_cast<long long>(val * 1000.0f));
     }
-    block_reduce_residual<SWEEP_THREADS>(qsum, emax, &a.red[slot]);
+    block_reduce_residual<TILE>(qsum, emax, &a.red[r.scene * MAX_RED_SLOTS + slot]);
 }
 
 // One density-solve iteration: density star (density.cpp:111-145), kappa_rho,
 // residual of max(0, (rho* - rho0) / rho0) (dfsph_p.cpp:82-122).
-__global__ __launch_bounds__(SWEEP_THREADS) void k_dfsph_dens(SweepArgs a, int slot) {
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+__global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a, int slot) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!a.stats[r.scene].active) { return; }
     unsigned long long qsum = 0;
     unsigned int emax = 0;
-    if (t < a.n) {
-        StepConsts const& c = a.c;
-        float const div = divergence_of(a, t, a.vel[t]);
+    if (r.valid) {
+        uint32_t const t = r.t;
+        StepConsts const c = a.scenes[r.scene].c;
+        float const div = divergence_of(a, c, t, a.vel[t]);
         float rs = a.rho[t] + (c.dt * div);
         rs = (0.0f < rs) ? rs : 0.0f;  // std::max(0.0f, ...)
         a.rho_star[t] = rs;
@@ -667,7 +821,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_dfsph_dens(SweepArgs a, int s
         emax = __float_as_uint(val);
         qsum = static_cast<unsigned long long>(static_cast<long long>(val * 1000.0f));
     }
-    block_reduce_residual<SWEEP_THREADS>(qsum, emax, &a.red[slot]);
+    block_reduce_residual<TILE>(qsum, emax, &a.red[r.scene * MAX_RED_SLOTS + slot]);
 }
 
 //------------------------------------------------------------------------------
@@ -675,10 +829,11 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_dfsph_dens(SweepArgs a, int s
 
 // :54-88 first predictor: v_ext = v + (dt/m) F_ext; v* = v_ext; x* = x + dt v*;
 // p = 0.
-__global__ __launch_bounds__(SWEEP_THREADS) void k_pci_init(SweepArgs a) {
-    uint32_t const t = blockIdx.x * SWEEP_THREADS + threadIdx.x;
-    if (t >= a.n) { return; }
-    StepConsts const& c = a.c;
+__global__ __launch_bounds__(TILE) void k_pci_init(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!r.valid) { return; }
+    uint32_t const t = r.t;
+    StepConsts const c = a.scenes[r.scene].c;
     V2 const f{0.0f, 0.0f - c.mass * c.gravity};
     float const dtm = c.dt / c.mass;
     V2 const vext = add(a.vel[t], scale(dtm, f));
@@ -690,12 +845,14 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_pci_init(SweepArgs a) {
 
 // :90-115 geometry at x* (neighborhoods_from_state.cpp:103-128), densities
 // (pcisph_ops.cpp:18-63), pressure update (:65-76) and the max density error
-// (common.cpp:78-85).
-__global__ __launch_bounds__(SWEEP_THREADS) void k_pci_density(SweepArgs a, int slot) {
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
+// (common.cpp:78-85). PCISPH densities are kept in rho.
+__global__ __launch_bounds__(TILE) void k_pci_density(Batch a, int slot) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!a.stats[r.scene].active) { return; }
     unsigned int emax = 0;
-    if (t < a.n) {
-        StepConsts const& c = a.c;
+    if (r.valid) {
+        uint32_t const t = r.t;
+        StepConsts const c = a.scenes[r.scene].c;
         V2 const xs = a.xstar[t];
         int const fc = a.fl.count[t];
         float w = c.kc.w0;
@@ -723,25 +880,31 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_pci_density(SweepArgs a, int 
             }
             rho += c.rho0 * vf;
         }
-        a.rho_star[t] = rho;  // PCISPH densities live in the rho_star buffer
+        a.rho[t] = rho;
         float e = rho - c.rho0;
         e = (e < 0.0f) ? 0.0f : e;  // std::max(rho - rho0, 0.0f)
         a.p_out[t] = a.p_out[t] + e / c.pci_denom;
         emax = __float_as_uint(e);
     }
-    block_reduce_max<SWEEP_THREADS>(emax, &a.red[slot].max_bits);
+    block_reduce_max<TILE>(emax, &a.red[r.scene * MAX_RED_SLOTS + slot].max_bits);
 }
 
 // :117-142 pressure + artificial viscosity forces, solid forces, then the next
-// predictor (:73-88) from the new force.
-__global__ __launch_bounds__(SWEEP_THREADS) void k_pci_force(SweepArgs a) {
-    uint32_t const t = xcd_tile(blockIdx.x, gridDim.x) * SWEEP_THREADS + threadIdx.x;
-    if (t >= a.n) { return; }
-    StepConsts const& c = a.c;
+// predictor (:73-88) from the new force. A converged scene carries its last
+// predictor forward.
+__global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!r.valid) { return; }
+    uint32_t const t = r.t;
+    if (!a.stats[r.scene].active) {
+        a.vstar_out[t] = a.vstar_in[t];
+        return;
+    }
+    StepConsts const c = a.scenes[r.scene].c;
     float const H = c.kc.h;
     float const M = c.mass;
     float const N2 = 0.01f * H * H;
-    float const* rho = a.rho_star;
+    float const* rho = a.rho;
     float const* p = a.p_out;
     V2 F{0.0f, 0.0f};
     float const density = rho[t];
@@ -753,12 +916,12 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_pci_force(SweepArgs a) {
         for (int k = 0; k < fc; ++k) {
             size_t const o = static_cast<size_t>(k) * a.stride + t;
             uint32_t const j = a.fl.idx[o];
-            V2 const r = a.fl.vec[o];
+            V2 const rv = a.fl.vec[o];
             V2 const gw = a.fl.gw[o];
             float const od = rho[j];
             f = add(f, scale(por + (p[j] / (od * od)), gw));
             V2 const dv = sub(a.vstar_in[j], v);
-            float const dvr = dot(r, dv);
+            float const dvr = dot(rv, dv);
             if (dvr < 0.0f) {
                 float const dist = a.fl.dist[o];
                 float const mu = H * dvr / (dist + N2);
@@ -788,25 +951,28 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_pci_force(SweepArgs a) {
 
 // :151-163 final update: v = v_ext + (dt/m) F; x += dt v. Both equal the last
 // predictor written by k_pci_force.
-__global__ __launch_bounds__(SWEEP_THREADS) void k_pci_finalize(SweepArgs a) {
-    uint32_t const t = blockIdx.x * SWEEP_THREADS + threadIdx.x;
+__global__ __launch_bounds__(TILE) void k_pci_finalize(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
     unsigned int vm = 0;
-    if (t < a.n) {
+    if (r.valid) {
+        uint32_t const t = r.t;
         V2 const v = a.vstar_in[t];
         uint32_t const o = a.orig[t];
         a.pos_orig[o] = a.xstar[t];
         a.vel_orig[o] = v;
         vm = __float_as_uint(dot(v, v));
     }
-    block_reduce_max<SWEEP_THREADS>(vm, a.vmax2_bits);
+    block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
 }
 
 //------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_vmax2(V2 const* vel, uint32_t n, unsigned int* out) {
-    uint32_t const t = blockIdx.x * 256 + threadIdx.x;
+// max |v|^2 of each scene over its original-order velocities (cfl.cpp:12-21);
+// the tiles cover original indices with the same scene ranges.
+__global__ __launch_bounds__(TILE) void k_vmax2(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
     unsigned int vm = 0;
-    if (t < n) { vm = __float_as_uint(dot(vel[t], vel[t])); }
-    block_reduce_max<256>(vm, out);
+    if (r.valid) { vm = __float_as_uint(dot(a.vel_orig[r.t], a.vel_orig[r.t])); }
+    block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
 }
 
 __global__ __launch_bounds__(256) void k_scatter_f32(float const* src, uint32_t const* orig,
@@ -823,6 +989,8 @@ __global__ __launch_bounds__(256) void k_scatter_v2(V2 const* src, uint32_t cons
 __global__ __launch_bounds__(256) void k_export_lists(Lists l, uint32_t n, uint32_t stride,
                                                       uint32_t const* orig_self,
                                                       uint32_t const* orig_other,
+                                                      uint32_t const* other_scene_of,
+                                                      SceneDev const* scenes, int solid_other,
                                                       uint8_t* counts, uint32_t* idx,
                                                       float* dist, V2* vec) {
     uint32_t const t = blockIdx.x * 256 + threadIdx.x;
@@ -834,7 +1002,9 @@ __global__ __launch_bounds__(256) void k_export_lists(Lists l, uint32_t n, uint3
         size_t const src = static_cast<size_t>(k) * stride + t;
         size_t const dst = static_cast<size_t>(o) * MAXN + k;
         if (k < cnt) {
-            idx[dst] = orig_other[l.idx[src]];
+            uint32_t const og = orig_other[l.idx[src]];
+            SceneDev const& sc = scenes[other_scene_of[og]];
+            idx[dst] = og - (solid_other ? sc.soffset : sc.offset);
             if (dist) { dist[dst] = l.dist[src]; }
             if (vec) { vec[dst] = l.vec[src]; }
         } else {
@@ -846,56 +1016,56 @@ __global__ __launch_bounds__(256) void k_export_lists(Lists l, uint32_t n, uint3
 }
 
 //------------------------------------------------------------------------------
-static inline dim3 sweep_grid(uint32_t n) { return dim3((n + SWEEP_THREADS - 1) / SWEEP_THREADS); }
+static inline dim3 tile_grid(Batch const& b) { return dim3(b.ntiles ? b.ntiles : 1); }
 
-void launch_iisph_aii(SweepArgs const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_iisph_aii, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+void launch_iisph_aii(Batch const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_iisph_aii, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_iisph_sdp(SweepArgs const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_iisph_sdp, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+void launch_iisph_sdp(Batch const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_iisph_sdp, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_iisph_beta_p(SweepArgs const& a, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_iisph_beta_p, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a, slot);
+void launch_iisph_beta_p(Batch const& a, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a, slot);
 }
-void launch_apply(SweepArgs const& a, int with_gravity, int finalize, hipStream_t s) {
+void launch_apply(Batch const& a, int with_gravity, int finalize, int all_scenes,
+                  hipStream_t s) {
     if (with_gravity && finalize) {
-        hipLaunchKernelGGL((k_apply<1, 1>), sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_apply<1, 1>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
     } else if (with_gravity) {
-        hipLaunchKernelGGL((k_apply<1, 0>), sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_apply<1, 0>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
     } else if (finalize) {
-        hipLaunchKernelGGL((k_apply<0, 1>), sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_apply<0, 1>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
     } else {
-        hipLaunchKernelGGL((k_apply<0, 0>), sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_apply<0, 0>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
     }
 }
-void launch_finalize(SweepArgs const& a, int dfsph, hipStream_t s) {
+void launch_finalize(Batch const& a, int dfsph, hipStream_t s) {
     if (dfsph) {
-        hipLaunchKernelGGL(k_finalize<1>, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+        hipLaunchKernelGGL(k_finalize<1>, tile_grid(a), dim3(TILE), 0, s, a);
     } else {
-        hipLaunchKernelGGL(k_finalize<0>, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+        hipLaunchKernelGGL(k_finalize<0>, tile_grid(a), dim3(TILE), 0, s, a);
     }
 }
-void launch_dfsph_div(SweepArgs const& a, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_dfsph_div, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a, slot);
+void launch_dfsph_div(Batch const& a, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_dfsph_div, tile_grid(a), dim3(TILE), 0, s, a, slot);
 }
-void launch_dfsph_dens(SweepArgs const& a, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_dfsph_dens, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a, slot);
+void launch_dfsph_dens(Batch const& a, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_dfsph_dens, tile_grid(a), dim3(TILE), 0, s, a, slot);
 }
-void launch_pci_init(SweepArgs const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_pci_init, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+void launch_pci_init(Batch const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_init, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_pci_density(SweepArgs const& a, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_pci_density, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a, slot);
+void launch_pci_density(Batch const& a, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_density, tile_grid(a), dim3(TILE), 0, s, a, slot);
 }
-void launch_pci_force(SweepArgs const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_pci_force, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+void launch_pci_force(Batch const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_force, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_pci_finalize(SweepArgs const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_pci_finalize, sweep_grid(a.n), dim3(SWEEP_THREADS), 0, s, a);
+void launch_pci_finalize(Batch const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_finalize, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_vmax2(V2 const* vel, uint32_t n, unsigned int* out, hipStream_t s) {
-    if (!n) { return; }
-    hipLaunchKernelGGL(k_vmax2, dim3((n + 255) / 256), dim3(256), 0, s, vel, n, out);
+void launch_vmax2(Batch const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_vmax2, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_scatter_f32(float const* src, uint32_t const* orig, float* dst, uint32_t n,
                         hipStream_t s) {
@@ -908,11 +1078,13 @@ void launch_scatter_v2(V2 const* src, uint32_t const* orig, V2* dst, uint32_t n,
     hipLaunchKernelGGL(k_scatter_v2, dim3((n + 255) / 256), dim3(256), 0, s, src, orig, dst, n);
 }
 void launch_export_lists(Lists const& l, uint32_t n, uint32_t stride, uint32_t const* orig_self,
-                         uint32_t const* orig_other, uint8_t* counts, uint32_t* idx, float* dist,
-                         V2* vec, hipStream_t s) {
+                         uint32_t const* orig_other, uint32_t const* other_scene_of,
+                         SceneDev const* scenes, int solid_other, uint8_t* counts, uint32_t* idx,
+                         float* dist, V2* vec, hipStream_t s) {
     if (!n) { return; }
     hipLaunchKernelGGL(k_export_lists, dim3((n + 255) / 256), dim3(256), 0, s, l, n, stride,
-                       orig_self, orig_other, counts, idx, dist, vec);
+                       orig_self, orig_other, other_scene_of, scenes, solid_other, counts, idx,
+                       dist, vec);
 }
 
 }  // namespace emsph

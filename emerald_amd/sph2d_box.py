@@ -350,3 +350,71 @@ class EZ_EXAMPLE_SIM:  # noqa: N801 - the reference's type name
     def fetch_state(self) -> State:
         self.state = self.ctx.get_state()
         return self.state
+
+
+class Batch:
+    """Many independent scenes stepped together (emsph_batch): one launch per
+    sub-step stage for all scenes; each scene keeps its own parameters,
+    solids, particle count, dt and solver iteration count."""
+
+    def __init__(self, configs, solid_states, states):
+        m = len(configs)
+        assert m == len(solid_states) == len(states) and m > 0
+        cfgs = (_capi.Config * m)(*[c.to_c() for c in configs])
+        self._solid = [np.ascontiguousarray(s.positions, np.float32).reshape(-1, 2)
+                       for s in solid_states]
+        sptr = (ctypes.c_void_p * m)(*[_ptr(a) for a in self._solid])
+        scnt = (ctypes.c_size_t * m)(*[len(a) for a in self._solid])
+        fcnt = (ctypes.c_size_t * m)(*[len(s.positions) for s in states])
+        h = ctypes.c_void_p()
+        check(lib().emsph_batch_create(m, cfgs, sptr, scnt, fcnt, ctypes.byref(h)),
+              "emsph_batch_create")
+        self._h = h
+        self.m = m
+        self.counts = [len(s.positions) for s in states]
+        for i, s in enumerate(states):
+            self.set_state(i, s)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().emsph_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, i: int, state: State) -> None:
+        pos = np.ascontiguousarray(state.positions, np.float32).reshape(-1, 2)
+        vel = None if state.velocities is None else np.ascontiguousarray(
+            state.velocities, np.float32).reshape(-1, 2)
+        check(lib().emsph_batch_set_state(self._h, i, _ptr(pos), _ptr(vel)), "batch_set_state")
+
+    def get_state(self, i: int) -> State:
+        n = self.counts[i]
+        pos = np.empty((n, 2), np.float32)
+        vel = np.empty((n, 2), np.float32)
+        check(lib().emsph_batch_get_state(self._h, i, _ptr(pos), _ptr(vel)), "batch_get_state")
+        return State(pos, vel)
+
+    def field(self, i: int, name: str) -> np.ndarray:
+        fid, comps = _capi.FIELDS[name]
+        out = np.empty((self.counts[i], comps), np.float32)
+        check(lib().emsph_batch_get_field(self._h, i, fid, _ptr(out)), "batch_get_field")
+        return out[:, 0].copy() if comps == 1 else out
+
+    def frame_init(self, method: str) -> None:
+        check(lib().emsph_batch_frame_init(self._h, _capi.METHODS[method]), "batch_frame_init")
+
+    def sub_step(self, method: str):
+        st = (_capi.StepStats * self.m)()
+        check(lib().emsph_batch_sub_step(self._h, _capi.METHODS[method], st), "batch_sub_step")
+        return list(st)
+
+    def run_sub_steps(self, method: str, n: int) -> float:
+        ms = ctypes.c_float()
+        check(lib().emsph_batch_run_sub_steps(self._h, _capi.METHODS[method], int(n),
+                                              ctypes.byref(ms)), "batch_run_sub_steps")
+        return float(ms.value)

@@ -200,6 +200,30 @@ emsph_status emsph_set_kernel_timing(emsph_context* ctx, int enable);
 int emsph_get_kernel_timing(emsph_context* ctx, const char** names, double* total_ms,
                             int64_t* calls, int max);
 
+/* Batches: many independent scenes (own parameters, solids, particle count and
+ * dt) stepped together, one launch per sub-step stage for all of them. Every
+ * scene's result is bit-identical to the same scene run alone: scenes never
+ * see each other's particles, and each scene's solver loop ends at its own
+ * iteration (the exit test runs per scene on the device). The batch sub-step
+ * uses each scene's config.seconds_per_sub_step as its dt. No counterpart in
+ * the reference (it steps one scene at a time). */
+typedef struct emsph_batch emsph_batch;
+emsph_status emsph_batch_create(int n_scenes, const emsph_config* configs,
+                                const float* const* solid_xy, const size_t* solid_counts,
+                                const size_t* fluid_counts, emsph_batch** out);
+emsph_status emsph_batch_destroy(emsph_batch* b);
+int emsph_batch_scene_count(const emsph_batch* b);
+/* Scene state (exactly fluid_counts[scene] particles; vel may be NULL). */
+emsph_status emsph_batch_set_state(emsph_batch* b, int scene, const float* pos_xy,
+                                   const float* vel_xy);
+emsph_status emsph_batch_get_state(emsph_batch* b, int scene, float* pos_xy, float* vel_xy);
+emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* out);
+emsph_status emsph_batch_frame_init(emsph_batch* b, int method);
+/* One sub-step of every scene; stats (array of n_scenes, may be NULL). */
+emsph_status emsph_batch_sub_step(emsph_batch* b, int method, emsph_step_stats* stats);
+emsph_status emsph_batch_run_sub_steps(emsph_batch* b, int method, int n_sub_steps,
+                                       float* elapsed_ms);
+
 /* Stand-alone z-index sort and neighbour search on one point set (no solids,
  * no dynamics): the reference's compute_all_neighbhorhoods fluid part. */
 typedef struct emsph_search emsph_search;
