@@ -30,28 +30,44 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per particle for one launch of each kernel class
-# (FP32 SoA fields, each per-particle field read or written once per sweep,
-# neighbour gathers assumed cache-resident, neighbour lists not counted).
+# Algorithmic HBM bytes per fluid particle for one launch of each timed kernel
+# class, per solver. Conventions: FP32 SoA; every per-particle field read or
+# written once per sweep (neighbour gathers hit cache after the Morton sort);
+# the uniform fluid volume costs nothing; neighbour lists and solid-field reads
+# are not counted. Neighbour build NB = keys 16 + sort 16 + cells 8 = 40.
+# Per sub-step the tables sum to IISPH 184 + 68k, DFSPH_P 172 + 68k_v + 72k_rho,
+# PCISPH 116 + 108k.
+NB = {"keys": 16, "radix_sort": 16, "cells": 8, "step_begin": 0, "loop": 0}
 ALGO_BYTES = {
-    "keys": 8 + 8 + 4,            # x read, key write, index write
-    "radix_sort": 3 * 2 * 12,     # 3 passes x (read + write) of key + index
-    "cells": 12 + 16 + 16 + 8,    # sorted pair read, head/scan, gather x,v in, out
-    "build": 8 + 8 + 8 + 4 + 8,   # x, v in; v out; rho; d_ii (IISPH build)
-    "iisph_aii": 4 + 8 + 8 + 4 + 4 + 4,   # rho, d_ii, v in; a_ii, rho*, p out
-    "iisph_sdp": 4 + 4 + 8,       # rho, p in; sum d_ij p_j out
-    "iisph_beta_p": 4 + 4 + 8 + 8 + 4 + 4 + 4 + 4,  # rho,p,s,dii,aii,rho* in; p' out (+residual)
-    "apply_integrate": 4 + 4 + 8 + 8 + 4 + 8 + 8,   # rho,p,x,v in; orig; x,v out
-    "apply": 4 + 4 + 8 + 8,
-    "dfsph_div": 8 + 4 + 4 + 4,
-    "dfsph_dens": 8 + 4 + 4 + 4 + 4,
-    "finalize": 8 + 8 + 4 + 8 + 8,
-    "pci_init": 8 + 8 + 8 + 8 + 8 + 4,
-    "pci_density": 8 + 4 + 4,
-    "pci_force": 8 + 4 + 4 + 8 + 8 + 8 + 8,
-    "pci_finalize": 8 + 8 + 4 + 8 + 8,
-    "key_minmax": 8,
+    "iisph": {**NB,
+              "build": 48,            # rho 12 + (v += g) 16 + d_ii 20
+              "iisph_aii": 40,        # (a_ii, rho*) 36 + (p = 0) 4
+              "iisph_sdp": 24,        # sum_j d_ij p_j, per iteration
+              "iisph_beta_p": 44,     # beta, p', residual, per iteration
+              "apply_integrate": 56},  # apply 32 + x 24
+    "dfsph_p": {**NB,
+                "build": 28,          # rho 12 + alpha 16
+                "apply": 32,          # warm start / per-iteration kappa application
+                "apply_gravity": 48,  # warm start 32 + g 16
+                "dfsph_div": 36,      # divergence iteration 68 - apply 32
+                "dfsph_dens": 40,     # density iteration 72 - apply 32
+                "finalize": 24},      # x 24
+    "pcisph": {**NB,
+               "build": 0,            # index lists only (not counted)
+               "pci_init": 28,        # v_ext 16 + init 12
+               "pci_density": 48,     # x* 24 + rho 12 + p 12
+               "pci_force": 60,       # F 32 + v* 24 + max error 4
+               "pci_finalize": 48},
 }
+
+
+def step_bytes(method: str, k: float, k_div: float) -> float:
+    """Algorithmic bytes per particle-step at the measured mean iteration counts."""
+    if method == "iisph":
+        return 184 + 68 * k
+    if method == "dfsph_p":
+        return 172 + 68 * k_div + 72 * k
+    return 116 + 108 * k
 
 
 def log(msg: str) -> None:
@@ -137,21 +153,29 @@ def main() -> int:
     ctx.run_sub_steps(method, scene.dt, args.warmup)
 
     log(f"timing {args.steps} sub-steps")
+    ctx.iteration_totals(reset=True)
     t0 = time.perf_counter()
     dev_ms, st = ctx.run_sub_steps(method, scene.dt, args.steps)
     wall = time.perf_counter() - t0
     value = n * args.steps / wall
+    _, iters, div_iters = ctx.iteration_totals(reset=True)
+    k_mean = iters / args.steps
+    kdiv_mean = div_iters / args.steps
+    bytes_step = step_bytes(method, k_mean, kdiv_mean)
 
-    # Per-kernel timing pass (HIP events around each launch on the context stream).
+    # Per-kernel timing pass over the same number of sub-steps: HIP events
+    # recorded around each launch on the context's stream, resolved afterwards.
+    log(f"kernel timing pass, {args.steps} sub-steps")
     ctx.set_kernel_timing(True)
-    prof_steps = 10
+    prof_steps = args.steps
     ctx.run_sub_steps(method, scene.dt, prof_steps)
     timing = ctx.kernel_timing()
     ctx.set_kernel_timing(False)
+    table = ALGO_BYTES[method]
     dom = max(timing.items(), key=lambda kv: kv[1][0])
     dom_name, (dom_ms, dom_calls) = dom
     avg_s = dom_ms / dom_calls / 1e3
-    per = ALGO_BYTES.get(dom_name)
+    per = table.get(dom_name)
     achieved = (per * n / avg_s / 1e9) if per else None
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -169,9 +193,22 @@ def main() -> int:
         "traffic": traffic,
         "avg_launch_us": avg_s * 1e6,
         "algorithmic_bytes_per_particle": per,
+        "particles_per_launch": n,
     }
-    kernels = {k: {"ms_per_step": v[0] / prof_steps, "calls_per_step": v[1] / prof_steps}
-               for k, v in sorted(timing.items(), key=lambda kv: -kv[1][0])}
+    kernels = {}
+    for k, v in sorted(timing.items(), key=lambda kv: -kv[1][0]):
+        e = {"ms_per_step": v[0] / prof_steps, "calls_per_step": v[1] / prof_steps,
+             "avg_launch_us": v[0] / v[1] * 1e3}
+        if table.get(k):
+            e["algorithmic_GBps"] = table[k] * n / (v[0] / v[1] / 1e3) / 1e9
+        kernels[k] = e
+    whole_step = {
+        "algorithmic_bytes_per_particle_step": bytes_step,
+        "mean_iterations": k_mean,
+        "mean_divergence_iterations": kdiv_mean,
+        "achieved_GBps": bytes_step * value / 1e9,
+        "frac_of_hbm_peak": bytes_step * value / 1e9 / HBM_PEAK_GBS,
+    }
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -204,6 +241,7 @@ def main() -> int:
             "cells": int(st.cell_count),
         },
         "roofline": roofline,
+        "whole_step": whole_step,
         "cpu_baseline": cpu,
         "kernels": kernels,
     }

@@ -131,6 +131,7 @@ SIGNATURES = {
     "emsph_run_sub_steps": (_ST, [_P, _I, _F, _I, _P, _P]),
     "emsph_set_kernel_timing": (_ST, [_P, _I]),
     "emsph_get_kernel_timing": (_I, [_P, _P, _P, _P, _I]),
+    "emsph_get_iteration_totals": (_ST, [_P, _I, _P]),
     "emsph_batch_create": (_ST, [_I, _P, _P, _P, _P, _P]),
     "emsph_batch_destroy": (_ST, [_P]),
     "emsph_batch_scene_count": (_I, [_P]),

@@ -179,15 +179,19 @@ struct SpatialIndex {
     }
 };
 
+// Event pairs recorded around each timed launch, resolved only when the pool
+// is full or the totals are read, so timing adds no host synchronisation.
 struct KernelTimer {
+    static constexpr size_t POOL = 8192;
     bool enabled = false;
     std::map<std::string, std::pair<double, int64_t>> acc;
-    hipEvent_t a = nullptr, b = nullptr;
+    std::vector<hipEvent_t> ev;        // 2 per record
+    std::vector<char const*> names;    // string literals
+    size_t used = 0;
 };
 
 // Pinned read-back block.
 struct Pinned {
-    KeyState keys_reset;
     KeyState keys;
 };
 
@@ -237,6 +241,7 @@ struct Engine {
     std::vector<SceneStats> host_stats;
     DBuf<Reduction> red;
     DBuf<KeyState> keystate;
+    DBuf<unsigned long long> key_partials;
 
     DBuf<V2> pos, vel;
     DBuf<float> kv, kd;
@@ -261,11 +266,15 @@ struct Engine {
     bool keys_checked = false;
     bool sort_short = false;
     KernelTimer timer;
+    // Solver iterations launched by the last attempt of the current sub-step
+    // (the maximum over the batch's scenes), and running totals.
+    int last_iterations = 0;
+    int last_div_iterations = 0;
+    int64_t totals[3] = {0, 0, 0};  // sub-steps, pressure/density iterations, divergence iterations
 
     ~Engine() {
         if (host) { (void)hipHostFree(host); }
-        if (timer.a) { (void)hipEventDestroy(timer.a); }
-        if (timer.b) { (void)hipEventDestroy(timer.b); }
+        for (hipEvent_t e : timer.ev) { (void)hipEventDestroy(e); }
         if (stream) { (void)hipStreamDestroy(stream); }
     }
 };
@@ -287,22 +296,49 @@ struct emsph_search {
 
 namespace {
 
+hipError_t timer_flush(Engine* g) {
+    KernelTimer& tm = g->timer;
+    if (!tm.used) { return hipSuccess; }
+    hipError_t e = hipEventSynchronize(tm.ev[2 * tm.used - 1]);
+    if (e) { return e; }
+    for (size_t k = 0; k < tm.used; ++k) {
+        float ms = 0.0f;
+        if ((e = hipEventElapsedTime(&ms, tm.ev[2 * k], tm.ev[2 * k + 1]))) { return e; }
+        auto& slot = tm.acc[tm.names[k]];
+        slot.first += ms;
+        slot.second += 1;
+    }
+    tm.used = 0;
+    return hipSuccess;
+}
+
 template <class F>
 hipError_t timed(Engine* g, char const* name, F&& launch) {
     if (!g->timer.enabled) {
         launch();
         return hipGetLastError();
     }
-    (void)hipEventRecord(g->timer.a, g->stream);
+    KernelTimer& tm = g->timer;
+    if (tm.used == KernelTimer::POOL) {
+        hipError_t const e = timer_flush(g);
+        if (e) { return e; }
+    }
+    size_t const k = tm.used;
+    while (tm.ev.size() < 2 * (k + 1)) {
+        hipEvent_t e = nullptr;
+        hipError_t const r = hipEventCreate(&e);
+        if (r) { return r; }
+        tm.ev.push_back(e);
+    }
+    if (tm.names.size() <= k) { tm.names.resize(k + 1); }
+    tm.names[k] = name;
+    hipError_t e = hipEventRecord(tm.ev[2 * k], g->stream);
+    if (e) { return e; }
     launch();
-    (void)hipEventRecord(g->timer.b, g->stream);
-    hipError_t const e = hipEventSynchronize(g->timer.b);
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, g->timer.a, g->timer.b);
-    auto& slot = g->timer.acc[name];
-    slot.first += ms;
-    slot.second += 1;
-    return e;
+    if ((e = hipGetLastError())) { return e; }
+    if ((e = hipEventRecord(tm.ev[2 * k + 1], g->stream))) { return e; }
+    tm.used = k + 1;
+    return hipSuccess;
 }
 
 emsph_status engine_init(Engine* g) {
@@ -313,9 +349,8 @@ emsph_status engine_init(Engine* g) {
     HIP_TRY(hipGetDevice(&g->device));
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g->host), sizeof(Pinned)));
-    HIP_TRY(hipEventCreate(&g->timer.a));
-    HIP_TRY(hipEventCreate(&g->timer.b));
     HIP_TRY(g->keystate.alloc(1));
+    HIP_TRY(g->key_partials.alloc(2 * KEY_PARTS));
     return EMSPH_OK;
 }
 
@@ -464,11 +499,7 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     a.vals = si.vals0.p;
     a.grid = si.grid.p;
     a.keystate = g->keystate.p;
-    g->host->keys_reset = KeyState{~0ull, 0ull, 0u, 0u};
-    if ((e = hipMemcpyAsync(g->keystate.p, &g->host->keys_reset, sizeof(KeyState),
-                            hipMemcpyHostToDevice, s))) {
-        return e;
-    }
+    a.key_partials = g->key_partials.p;
     if ((e = run("keys", [&] { launch_keys(a, s); }))) { return e; }
     if (passes_hint < 0) {
         if ((e = hipMemcpyAsync(&g->host->keys, g->keystate.p, sizeof(KeyState),
@@ -634,6 +665,8 @@ emsph_status prepare(Engine* g, int build_mode, bool dfsph) {
     }
     g->keys_checked = g->passes_hint < 0;
     g->sort_short = false;
+    g->last_iterations = 0;
+    g->last_div_iterations = 0;
     Batch b = fluid_batch(g);
     HIP_TRY(timed(g, "step_begin", [&] { launch_step_begin(b, g->stream); }));
     HIP_TRY(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->passes_hint, ga,
@@ -658,6 +691,7 @@ emsph_status iisph_sub_step(Engine* g) {
                                            MAX_RED_SLOTS));
     }
     for (int it = 0; it < max_it; ++it) {
+        g->last_iterations = it + 1;
         b.p_in = pa;
         b.p_out = pb;
         HIP_TRY(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
@@ -695,6 +729,7 @@ emsph_status dfsph_sub_step(Engine* g) {
     HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 1, g->stream); }));
     HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
     for (int it = 0; it < max_it; ++it) {
+        g->last_div_iterations = it + 1;
         HIP_TRY(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, it, g->stream); }));
         HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
         HIP_TRY(timed(g, "loop", [&] {
@@ -710,9 +745,10 @@ emsph_status dfsph_sub_step(Engine* g) {
     int const base = MAX_RED_SLOTS / 2;
     b.p_in = g->kds.p;
     b.kappa = g->kds.p;
-    HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 1, 0, 1, g->stream); }));
+    HIP_TRY(timed(g, "apply_gravity", [&] { launch_apply(b, 1, 0, 1, g->stream); }));
     HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
     for (int it = 0; it < max_it; ++it) {
+        g->last_iterations = it + 1;
         HIP_TRY(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, base + it, g->stream); }));
         HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
         HIP_TRY(timed(g, "loop", [&] {
@@ -746,6 +782,7 @@ emsph_status pcisph_sub_step(Engine* g) {
     HIP_TRY(timed(g, "pci_init", [&] { launch_pci_init(b, g->stream); }));
     HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
     for (int it = 0; it < 6; ++it) {
+        g->last_iterations = it + 1;
         HIP_TRY(timed(g, "pci_density", [&] { launch_pci_density(b, it, g->stream); }));
         b.vstar_in = vin;
         b.vstar_out = vout;
@@ -793,6 +830,9 @@ emsph_status engine_sub_step(Engine* g, int method) {
         if (!g->sort_short) { break; }
     }
     g->last_method = method;
+    g->totals[0] += 1;
+    g->totals[1] += g->last_iterations;
+    g->totals[2] += g->last_div_iterations;
     return EMSPH_OK;
 }
 
@@ -1193,8 +1233,20 @@ emsph_status emsph_run_sub_steps(emsph_context* c, int method, float dt, int n_s
     return EMSPH_OK;
 }
 
+emsph_status emsph_get_iteration_totals(emsph_context* c, int reset, int64_t* out) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    if (out) {
+        for (int k = 0; k < 3; ++k) { out[k] = c->e.totals[k]; }
+    }
+    if (reset) {
+        for (int k = 0; k < 3; ++k) { c->e.totals[k] = 0; }
+    }
+    return EMSPH_OK;
+}
+
 emsph_status emsph_set_kernel_timing(emsph_context* c, int enable) {
     if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    HIP_TRY(timer_flush(&c->e));
     c->e.timer.enabled = enable != 0;
     c->e.timer.acc.clear();
     return EMSPH_OK;
@@ -1203,6 +1255,7 @@ emsph_status emsph_set_kernel_timing(emsph_context* c, int enable) {
 int emsph_get_kernel_timing(emsph_context* c, const char** names, double* total_ms,
                             int64_t* calls, int max) {
     if (!c) { return 0; }
+    if (timer_flush(&c->e)) { return -1; }
     int k = 0;
     for (auto const& kv : c->e.timer.acc) {
         if (k >= max) { break; }
@@ -1440,6 +1493,7 @@ emsph_status emsph_search_get(emsph_search* s, int32_t* grid_xy, uint64_t* keys,
         ka.vals = v2.p;
         ka.grid = g2.p;
         ka.keystate = g->keystate.p;
+        ka.key_partials = g->key_partials.p;
         launch_keys(ka, st);
         HIP_TRY(hipGetLastError());
         if (keys) { HIP_TRY(hipMemcpyAsync(keys, k2.p, n * 8, hipMemcpyDeviceToHost, st)); }

@@ -81,6 +81,9 @@ struct SceneStats {
 
 // Key range of one sort and the flag raised when the digits that were sorted
 // do not cover it (the work is then redone with the exact pass count).
+// Workgroups of the key kernel (grid-stride), i.e. partial key ranges.
+constexpr uint32_t KEY_PARTS = 2048;
+
 struct KeyState {
     unsigned long long kmin;
     unsigned long long kmax;
@@ -189,6 +192,7 @@ struct IndexArgs {
     uint32_t* vals;
     int2* grid;                // optional
     KeyState* keystate;
+    unsigned long long* key_partials;  // 2 * KEY_PARTS: per-workgroup key min, max
     // cells
     int key_passes;
     int check_keys;

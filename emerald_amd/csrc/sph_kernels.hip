@@ -92,19 +92,22 @@ __device__ __forceinline__ void block_reduce_max(unsigned int m, unsigned int* o
 //------------------------------------------------------------------------------
 // Keys: sph_common/common.cpp:43-76 (grid coordinate, z key) and
 // neighborhood.cpp:10-16 (index pairs), with the key range reduced on the fly.
+// Each workgroup stores its points' key range in its own partial slot (no
+// atomics: one address taken by every wave serialises at the device-coherent
+// level); k_key_range folds the partials.
 __global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long key_min = ~0ull;
     unsigned long long key_max = 0ull;
-    if (i < a.n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += gridDim.x * blockDim.x) {
         float const cell = a.scenes[a.scene_of[i]].c.cell;
         int2 const g = grid_coord(a.pos[i], cell);
         uint64_t const key = morton2(g.x, g.y);
         a.keys[i] = key;
         a.vals[i] = i;
         if (a.grid) { a.grid[i] = g; }
-        key_min = key;
-        key_max = key;
+        key_min = key < key_min ? key : key_min;
+        key_max = key > key_max ? key : key_max;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -113,15 +116,63 @@ __global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
         key_min = lo < key_min ? lo : key_min;
         key_max = hi > key_max ? hi : key_max;
     }
-    if (__lane_id() == 0 && key_min <= key_max) {
-        atomicMin(&a.keystate->kmin, key_min);
-        atomicMax(&a.keystate->kmax, key_max);
+    __shared__ unsigned long long smin[4], smax[4];
+    if (__lane_id() == 0) {
+        smin[threadIdx.x / 64] = key_min;
+        smax[threadIdx.x / 64] = key_max;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            key_min = smin[w] < key_min ? smin[w] : key_min;
+            key_max = smax[w] > key_max ? smax[w] : key_max;
+        }
+        a.key_partials[2 * blockIdx.x] = key_min;
+        a.key_partials[2 * blockIdx.x + 1] = key_max;
+    }
+}
+
+// One workgroup: the key range over the partials; clears the short-sort flag
+// of this indexing run.
+__global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) {
+    unsigned long long key_min = ~0ull;
+    unsigned long long key_max = 0ull;
+    for (uint32_t k = threadIdx.x; k < parts; k += blockDim.x) {
+        unsigned long long const lo = a.key_partials[2 * k];
+        unsigned long long const hi = a.key_partials[2 * k + 1];
+        key_min = lo < key_min ? lo : key_min;
+        key_max = hi > key_max ? hi : key_max;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long const lo = __shfl_xor(key_min, off);
+        unsigned long long const hi = __shfl_xor(key_max, off);
+        key_min = lo < key_min ? lo : key_min;
+        key_max = hi > key_max ? hi : key_max;
+    }
+    __shared__ unsigned long long smin[4], smax[4];
+    if (__lane_id() == 0) {
+        smin[threadIdx.x / 64] = key_min;
+        smax[threadIdx.x / 64] = key_max;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            key_min = smin[w] < key_min ? smin[w] : key_min;
+            key_max = smax[w] > key_max ? smax[w] : key_max;
+        }
+        a.keystate->kmin = key_min;
+        a.keystate->kmax = key_max;
+        a.keystate->sort_short = 0u;
     }
 }
 
 void launch_keys(IndexArgs const& a, hipStream_t s) {
-    uint32_t const blocks = (a.n + 255) / 256;
-    hipLaunchKernelGGL(k_keys, dim3(blocks ? blocks : 1), dim3(256), 0, s, a);
+    uint32_t blocks = (a.n + 255) / 256;
+    if (blocks > KEY_PARTS) { blocks = KEY_PARTS; }
+    if (!blocks) { blocks = 1; }
+    hipLaunchKernelGGL(k_keys, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_key_range, dim3(1), dim3(256), 0, s, a, blocks);
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {

@@ -284,6 +284,14 @@ class Context:
                                         ctypes.byref(ms), ctypes.byref(st)), "run_sub_steps")
         return float(ms.value), st
 
+    def iteration_totals(self, reset: bool = False) -> tuple:
+        """(sub-steps, pressure or density iterations, divergence iterations)
+        since creation or the last reset."""
+        out = (ctypes.c_int64 * 3)()
+        check(lib().emsph_get_iteration_totals(self._h, 1 if reset else 0, out),
+              "get_iteration_totals")
+        return int(out[0]), int(out[1]), int(out[2])
+
     def set_kernel_timing(self, enable: bool) -> None:
         check(lib().emsph_set_kernel_timing(self._h, 1 if enable else 0), "set_kernel_timing")
 
