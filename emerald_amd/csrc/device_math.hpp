@@ -145,38 +145,32 @@ __device__ __host__ __forceinline__ uint64_t spread_bits_2(uint32_t v) {
     x = (x | (x << 1)) & 0x5555555555555555ull;
     return x;
 }
+// Inverse of spread_bits_2: the even bits of x packed into 32 bits.
+__device__ __host__ __forceinline__ uint32_t compact_bits_2(uint64_t x) {
+    x &= 0x5555555555555555ull;
+    x = (x | (x >> 1)) & 0x3333333333333333ull;
+    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+    return static_cast<uint32_t>(x);
+}
+
 __device__ __host__ __forceinline__ uint64_t morton2(int32_t gx, int32_t gy) {
     return spread_bits_2(static_cast<uint32_t>(gx)) |
            (spread_bits_2(static_cast<uint32_t>(gy)) << 1);
 }
 
-// Open-addressed cell table: (cell key, scene tag) -> cell ordinal. A slot is
-// empty while its value is EMPTY_CELL; key and tag are written by the thread
-// that claims the slot (inserts and lookups run in different kernels).
-constexpr uint32_t EMPTY_CELL = 0xFFFFFFFFu;
-
-__device__ __forceinline__ uint32_t cell_hash(uint64_t key, uint32_t tag, uint32_t mask) {
-    key ^= static_cast<uint64_t>(tag) * 0x9E3779B97F4A7C15ull;
-    key ^= key >> 33;
-    key *= 0xff51afd7ed558ccdull;
-    key ^= key >> 33;
-    key *= 0xc4ceb9fe1a85ec53ull;
-    key ^= key >> 33;
-    return static_cast<uint32_t>(key) & mask;
-}
-
-__device__ __forceinline__ uint32_t cell_lookup(uint64_t key, uint32_t tag,
-                                                uint64_t const* __restrict__ keys,
-                                                uint32_t const* __restrict__ tags,
-                                                uint32_t const* __restrict__ vals,
-                                                uint32_t mask) {
-    uint32_t s = cell_hash(key, tag, mask);
-    for (;;) {
-        uint32_t const v = vals[s];
-        if (v == EMPTY_CELL) { return EMPTY_CELL; }
-        if (keys[s] == key && tags[s] == tag) { return v; }
-        s = (s + 1) & mask;
-    }
+// Hash of a (cell key, scene) pair for the static solid cell set (64-bit
+// finalizer; the same on host and device).
+__device__ __host__ __forceinline__ uint32_t cell_set_hash(uint64_t key, uint32_t tag) {
+    uint64_t h = key ^ (static_cast<uint64_t>(tag) * 0x9E3779B97F4A7C15ull);
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    return static_cast<uint32_t>(h);
 }
 
 // Workgroup -> tile remap so that each XCD (workgroups are dealt round-robin

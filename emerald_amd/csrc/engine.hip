@@ -5,8 +5,10 @@
 //   1. keys      grid coords + 64-bit Morton keys, in original order
 //   2. sort      stable LSD radix sort of (key, index) over the significant key
 //                digits, then over the scene id (scenes stay contiguous)
-//   3. cells     run heads, exclusive scan -> block indices, cell ranges,
-//                (key, scene) cell table; gather of the state into sorted order
+//   3. cells     run heads, exclusive scan -> block indices, cell ranges in
+//                sorted (scene, key) order; gather of the state into sorted
+//                order; per cell, the slot ranges of its 3x3 neighbour cells
+//                (fluid and solid), found by searching the sorted cell lists
 //   4. build     64-capped neighbour lists in reference order (fluid and solid),
 //                with the first per-particle sums of the method fused in
 //   5. solver    method sweeps over all scenes at once; after each iteration
@@ -14,7 +16,7 @@
 //                and drops out of later sweeps when it passes; the host only
 //                reads how many scenes are still iterating
 //   6. finalize  x += dt v, state scattered back to original order
-// Solids are static: their cell table, lists and volumes are built once at
+// Solids are static: their cells, lists and volumes are built once at
 // configuration (solids.cpp:76-137) with the same kernels.
 #include <hip/hip_runtime.h>
 
@@ -107,12 +109,6 @@ Thresholds thresholds(emsph_parameters const& p) {
     return t;
 }
 
-uint32_t next_pow2(uint64_t v) {
-    uint64_t p = 1;
-    while (p < v) { p <<= 1; }
-    return static_cast<uint32_t>(p);
-}
-
 uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 
 // A device allocation that frees itself.
@@ -139,17 +135,19 @@ struct DBuf {
     hipError_t ensure(size_t count) { return (p && n >= count) ? hipSuccess : alloc(count); }
 };
 
-// Keys, sort and cell table of one point set (fluid or solid).
+// Keys, sort and cells of one point set (fluid or solid).
 struct SpatialIndex {
     uint32_t cap = 0;
-    uint32_t table_cap = 0;  // power of two
     DBuf<uint64_t> keys0, keys1;
     DBuf<uint32_t> vals0, vals1;
     DBuf<uint32_t> counts, scan_scratch;
-    DBuf<uint32_t> heads, head_scan, block_index, cell_begin, cell_end;
-    DBuf<uint64_t> table_keys;
-    DBuf<uint32_t> table_tags, table_vals;
+    DBuf<uint32_t> heads, head_scan, block_index, cell_begin, cell_end, cell_tag;
+    DBuf<uint64_t> cell_key;
+    DBuf<uint32_t> ncells;  // device cell count of the last indexing run
+    DBuf<uint4> set;        // static cell set (solids), see CellSet
+    uint32_t set_mask = 0;
     DBuf<uint32_t> orig, rank;
+    DBuf<uint2> win;        // [cell][9] slot ranges of the 3x3 neighbour cells
     DBuf<int2> grid;
     uint64_t* sorted_keys = nullptr;
     uint32_t* sorted_vals = nullptr;
@@ -158,24 +156,23 @@ struct SpatialIndex {
 
     hipError_t alloc(uint32_t c, bool with_grid) {
         cap = c;
-        table_cap = next_pow2(std::max<uint64_t>(2ull * c, 1024));
         hipError_t e;
         if ((e = keys0.alloc(c)) || (e = keys1.alloc(c)) || (e = vals0.alloc(c)) ||
             (e = vals1.alloc(c)) || (e = counts.alloc(radix_counts_words(c) + 256)) ||
             (e = scan_scratch.alloc(scan_scratch_words(radix_counts_words(c) + c) + 64)) ||
             (e = heads.alloc(c)) || (e = head_scan.alloc(c)) || (e = block_index.alloc(c)) ||
             (e = cell_begin.alloc(c)) || (e = cell_end.alloc(c)) ||
-            (e = table_keys.alloc(table_cap)) || (e = table_tags.alloc(table_cap)) ||
-            (e = table_vals.alloc(table_cap)) || (e = orig.alloc(c)) || (e = rank.alloc(c))) {
+            (e = cell_key.alloc(c)) || (e = cell_tag.alloc(c)) ||
+            (e = ncells.alloc(1)) || (e = orig.alloc(c)) || (e = rank.alloc(c)) ||
+            (e = win.alloc(static_cast<size_t>(c) * 9))) {
             return e;
         }
         if (with_grid) { return grid.alloc(c); }
         return hipSuccess;
     }
 
-    CellTable table() const {
-        return CellTable{table_keys.p, table_tags.p, table_vals.p, cell_begin.p, cell_end.p,
-                         table_cap - 1};
+    CellList cells() const {
+        return CellList{cell_key.p, cell_tag.p, cell_begin.p, cell_end.p, ncells.p};
     }
 };
 
@@ -253,6 +250,9 @@ struct Engine {
     DBuf<uint32_t> fl_idx, sl_idx;
     DBuf<V2> fl_gw, sl_gw, fl_vec, sl_vec;
     DBuf<float> fl_dist, sl_dist;
+    DBuf<uint2> swin;       // [fluid cell][9] solid slot ranges
+    DBuf<uint2> overflow;   // particles deferred by the capped neighbour build
+    int build_cap = 32;     // LDS list capacity of the next build (from the last max count)
 
     SpatialIndex sidx;
     DBuf<uint32_t> s_scene_of;
@@ -368,10 +368,12 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     b.vel = g->svel.p;
     b.orig = g->fidx.orig.p;
     b.rank = g->fidx.rank.p;
-    b.table = g->fidx.table();
+    b.cell_of = g->fidx.block_index.p;
+    b.win = g->fidx.win.p;
+    b.swin = g->swin.p;
+    b.overflow = g->overflow.p;
     if (g->NS) {
-        b.solids = SolidView{g->s_pos.p, g->s_vel.p, g->s_vol.p, g->sidx.rank.p, g->sidx.orig.p,
-                             g->sidx.table()};
+        b.solids = SolidView{g->s_pos.p, g->s_vel.p, g->s_vol.p, g->sidx.rank.p, g->sidx.orig.p};
     }
     b.fl = Lists{g->fcount.p, g->fl_idx.p, g->fl_gw.p, g->fl_dist.p, g->fl_vec.p};
     b.sl = Lists{g->scount.p, g->sl_idx.p, g->sl_gw.p, g->sl_dist.p, g->sl_vec.p};
@@ -454,7 +456,8 @@ hipError_t alloc_fluid(Engine* g, uint32_t cap) {
         (e = g->p1.alloc(cap)) || (e = g->alpha.alloc(cap)) || (e = g->kvs.alloc(cap)) ||
         (e = g->kds.alloc(cap)) || (e = g->fcount.alloc(g->stride)) ||
         (e = g->scount.alloc(g->stride)) || (e = g->fl_idx.alloc(L)) ||
-        (e = g->sl_idx.alloc(L)) || (e = g->fl_gw.alloc(L)) || (e = g->sl_gw.alloc(L))) {
+        (e = g->sl_idx.alloc(L)) || (e = g->fl_gw.alloc(L)) || (e = g->sl_gw.alloc(L)) ||
+        (e = g->swin.alloc(static_cast<size_t>(cap) * 9)) || (e = g->overflow.alloc(cap))) {
         return e;
     }
     if ((e = hipMemsetAsync(g->kv.p, 0, cap * sizeof(float), g->stream))) { return e; }
@@ -480,9 +483,13 @@ hipError_t ensure_geometry(Engine* g) {
 // digits without waiting; k_cell_heads then flags (KeyState::sort_short) a key
 // range those digits do not cover, and the caller redoes the work with the
 // exact count before anything it produced is used.
+//
+// The cells stage ends with the cell windows of the index against itself and,
+// when other is given, against other's cells into owin.
 hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
                         uint32_t const* scene_of, uint32_t nscenes, int passes_hint,
-                        IndexArgs gathers, bool time_it) {
+                        IndexArgs gathers, bool time_it, SpatialIndex const* other = nullptr,
+                        uint2* owin = nullptr) {
     hipStream_t const s = g->stream;
     auto run = [&](char const* name, auto&& f) -> hipError_t {
         if (time_it) { return timed(g, name, f); }
@@ -531,18 +538,58 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     a.block_index = si.block_index.p;
     a.cell_begin = si.cell_begin.p;
     a.cell_end = si.cell_end.p;
-    a.table_keys = si.table_keys.p;
-    a.table_tags = si.table_tags.p;
-    a.table_vals = si.table_vals.p;
-    a.mask = si.table_cap - 1;
+    a.cell_key = si.cell_key.p;
+    a.cell_tag = si.cell_tag.p;
     a.orig = si.orig.p;
     a.rank = si.rank.p;
+    WindowArgs w{};
+    w.self = si.cells();
+    if (other && other->set.p) {
+        w.other = other->cells();
+        w.other_set = CellSet{other->set.p, other->set_mask};
+    }
+    w.max_cells = n;
+    w.win = si.win.p;
+    w.owin = owin;
+    w.report = &g->keystate.p->cells;
     return run("cells", [&] {
         launch_cell_heads(a, s);
-        launch_exclusive_scan(si.heads.p, si.head_scan.p, n, si.scan_scratch.p, nullptr, s);
-        launch_fill_u32(si.table_vals.p, EMPTY_CELL, si.table_cap, s);
+        launch_exclusive_scan(si.heads.p, si.head_scan.p, n, si.scan_scratch.p, si.ncells.p, s);
         launch_cells_finalize(a, s);
+        launch_cell_windows(w, s);
     });
+}
+
+// The static hash set of an index's cells (built on the host from the cell
+// list; load factor at most 1/4).
+emsph_status build_cell_set(Engine* g, SpatialIndex& si) {
+    uint32_t nc = 0;
+    HIP_TRY(hipMemcpyAsync(&nc, si.ncells.p, sizeof(nc), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    std::vector<uint64_t> keys(nc);
+    std::vector<uint32_t> tags(nc);
+    if (nc) {
+        HIP_TRY(hipMemcpyAsync(keys.data(), si.cell_key.p, nc * sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(tags.data(), si.cell_tag.p, nc * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    uint32_t cap = 64;
+    while (cap < 4ull * nc) { cap <<= 1; }
+    std::vector<uint4> slots(cap, make_uint4(0u, 0u, 0u, EMPTY_SLOT));
+    for (uint32_t c = 0; c < nc; ++c) {
+        uint32_t s = cell_set_hash(keys[c], tags[c]) & (cap - 1);
+        while (slots[s].w != EMPTY_SLOT) { s = (s + 1) & (cap - 1); }
+        slots[s] = make_uint4(static_cast<uint32_t>(keys[c]), static_cast<uint32_t>(keys[c] >> 32),
+                              tags[c], c);
+    }
+    HIP_TRY(si.set.alloc(cap));
+    HIP_TRY(hipMemcpyAsync(si.set.p, slots.data(), cap * sizeof(uint4), hipMemcpyHostToDevice,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    si.set_mask = cap - 1;
+    return EMSPH_OK;
 }
 
 // Solids of every scene: index, solid-solid lists with geometry, volumes
@@ -592,6 +639,7 @@ emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
     IndexArgs ga{};
     ga.pos_out = g->s_pos.p;
     HIP_TRY(index_points(g, g->sidx, g->s_pos_orig.p, NS, g->s_scene_of.p, g->M, -1, ga, false));
+    ST_TRY(build_cell_set(g, g->sidx));
     Batch b{};
     b.scenes = g->scenes.p;
     b.tiles = g->stiles.p;
@@ -601,12 +649,14 @@ emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
     b.pos = g->s_pos.p;
     b.orig = g->sidx.orig.p;
     b.rank = g->sidx.rank.p;
-    b.table = g->sidx.table();
+    b.cell_of = g->sidx.block_index.p;
+    b.win = g->sidx.win.p;
+    b.keystate = g->keystate.p;
     b.fl = Lists{scnt.p, sli.p, nullptr, sld.p, slv.p};
     b.sl = Lists{};  // solid-solid lists only
     b.rho = g->s_vol.p;
     b.store_geometry = 1;
-    launch_build(BUILD_LISTS, b, g->stream);
+    launch_build(BUILD_LISTS, MAXN, b, g->stream);
     launch_solid_volumes(b, NS, g->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(g->stream));
@@ -638,11 +688,21 @@ emsph_status engine_configure(Engine* g, std::vector<SceneHost> scenes,
 
 //------------------------------------------------------------------------------
 // KeyState read-backs.
+// The smallest LDS list capacity that held the last build's longest list with
+// a margin of two (a longer list in the next build is deferred, not lost).
+int pick_build_cap(unsigned int max_count) {
+    for (int c : BUILD_CAPS) {
+        if (max_count + 2u <= static_cast<unsigned int>(c)) { return c; }
+    }
+    return MAXN;
+}
+
 hipError_t read_keystate(Engine* g) {
     hipError_t e = hipMemcpyAsync(&g->host->keys, g->keystate.p, sizeof(KeyState),
                                   hipMemcpyDeviceToHost, g->stream);
     if (e) { return e; }
     if ((e = hipStreamSynchronize(g->stream))) { return e; }
+    g->build_cap = pick_build_cap(g->host->keys.max_count);
     if (!g->keys_checked) {
         g->keys_checked = true;
         g->sort_short = g->host->keys.sort_short != 0u;
@@ -670,9 +730,9 @@ emsph_status prepare(Engine* g, int build_mode, bool dfsph) {
     Batch b = fluid_batch(g);
     HIP_TRY(timed(g, "step_begin", [&] { launch_step_begin(b, g->stream); }));
     HIP_TRY(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->passes_hint, ga,
-                         g->timer.enabled));
+                         g->timer.enabled, g->NS ? &g->sidx : nullptr, g->swin.p));
     if (g->passes_hint < 0) { g->passes_hint = g->fidx.key_passes; }
-    HIP_TRY(timed(g, "build", [&] { launch_build(build_mode, b, g->stream); }));
+    HIP_TRY(timed(g, "build", [&] { launch_build(build_mode, g->build_cap, b, g->stream); }));
     g->have_lists = true;
     return EMSPH_OK;
 }
@@ -854,6 +914,8 @@ void to_public(SceneStats const& s, Engine const* g, emsph_step_stats* out) {
     out->divergence_error_max = s.divergence_error_max;
     std::memcpy(&out->max_velocity_sq, &s.vmax2_bits, 4);
     out->sort_passes = static_cast<uint32_t>(g->fidx.key_passes + g->fidx.scene_passes);
+    out->cell_count = g->host->keys.cells;
+    out->max_neighbors = g->host->keys.max_count;
 }
 
 emsph_status engine_frame_init(Engine* g, int method) {
@@ -1442,7 +1504,7 @@ emsph_status emsph_search_run(emsph_search* s, float cell_size, int store_geomet
         HIP_TRY(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, 1, -1, ga, false));
         Batch b = fluid_batch(g);
         b.fl.gw = nullptr;
-        launch_build(BUILD_LISTS, b, g->stream);
+        launch_build(BUILD_LISTS, g->build_cap, b, g->stream);
         HIP_TRY(hipGetLastError());
         g->have_lists = true;
     }
@@ -1453,6 +1515,7 @@ emsph_status emsph_search_run(emsph_search* s, float cell_size, int store_geomet
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (elapsed_ms) { *elapsed_ms = ms; }
+    if (g->N) { HIP_TRY(read_keystate(g)); }  // list capacity of the next run
     s->ran = true;
     return EMSPH_OK;
 }

@@ -79,25 +79,44 @@ struct SceneStats {
     unsigned int active;   // 1 while the current solver loop still iterates
 };
 
-// Key range of one sort and the flag raised when the digits that were sorted
-// do not cover it (the work is then redone with the exact pass count).
 // Workgroups of the key kernel (grid-stride), i.e. partial key ranges.
 constexpr uint32_t KEY_PARTS = 2048;
 
+// Key range of one sort and the flag raised when the digits that were sorted
+// do not cover it (the work is then redone with the exact pass count), plus
+// the solver-loop and neighbour-build counters the host reads back.
 struct KeyState {
     unsigned long long kmin;
     unsigned long long kmax;
     unsigned int sort_short;
     unsigned int active_count;  // scenes still iterating (written by k_loop_*)
+    unsigned int overflow_count;  // particles the last build deferred to its full-cap pass
+    unsigned int max_count;       // largest fluid or solid list of the last build
+    unsigned int cells;           // cells of the last indexing run
+    unsigned int pad;
 };
 
-// Cell table of one particle array: (cell key, scene) -> cell ordinal -> [begin, end).
-struct CellTable {
-    uint64_t const* keys;
-    uint32_t const* tags;
-    uint32_t const* vals;
+// Neighbour-list capacities of the build kernel's LDS list (the full one is
+// the reference's NEIGHBORHOOD_MAX_COUNT).
+constexpr int BUILD_CAPS[3] = {16, 32, 64};
+
+// Cells of one point set in sorted (scene, key) order: cell c is the slot
+// range [begin[c], end[c]) of key key[c] in scene tag[c]; *count cells.
+struct CellList {
+    uint64_t const* key;
+    uint32_t const* tag;
     uint32_t const* begin;
     uint32_t const* end;
+    uint32_t const* count;
+};
+
+// Static set of cells (the solid cells): open addressing over 16-byte slots
+// {key low, key high, scene, cell ordinal}, ordinal EMPTY_SLOT when free.
+// Built once on the host with a load factor of at most 1/4, so a lookup of a
+// cell that is not in the set mostly ends at its first slot.
+constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
+struct CellSet {
+    uint4 const* slots;
     uint32_t mask;
 };
 
@@ -108,7 +127,6 @@ struct SolidView {
     float const* vol;
     uint32_t const* rank;   // original solid index -> sorted slot
     uint32_t const* orig;   // sorted slot -> original solid index
-    CellTable table;
 };
 
 // Neighbour lists in slot-major layout: entry k of slot t at [k * stride + t].
@@ -135,8 +153,14 @@ struct Batch {
     V2* vel;
     uint32_t const* orig;   // sorted slot -> original index
     uint32_t const* rank;   // original index -> sorted slot
-    CellTable table;
     SolidView solids;
+    // Cell windows: for the cell of each fluid slot, the [begin, end) slot
+    // ranges of its 3x3 neighbour cells (scan order j-outer, i-inner) in the
+    // fluid (win) and solid (swin) index; cell_of maps slot -> cell.
+    uint32_t const* cell_of;
+    uint2 const* win;
+    uint2 const* swin;
+    uint2* overflow;        // {slot, scene} of particles deferred to the full-cap pass
     Lists fl;
     Lists sl;
     float* rho;
@@ -203,10 +227,8 @@ struct IndexArgs {
     uint32_t* block_index;
     uint32_t* cell_begin;
     uint32_t* cell_end;
-    uint64_t* table_keys;
-    uint32_t* table_tags;
-    uint32_t* table_vals;
-    uint32_t mask;
+    uint64_t* cell_key;
+    uint32_t* cell_tag;
     uint32_t* orig;
     uint32_t* rank;
     // gathers into sorted order (optional)
@@ -218,6 +240,20 @@ struct IndexArgs {
     float* kv_out;
     float* kd_out;
 };
+
+// Cell windows of one index against itself (win) and optionally another
+// index (owin), e.g. fluid cells against the solid cells.
+struct WindowArgs {
+    CellList self;           // windows of these cells against themselves -> win
+    CellList other;          // ranges of the other cells
+    CellSet other_set;       // lookup of the other cells; slots == nullptr: none
+    uint32_t max_cells;      // launch bound
+    uint2* win;
+    uint2* owin;
+    unsigned int* report;    // optional: receives the cell count
+};
+void launch_cell_windows(WindowArgs const& w, hipStream_t s);
+
 void launch_keys(IndexArgs const& a, hipStream_t s);
 void launch_cell_heads(IndexArgs const& a, hipStream_t s);
 void launch_cells_finalize(IndexArgs const& a, hipStream_t s);
@@ -231,7 +267,9 @@ void launch_loop_begin(Batch const& b, hipStream_t s);
 // reference's exit test to its residual slot and drops out when it passes.
 void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t s);
 
-void launch_build(int mode, Batch const& b, hipStream_t s);
+// cap: LDS list capacity (one of BUILD_CAPS). Below 64 a particle whose list
+// would exceed it is deferred and rebuilt by a full-capacity pass.
+void launch_build(int mode, int cap, Batch const& b, hipStream_t s);
 void launch_iisph_aii(Batch const& b, hipStream_t s);
 void launch_iisph_sdp(Batch const& b, hipStream_t s);
 void launch_iisph_beta_p(Batch const& b, int slot, hipStream_t s);

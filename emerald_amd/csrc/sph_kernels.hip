@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
 }
 
 // One workgroup: the key range over the partials; clears the short-sort flag
-// of this indexing run.
+// and the neighbour-build counters of this indexing run.
 __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) {
     unsigned long long key_min = ~0ull;
     unsigned long long key_max = 0ull;
@@ -164,6 +164,8 @@ __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) 
         a.keystate->kmin = key_min;
         a.keystate->kmax = key_max;
         a.keystate->sort_short = 0u;
+        a.keystate->overflow_count = 0u;
+        a.keystate->max_count = 0u;
     }
 }
 
@@ -221,16 +223,8 @@ __global__ __launch_bounds__(256) void k_cells_finalize(IndexArgs a) {
     a.block_index[t] = b;
     if (head) {
         a.cell_begin[b] = t;
-        uint32_t s = cell_hash(key, tag, a.mask);
-        for (;;) {
-            uint32_t const prev = atomicCAS(&a.table_vals[s], EMPTY_CELL, b);
-            if (prev == EMPTY_CELL) {
-                a.table_keys[s] = key;
-                a.table_tags[s] = tag;
-                break;
-            }
-            s = (s + 1) & a.mask;
-        }
+        a.cell_key[b] = key;
+        a.cell_tag[b] = tag;
     }
     bool const tail = (t + 1 == a.n) || a.sorted_keys[t + 1] != key ||
                       a.scene_of[a.sorted_vals[t + 1]] != tag;
@@ -361,75 +355,203 @@ void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t 
 }
 
 //------------------------------------------------------------------------------
+// Cell windows. One thread per (cell, neighbour cell): the cell's key gives
+// back its grid coordinate (the Morton code inverts exactly), and each of the
+// 3x3 neighbour cells is looked up once per cell instead of once per particle.
+// Cells are sorted by (scene, key), so a lookup among the same cells is a
+// search of the cell list, galloping from the cell itself (Morton neighbours
+// are mostly a few cells away, so the probes stay on nearby cache lines). The
+// other (solid) cells are looked up in their static hash set.
+__device__ __forceinline__ bool cell_before(CellList const& cl, uint32_t i, uint32_t tag,
+                                            uint64_t key) {
+    uint32_t const ti = cl.tag[i];
+    return ti < tag || (ti == tag && cl.key[i] < key);
+}
+
+// First cell i in [lo, hi) that is not before (tag, key); hi when none.
+__device__ __forceinline__ uint32_t cell_bisect(CellList const& cl, uint32_t lo, uint32_t hi,
+                                                uint32_t tag, uint64_t key) {
+    while (lo < hi) {
+        uint32_t const mid = lo + (hi - lo) / 2;
+        if (cell_before(cl, mid, tag, key)) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+// Same over [0, n), probing at hint +- 1, 2, 4, ... first: O(log |i - hint|).
+__device__ __forceinline__ uint32_t cell_gallop(CellList const& cl, uint32_t n, uint32_t hint,
+                                                uint32_t tag, uint64_t key) {
+    uint32_t lo, hi;
+    if (cell_before(cl, hint, tag, key)) {
+        lo = hint + 1;
+        hi = n;
+        for (uint32_t step = 1;; step <<= 1) {
+            uint32_t const probe = hint + step;
+            if (probe >= n) { break; }
+            if (!cell_before(cl, probe, tag, key)) {
+                hi = probe;
+                break;
+            }
+            lo = probe + 1;
+        }
+    } else {
+        lo = 0;
+        hi = hint;
+        for (uint32_t step = 1; step <= hint; step <<= 1) {
+            uint32_t const probe = hint - step;
+            if (cell_before(cl, probe, tag, key)) {
+                lo = probe + 1;
+                break;
+            }
+            hi = probe;
+        }
+    }
+    return cell_bisect(cl, lo, hi, tag, key);
+}
+
+__device__ __forceinline__ uint2 cell_range(CellList const& cl, uint32_t i, uint32_t n,
+                                            uint32_t tag, uint64_t key) {
+    if (i < n && cl.tag[i] == tag && cl.key[i] == key) {
+        return make_uint2(cl.begin[i], cl.end[i]);
+    }
+    return make_uint2(0u, 0u);
+}
+
+__device__ __forceinline__ uint2 cell_set_range(CellSet const& cs, CellList const& cl,
+                                                uint64_t key, uint32_t tag) {
+    uint32_t const lo = static_cast<uint32_t>(key);
+    uint32_t const hi = static_cast<uint32_t>(key >> 32);
+    uint32_t s = cell_set_hash(key, tag) & cs.mask;
+    for (;;) {
+        uint4 const e = cs.slots[s];
+        if (e.w == EMPTY_SLOT) { return make_uint2(0u, 0u); }
+        if (e.x == lo && e.y == hi && e.z == tag) { return make_uint2(cl.begin[e.w], cl.end[e.w]); }
+        s = (s + 1) & cs.mask;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_windows(WindowArgs w) {
+    uint32_t const n = *w.self.count;
+    size_t const total = static_cast<size_t>(n) * 9;
+    size_t const first = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+    if (first == 0 && w.report) { *w.report = n; }
+    for (size_t item = first; item < total; item += size_t(gridDim.x) * blockDim.x) {
+        uint32_t const c = static_cast<uint32_t>(item / 9);
+        uint32_t const k = static_cast<uint32_t>(item - static_cast<size_t>(c) * 9);
+        uint64_t const key = w.self.key[c];
+        uint32_t const tag = w.self.tag[c];
+        // neighborhood.h:233-243: j outer, i inner, each from g - 1 to g + 1
+        // (int32 arithmetic; the key takes the coordinates as uint32).
+        uint32_t const gx = compact_bits_2(key) + (k % 3u) - 1u;
+        uint32_t const gy = compact_bits_2(key >> 1) + (k / 3u) - 1u;
+        uint64_t const nk = morton2(static_cast<int32_t>(gx), static_cast<int32_t>(gy));
+        uint32_t const i = cell_gallop(w.self, n, c, tag, nk);
+        w.win[item] = cell_range(w.self, i, n, tag, nk);
+        if (w.owin) {
+            w.owin[item] = w.other_set.slots ? cell_set_range(w.other_set, w.other, nk, tag)
+                                             : make_uint2(0u, 0u);
+        }
+    }
+}
+
+void launch_cell_windows(WindowArgs const& w, hipStream_t s) {
+    if (!w.max_cells) { return; }
+    size_t blocks = (static_cast<size_t>(w.max_cells) * 9 + 255) / 256;
+    if (blocks > 4096) { blocks = 4096; }
+    hipLaunchKernelGGL(k_cell_windows, dim3(blocks), dim3(256), 0, s, w);
+}
+
+//------------------------------------------------------------------------------
 // Neighbour build: sph_common/neighborhood.h:198-282.
 //
 // Candidates are scanned in the reference's order (cells j-outer, i-inner; in a
 // cell, sorted slots, i.e. increasing original index) and accepted while
 // distance < cut-off, up to 64 in scan order. Each thread keeps its accepted
-// list sorted by (distance bits, original index) in LDS by insertion, which is
-// the reference's neighbor_order (neighborhood.h:172-186): distances are
-// non-negative, so their bit patterns order like the floats.
-__device__ __forceinline__ int collect_candidates(uint64_t (*ent)[BUILD_THREADS], uint32_t lane,
-                                                  V2 p, int2 g, uint32_t self, bool same,
-                                                  uint32_t tag, CellTable const& tb,
-                                                  V2 const* __restrict__ opos,
-                                                  uint32_t const* __restrict__ oorig, float maxd) {
+// list in LDS as (distance bits << 32 | slot), sorted by insertion in the
+// reference's neighbor_order (neighborhood.h:172-186): by distance, whose
+// non-negative bit patterns order like the floats, then by original index.
+
+// True when entry a sorts after entry b.
+__device__ __forceinline__ bool entry_after(uint64_t a, uint64_t b,
+                                            uint32_t const* __restrict__ oorig) {
+    uint32_t const da = static_cast<uint32_t>(a >> 32);
+    uint32_t const db = static_cast<uint32_t>(b >> 32);
+    if (da != db) { return da > db; }
+    return oorig[static_cast<uint32_t>(a)] > oorig[static_cast<uint32_t>(b)];
+}
+
+// Accepted candidates of one particle over its 3x3 cell window. Returns the
+// count, or -1 when CAP < 64 and more than CAP candidates are accepted.
+template <int CAP>
+__device__ __forceinline__ int collect_window(uint64_t (*ent)[BUILD_THREADS], uint32_t lane,
+                                              V2 p, uint32_t self, bool same,
+                                              uint2 const* __restrict__ win,
+                                              V2 const* __restrict__ opos,
+                                              uint32_t const* __restrict__ oorig, float maxd) {
     int cnt = 0;
-    for (int32_t j = g.y - 1; cnt < MAXN && j <= g.y + 1; ++j) {
-        for (int32_t i = g.x - 1; cnt < MAXN && i <= g.x + 1; ++i) {
-            uint32_t const ord =
-              cell_lookup(morton2(i, j), tag, tb.keys, tb.tags, tb.vals, tb.mask);
-            if (ord == EMPTY_CELL) { continue; }
-            uint32_t const e = tb.end[ord];
-            for (uint32_t s = tb.begin[ord]; cnt < MAXN && s != e; ++s) {
-                if (same && s == self) { continue; }
-                V2 const v = sub(opos[s], p);
-                float const d = imath_length(v);
-                if (d >= maxd) { continue; }
-                uint64_t const key =
-                  (static_cast<uint64_t>(__float_as_uint(d)) << 32) | oorig[s];
-                int k = cnt;
-                while (k > 0) {
-                    uint64_t const prev = ent[k - 1][lane];
-                    if (prev <= key) { break; }
-                    ent[k][lane] = prev;
-                    --k;
+    uint2 cur = win[0];
+    for (int k = 0; k < 9; ++k) {
+        uint2 const nxt = win[k < 8 ? k + 1 : 8];  // next range in flight
+        uint32_t const e = cur.y;
+        if (cur.x < e) {
+            V2 q = opos[cur.x];
+            for (uint32_t s = cur.x; s < e; ++s) {
+                V2 const qn = opos[s + 1 < e ? s + 1 : s];  // next candidate in flight
+                if (!(same && s == self)) {
+                    V2 const v = sub(q, p);
+                    float const d = imath_length(v);
+                    if (d < maxd) {
+                        if (CAP < MAXN && cnt == CAP) { return -1; }
+                        uint64_t const key = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | s;
+                        int kk = cnt;
+                        while (kk > 0) {
+                            uint64_t const prev = ent[kk - 1][lane];
+                            if (!entry_after(prev, key, oorig)) { break; }
+                            ent[kk][lane] = prev;
+                            --kk;
+                        }
+                        ent[kk][lane] = key;
+                        ++cnt;
+                        if (CAP == MAXN && cnt == MAXN) { return cnt; }
+                    }
                 }
-                ent[k][lane] = key;
-                ++cnt;
+                q = qn;
             }
         }
+        cur = nxt;
     }
     return cnt;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
-    __shared__ uint64_t ent[MAXN][BUILD_THREADS];
-    uint32_t const lane = threadIdx.x;
-    // Build tiles are the fluid tiles split in quarters (64 threads).
-    uint32_t const k = xcd_tile(blockIdx.x, gridDim.x);
-    Tile const tl = a.tiles[k >> 2];
-    uint32_t const sub_off = (k & 3u) * BUILD_THREADS + lane;
-    if (sub_off >= tl.count) { return; }
-    uint32_t const t = tl.start + sub_off;
-    uint32_t const scene = tl.scene;
+// Lists and the fused per-particle sums of slot t. Returns false (nothing of
+// the particle's state changed) when a list would exceed CAP.
+template <int MODE, int CAP>
+__device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_THREADS],
+                                          uint32_t lane, uint32_t t, uint32_t scene,
+                                          int& max_count) {
     StepConsts const c = a.scenes[scene].c;
     V2 const p = a.pos[t];
-    int2 const g = grid_coord(p, c.cell);
+    uint32_t const cell = a.cell_of[t];
     float const mf = c.rho0 * c.fluid_volume;  // rho0 * V_j (uniform fluid volume)
 
     // Fluid neighbours.
-    int const fc =
-      collect_candidates(ent, lane, p, g, t, true, scene, a.table, a.pos, a.orig, c.cell);
-    a.fl.count[t] = static_cast<uint8_t>(fc);
+    int const fc = collect_window<CAP>(ent, lane, p, t, true, a.win + static_cast<size_t>(cell) * 9,
+                                       a.pos, a.orig, c.cell);
+    if (fc < 0) {
+        max_count = CAP + 1;
+        return false;
+    }
     float dens = c.fluid_volume * c.rho0 * c.kc.w0;  // density.cpp:57-59
     V2 msum_f{0.0f, 0.0f};
     V2 sum_mg{0.0f, 0.0f};    // DFSPH: sum of m grad W over fluid and solid
     float sum_mg2 = 0.0f;     // DFSPH: sum of |m grad W|^2 over fluid
     for (int k2 = 0; k2 < fc; ++k2) {
         uint64_t const e = ent[k2][lane];
-        uint32_t const s = a.rank[static_cast<uint32_t>(e)];
+        uint32_t const s = static_cast<uint32_t>(e);
         float const d = __uint_as_float(static_cast<uint32_t>(e >> 32));
         V2 const vec = sub(a.pos[s], p);
         size_t const o = static_cast<size_t>(k2) * a.stride + t;
@@ -452,21 +574,31 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
         }
     }
 
-    // Solid neighbours (no self exclusion: different arrays).
-    // A batch without solid lists (the solid-solid build) skips this part.
-    if (!a.sl.count) { return; }
+    // Solid neighbours (no self exclusion: different arrays). A batch without
+    // solid lists (the solid-solid build) stops here.
+    if (!a.sl.count) {
+        a.fl.count[t] = static_cast<uint8_t>(fc);
+        max_count = fc;
+        return true;
+    }
     SolidView const& sv = a.solids;
     int sc = 0;
-    if (sv.table.keys) {
-        sc = collect_candidates(ent, lane, p, g, t, false, scene, sv.table, sv.pos, sv.orig,
-                                c.cell);
+    if (a.swin && sv.pos) {
+        sc = collect_window<CAP>(ent, lane, p, 0xFFFFFFFFu, false,
+                                 a.swin + static_cast<size_t>(cell) * 9, sv.pos, sv.orig, c.cell);
+        if (sc < 0) {
+            max_count = CAP + 1;
+            return false;
+        }
     }
+    a.fl.count[t] = static_cast<uint8_t>(fc);
     a.sl.count[t] = static_cast<uint8_t>(sc);
+    max_count = fc > sc ? fc : sc;
     float dens_s = 0.0f;
     V2 msum_s{0.0f, 0.0f};
     for (int k2 = 0; k2 < sc; ++k2) {
         uint64_t const e = ent[k2][lane];
-        uint32_t const b = sv.rank[static_cast<uint32_t>(e)];
+        uint32_t const b = static_cast<uint32_t>(e);
         float const d = __uint_as_float(static_cast<uint32_t>(e >> 32));
         V2 const vec = sub(sv.pos[b], p);
         size_t const o = static_cast<size_t>(k2) * a.stride + t;
@@ -484,7 +616,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
             if (MODE == BUILD_DFSPH) { sum_mg = add(sum_mg, scale(ms, gw)); }
         }
     }
-    if (MODE == BUILD_LISTS) { return; }
+    if (MODE == BUILD_LISTS) { return true; }
 
     float rho = dens;
     if (sc) { rho += dens_s; }  // density.cpp:76-80
@@ -521,21 +653,95 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
         }
         a.alpha[t] = alpha;
     }
+    return true;
 }
 
-void launch_build(int mode, Batch const& a, hipStream_t s) {
+// Wave maximum of the list counts into KeyState::max_count (read first: the
+// value rarely grows, so few waves take the atomic).
+__device__ __forceinline__ void note_max_count(Batch const& a, int m) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int const o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if (__lane_id() == 0) {
+        unsigned int const mu = static_cast<unsigned int>(m);
+        if (mu > __atomic_load_n(&a.keystate->max_count, __ATOMIC_RELAXED)) {
+            atomicMax(&a.keystate->max_count, mu);
+        }
+    }
+}
+
+template <int MODE, int CAP>
+__global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
+    __shared__ uint64_t ent[CAP][BUILD_THREADS];
+    uint32_t const lane = threadIdx.x;
+    // Build tiles are the fluid tiles split in quarters (64 threads).
+    uint32_t const k = xcd_tile(blockIdx.x, gridDim.x);
+    Tile const tl = a.tiles[k >> 2];
+    uint32_t const sub_off = (k & 3u) * BUILD_THREADS + lane;
+    if ((k & 3u) * BUILD_THREADS >= tl.count) { return; }  // the whole wave
+    // Every lane stays for the cross-lane reductions below.
+    bool const valid = sub_off < tl.count;
+    uint32_t const t = tl.start + sub_off;
+    int m = 0;
+    bool const ok = valid ? build_one<MODE, CAP>(a, ent, lane, t, tl.scene, m) : true;
+    note_max_count(a, m);
+    if (CAP < MAXN) {
+        // Deferred particles, appended with one atomic per wave.
+        unsigned long long const mask = __ballot(!ok);
+        if (mask) {
+            int const leader = __ffsll(mask) - 1;
+            unsigned int base = 0;
+            if (static_cast<int>(lane) == leader) {
+                base = atomicAdd(&a.keystate->overflow_count,
+                                 static_cast<unsigned int>(__popcll(mask)));
+            }
+            base = __shfl(base, leader);
+            if (!ok) {
+                unsigned int const r =
+                  static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
+                a.overflow[base + r] = make_uint2(t, tl.scene);
+            }
+        }
+    }
+}
+
+// Full-capacity pass over the particles the capped build deferred.
+template <int MODE>
+__global__ __launch_bounds__(BUILD_THREADS) void k_build_deferred(Batch a) {
+    __shared__ uint64_t ent[MAXN][BUILD_THREADS];
+    uint32_t const n = a.keystate->overflow_count;
+    for (uint32_t i = blockIdx.x * BUILD_THREADS + threadIdx.x; i < n;
+         i += gridDim.x * BUILD_THREADS) {
+        uint2 const e = a.overflow[i];
+        int m = 0;
+        (void)build_one<MODE, MAXN>(a, ent, threadIdx.x, e.x, e.y, m);
+        if (m > 0) { atomicMax(&a.keystate->max_count, static_cast<unsigned int>(m)); }
+    }
+}
+
+template <int MODE>
+void launch_build_mode(int cap, Batch const& a, hipStream_t s) {
     uint32_t const blocks = a.ntiles * 4;
-    if (!blocks) { return; }
+    if (cap <= 16) {
+        hipLaunchKernelGGL((k_build<MODE, 16>), dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+    } else if (cap <= 32) {
+        hipLaunchKernelGGL((k_build<MODE, 32>), dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_build<MODE, MAXN>), dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+        return;
+    }
+    uint32_t const dblocks = blocks < 1024 ? blocks : 1024;
+    hipLaunchKernelGGL(k_build_deferred<MODE>, dim3(dblocks), dim3(BUILD_THREADS), 0, s, a);
+}
+
+void launch_build(int mode, int cap, Batch const& a, hipStream_t s) {
+    if (!a.ntiles) { return; }
     switch (mode) {
-    case BUILD_IISPH:
-        hipLaunchKernelGGL(k_build<BUILD_IISPH>, dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
-        break;
-    case BUILD_DFSPH:
-        hipLaunchKernelGGL(k_build<BUILD_DFSPH>, dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
-        break;
-    default:
-        hipLaunchKernelGGL(k_build<BUILD_LISTS>, dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
-        break;
+    case BUILD_IISPH: launch_build_mode<BUILD_IISPH>(cap, a, s); break;
+    case BUILD_DFSPH: launch_build_mode<BUILD_DFSPH>(cap, a, s); break;
+    default: launch_build_mode<BUILD_LISTS>(cap, a, s); break;
     }
 }
 

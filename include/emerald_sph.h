@@ -97,9 +97,9 @@ typedef struct emsph_step_stats {
     float divergence_error_average;
     float divergence_error_max;
     float max_velocity_sq;         /* max |v|^2 after the sub-step */
-    uint32_t cell_count;           /* occupied grid cells */
+    uint32_t cell_count;           /* occupied grid cells (all scenes of a batch) */
     uint32_t sort_passes;          /* 8-bit radix passes of the key sort */
-    uint32_t max_neighbors;        /* reserved (0) */
+    uint32_t max_neighbors;        /* longest fluid or solid neighbour list */
 } emsph_step_stats;
 
 typedef struct emsph_frame_stats {

@@ -16,11 +16,13 @@ from helpers import assert_bits_equal
 pytestmark = pytest.mark.gpu
 
 
-def _compare(xy, cell, geometry=True):
+def _compare(xy, cell, geometry=True, s=None):
     xy = np.ascontiguousarray(xy, np.float32)
     n = len(xy)
     print(f"[search parity] n={n} cell={cell}", flush=True)
-    s = NeighborSearch(max(n, 1))
+    own = s is None
+    if own:
+        s = NeighborSearch(max(n, 1))
     s.set_points(xy)
     s.run(cell, store_geometry=geometry)
     g = s.results(geometry=geometry)
@@ -37,7 +39,8 @@ def _compare(xy, cell, geometry=True):
         assert_bits_equal(np.where(mask, g["distances"], 0), np.where(mask, od, 0), "distances")
         m2 = mask[:, :, None]
         assert_bits_equal(np.where(m2, g["vectors"], 0), np.where(m2, ov, 0), "vectors")
-    s.close()
+    if own:
+        s.close()
     return g
 
 
@@ -57,6 +60,24 @@ def test_clustered_points_cap_64(gpu):
     xy = np.concatenate([bg, core])
     g = _compare(xy, 1.0 / 256.0)
     assert g["counts"].max() == 64
+
+
+def test_list_capacity_switches(gpu):
+    # One search object run by turns on dense and sparse points: the build's
+    # LDS list capacity follows the previous run's longest list (32 on the
+    # first run, so the dense run defers its long lists to the full-capacity
+    # pass; then 64; then back down to 16 on sparse points).
+    rng = np.random.default_rng(11)
+    core = (0.5 + 0.002 * rng.standard_normal((6000, 2))).astype(np.float32)
+    dense = np.concatenate([rng.random((20000, 2)).astype(np.float32), core])
+    sparse = rng.random((len(dense), 2)).astype(np.float32)
+    s = NeighborSearch(len(dense))
+    try:
+        for xy, cell in ((dense, 1.0 / 256.0), (dense, 1.0 / 256.0), (sparse, 1.0 / 512.0),
+                         (sparse, 1.0 / 512.0), (dense, 1.0 / 256.0)):
+            _compare(xy, cell, s=s)
+    finally:
+        s.close()
 
 
 def test_lattice_distance_ties(gpu):
