@@ -359,9 +359,11 @@ void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t 
 // back its grid coordinate (the Morton code inverts exactly), and each of the
 // 3x3 neighbour cells is looked up once per cell instead of once per particle.
 // Cells are sorted by (scene, key), so a lookup among the same cells is a
-// search of the cell list, galloping from the cell itself (Morton neighbours
-// are mostly a few cells away, so the probes stay on nearby cache lines). The
-// other (solid) cells are looked up in their static hash set.
+// search of the cell list, galloping from a hint: the cell's own ordinal moved
+// by the key difference, which is exact where every cell in between is
+// occupied (the interior of a fluid) and otherwise a nearby start, so the
+// probes stay on a few cache lines. The other (solid) cells are looked up in
+// their static hash set.
 __device__ __forceinline__ bool cell_before(CellList const& cl, uint32_t i, uint32_t tag,
                                             uint64_t key) {
     uint32_t const ti = cl.tag[i];
@@ -449,7 +451,18 @@ __global__ __launch_bounds__(256) void k_cell_windows(WindowArgs w) {
         uint32_t const gx = compact_bits_2(key) + (k % 3u) - 1u;
         uint32_t const gy = compact_bits_2(key >> 1) + (k / 3u) - 1u;
         uint64_t const nk = morton2(static_cast<int32_t>(gx), static_cast<int32_t>(gy));
-        uint32_t const i = cell_gallop(w.self, n, c, tag, nk);
+        // Hint: ordinals advance at most as fast as keys.
+        uint32_t hint = c;
+        if (nk > key) {
+            uint64_t const d = nk - key;
+            hint = d < static_cast<uint64_t>(n - 1u - c) ? c + static_cast<uint32_t>(d) : n - 1u;
+        } else {
+            uint64_t const d = key - nk;
+            hint = d < static_cast<uint64_t>(c) ? c - static_cast<uint32_t>(d) : 0u;
+        }
+        uint32_t const i = (w.self.key[hint] == nk && w.self.tag[hint] == tag)
+                             ? hint
+                             : cell_gallop(w.self, n, hint, tag, nk);
         w.win[item] = cell_range(w.self, i, n, tag, nk);
         if (w.owin) {
             w.owin[item] = w.other_set.slots ? cell_set_range(w.other_set, w.other, nk, tag)
@@ -768,6 +781,33 @@ void launch_solid_volumes(Batch const& b, uint32_t n, hipStream_t s) {
 }
 
 //------------------------------------------------------------------------------
+// Neighbour loops of the sweeps walk a list in batches of NB_BATCH entries:
+// the batch's indices and gradients are loaded together, then the gathers of
+// the neighbours' fields, then the entries are summed in list order. This
+// keeps several dependent load chains in flight per thread without changing
+// the order of any floating-point sum.
+constexpr int NB_BATCH = 4;
+
+struct NbrBatch {
+    uint32_t j[NB_BATCH];
+    V2 g[NB_BATCH];
+};
+
+__device__ __forceinline__ void load_batch(Lists const& l, uint32_t stride, uint32_t t, int k0,
+                                           int cnt, NbrBatch& nb) {
+#pragma unroll
+    for (int u = 0; u < NB_BATCH; ++u) {
+        nb.j[u] = 0u;
+        nb.g[u] = V2{0.0f, 0.0f};
+        if (k0 + u < cnt) {
+            size_t const o = static_cast<size_t>(k0 + u) * stride + t;
+            nb.j[u] = l.idx[o];
+            nb.g[u] = l.gw[o];
+        }
+    }
+}
+
+//------------------------------------------------------------------------------
 // IISPH sweeps: sph2d_box/iisph_ops.cpp.
 
 // :94-221 a_ii and density stars; also zeroes the pressures (iisph.cpp:83).
@@ -791,19 +831,34 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
     } else {
         float acc = 0.0f;
         float dd = 0.0f;
-        for (int k = 0; k < fc; ++k) {
-            size_t const o = static_cast<size_t>(k) * a.stride + t;
-            uint32_t const s = a.fl.idx[o];
-            V2 const gw = a.fl.gw[o];
-            V2 const numer = scale(kf, gw);
-            float const rj = a.rho[s];
-            float const denom = rj * rj;
-            if (is_safe_div(numer, denom)) {
-                V2 const dij = divs(numer, denom);
-                V2 const dji = neg(dij);
-                acc += mf * dot(sub(d, dji), gw);
+        for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
+            NbrBatch nb;
+            load_batch(a.fl, a.stride, t, k0, fc, nb);
+            float rj[NB_BATCH];
+            V2 vj[NB_BATCH];
+#pragma unroll
+            for (int u = 0; u < NB_BATCH; ++u) {
+                rj[u] = 1.0f;
+                vj[u] = V2{0.0f, 0.0f};
+                if (k0 + u < fc) {
+                    rj[u] = a.rho[nb.j[u]];
+                    vj[u] = a.vel[nb.j[u]];
+                }
             }
-            dd += dtm * dot(sub(vi, a.vel[s]), gw);
+#pragma unroll
+            for (int u = 0; u < NB_BATCH; ++u) {
+                if (k0 + u < fc) {
+                    V2 const gw = nb.g[u];
+                    V2 const numer = scale(kf, gw);
+                    float const denom = rj[u] * rj[u];
+                    if (is_safe_div(numer, denom)) {
+                        V2 const dij = divs(numer, denom);
+                        V2 const dji = neg(dij);
+                        acc += mf * dot(sub(d, dji), gw);
+                    }
+                    dd += dtm * dot(sub(vi, vj[u]), gw);
+                }
+            }
         }
         aii = acc;
         rs = a.rho[t] + dd;
@@ -837,14 +892,27 @@ __global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
     float const kf = -(c.dt * c.dt) * (c.fluid_volume * c.rho0);
     int const fc = a.fl.count[t];
     V2 s{0.0f, 0.0f};
-    for (int k = 0; k < fc; ++k) {
-        size_t const o = static_cast<size_t>(k) * a.stride + t;
-        uint32_t const j = a.fl.idx[o];
-        V2 const gw = a.fl.gw[o];
-        float const rj = a.rho[j];
-        V2 const numer = scale(kf * a.p_in[j], gw);
-        float const denom = rj * rj;
-        if (is_safe_div(numer, denom)) { s = add(s, divs(numer, denom)); }
+    for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
+        NbrBatch nb;
+        load_batch(a.fl, a.stride, t, k0, fc, nb);
+        float rj[NB_BATCH], pj[NB_BATCH];
+#pragma unroll
+        for (int u = 0; u < NB_BATCH; ++u) {
+            rj[u] = 1.0f;
+            pj[u] = 0.0f;
+            if (k0 + u < fc) {
+                rj[u] = a.rho[nb.j[u]];
+                pj[u] = a.p_in[nb.j[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NB_BATCH; ++u) {
+            if (k0 + u < fc) {
+                V2 const numer = scale(kf * pj[u], nb.g[u]);
+                float const denom = rj[u] * rj[u];
+                if (is_safe_div(numer, denom)) { s = add(s, divs(numer, denom)); }
+            }
+        }
     }
     a.sdp[t] = s;
 }
@@ -870,18 +938,39 @@ __global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, int slot) {
         V2 const si = a.sdp[t];
         int const fc = a.fl.count[t];
         float beta = 0.0f;
-        for (int k = 0; k < fc; ++k) {
-            size_t const o = static_cast<size_t>(k) * a.stride + t;
-            uint32_t const j = a.fl.idx[o];
-            V2 const gw = a.fl.gw[o];
-            float const rj = a.rho[j];
-            V2 const numer = scale(kf, gw);
-            float const denom = rj * rj;
-            V2 const dij = is_safe_div(numer, denom) ? divs(numer, denom) : V2{0.0f, 0.0f};
-            V2 const dji = neg(dij);
-            V2 const sjk = sub(a.sdp[j], scale(pi, dji));
-            V2 const djj_pj = scale(a.p_in[j], a.dii[j]);
-            beta += mf * dot(sub(sub(si, djj_pj), sjk), gw);
+        for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
+            NbrBatch nb;
+            load_batch(a.fl, a.stride, t, k0, fc, nb);
+            float rj[NB_BATCH], pj[NB_BATCH];
+            V2 sj[NB_BATCH], djj[NB_BATCH];
+#pragma unroll
+            for (int u = 0; u < NB_BATCH; ++u) {
+                rj[u] = 1.0f;
+                pj[u] = 0.0f;
+                sj[u] = V2{0.0f, 0.0f};
+                djj[u] = V2{0.0f, 0.0f};
+                if (k0 + u < fc) {
+                    uint32_t const j = nb.j[u];
+                    rj[u] = a.rho[j];
+                    pj[u] = a.p_in[j];
+                    sj[u] = a.sdp[j];
+                    djj[u] = a.dii[j];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NB_BATCH; ++u) {
+                if (k0 + u < fc) {
+                    V2 const gw = nb.g[u];
+                    V2 const numer = scale(kf, gw);
+                    float const denom = rj[u] * rj[u];
+                    V2 const dij =
+                      is_safe_div(numer, denom) ? divs(numer, denom) : V2{0.0f, 0.0f};
+                    V2 const dji = neg(dij);
+                    V2 const sjk = sub(sj[u], scale(pi, dji));
+                    V2 const djj_pj = scale(pj[u], djj[u]);
+                    beta += mf * dot(sub(sub(si, djj_pj), sjk), gw);
+                }
+            }
         }
         int const sc = a.sl.count[t];
         if (sc) {
@@ -941,14 +1030,27 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
             if (fc) {
                 float const mf = c.rho0 * c.fluid_volume;
                 V2 acc{0.0f, 0.0f};
-                for (int k = 0; k < fc; ++k) {
-                    size_t const o = static_cast<size_t>(k) * a.stride + t;
-                    uint32_t const j = a.fl.idx[o];
-                    float const rj = a.rho[j];
-                    float const rj2 = rj * rj;
-                    float const pj = a.p_in[j];
-                    float const porj = is_safe_div(pj, rj2) ? pj / rj2 : 0.0f;
-                    acc = add(acc, scale(-mf * (pori + porj), a.fl.gw[o]));
+                for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
+                    NbrBatch nb;
+                    load_batch(a.fl, a.stride, t, k0, fc, nb);
+                    float rj[NB_BATCH], pj[NB_BATCH];
+#pragma unroll
+                    for (int u = 0; u < NB_BATCH; ++u) {
+                        rj[u] = 1.0f;
+                        pj[u] = 0.0f;
+                        if (k0 + u < fc) {
+                            rj[u] = a.rho[nb.j[u]];
+                            pj[u] = a.p_in[nb.j[u]];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < NB_BATCH; ++u) {
+                        if (k0 + u < fc) {
+                            float const rj2 = rj[u] * rj[u];
+                            float const porj = is_safe_div(pj[u], rj2) ? pj[u] / rj2 : 0.0f;
+                            acc = add(acc, scale(-mf * (pori + porj), nb.g[u]));
+                        }
+                    }
                 }
                 v = add(v, scale(c.dt, acc));
             }
@@ -1007,10 +1109,19 @@ __device__ __forceinline__ float divergence_of(Batch const& a, StepConsts const&
     float const mf = c.rho0 * c.fluid_volume;
     int const fc = a.fl.count[t];
     float div = 0.0f;
-    for (int k = 0; k < fc; ++k) {
-        size_t const o = static_cast<size_t>(k) * a.stride + t;
-        uint32_t const j = a.fl.idx[o];
-        div += mf * dot(sub(vi, a.vel[j]), a.fl.gw[o]);
+    for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
+        NbrBatch nb;
+        load_batch(a.fl, a.stride, t, k0, fc, nb);
+        V2 vj[NB_BATCH];
+#pragma unroll
+        for (int u = 0; u < NB_BATCH; ++u) {
+            vj[u] = V2{0.0f, 0.0f};
+            if (k0 + u < fc) { vj[u] = a.vel[nb.j[u]]; }
+        }
+#pragma unroll
+        for (int u = 0; u < NB_BATCH; ++u) {
+            if (k0 + u < fc) { div += mf * dot(sub(vi, vj[u]), nb.g[u]); }
+        }
     }
     int const sc = a.sl.count[t];
     if (sc) {
