@@ -110,6 +110,65 @@ def cpu_baseline(scene, cfg, method: str, gpu_state_after, budget_s: float = 12.
     }
 
 
+def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict:
+    """256 dam breaks of uneven size (log-uniform 2k-200k particles, aspect
+    0.25-2, per-scene CFL 0.4 dt) stepped as one batch (one launch per sub-step
+    stage for all scenes) and as 256 plain per-scene contexts, over the same
+    sub-steps from the same states; every scene's state must come out
+    bit-identical in both forms."""
+    from emerald_amd import scenes
+    from emerald_amd import sph2d_box as sb
+    log(f"batched form: generating {count} scenes")
+    scs = scenes.batch_dam_breaks(count)
+    cfgs = [scenes.config_for(sc, method) for sc in scs]
+    n_total = sum(len(sc.positions) for sc in scs)
+    solids = [sb.Solid_state(sc.solids) for sc in scs]
+    states = [sb.State(sc.positions, sc.velocities) for sc in scs]
+
+    log(f"batched form: one batch of {count} scenes, {n_total} particles")
+    batch = sb.Batch(cfgs, solids, states)
+    batch.set_cfl(0.4)
+    batch.frame_init(method)
+    batch.run_sub_steps(method, warmup)
+    t0 = time.perf_counter()
+    batch.run_sub_steps(method, steps)
+    t_batch = time.perf_counter() - t0
+    last = batch.sub_step(method)  # one more, for the per-scene statistics
+    iters = [st.iterations for st in last]
+
+    log(f"batched form: {count} plain per-scene contexts")
+    ctxs = []
+    for sc, cfg, sol, st in zip(scs, cfgs, solids, states):
+        c = sb.Context(cfg, sol, len(sc.positions))
+        c.set_cfl(0.4)
+        c.set_state(st)
+        c.frame_init(method)
+        c.run_sub_steps(method, cfg.seconds_per_sub_step, warmup)
+        ctxs.append(c)
+    t0 = time.perf_counter()
+    for c, cfg in zip(ctxs, cfgs):
+        c.run_sub_steps(method, cfg.seconds_per_sub_step, steps)
+    t_plain = time.perf_counter() - t0
+    same = True
+    for i, (c, cfg) in enumerate(zip(ctxs, cfgs)):
+        c.run_sub_steps(method, cfg.seconds_per_sub_step, 1)
+        a, b = c.get_state(), batch.get_state(i)
+        same = same and np.array_equal(a.positions.view(np.uint32), b.positions.view(np.uint32)) \
+            and np.array_equal(a.velocities.view(np.uint32), b.velocities.view(np.uint32))
+        c.close()
+    batch.close()
+    v_batch = n_total * steps / t_batch
+    v_plain = n_total * steps / t_plain
+    return {
+        "scenes": count, "particles": n_total, "sub_steps": steps, "method": method,
+        "value": v_batch, "unit": "particle-steps/s", "ms_per_sub_step": t_batch * 1e3 / steps,
+        "plain_value": v_plain, "plain_ms_per_sub_step": t_plain * 1e3 / steps,
+        "speedup_vs_plain": v_batch / v_plain, "bit_identical_to_plain": bool(same),
+        "iterations_min_mean_max": [min(iters), float(np.mean(iters)), max(iters)],
+        "dt": "per scene, CFL 0.4 on the device",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +177,9 @@ def main() -> int:
     ap.add_argument("--method", default="iisph", choices=["iisph", "dfsph_p", "pcisph"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--batched-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are "
@@ -131,6 +193,9 @@ def main() -> int:
         raise SystemExit("bench: no HIP device")
 
     method = args.method
+    if args.batched_only:
+        print(json.dumps(batched_form(method, args.batched_steps, 2)), flush=True)
+        return 0
     log("building scene (1024 x 1024 lattice dam break)")
     scene = scenes.dam_break_1m()
     cfg = scenes.config_for(scene, method)
@@ -215,6 +280,11 @@ def main() -> int:
         log("cpu baseline (oracle on host cores)")
         cpu = cpu_baseline(scene, cfg, method, gpu_after, args.cpu_budget)
 
+    batched = None
+    if not args.no_batched:
+        ctx.close()
+        batched = batched_form(method, args.batched_steps, 2)
+
     out = {
         "metric": "IISPH particle-steps/s @1M (1 GPU)" if method == "iisph"
         else f"{method.upper()} particle-steps/s @1M (1 GPU)",
@@ -243,6 +313,7 @@ def main() -> int:
         "roofline": roofline,
         "whole_step": whole_step,
         "cpu_baseline": cpu,
+        "batched": batched,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

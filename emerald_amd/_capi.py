@@ -82,6 +82,10 @@ class StepStats(ctypes.Structure):
         ("cell_count", ctypes.c_uint32),
         ("sort_passes", ctypes.c_uint32),
         ("max_neighbors", ctypes.c_uint32),
+        ("dt", ctypes.c_float),
+        ("reserved", ctypes.c_uint32),
+        ("kinetic_energy", ctypes.c_double),
+        ("potential_energy", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -129,6 +133,8 @@ SIGNATURES = {
     "emsph_sub_step": (_ST, [_P, _I, _F, _F, _P]),
     "emsph_frame_step": (_ST, [_P, _I, _I64, _P]),
     "emsph_run_sub_steps": (_ST, [_P, _I, _F, _I, _P, _P]),
+    "emsph_set_cfl": (_ST, [_P, _F]),
+    "emsph_batch_set_cfl": (_ST, [_P, _F]),
     "emsph_set_kernel_timing": (_ST, [_P, _I]),
     "emsph_get_kernel_timing": (_I, [_P, _P, _P, _P, _I]),
     "emsph_get_iteration_totals": (_ST, [_P, _I, _P]),

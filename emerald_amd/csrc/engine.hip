@@ -199,7 +199,8 @@ struct SceneHost {
     uint32_t ns = 0;      // solid particles
     uint32_t offset = 0;  // first fluid slot
     uint32_t soffset = 0; // first solid slot
-    float dt = 0.0f;      // batch sub-step dt
+    float dt = 0.0f;      // sub-step dt (the largest one when cfl > 0)
+    float cfl = 0.0f;     // > 0: per-sub-step dt from the scene's CFL number
 };
 
 int scene_pass_count(uint32_t m) {
@@ -252,6 +253,7 @@ struct Engine {
     DBuf<float> fl_dist, sl_dist;
     DBuf<uint2> swin;       // [fluid cell][9] solid slot ranges
     DBuf<uint2> overflow;   // particles deferred by the capped neighbour build
+    DBuf<double2> tile_energy;
     int build_cap = 32;     // LDS list capacity of the next build (from the last max count)
 
     SpatialIndex sidx;
@@ -372,6 +374,7 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     b.win = g->fidx.win.p;
     b.swin = g->swin.p;
     b.overflow = g->overflow.p;
+    b.tile_energy = g->tile_energy.p;
     if (g->NS) {
         b.solids = SolidView{g->s_pos.p, g->s_vel.p, g->s_vol.p, g->sidx.rank.p, g->sidx.orig.p};
     }
@@ -412,6 +415,10 @@ emsph_status upload_layout(Engine* g) {
         sd[s].n = h.n;
         sd[s].soffset = h.soffset;
         sd[s].ns = h.ns;
+        sd[s].first_tile = static_cast<uint32_t>(tl.size());
+        sd[s].ntiles = (h.n + TILE - 1) / TILE;
+        sd[s].cfl = h.cfl;
+        sd[s].dt_max = h.dt;
         for (uint32_t k = 0; k < h.n; k += TILE) {
             tl.push_back(Tile{s, h.offset + k, std::min<uint32_t>(TILE, h.n - k), 0});
         }
@@ -420,6 +427,7 @@ emsph_status upload_layout(Engine* g) {
     g->ntiles = static_cast<uint32_t>(tl.size());
     HIP_TRY(g->scenes.ensure(std::max<uint32_t>(g->M, 1)));
     HIP_TRY(g->tiles.ensure(std::max<size_t>(tl.size(), 1)));
+    HIP_TRY(g->tile_energy.ensure(std::max<size_t>(tl.size(), 1)));
     HIP_TRY(g->scene_of.ensure(std::max<uint32_t>(g->cap, 1)));
     HIP_TRY(g->stats.ensure(std::max<uint32_t>(g->M, 1)));
     HIP_TRY(g->red.ensure(static_cast<size_t>(std::max<uint32_t>(g->M, 1)) * MAX_RED_SLOTS));
@@ -769,9 +777,12 @@ emsph_status iisph_sub_step(Engine* g) {
         HIP_TRY(read_keystate(g));
         if (g->sort_short) { return EMSPH_OK; }
     }
-    b.p_in = pa;
+    // Each scene's last pressure buffer follows its own iteration count.
+    b.p_in = g->p0.p;
+    b.p_alt = g->p1.p;
     HIP_TRY(timed(g, "apply_integrate", [&] { launch_apply(b, 0, 1, 1, g->stream); }));
-    g->cur_p = pa;
+    launch_scene_energy(b, g->stream);
+    g->cur_p = g->p0.p;
     return EMSPH_OK;
 }
 
@@ -825,6 +836,7 @@ emsph_status dfsph_sub_step(Engine* g) {
         if (g->sort_short) { return EMSPH_OK; }
     }
     HIP_TRY(timed(g, "finalize", [&] { launch_finalize(b, 1, g->stream); }));
+    launch_scene_energy(b, g->stream);
     g->cur_p = g->kds.p;
     return EMSPH_OK;
 }
@@ -859,8 +871,11 @@ emsph_status pcisph_sub_step(Engine* g) {
         HIP_TRY(read_keystate(g));
         if (g->sort_short) { return EMSPH_OK; }
     }
-    b.vstar_in = vin;
+    // Each scene's last predictor follows its own iteration count.
+    b.vstar_in = g->vstar0.p;
+    b.vstar_alt = g->vstar1.p;
     HIP_TRY(timed(g, "pci_finalize", [&] { launch_pci_finalize(b, g->stream); }));
+    launch_scene_energy(b, g->stream);
     g->cur_p = g->p0.p;
     return EMSPH_OK;
 }
@@ -916,6 +931,9 @@ void to_public(SceneStats const& s, Engine const* g, emsph_step_stats* out) {
     out->sort_passes = static_cast<uint32_t>(g->fidx.key_passes + g->fidx.scene_passes);
     out->cell_count = g->host->keys.cells;
     out->max_neighbors = g->host->keys.max_count;
+    out->dt = s.dt;
+    out->kinetic_energy = s.kinetic_energy;
+    out->potential_energy = s.potential_energy;
 }
 
 emsph_status engine_frame_init(Engine* g, int method) {
@@ -1030,7 +1048,21 @@ emsph_status engine_set_state(Engine* g, uint32_t s, float const* pos_xy, float 
     }
     g->have_lists = false;
     g->passes_hint = -1;
+    // max |v|^2 of the new state, for the CFL dt of the next sub-step.
+    if (g->N) {
+        Batch b = fluid_batch(g);
+        launch_step_begin(b, g->stream);
+        launch_vmax2(b, g->stream);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+emsph_status engine_set_cfl(Engine* g, float factor) {
+    if (!(factor >= 0.0f)) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "CFL factor < 0"); }
+    for (auto& h : g->sh) { h.cfl = factor; }
+    ST_TRY(upload_layout(g));
     return EMSPH_OK;
 }
 
@@ -1306,6 +1338,11 @@ emsph_status emsph_get_iteration_totals(emsph_context* c, int reset, int64_t* ou
     return EMSPH_OK;
 }
 
+emsph_status emsph_set_cfl(emsph_context* c, float factor) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    return engine_set_cfl(&c->e, factor);
+}
+
 emsph_status emsph_set_kernel_timing(emsph_context* c, int enable) {
     if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
     HIP_TRY(timer_flush(&c->e));
@@ -1399,6 +1436,11 @@ emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* 
         return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument");
     }
     return engine_get_field(&b->e, scene, field, out);
+}
+
+emsph_status emsph_batch_set_cfl(emsph_batch* b, float factor) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    return engine_set_cfl(&b->e, factor);
 }
 
 emsph_status emsph_batch_frame_init(emsph_batch* b, int method) {

@@ -46,10 +46,14 @@ struct Thresholds {
 struct SceneDev {
     StepConsts c;
     Thresholds thr;
-    uint32_t offset;   // first fluid slot
-    uint32_t n;        // fluid particles
-    uint32_t soffset;  // first solid slot
-    uint32_t ns;       // solid particles
+    uint32_t offset;      // first fluid slot
+    uint32_t n;           // fluid particles
+    uint32_t soffset;     // first solid slot
+    uint32_t ns;          // solid particles
+    uint32_t first_tile;  // the scene's fluid tiles are [first_tile, first_tile + ntiles)
+    uint32_t ntiles;
+    float cfl;            // > 0: c.dt = min(dt_max, cfl * h / max |v|) at each sub-step
+    float dt_max;
 };
 
 struct Tile {
@@ -77,6 +81,10 @@ struct SceneStats {
     float divergence_error_max;
     unsigned int vmax2_bits;
     unsigned int active;   // 1 while the current solver loop still iterates
+    float dt;              // dt of the sub-step
+    unsigned int pad;
+    double kinetic_energy;    // sum of m |v|^2 / 2 after the sub-step (FP64)
+    double potential_energy;  // sum of m g y after the sub-step (FP64)
 };
 
 // Workgroups of the key kernel (grid-stride), i.e. partial key ranges.
@@ -161,6 +169,7 @@ struct Batch {
     uint2 const* win;
     uint2 const* swin;
     uint2* overflow;        // {slot, scene} of particles deferred to the full-cap pass
+    double2* tile_energy;   // per fluid tile: FP64 kinetic, potential energy
     Lists fl;
     Lists sl;
     float* rho;
@@ -170,6 +179,10 @@ struct Batch {
     V2* sdp;
     float const* p_in;
     float* p_out;
+    // Second buffer of a solver loop's ping-pong: a scene that ran an odd number
+    // of iterations left its result there (converged scenes stop writing, so
+    // scenes of one batch end in different buffers). Read by the final kernels.
+    float* p_alt;
     float* alpha;
     float* kappa;
     // original-order state (gathered in by the cells stage, scattered back at the end)
@@ -184,6 +197,7 @@ struct Batch {
     V2* xstar;
     V2 const* vstar_in;
     V2* vstar_out;
+    V2 const* vstar_alt;
     V2* force;
     int store_geometry;
 };
@@ -259,8 +273,11 @@ void launch_cell_heads(IndexArgs const& a, hipStream_t s);
 void launch_cells_finalize(IndexArgs const& a, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
-// Per-sub-step resets: stats, residual slots, vmax2 of every scene.
+// Per-sub-step resets: stats, residual slots, vmax2 of every scene; the CFL dt
+// of scenes that use one.
 void launch_step_begin(Batch const& b, hipStream_t s);
+// Per-scene energy sums over the tile partials written by the final kernel.
+void launch_scene_energy(Batch const& b, hipStream_t s);
 // Start a solver loop: every non-empty scene active; KeyState::active_count set.
 void launch_loop_begin(Batch const& b, hipStream_t s);
 // After iteration `it` of loop `kind`: each active scene applies the

@@ -29,6 +29,7 @@ constexpr int BUILD_THREADS = 64;
 struct TileRef {
     uint32_t scene;
     uint32_t t;       // this thread's slot
+    uint32_t tile;
     bool valid;
 };
 __device__ __forceinline__ TileRef tile_ref(Batch const& b, uint32_t threads) {
@@ -37,6 +38,7 @@ __device__ __forceinline__ TileRef tile_ref(Batch const& b, uint32_t threads) {
     (void)threads;
     TileRef r;
     r.scene = tl.scene;
+    r.tile = k;
     r.t = tl.start + threadIdx.x;
     r.valid = threadIdx.x < tl.count;
     return r;
@@ -87,6 +89,39 @@ __device__ __forceinline__ void block_reduce_max(unsigned int m, unsigned int* o
         for (int w = 0; w < THREADS / 64; ++w) { tm = sm[w] > tm ? sm[w] : tm; }
         if (tm) { atomicMax(out, tm); }
     }
+}
+
+// FP64 energy of a workgroup's particles into its tile's partial (fixed
+// shuffle order, so the sum is the same in every run). Every thread calls it.
+template <int THREADS>
+__device__ __forceinline__ void block_energy(double ke, double pe, double2* out) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ke += __shfl_xor(ke, off);
+        pe += __shfl_xor(pe, off);
+    }
+    __shared__ double ske[THREADS / 64], spe[THREADS / 64];
+    if (__lane_id() == 0) {
+        ske[threadIdx.x / 64] = ke;
+        spe[threadIdx.x / 64] = pe;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tk = 0.0, tp = 0.0;
+        for (int w = 0; w < THREADS / 64; ++w) {
+            tk += ske[w];
+            tp += spe[w];
+        }
+        *out = make_double2(tk, tp);
+    }
+}
+
+// Kinetic and potential energy of one particle in FP64.
+__device__ __forceinline__ void particle_energy(StepConsts const& c, V2 x, V2 v, double& ke,
+                                                double& pe) {
+    double const m = c.mass;
+    ke = 0.5 * m * (static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y);
+    pe = m * static_cast<double>(c.gravity) * static_cast<double>(x.y);
 }
 
 //------------------------------------------------------------------------------
@@ -249,7 +284,21 @@ void launch_cells_finalize(IndexArgs const& a, hipStream_t s) {
 __global__ __launch_bounds__(256) void k_step_begin(Batch b) {
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < b.nscenes;
          s += gridDim.x * blockDim.x) {
+        SceneDev* sd = const_cast<SceneDev*>(b.scenes) + s;
+        if (sd->cfl > 0.0f) {
+            // cfl.cpp:12-21: factor * h / sqrt(max |v|^2) in double, over the
+            // velocities the previous sub-step left (dt_max while at rest).
+            float const vmax2 = __uint_as_float(b.stats[s].vmax2_bits);
+            float dt = sd->dt_max;
+            if (vmax2 > 0.0f) {
+                double const d = static_cast<double>(sd->cfl) * sd->c.kc.h /
+                                 sqrt(static_cast<double>(vmax2));
+                if (d < static_cast<double>(dt)) { dt = static_cast<float>(d); }
+            }
+            sd->c.dt = dt;
+        }
         SceneStats st{};
+        st.dt = sd->c.dt;
         b.stats[s] = st;
         for (int k = 0; k < MAX_RED_SLOTS; ++k) {
             b.red[s * MAX_RED_SLOTS + k].isum = 0;
@@ -918,15 +967,11 @@ __global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
 }
 
 // :261-353 beta (into the new-pressure buffer) fused with :357-425 new
-// pressure, quantised error sum and error max. A converged scene copies its
-// pressures forward so that the last buffer holds every scene's result.
+// pressure, quantised error sum and error max. A converged scene skips the
+// sweep; its result stays in the buffer of its last iteration.
 __global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, int slot) {
     TileRef const r = tile_ref(a, TILE);
-    bool const active = a.stats[r.scene].active != 0u;
-    if (!active) {
-        if (r.valid) { a.p_out[r.t] = a.p_in[r.t]; }
-        return;  // the whole workgroup takes this branch
-    }
+    if (!a.stats[r.scene].active) { return; }  // the whole workgroup
     unsigned long long qsum = 0;
     unsigned int emax = 0;
     if (r.valid) {
@@ -1010,6 +1055,7 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
     TileRef const r = tile_ref(a, TILE);
     if (!all_scenes && !a.stats[r.scene].active) { return; }  // whole workgroup
     unsigned int vm = 0;
+    double ke = 0.0, pe = 0.0;
     if (r.valid) {
         uint32_t const t = r.t;
         StepConsts const c = a.scenes[r.scene].c;
@@ -1022,10 +1068,18 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
         }
         int const fc = a.fl.count[t];
         int const sc = a.sl.count[t];
+        // The scene's last pressure buffer (odd iteration count: p_alt); it is
+        // moved into p_in for the field read-back, which no thread of this
+        // scene reads in this kernel.
+        float const* P = a.p_in;
+        if (a.p_alt && (a.stats[r.scene].iterations & 1)) {
+            P = a.p_alt;
+            const_cast<float*>(a.p_in)[t] = P[t];
+        }
         if (fc || sc) {
             float const ri = a.rho[t];
             float const ri2 = ri * ri;
-            float const pi = a.p_in[t];
+            float const pi = P[t];
             float const pori = is_safe_div(pi, ri2) ? pi / ri2 : 0.0f;
             if (fc) {
                 float const mf = c.rho0 * c.fluid_volume;
@@ -1040,7 +1094,7 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
                         pj[u] = 0.0f;
                         if (k0 + u < fc) {
                             rj[u] = a.rho[nb.j[u]];
-                            pj[u] = a.p_in[nb.j[u]];
+                            pj[u] = P[nb.j[u]];
                         }
                     }
 #pragma unroll
@@ -1072,11 +1126,15 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
             a.pos_orig[o] = x;
             a.vel_orig[o] = v;
             vm = __float_as_uint(dot(v, v));
+            particle_energy(c, x, v, ke, pe);
         } else {
             a.vel[t] = v;
         }
     }
-    if (FINAL) { block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits); }
+    if (FINAL) {
+        block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
+        block_energy<TILE>(ke, pe, a.tile_energy + r.tile);
+    }
 }
 
 // x += dt v and scatter the sorted state back to original order
@@ -1085,10 +1143,12 @@ template <int DFSPH>
 __global__ __launch_bounds__(TILE) void k_finalize(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     unsigned int vm = 0;
+    double ke = 0.0, pe = 0.0;
     if (r.valid) {
         uint32_t const t = r.t;
+        StepConsts const c = a.scenes[r.scene].c;
         V2 const v = a.vel[t];
-        V2 const x = add(a.pos[t], scale(a.scenes[r.scene].c.dt, v));
+        V2 const x = add(a.pos[t], scale(c.dt, v));
         uint32_t const o = a.orig[t];
         a.pos_orig[o] = x;
         a.vel_orig[o] = v;
@@ -1097,8 +1157,10 @@ __global__ __launch_bounds__(TILE) void k_finalize(Batch a) {
             a.kd_orig[o] = a.kds[t];
         }
         vm = __float_as_uint(dot(v, v));
+        particle_energy(c, x, v, ke, pe);
     }
     block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
+    block_energy<TILE>(ke, pe, a.tile_energy + r.tile);
 }
 
 //------------------------------------------------------------------------------
@@ -1264,10 +1326,7 @@ __global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     if (!r.valid) { return; }
     uint32_t const t = r.t;
-    if (!a.stats[r.scene].active) {
-        a.vstar_out[t] = a.vstar_in[t];
-        return;
-    }
+    if (!a.stats[r.scene].active) { return; }  // last predictor stays in place
     StepConsts const c = a.scenes[r.scene].c;
     float const H = c.kc.h;
     float const M = c.mass;
@@ -1318,19 +1377,41 @@ __global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
 }
 
 // :151-163 final update: v = v_ext + (dt/m) F; x += dt v. Both equal the last
-// predictor written by k_pci_force.
+// predictor written by k_pci_force (in vstar_alt after an odd number of
+// iterations).
 __global__ __launch_bounds__(TILE) void k_pci_finalize(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     unsigned int vm = 0;
+    double ke = 0.0, pe = 0.0;
     if (r.valid) {
         uint32_t const t = r.t;
-        V2 const v = a.vstar_in[t];
+        V2 const* vs = (a.stats[r.scene].iterations & 1) ? a.vstar_alt : a.vstar_in;
+        V2 const v = vs[t];
+        V2 const x = a.xstar[t];
         uint32_t const o = a.orig[t];
-        a.pos_orig[o] = a.xstar[t];
+        a.pos_orig[o] = x;
         a.vel_orig[o] = v;
         vm = __float_as_uint(dot(v, v));
+        particle_energy(a.scenes[r.scene].c, x, v, ke, pe);
     }
     block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
+    block_energy<TILE>(ke, pe, a.tile_energy + r.tile);
+}
+
+// Per-scene energies: the scene's tile partials summed in tile order.
+__global__ __launch_bounds__(256) void k_scene_energy(Batch a) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < a.nscenes;
+         s += gridDim.x * blockDim.x) {
+        SceneDev const& sd = a.scenes[s];
+        double ke = 0.0, pe = 0.0;
+        for (uint32_t k = 0; k < sd.ntiles; ++k) {
+            double2 const e = a.tile_energy[sd.first_tile + k];
+            ke += e.x;
+            pe += e.y;
+        }
+        a.stats[s].kinetic_energy = ke;
+        a.stats[s].potential_energy = pe;
+    }
 }
 
 //------------------------------------------------------------------------------
@@ -1431,6 +1512,10 @@ void launch_pci_force(Batch const& a, hipStream_t s) {
 }
 void launch_pci_finalize(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_finalize, tile_grid(a), dim3(TILE), 0, s, a);
+}
+void launch_scene_energy(Batch const& a, hipStream_t s) {
+    uint32_t const blocks = (a.nscenes + 255) / 256;
+    hipLaunchKernelGGL(k_scene_energy, dim3(blocks ? blocks : 1), dim3(256), 0, s, a);
 }
 void launch_vmax2(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_vmax2, tile_grid(a), dim3(TILE), 0, s, a);

@@ -14,6 +14,12 @@ Workloads (sizes are the benchmark's):
                               obstacles), rest density 1000, zero velocity.
       12,800 particles:  80 x 160 at s = 0.005, dt = 2e-4
       1,048,576:         1024 x 1024 at s = 1/2560, dt = 1.5625e-5 (same CFL number)
+  batch_dam_breaks(256)       256 dam breaks, N ~ log-U[2k, 200k], block aspect
+                              (width / height) ~ U[0.25, 2], spacing so that
+                              the block's longer side is 0.8, h = s, largest
+                              dt = CFL 0.4 at the free-fall speed sqrt(2 g H)
+                              of the block height H (the device CFL 0.4 over
+                              the live max |v| takes over when it is lower)
   search_points(n)            half uniform in [0,1)^2, half from 8 Gaussian
                               clusters (sigma 0.02) with centres U[0.1, 0.9]^2,
                               samples outside [0,1)^2 redrawn.
@@ -110,6 +116,24 @@ def dam_break_12800(seed: int = 1) -> DamBreak:
 
 def dam_break_1m(seed: int = 2) -> DamBreak:
     return dam_break(1024, 1024, 1.0 / 2560.0, 1.5625e-5, seed)
+
+
+def batch_dam_breaks(count: int = 256, seed: int = 4, n_min: int = 2000, n_max: int = 200_000,
+                     aspect_min: float = 0.25, aspect_max: float = 2.0,
+                     cfl: float = 0.4, gravity: float = 9.81) -> list:
+    """Independent dam-break scenes of uneven size for the batched form."""
+    u = uniform01(seed, 2 * count).reshape(count, 2)
+    out = []
+    for i in range(count):
+        n = math.exp(math.log(n_min) + u[i, 0] * (math.log(n_max) - math.log(n_min)))
+        aspect = aspect_min + u[i, 1] * (aspect_max - aspect_min)
+        nx = max(1, int(round(math.sqrt(n * aspect))))
+        ny = max(1, int(round(n / nx)))
+        s = 0.8 / max(nx, ny)
+        h = float(np.float32(s))
+        dt = cfl * h / math.sqrt(2.0 * gravity * ny * s)
+        out.append(dam_break(nx, ny, s, float(np.float32(dt)), seed=1000 + seed * 10007 + i))
+    return out
 
 
 def pcisph_denom(dt: float, mass: float, rho0: float, spacing: float, h: float) -> np.float32:

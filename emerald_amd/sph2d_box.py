@@ -284,6 +284,10 @@ class Context:
                                         ctypes.byref(ms), ctypes.byref(st)), "run_sub_steps")
         return float(ms.value), st
 
+    def set_cfl(self, factor: float) -> None:
+        """Per-sub-step dt = min(dt, factor * h / max |v|) on the device (0: off)."""
+        check(lib().emsph_set_cfl(self._h, float(factor)), "set_cfl")
+
     def iteration_totals(self, reset: bool = False) -> tuple:
         """(sub-steps, pressure or density iterations, divergence iterations)
         since creation or the last reset."""
@@ -412,6 +416,10 @@ class Batch:
         out = np.empty((self.counts[i], comps), np.float32)
         check(lib().emsph_batch_get_field(self._h, i, fid, _ptr(out)), "batch_get_field")
         return out[:, 0].copy() if comps == 1 else out
+
+    def set_cfl(self, factor: float) -> None:
+        """Per-scene, per-sub-step dt = min(config dt, factor * h / max |v|) (0: off)."""
+        check(lib().emsph_batch_set_cfl(self._h, float(factor)), "batch_set_cfl")
 
     def frame_init(self, method: str) -> None:
         check(lib().emsph_batch_frame_init(self._h, _capi.METHODS[method]), "batch_frame_init")

@@ -100,6 +100,10 @@ typedef struct emsph_step_stats {
     uint32_t cell_count;           /* occupied grid cells (all scenes of a batch) */
     uint32_t sort_passes;          /* 8-bit radix passes of the key sort */
     uint32_t max_neighbors;        /* longest fluid or solid neighbour list */
+    float dt;                      /* dt of the sub-step */
+    uint32_t reserved;
+    double kinetic_energy;         /* sum of m |v|^2 / 2 after the sub-step, FP64 */
+    double potential_energy;       /* sum of m g y after the sub-step, FP64 */
 } emsph_step_stats;
 
 typedef struct emsph_frame_stats {
@@ -196,6 +200,16 @@ emsph_status emsph_run_sub_steps(emsph_context* ctx, int method, float dt, int n
                                  float* elapsed_ms, emsph_step_stats* last);
 /* Per-kernel-class device timing (HIP events around each launch; adds
  * synchronisation, for measurement only). names/ms/calls arrays of size max. */
+/* Per-sub-step dt from the CFL condition (cfl.cpp:12-21): with factor > 0
+ * every sub-step uses dt = min(dt_max, factor * h / sqrt(max |v|^2)) over the
+ * velocities the previous sub-step left (or set_state gave), evaluated on the
+ * device, where dt_max is the dt passed to the step call (emsph_sub_step,
+ * emsph_run_sub_steps) or config.seconds_per_sub_step (batches). 0 turns it
+ * off (the default). The dt used is reported in emsph_step_stats.dt. This is
+ * the reference's cfl_maximum_time_step applied per sub-step rather than its
+ * per-frame quantisation (adaptive_time_step.h:35-122, which emsph_frame_step
+ * keeps). */
+emsph_status emsph_set_cfl(emsph_context* ctx, float factor);
 emsph_status emsph_set_kernel_timing(emsph_context* ctx, int enable);
 int emsph_get_kernel_timing(emsph_context* ctx, const char** names, double* total_ms,
                             int64_t* calls, int max);
@@ -222,6 +236,7 @@ emsph_status emsph_batch_set_state(emsph_batch* b, int scene, const float* pos_x
                                    const float* vel_xy);
 emsph_status emsph_batch_get_state(emsph_batch* b, int scene, float* pos_xy, float* vel_xy);
 emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* out);
+emsph_status emsph_batch_set_cfl(emsph_batch* b, float factor);  /* as emsph_set_cfl, per scene */
 emsph_status emsph_batch_frame_init(emsph_batch* b, int method);
 /* One sub-step of every scene; stats (array of n_scenes, may be NULL). */
 emsph_status emsph_batch_sub_step(emsph_batch* b, int method, emsph_step_stats* stats);

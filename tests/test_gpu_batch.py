@@ -54,3 +54,61 @@ def test_batch_equals_alone(gpu, method):
         assert_bits_equal(b.velocities, a.velocities, f"scene {i} batch vs alone velocities")
         assert_bits_equal(b.positions, pos, f"scene {i} batch vs oracle positions")
         assert_bits_equal(batch.field(i, "density"), ctx.field("density"), f"scene {i} density")
+
+
+def test_batch_cfl_dt_and_energy(gpu):
+    # Per-scene CFL dt on the device and the FP64 energy sums: identical alone
+    # and in the batch, dt per cfl.cpp:12-21 over the previous max |v|^2 (the
+    # first from the uploaded velocities), and the energies equal to a host
+    # FP64 sum of the downloaded state. The scenes start in a uniform flow fast
+    # enough for the CFL bound to be below the largest dt.
+    method = "iisph"
+    factor = 0.04
+    scs = _mixed_scenes()
+    cfgs = [scenes.config_for(sc, method) for sc in scs]
+    states = []
+    for k, sc in enumerate(scs):
+        u = scenes.uniform01(77 + k, 2 * len(sc.positions)).reshape(-1, 2)
+        v = np.array([1.5, -0.5]) + 0.2 * (u - 0.5)
+        states.append(sb.State(sc.positions, v.astype(np.float32)))
+    batch = sb.Batch(cfgs, [sb.Solid_state(s.solids) for s in scs], states)
+    batch.set_cfl(factor)
+    batch.frame_init(method)
+    alone = []
+    for sc, cfg, st0 in zip(scs, cfgs, states):
+        ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(sc.positions))
+        ctx.set_cfl(factor)
+        ctx.set_state(st0)
+        ctx.frame_init(method)
+        alone.append(ctx)
+    prev_vmax2 = [float(np.max(np.sum(s.velocities.astype(np.float32) ** 2, axis=1)))
+                  for s in states]
+    cfl_bound = 0
+    for k in range(20):
+        bst = batch.sub_step(method)
+        for i, (sc, cfg, ctx) in enumerate(zip(scs, cfgs, alone)):
+            st = ctx.sub_step(method, 0.0, cfg.seconds_per_sub_step)
+            assert bst[i].dt == st.dt and bst[i].iterations == st.iterations, (k, i)
+            assert bst[i].kinetic_energy == st.kinetic_energy, (k, i)
+            assert bst[i].potential_energy == st.potential_energy, (k, i)
+            dt_max = np.float32(cfg.seconds_per_sub_step)
+            want = dt_max
+            if prev_vmax2[i] > 0:
+                # The factor is held as a float on the device.
+                d = (float(np.float32(factor)) * float(np.float32(sc.support))
+                     / np.sqrt(np.float64(prev_vmax2[i])))
+                if d < float(dt_max):
+                    want = np.float32(d)
+                    cfl_bound += 1
+            assert np.float32(st.dt) == want, (k, i, st.dt, want)
+            prev_vmax2[i] = float(np.float32(st.max_velocity_sq))
+            s = ctx.get_state()
+            m = np.float64(np.float32(cfg.mass_per_particle))
+            v = s.velocities.astype(np.float64)
+            ke = 0.5 * m * np.sum(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1])
+            pe = m * np.float64(np.float32(sc.gravity)) * np.sum(s.positions[:, 1].astype(np.float64))
+            assert abs(st.kinetic_energy - ke) <= 1e-12 * abs(ke), (k, i)
+            assert abs(st.potential_energy - pe) <= 1e-12 * abs(pe), (k, i)
+    assert cfl_bound > 0, "the CFL bound never went below the largest dt"
+    for i, ctx in enumerate(alone):
+        assert_bits_equal(batch.get_state(i).positions, ctx.get_state().positions, f"scene {i}")
