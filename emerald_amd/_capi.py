@@ -147,6 +147,16 @@ SIGNATURES = {
     "emsph_batch_frame_init": (_ST, [_P, _I]),
     "emsph_batch_sub_step": (_ST, [_P, _I, _P]),
     "emsph_batch_run_sub_steps": (_ST, [_P, _I, _I, _P]),
+    "emsph_pool_create": (_ST, [_SZ, _SZ, _I, _I, _P]),
+    "emsph_pool_destroy": (_ST, [_P]),
+    "emsph_pool_add": (_ST, [_P, _P, _P, _SZ, _P, _P, _SZ, _P]),
+    "emsph_pool_remove": (_ST, [_P, _I]),
+    "emsph_pool_commit": (_ST, [_P]),
+    "emsph_pool_live": (_I, [_P, _P, _I]),
+    "emsph_pool_set_cfl": (_ST, [_P, _F]),
+    "emsph_pool_sub_step": (_ST, [_P, _P]),
+    "emsph_pool_get_state": (_ST, [_P, _I, _P, _P]),
+    "emsph_device_bytes": (_I64, []),
     "emsph_search_create": (_ST, [_SZ, _P]),
     "emsph_search_destroy": (_ST, [_P]),
     "emsph_search_set_points": (_ST, [_P, _P, _SZ]),

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -111,6 +112,9 @@ Thresholds thresholds(emsph_parameters const& p) {
 
 uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 
+// Device bytes held by the library (all DBufs of all engines).
+std::atomic<int64_t> g_device_bytes{0};
+
 // A device allocation that frees itself.
 template <class T>
 struct DBuf {
@@ -121,7 +125,10 @@ struct DBuf {
     DBuf& operator=(DBuf const&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) { (void)hipFree(p); }
+        if (p) {
+            (void)hipFree(p);
+            g_device_bytes -= static_cast<int64_t>(n * sizeof(T));
+        }
         p = nullptr;
         n = 0;
     }
@@ -129,10 +136,19 @@ struct DBuf {
         release();
         if (count == 0) { count = 1; }
         hipError_t const e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
-        if (e == hipSuccess) { n = count; }
+        if (e == hipSuccess) {
+            n = count;
+            g_device_bytes += static_cast<int64_t>(n * sizeof(T));
+        } else {
+            p = nullptr;
+        }
         return e;
     }
     hipError_t ensure(size_t count) { return (p && n >= count) ? hipSuccess : alloc(count); }
+    void swap(DBuf& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
 };
 
 // Keys, sort and cells of one point set (fluid or solid).
@@ -153,6 +169,9 @@ struct SpatialIndex {
     uint32_t* sorted_vals = nullptr;
     int key_passes = 0;
     int scene_passes = 0;
+
+    // Keeps the arrays when they already hold c points.
+    hipError_t ensure(uint32_t c) { return (keys0.p && cap >= c) ? hipSuccess : alloc(c, false); }
 
     hipError_t alloc(uint32_t c, bool with_grid) {
         cap = c;
@@ -294,6 +313,32 @@ struct emsph_search {
     Engine e;
     float cell = 0.0f;
     bool ran = false;
+};
+
+// A scene of a pool: its parameters and static solids (kept on the host for
+// the solid rebuild of each commit) and, until the commit that uploads it, its
+// initial state.
+struct PoolScene {
+    int id = 0;
+    emsph_config cfg{};
+    std::vector<float> solids;
+    std::vector<float> pos, vel;
+    uint32_t n = 0;
+    uint32_t ns = 0;
+};
+
+// A batch whose scenes come and go (the served form).
+struct emsph_pool {
+    Engine e;
+    int method = 1;
+    uint32_t fluid_cap = 0;
+    uint32_t solid_cap = 0;
+    uint32_t max_scenes = 0;
+    float cfl = 0.0f;
+    int next_id = 1;
+    std::vector<PoolScene> live;     // in slot order
+    std::vector<PoolScene> adds;     // staged
+    std::vector<int> removes;        // staged
 };
 
 namespace {
@@ -592,7 +637,7 @@ emsph_status build_cell_set(Engine* g, SpatialIndex& si) {
         slots[s] = make_uint4(static_cast<uint32_t>(keys[c]), static_cast<uint32_t>(keys[c] >> 32),
                               tags[c], c);
     }
-    HIP_TRY(si.set.alloc(cap));
+    HIP_TRY(si.set.ensure(cap));
     HIP_TRY(hipMemcpyAsync(si.set.p, slots.data(), cap * sizeof(uint4), hipMemcpyHostToDevice,
                            g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
@@ -618,9 +663,9 @@ emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
     DBuf<float> sld;
     DBuf<V2> slv;
     hipError_t e;
-    if ((e = g->sidx.alloc(NS, false)) || (e = g->s_pos.alloc(NS)) || (e = g->s_vel.alloc(NS)) ||
-        (e = g->s_pos_orig.alloc(NS)) || (e = g->s_vol.alloc(NS)) ||
-        (e = g->s_scene_of.alloc(NS)) || (e = scnt.alloc(sstride)) || (e = sli.alloc(SL)) ||
+    if ((e = g->sidx.ensure(NS)) || (e = g->s_pos.ensure(NS)) || (e = g->s_vel.ensure(NS)) ||
+        (e = g->s_pos_orig.ensure(NS)) || (e = g->s_vol.ensure(NS)) ||
+        (e = g->s_scene_of.ensure(NS)) || (e = scnt.alloc(sstride)) || (e = sli.alloc(SL)) ||
         (e = sld.alloc(SL)) || (e = slv.alloc(SL))) {
         return fail(EMSPH_ERR_OUT_OF_MEMORY, "solid allocation");
     }
@@ -638,7 +683,7 @@ emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
         }
     }
     g->nstiles = static_cast<uint32_t>(tl.size());
-    HIP_TRY(g->stiles.alloc(tl.size()));
+    HIP_TRY(g->stiles.ensure(tl.size()));
     HIP_TRY(hipMemcpyAsync(g->stiles.p, tl.data(), tl.size() * sizeof(Tile),
                            hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipMemcpyAsync(g->s_scene_of.p, so.data(), NS * sizeof(uint32_t),
@@ -1119,6 +1164,130 @@ emsph_status engine_vmax2(Engine* g, float* out) {
     ST_TRY(read_stats(g));
     std::memcpy(out, &g->host_stats[0].vmax2_bits, 4);
     return EMSPH_OK;
+}
+
+// Totals the pool will hold after its staged adds and removes.
+void pool_projection(emsph_pool const* P, uint64_t& n, uint64_t& ns, uint64_t& m) {
+    n = ns = m = 0;
+    for (auto const& sc : P->live) {
+        if (std::find(P->removes.begin(), P->removes.end(), sc.id) != P->removes.end()) {
+            continue;
+        }
+        n += sc.n;
+        ns += sc.ns;
+        ++m;
+    }
+    for (auto const& sc : P->adds) {
+        n += sc.n;
+        ns += sc.ns;
+        ++m;
+    }
+}
+
+// Applies the staged adds and removes: survivors keep their order and are
+// repacked from slot 0 (state copied through the sorted-space scratch arrays,
+// which then swap roles with the state arrays), added scenes follow, and the
+// solids of the live scenes are re-indexed.
+emsph_status pool_commit(emsph_pool* P) {
+    Engine* g = &P->e;
+    if (P->adds.empty() && P->removes.empty()) { return EMSPH_OK; }
+    HIP_TRY(hipSetDevice(g->device));
+    std::vector<PoolScene> next;
+    std::vector<uint32_t> old_off;
+    {
+        uint32_t off = 0;
+        for (auto& sc : P->live) {
+            uint32_t const n = sc.n;
+            bool const gone =
+              std::find(P->removes.begin(), P->removes.end(), sc.id) != P->removes.end();
+            if (!gone) {
+                old_off.push_back(off);
+                next.push_back(std::move(sc));
+            }
+            off += n;
+        }
+    }
+    uint32_t off = 0;
+    hipStream_t const st = g->stream;
+    for (size_t k = 0; k < next.size(); ++k) {
+        uint32_t const n = next[k].n;
+        HIP_TRY(hipMemcpyAsync(g->spos.p + off, g->pos.p + old_off[k], n * sizeof(V2),
+                               hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->svel.p + off, g->vel.p + old_off[k], n * sizeof(V2),
+                               hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->kvs.p + off, g->kv.p + old_off[k], n * sizeof(float),
+                               hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->kds.p + off, g->kd.p + old_off[k], n * sizeof(float),
+                               hipMemcpyDeviceToDevice, st));
+        off += n;
+    }
+    for (auto& sc : P->adds) {
+        uint32_t const n = sc.n;
+        HIP_TRY(hipMemcpyAsync(g->spos.p + off, sc.pos.data(), n * sizeof(V2),
+                               hipMemcpyHostToDevice, st));
+        if (!sc.vel.empty()) {
+            HIP_TRY(hipMemcpyAsync(g->svel.p + off, sc.vel.data(), n * sizeof(V2),
+                                   hipMemcpyHostToDevice, st));
+        } else {
+            HIP_TRY(hipMemsetAsync(g->svel.p + off, 0, n * sizeof(V2), st));
+        }
+        HIP_TRY(hipMemsetAsync(g->kvs.p + off, 0, n * sizeof(float), st));
+        HIP_TRY(hipMemsetAsync(g->kds.p + off, 0, n * sizeof(float), st));
+        off += n;
+    }
+    // The host copies of the added states must outlive the copies.
+    HIP_TRY(hipStreamSynchronize(st));
+    for (auto& sc : P->adds) {
+        std::vector<float>().swap(sc.pos);
+        std::vector<float>().swap(sc.vel);
+        next.push_back(std::move(sc));
+    }
+    P->adds.clear();
+    P->removes.clear();
+    g->pos.swap(g->spos);
+    g->vel.swap(g->svel);
+    g->kv.swap(g->kvs);
+    g->kd.swap(g->kds);
+    P->live = std::move(next);
+
+    std::vector<SceneHost> hs(P->live.size());
+    std::vector<float const*> sx(P->live.size(), nullptr);
+    uint32_t o = 0;
+    for (size_t k = 0; k < P->live.size(); ++k) {
+        PoolScene const& sc = P->live[k];
+        hs[k].cfg = sc.cfg;
+        hs[k].cap = sc.n;
+        hs[k].n = sc.n;
+        hs[k].ns = sc.ns;
+        hs[k].offset = o;
+        hs[k].dt = sc.cfg.seconds_per_sub_step;
+        hs[k].cfl = P->cfl;
+        sx[k] = sc.solids.empty() ? nullptr : sc.solids.data();
+        o += sc.n;
+    }
+    g->sh = std::move(hs);
+    g->M = static_cast<uint32_t>(g->sh.size());
+    g->N = o;
+    ST_TRY(upload_layout(g));
+    ST_TRY(build_solids(g, sx));
+    ST_TRY(upload_layout(g));  // solid offsets are now known
+    g->passes_hint = -1;
+    g->have_lists = false;
+    if (g->N) {
+        Batch b = fluid_batch(g);
+        launch_step_begin(b, st);
+        launch_vmax2(b, st);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return EMSPH_OK;
+}
+
+int pool_index(emsph_pool const* P, int id) {
+    for (size_t k = 0; k < P->live.size(); ++k) {
+        if (P->live[k].id == id) { return static_cast<int>(k); }
+    }
+    return -1;
 }
 
 }  // namespace
@@ -1630,6 +1799,139 @@ emsph_status emsph_search_get(emsph_search* s, int32_t* grid_xy, uint64_t* keys,
         ST_TRY(engine_get_neighborhood(g, 0, 0, counts, indices, distances, vectors_xy));
     }
     return EMSPH_OK;
+}
+
+//------------------------------------------------------------------------------
+// Pools (the served form).
+int64_t emsph_device_bytes(void) { return g_device_bytes.load(); }
+
+emsph_status emsph_pool_create(size_t fluid_capacity, size_t solid_capacity, int max_scenes,
+                               int method, emsph_pool** out) {
+    if (!out || fluid_capacity == 0 || max_scenes <= 0 || fluid_capacity > (1ull << 30) ||
+        solid_capacity > (1ull << 30)) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument");
+    }
+    ST_TRY(check_method(method));
+    auto* P = new emsph_pool();
+    P->method = method;
+    P->fluid_cap = static_cast<uint32_t>(fluid_capacity);
+    P->solid_cap = static_cast<uint32_t>(solid_capacity);
+    P->max_scenes = static_cast<uint32_t>(max_scenes);
+    Engine* g = &P->e;
+    emsph_status st = engine_init(g);
+    if (st) {
+        delete P;
+        return st;
+    }
+    hipError_t e = alloc_fluid(g, P->fluid_cap);
+    if (!e && solid_capacity) {
+        uint32_t const NS = P->solid_cap;
+        if (!e) { e = g->sidx.ensure(NS); }
+        if (!e) { e = g->s_pos.ensure(NS); }
+        if (!e) { e = g->s_vel.ensure(NS); }
+        if (!e) { e = g->s_pos_orig.ensure(NS); }
+        if (!e) { e = g->s_vol.ensure(NS); }
+        if (!e) { e = g->s_scene_of.ensure(NS); }
+    }
+    if (e) {
+        delete P;
+        return fail(e == hipErrorOutOfMemory ? EMSPH_ERR_OUT_OF_MEMORY : EMSPH_ERR_HIP,
+                    std::string("pool allocation: ") + hipGetErrorString(e));
+    }
+    g->M = 0;
+    g->N = 0;
+    st = upload_layout(g);
+    if (st) {
+        delete P;
+        return st;
+    }
+    *out = P;
+    return EMSPH_OK;
+}
+
+emsph_status emsph_pool_destroy(emsph_pool* P) {
+    if (P) {
+        (void)hipStreamSynchronize(P->e.stream);
+        delete P;
+    }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_pool_add(emsph_pool* P, const emsph_config* config, const float* solid_xy,
+                            size_t solid_count, const float* pos_xy, const float* vel_xy,
+                            size_t count, int* id) {
+    if (!P || !config || !pos_xy || count == 0 || (solid_count && !solid_xy)) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument");
+    }
+    uint64_t n, ns, m;
+    pool_projection(P, n, ns, m);
+    if (n + count > P->fluid_cap || ns + solid_count > P->solid_cap || m + 1 > P->max_scenes) {
+        return fail(EMSPH_ERR_CAPACITY, "pool capacity");
+    }
+    PoolScene sc;
+    sc.id = P->next_id++;
+    sc.cfg = *config;
+    sc.n = static_cast<uint32_t>(count);
+    sc.ns = static_cast<uint32_t>(solid_count);
+    sc.solids.assign(solid_xy, solid_xy + 2 * solid_count);
+    sc.pos.assign(pos_xy, pos_xy + 2 * count);
+    if (vel_xy) { sc.vel.assign(vel_xy, vel_xy + 2 * count); }
+    if (id) { *id = sc.id; }
+    P->adds.push_back(std::move(sc));
+    return EMSPH_OK;
+}
+
+emsph_status emsph_pool_remove(emsph_pool* P, int id) {
+    if (!P) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null pool"); }
+    for (size_t k = 0; k < P->adds.size(); ++k) {
+        if (P->adds[k].id == id) {
+            P->adds.erase(P->adds.begin() + static_cast<std::ptrdiff_t>(k));
+            return EMSPH_OK;
+        }
+    }
+    if (pool_index(P, id) < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no such scene"); }
+    if (std::find(P->removes.begin(), P->removes.end(), id) == P->removes.end()) {
+        P->removes.push_back(id);
+    }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_pool_commit(emsph_pool* P) {
+    if (!P) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null pool"); }
+    return pool_commit(P);
+}
+
+int emsph_pool_live(const emsph_pool* P, int* ids, int max) {
+    if (!P) { return 0; }
+    int const k = static_cast<int>(P->live.size());
+    for (int i = 0; ids && i < k && i < max; ++i) { ids[i] = P->live[i].id; }
+    return k;
+}
+
+emsph_status emsph_pool_set_cfl(emsph_pool* P, float factor) {
+    if (!P) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null pool"); }
+    if (!(factor >= 0.0f)) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "CFL factor < 0"); }
+    P->cfl = factor;
+    return engine_set_cfl(&P->e, factor);
+}
+
+emsph_status emsph_pool_sub_step(emsph_pool* P, emsph_step_stats* stats) {
+    if (!P) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null pool"); }
+    Engine* g = &P->e;
+    if (!g->M) { return EMSPH_OK; }
+    ST_TRY(engine_sub_step(g, P->method));
+    if (stats) {
+        ST_TRY(read_stats(g));
+        for (uint32_t s = 0; s < g->M; ++s) { to_public(g->host_stats[s], g, &stats[s]); }
+    }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_pool_get_state(emsph_pool* P, int id, float* pos_xy, float* vel_xy) {
+    if (!P) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null pool"); }
+    int const k = pool_index(P, id);
+    if (k < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no such live scene"); }
+    return engine_get_state(&P->e, static_cast<uint32_t>(k), pos_xy, vel_xy);
 }
 
 }  // extern "C"

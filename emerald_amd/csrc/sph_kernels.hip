@@ -1398,19 +1398,35 @@ __global__ __launch_bounds__(TILE) void k_pci_finalize(Batch a) {
     block_energy<TILE>(ke, pe, a.tile_energy + r.tile);
 }
 
-// Per-scene energies: the scene's tile partials summed in tile order.
+// Per-scene energies from the scene's tile partials: one workgroup per scene;
+// thread i sums tiles i, i + 256, ... of the scene in order, then a fixed
+// tree. The order depends only on the scene's own tiles, so the result is the
+// same alone and in any batch.
 __global__ __launch_bounds__(256) void k_scene_energy(Batch a) {
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < a.nscenes;
-         s += gridDim.x * blockDim.x) {
+    for (uint32_t s = blockIdx.x; s < a.nscenes; s += gridDim.x) {
         SceneDev const& sd = a.scenes[s];
         double ke = 0.0, pe = 0.0;
-        for (uint32_t k = 0; k < sd.ntiles; ++k) {
+        for (uint32_t k = threadIdx.x; k < sd.ntiles; k += blockDim.x) {
             double2 const e = a.tile_energy[sd.first_tile + k];
             ke += e.x;
             pe += e.y;
         }
-        a.stats[s].kinetic_energy = ke;
-        a.stats[s].potential_energy = pe;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            ke += __shfl_xor(ke, off);
+            pe += __shfl_xor(pe, off);
+        }
+        __shared__ double ske[4], spe[4];
+        if (__lane_id() == 0) {
+            ske[threadIdx.x / 64] = ke;
+            spe[threadIdx.x / 64] = pe;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            a.stats[s].kinetic_energy = (ske[0] + ske[1]) + (ske[2] + ske[3]);
+            a.stats[s].potential_energy = (spe[0] + spe[1]) + (spe[2] + spe[3]);
+        }
+        __syncthreads();
     }
 }
 
@@ -1514,7 +1530,7 @@ void launch_pci_finalize(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_finalize, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_scene_energy(Batch const& a, hipStream_t s) {
-    uint32_t const blocks = (a.nscenes + 255) / 256;
+    uint32_t const blocks = a.nscenes < 4096 ? a.nscenes : 4096;
     hipLaunchKernelGGL(k_scene_energy, dim3(blocks ? blocks : 1), dim3(256), 0, s, a);
 }
 void launch_vmax2(Batch const& a, hipStream_t s) {

@@ -434,3 +434,77 @@ class Batch:
         check(lib().emsph_batch_run_sub_steps(self._h, _capi.METHODS[method], int(n),
                                               ctypes.byref(ms)), "batch_run_sub_steps")
         return float(ms.value)
+
+
+class Pool:
+    """Scenes that come and go, stepped together (emsph_pool): one solver per
+    pool; adds and removes are staged and applied by commit() between
+    sub-steps. Capacity (the HBM budget) is allocated at creation."""
+
+    def __init__(self, fluid_capacity: int, solid_capacity: int, max_scenes: int, method: str):
+        h = ctypes.c_void_p()
+        before = lib().emsph_device_bytes()
+        check(lib().emsph_pool_create(int(fluid_capacity), int(solid_capacity), int(max_scenes),
+                                      _capi.METHODS[method], ctypes.byref(h)), "emsph_pool_create")
+        self._h = h
+        self.method = method
+        self.device_bytes = lib().emsph_device_bytes() - before
+        self.counts = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().emsph_pool_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, config: Simulation_config, solid_state: Solid_state, state: State) -> int:
+        """Stage a scene; returns its id. Raises EmsphError (EMSPH_ERR_CAPACITY)
+        when it does not fit."""
+        c = config.to_c()
+        sol = np.ascontiguousarray(solid_state.positions, np.float32).reshape(-1, 2)
+        pos = np.ascontiguousarray(state.positions, np.float32).reshape(-1, 2)
+        vel = None if state.velocities is None else np.ascontiguousarray(
+            state.velocities, np.float32).reshape(-1, 2)
+        sid = ctypes.c_int()
+        check(lib().emsph_pool_add(self._h, ctypes.byref(c), _ptr(sol), len(sol), _ptr(pos),
+                                   _ptr(vel), len(pos), ctypes.byref(sid)), "pool_add")
+        self.counts[sid.value] = len(pos)
+        return sid.value
+
+    def remove(self, sid: int) -> None:
+        check(lib().emsph_pool_remove(self._h, int(sid)), "pool_remove")
+
+    def commit(self) -> None:
+        check(lib().emsph_pool_commit(self._h), "pool_commit")
+
+    def live(self) -> list:
+        n = lib().emsph_pool_live(self._h, None, 0)
+        ids = (ctypes.c_int * max(n, 1))()
+        lib().emsph_pool_live(self._h, ids, n)
+        return list(ids[:n])
+
+    def set_cfl(self, factor: float) -> None:
+        check(lib().emsph_pool_set_cfl(self._h, float(factor)), "pool_set_cfl")
+
+    def sub_step(self, with_stats: bool = True):
+        """One sub-step of every live scene; stats in live() order."""
+        if not with_stats:
+            check(lib().emsph_pool_sub_step(self._h, None), "pool_sub_step")
+            return None
+        n = lib().emsph_pool_live(self._h, None, 0)
+        st = (_capi.StepStats * max(n, 1))()
+        check(lib().emsph_pool_sub_step(self._h, st), "pool_sub_step")
+        return list(st[:n])
+
+    def get_state(self, sid: int) -> State:
+        n = self.counts[sid]
+        pos = np.empty((n, 2), np.float32)
+        vel = np.empty((n, 2), np.float32)
+        check(lib().emsph_pool_get_state(self._h, int(sid), _ptr(pos), _ptr(vel)), "pool_get_state")
+        return State(pos, vel)
+

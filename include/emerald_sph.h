@@ -243,6 +243,39 @@ emsph_status emsph_batch_sub_step(emsph_batch* b, int method, emsph_step_stats* 
 emsph_status emsph_batch_run_sub_steps(emsph_batch* b, int method, int n_sub_steps,
                                        float* elapsed_ms);
 
+/* Pools: a batch whose scenes come and go, for serving. Device memory for
+ * fluid_capacity fluid and solid_capacity solid particles is allocated at
+ * creation (the admission budget); one solver (method) steps every live scene
+ * per sub-step, each with its own parameters, solids and dt (its
+ * config.seconds_per_sub_step, or the CFL dt of emsph_pool_set_cfl). Adds and
+ * removes are staged and take effect at emsph_pool_commit, between sub-steps;
+ * emsph_pool_add fails with EMSPH_ERR_CAPACITY when the pool would exceed its
+ * capacity after the staged changes. A scene's results do not depend on the
+ * other live scenes: they are bit-identical to the scene stepped alone. Scene
+ * ids are positive and never reused. One pool has one HIP stream and is not
+ * re-entrant; distinct pools may run from distinct threads and overlap on the
+ * device. No counterpart in the reference. */
+typedef struct emsph_pool emsph_pool;
+emsph_status emsph_pool_create(size_t fluid_capacity, size_t solid_capacity, int max_scenes,
+                               int method, emsph_pool** out);
+emsph_status emsph_pool_destroy(emsph_pool* pool);
+/* Stage a scene (state copied now; vel may be NULL). *id receives its id. */
+emsph_status emsph_pool_add(emsph_pool* pool, const emsph_config* config, const float* solid_xy,
+                            size_t solid_count, const float* pos_xy, const float* vel_xy,
+                            size_t count, int* id);
+/* Stage the removal of a live scene (or drop a staged add). */
+emsph_status emsph_pool_remove(emsph_pool* pool, int id);
+emsph_status emsph_pool_commit(emsph_pool* pool);
+/* Live scene ids in slot order (the order of emsph_pool_sub_step's stats);
+ * returns the live count. ids may be NULL. */
+int emsph_pool_live(const emsph_pool* pool, int* ids, int max);
+emsph_status emsph_pool_set_cfl(emsph_pool* pool, float factor);
+/* One sub-step of every live scene; stats: one per live scene (may be NULL). */
+emsph_status emsph_pool_sub_step(emsph_pool* pool, emsph_step_stats* stats);
+emsph_status emsph_pool_get_state(emsph_pool* pool, int id, float* pos_xy, float* vel_xy);
+/* Device bytes currently held by the library (all contexts, batches, pools). */
+int64_t emsph_device_bytes(void);
+
 /* Stand-alone z-index sort and neighbour search on one point set (no solids,
  * no dynamics): the reference's compute_all_neighbhorhoods fluid part. */
 typedef struct emsph_search emsph_search;
