@@ -169,6 +169,48 @@ def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict
     }
 
 
+def served_form(budget_gb: float, window: float, rate: float = 200.0,
+                plain_steps: int = 2) -> dict:
+    """Poisson stream of dam-break requests (200/s, 5k-500k particles, IISPH or
+    DFSPH_P, 100-2000 steps) served from two device-resident pools (one per
+    solver) with continuous batching under an HBM budget. After the window,
+    the scenes still live are also stepped as plain per-scene contexts (one at
+    a time) for the throughput comparison."""
+    from emerald_amd import serve
+    from emerald_amd import sph2d_box as sb
+    reqs = serve.poisson_requests(rate, window + 5.0)
+    res = serve.serve(reqs, budget_gb * 1e9, window, log=log)
+    loops = res["_loops"]
+    # Plain per-scene contexts on the pools' final live sets.
+    log("served form: plain per-scene comparison on the final live sets")
+    n_steps = 0
+    t_plain = 0.0
+    scenes_done = 0
+    for m, lp in loops.items():
+        live = list(lp.live.items())
+        states = {sid: lp.pool.get_state(sid) for sid, _ in live}
+        lp.pool.close()
+        for sid, v in live:
+            req = v[0]
+            sc, cfg = serve.request_scene(req)
+            c = sb.Context(cfg, sb.Solid_state(sc.solids), len(sc.positions))
+            c.set_cfl(0.4)
+            c.set_state(states[sid])
+            c.frame_init(m)
+            t0 = time.perf_counter()
+            c.run_sub_steps(m, cfg.seconds_per_sub_step, plain_steps)
+            t_plain += time.perf_counter() - t0
+            n_steps += len(sc.positions) * plain_steps
+            scenes_done += 1
+            c.close()
+    serve.close(res)
+    res["plain_value_final_live_set"] = n_steps / t_plain if t_plain > 0 else None
+    res["plain_scenes"] = scenes_done
+    res["speedup_vs_plain"] = (res["value"] / res["plain_value_final_live_set"]
+                               if res["plain_value_final_live_set"] else None)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +222,10 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-only", action="store_true")
+    ap.add_argument("--served-window", type=float, default=10.0)
+    ap.add_argument("--served-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are "
@@ -195,6 +241,9 @@ def main() -> int:
     method = args.method
     if args.batched_only:
         print(json.dumps(batched_form(method, args.batched_steps, 2)), flush=True)
+        return 0
+    if args.served_only:
+        print(json.dumps(served_form(args.served_budget_gb, args.served_window)), flush=True)
         return 0
     log("building scene (1024 x 1024 lattice dam break)")
     scene = scenes.dam_break_1m()
@@ -284,6 +333,10 @@ def main() -> int:
     if not args.no_batched:
         ctx.close()
         batched = batched_form(method, args.batched_steps, 2)
+    served = None
+    if not args.no_served:
+        ctx.close()
+        served = served_form(args.served_budget_gb, args.served_window)
 
     out = {
         "metric": "IISPH particle-steps/s @1M (1 GPU)" if method == "iisph"
@@ -314,6 +367,7 @@ def main() -> int:
         "whole_step": whole_step,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

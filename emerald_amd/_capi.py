@@ -119,6 +119,8 @@ SIGNATURES = {
     "emsph_emit_solid_box": (_SZ, [_F, _F, _F, _F, _F, _SZ, _P]),
     "emsph_world_walls_solid_positions": (_SZ, [_P, _I, _P, _SZ]),
     "emsph_dam_break_positions": (_SZ, [_P, _P, _SZ, _P, _SZ]),
+    "emsph_lattice_positions": (_SZ, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                      ctypes.c_uint64, ctypes.c_double, _P]),
     "emsph_create": (_ST, [_P, _P, _SZ, _SZ, _P]),
     "emsph_destroy": (_ST, [_P]),
     "emsph_set_store_geometry": (_ST, [_P, _I]),

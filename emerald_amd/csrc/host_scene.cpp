@@ -299,4 +299,37 @@ size_t emsph_dam_break_positions(const emsph_parameters* params, const float* so
     return kept;
 }
 
+// Square lattice block of nx * ny particles at ((i + 1/2) s, (j + 1/2) s),
+// each coordinate jittered by U(-1, 1) * jitter * s with u from SplitMix64
+// (state seed + k * 0x9E3779B97F4A7C15 for draw k = 1, 2, ...; u = (x >> 40)
+// * 2^-24; particle p = j * nx + i takes draws 2p + 1 (x) and 2p + 2 (y)),
+// evaluated in double and rounded to float: the synthetic dam-break input of
+// the benchmark workloads. No counterpart in the reference (its own dam break
+// is dam_break_initial_state, a hex block).
+size_t emsph_lattice_positions(uint32_t nx, uint32_t ny, double spacing, uint64_t seed,
+                               double jitter, float* out_xy) {
+    size_t const n = static_cast<size_t>(nx) * ny;
+    if (!out_xy) { return n; }
+    double const js = jitter * spacing;
+    auto draw = [seed](uint64_t k) {
+        uint64_t z = seed + k * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        return static_cast<double>(z >> 40) * (1.0 / 16777216.0);
+    };
+    for (uint32_t j = 0; j < ny; ++j) {
+        for (uint32_t i = 0; i < nx; ++i) {
+            size_t const p = static_cast<size_t>(j) * nx + i;
+            double const ux = draw(2 * p + 1);
+            double const uy = draw(2 * p + 2);
+            double const bx = (static_cast<double>(i) + 0.5) * spacing;
+            double const by = (static_cast<double>(j) + 0.5) * spacing;
+            out_xy[2 * p] = static_cast<float>(bx + (2.0 * ux - 1.0) * js);
+            out_xy[2 * p + 1] = static_cast<float>(by + (2.0 * uy - 1.0) * js);
+        }
+    }
+    return n;
+}
+
 }  // extern "C"

@@ -76,6 +76,19 @@ def lattice_mass(spacing: float, h: float, rho0: float) -> np.float32:
     return np.float32(rho0 / s)
 
 
+def lattice_positions(nx: int, ny: int, spacing: float, seed: int, jitter: float) -> np.ndarray:
+    """Square lattice at ((i + 1/2) s, (j + 1/2) s), particle p = j * nx + i,
+    jittered by (2u - 1) * jitter * s with u = uniform01(seed, 2 nx ny)[2p],
+    [2p + 1] (evaluated in FP64, rounded to FP32; the C ABI's
+    emsph_lattice_positions)."""
+    import ctypes
+    from ._capi import lib
+    out = np.empty((nx * ny, 2), np.float32)
+    lib().emsph_lattice_positions(nx, ny, float(spacing), int(seed) & MASK64, float(jitter),
+                                  out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
 @dataclasses.dataclass
 class DamBreak:
     positions: np.ndarray
@@ -97,10 +110,7 @@ def dam_break(nx: int, ny: int, spacing: float, dt: float, seed: int = 1,
     # of two lattice spacings is h = s. (h = 2s makes the reference's IISPH
     # relaxed-Jacobi solve diverge on this wall-bounded lattice.)
     h = s
-    u = uniform01(seed, 2 * nx * ny).reshape(ny * nx, 2)
-    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
-    base = np.stack([(ii.ravel() + 0.5) * s, (jj.ravel() + 0.5) * s], axis=1)
-    pos = (base + (2.0 * u - 1.0) * (jitter * s)).astype(np.float32)
+    pos = lattice_positions(nx, ny, s, seed, jitter)
     vel = np.zeros_like(pos)
     solids = np.zeros((0, 2), np.float32)
     if with_walls:

@@ -158,6 +158,11 @@ size_t emsph_world_walls_solid_positions(const emsph_parameters* params, int ran
  * NULL to query the count. Returns the count. */
 size_t emsph_dam_break_positions(const emsph_parameters* params, const float* solid_xy,
                                  size_t solid_count, float* out_xy, size_t capacity);
+/* Jittered square lattice block of nx * ny particles (the benchmark's dam-break
+ * input; SplitMix64 draws, see host_scene.cpp). out_xy may be NULL to query the
+ * count. Returns nx * ny. No counterpart in the reference. */
+size_t emsph_lattice_positions(uint32_t nx, uint32_t ny, double spacing, uint64_t seed,
+                               double jitter, float* out_xy);
 
 /* Create a context on the current HIP device with static solids (their
  * neighbourhoods and volumes are computed on the device, solids.cpp:76-137)

@@ -78,3 +78,18 @@ def test_world_walls_counts():
     w2 = sb.world_walls_initial_solid_state(p, random_obstacles=True).positions
     assert 0 < len(w) < len(w2)
     assert np.array_equal(w2[:len(w)], w)
+
+
+def test_lattice_positions_match_restatement():
+    # The C generator against a numpy restatement of the same SplitMix64
+    # stream and FP64 expression.
+    from emerald_amd import scenes
+    for nx, ny, s, seed, jit in [(80, 160, 0.005, 1, 1e-3), (33, 17, 0.004, 9, 1e-3),
+                                 (7, 5, 0.1, 2 ** 63 + 5, 0.25)]:
+        u = scenes.uniform01(seed, 2 * nx * ny).reshape(ny * nx, 2)
+        jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        base = np.stack([(ii.ravel() + 0.5) * s, (jj.ravel() + 0.5) * s], axis=1)
+        want = (base + (2.0 * u - 1.0) * (jit * s)).astype(np.float32)
+        got = scenes.lattice_positions(nx, ny, s, seed, jit)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (nx, ny)
+
