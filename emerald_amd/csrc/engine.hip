@@ -197,6 +197,20 @@ struct SpatialIndex {
 
 // Event pairs recorded around each timed launch, resolved only when the pool
 // is full or the totals are read, so timing adds no host synchronisation.
+// Solid-solid lists of the solid build (for the volumes only).
+struct SolidScratch {
+    DBuf<uint8_t> count;
+    DBuf<uint32_t> idx;
+    DBuf<float> dist;
+    DBuf<V2> vec;
+    void release() {
+        count.release();
+        idx.release();
+        dist.release();
+        vec.release();
+    }
+};
+
 struct KernelTimer {
     static constexpr size_t POOL = 8192;
     bool enabled = false;
@@ -271,6 +285,9 @@ struct Engine {
     DBuf<V2> fl_gw, sl_gw, fl_vec, sl_vec;
     DBuf<float> fl_dist, sl_dist;
     DBuf<uint2> swin;       // [fluid cell][9] solid slot ranges
+    SolidScratch solid_scratch;
+    bool keep_solid_scratch = false;
+    DBuf<uint3> repack_segments;
     DBuf<uint2> overflow;   // particles deferred by the capped neighbour build
     DBuf<double2> tile_energy;
     int build_cap = 32;     // LDS list capacity of the next build (from the last max count)
@@ -451,7 +468,6 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
 emsph_status upload_layout(Engine* g) {
     std::vector<SceneDev> sd(g->M);
     std::vector<Tile> tl;
-    std::vector<uint32_t> so(g->cap, 0);
     for (uint32_t s = 0; s < g->M; ++s) {
         SceneHost const& h = g->sh[s];
         sd[s].c = step_consts(h.cfg, h.dt);
@@ -467,7 +483,6 @@ emsph_status upload_layout(Engine* g) {
         for (uint32_t k = 0; k < h.n; k += TILE) {
             tl.push_back(Tile{s, h.offset + k, std::min<uint32_t>(TILE, h.n - k), 0});
         }
-        for (uint32_t k = 0; k < h.n; ++k) { so[h.offset + k] = s; }
     }
     g->ntiles = static_cast<uint32_t>(tl.size());
     HIP_TRY(g->scenes.ensure(std::max<uint32_t>(g->M, 1)));
@@ -482,10 +497,8 @@ emsph_status upload_layout(Engine* g) {
         HIP_TRY(hipMemcpyAsync(g->tiles.p, tl.data(), tl.size() * sizeof(Tile),
                                hipMemcpyHostToDevice, g->stream));
     }
-    if (g->cap) {
-        HIP_TRY(hipMemcpyAsync(g->scene_of.p, so.data(), so.size() * sizeof(uint32_t),
-                               hipMemcpyHostToDevice, g->stream));
-    }
+    launch_scene_of(g->tiles.p, g->ntiles, g->scene_of.p, g->stream);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->host_stats.assign(g->M, SceneStats{});
     // N is the packed extent: every slot up to the last scene's end. Scenes
@@ -658,30 +671,32 @@ emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
     g->scap = NS;
     uint32_t const sstride = round_up(NS, 256);
     size_t const SL = static_cast<size_t>(MAXN) * sstride;
-    DBuf<uint8_t> scnt;
-    DBuf<uint32_t> sli;
-    DBuf<float> sld;
-    DBuf<V2> slv;
+    // Solid-solid lists, only for the volumes: scratch that pools keep between
+    // commits (grow-only) and other engines release after the build.
+    SolidScratch& sc = g->solid_scratch;
     hipError_t e;
     if ((e = g->sidx.ensure(NS)) || (e = g->s_pos.ensure(NS)) || (e = g->s_vel.ensure(NS)) ||
         (e = g->s_pos_orig.ensure(NS)) || (e = g->s_vol.ensure(NS)) ||
-        (e = g->s_scene_of.ensure(NS)) || (e = scnt.alloc(sstride)) || (e = sli.alloc(SL)) ||
-        (e = sld.alloc(SL)) || (e = slv.alloc(SL))) {
+        (e = g->s_scene_of.ensure(NS)) || (e = sc.count.ensure(sstride)) ||
+        (e = sc.idx.ensure(SL)) || (e = sc.dist.ensure(SL)) || (e = sc.vec.ensure(SL))) {
         return fail(EMSPH_ERR_OUT_OF_MEMORY, "solid allocation");
     }
     std::vector<uint32_t> so(NS);
+    std::vector<float> staged(2 * static_cast<size_t>(NS));
     std::vector<Tile> tl;
     for (uint32_t s = 0; s < g->M; ++s) {
         SceneHost const& h = g->sh[s];
         if (h.ns) {
-            HIP_TRY(hipMemcpyAsync(g->s_pos_orig.p + h.soffset, sxy[s], h.ns * sizeof(V2),
-                                   hipMemcpyHostToDevice, g->stream));
+            std::memcpy(staged.data() + 2 * static_cast<size_t>(h.soffset), sxy[s],
+                        h.ns * sizeof(V2));
         }
         for (uint32_t k = 0; k < h.ns; ++k) { so[h.soffset + k] = s; }
         for (uint32_t k = 0; k < h.ns; k += TILE) {
             tl.push_back(Tile{s, h.soffset + k, std::min<uint32_t>(TILE, h.ns - k), 0});
         }
     }
+    HIP_TRY(hipMemcpyAsync(g->s_pos_orig.p, staged.data(), NS * sizeof(V2),
+                           hipMemcpyHostToDevice, g->stream));
     g->nstiles = static_cast<uint32_t>(tl.size());
     HIP_TRY(g->stiles.ensure(tl.size()));
     HIP_TRY(hipMemcpyAsync(g->stiles.p, tl.data(), tl.size() * sizeof(Tile),
@@ -705,7 +720,7 @@ emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
     b.cell_of = g->sidx.block_index.p;
     b.win = g->sidx.win.p;
     b.keystate = g->keystate.p;
-    b.fl = Lists{scnt.p, sli.p, nullptr, sld.p, slv.p};
+    b.fl = Lists{sc.count.p, sc.idx.p, nullptr, sc.dist.p, sc.vec.p};
     b.sl = Lists{};  // solid-solid lists only
     b.rho = g->s_vol.p;
     b.store_geometry = 1;
@@ -713,6 +728,7 @@ emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
     launch_solid_volumes(b, NS, g->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(g->stream));
+    if (!g->keep_solid_scratch) { sc.release(); }
     return EMSPH_OK;
 }
 
@@ -1184,55 +1200,61 @@ void pool_projection(emsph_pool const* P, uint64_t& n, uint64_t& ns, uint64_t& m
     }
 }
 
-// Applies the staged adds and removes: survivors keep their order and are
-// repacked from slot 0 (state copied through the sorted-space scratch arrays,
-// which then swap roles with the state arrays), added scenes follow, and the
-// solids of the live scenes are re-indexed.
+// Applies the staged adds and removes: survivors keep their order; those after
+// the first removed scene move down over it (gathered through the
+// sorted-space scratch arrays in one kernel), added scenes are appended, and
+// the solids of the live scenes are re-indexed.
 emsph_status pool_commit(emsph_pool* P) {
     Engine* g = &P->e;
     if (P->adds.empty() && P->removes.empty()) { return EMSPH_OK; }
     HIP_TRY(hipSetDevice(g->device));
-    std::vector<PoolScene> next;
-    std::vector<uint32_t> old_off;
-    {
-        uint32_t off = 0;
-        for (auto& sc : P->live) {
-            uint32_t const n = sc.n;
-            bool const gone =
-              std::find(P->removes.begin(), P->removes.end(), sc.id) != P->removes.end();
-            if (!gone) {
-                old_off.push_back(off);
-                next.push_back(std::move(sc));
-            }
-            off += n;
-        }
-    }
-    uint32_t off = 0;
     hipStream_t const st = g->stream;
-    for (size_t k = 0; k < next.size(); ++k) {
-        uint32_t const n = next[k].n;
-        HIP_TRY(hipMemcpyAsync(g->spos.p + off, g->pos.p + old_off[k], n * sizeof(V2),
-                               hipMemcpyDeviceToDevice, st));
-        HIP_TRY(hipMemcpyAsync(g->svel.p + off, g->vel.p + old_off[k], n * sizeof(V2),
-                               hipMemcpyDeviceToDevice, st));
-        HIP_TRY(hipMemcpyAsync(g->kvs.p + off, g->kv.p + old_off[k], n * sizeof(float),
-                               hipMemcpyDeviceToDevice, st));
-        HIP_TRY(hipMemcpyAsync(g->kds.p + off, g->kd.p + old_off[k], n * sizeof(float),
-                               hipMemcpyDeviceToDevice, st));
-        off += n;
+    std::vector<PoolScene> next;
+    std::vector<uint3> seg;  // {src, dst, n} of survivors that move
+    uint32_t src = 0, dst = 0;
+    for (auto& sc : P->live) {
+        uint32_t const n = sc.n;
+        bool const gone =
+          std::find(P->removes.begin(), P->removes.end(), sc.id) != P->removes.end();
+        if (!gone) {
+            if (src != dst) { seg.push_back(make_uint3(src, dst, n)); }
+            dst += n;
+            next.push_back(std::move(sc));
+        }
+        src += n;
     }
+    if (!seg.empty()) {
+        uint32_t const first = seg[0].y;
+        uint32_t const total = dst - first;
+        HIP_TRY(g->repack_segments.ensure(seg.size()));
+        HIP_TRY(hipMemcpyAsync(g->repack_segments.p, seg.data(), seg.size() * sizeof(uint3),
+                               hipMemcpyHostToDevice, st));
+        launch_repack(g->repack_segments.p, static_cast<uint32_t>(seg.size()), total, g->pos.p,
+                      g->vel.p, g->kv.p, g->kd.p, g->spos.p, g->svel.p, g->kvs.p, g->kds.p, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(g->pos.p + first, g->spos.p + first, total * sizeof(V2),
+                               hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->vel.p + first, g->svel.p + first, total * sizeof(V2),
+                               hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->kv.p + first, g->kvs.p + first, total * sizeof(float),
+                               hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->kd.p + first, g->kds.p + first, total * sizeof(float),
+                               hipMemcpyDeviceToDevice, st));
+        // The segment table is read by the kernel before the next commit rewrites it.
+    }
+    uint32_t off = dst;
     for (auto& sc : P->adds) {
         uint32_t const n = sc.n;
-        HIP_TRY(hipMemcpyAsync(g->spos.p + off, sc.pos.data(), n * sizeof(V2),
+        HIP_TRY(hipMemcpyAsync(g->pos.p + off, sc.pos.data(), n * sizeof(V2),
                                hipMemcpyHostToDevice, st));
         if (!sc.vel.empty()) {
-            HIP_TRY(hipMemcpyAsync(g->svel.p + off, sc.vel.data(), n * sizeof(V2),
+            HIP_TRY(hipMemcpyAsync(g->vel.p + off, sc.vel.data(), n * sizeof(V2),
                                    hipMemcpyHostToDevice, st));
         } else {
-            HIP_TRY(hipMemsetAsync(g->svel.p + off, 0, n * sizeof(V2), st));
+            HIP_TRY(hipMemsetAsync(g->vel.p + off, 0, n * sizeof(V2), st));
         }
-        HIP_TRY(hipMemsetAsync(g->kvs.p + off, 0, n * sizeof(float), st));
-        HIP_TRY(hipMemsetAsync(g->kds.p + off, 0, n * sizeof(float), st));
+        HIP_TRY(hipMemsetAsync(g->kv.p + off, 0, n * sizeof(float), st));
+        HIP_TRY(hipMemsetAsync(g->kd.p + off, 0, n * sizeof(float), st));
         off += n;
     }
     // The host copies of the added states must outlive the copies.
@@ -1244,10 +1266,6 @@ emsph_status pool_commit(emsph_pool* P) {
     }
     P->adds.clear();
     P->removes.clear();
-    g->pos.swap(g->spos);
-    g->vel.swap(g->svel);
-    g->kv.swap(g->kvs);
-    g->kd.swap(g->kds);
     P->live = std::move(next);
 
     std::vector<SceneHost> hs(P->live.size());
@@ -1840,6 +1858,7 @@ emsph_status emsph_pool_create(size_t fluid_capacity, size_t solid_capacity, int
     }
     g->M = 0;
     g->N = 0;
+    g->keep_solid_scratch = true;
     st = upload_layout(g);
     if (st) {
         delete P;

@@ -278,6 +278,13 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_step_begin(Batch const& b, hipStream_t s);
 // Per-scene energy sums over the tile partials written by the final kernel.
 void launch_scene_energy(Batch const& b, hipStream_t s);
+// scene_of of every slot covered by the tiles.
+void launch_scene_of(Tile const* tiles, uint32_t ntiles, uint32_t* scene_of, hipStream_t s);
+// Pool repacking: segments {src, dst, n} of the state arrays into the scratch
+// arrays, total slots from seg[0].dst on.
+void launch_repack(uint3 const* seg, uint32_t nseg, uint32_t total, V2 const* pos, V2 const* vel,
+                   float const* kv, float const* kd, V2* pos_o, V2* vel_o, float* kv_o,
+                   float* kd_o, hipStream_t s);
 // Start a solver loop: every non-empty scene active; KeyState::active_count set.
 void launch_loop_begin(Batch const& b, hipStream_t s);
 // After iteration `it` of loop `kind`: each active scene applies the

@@ -1529,6 +1529,55 @@ void launch_pci_force(Batch const& a, hipStream_t s) {
 void launch_pci_finalize(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_finalize, tile_grid(a), dim3(TILE), 0, s, a);
 }
+// scene_of[i] = scene of slot i, from the tile table (one workgroup per tile).
+__global__ __launch_bounds__(TILE) void k_scene_of(Tile const* tiles, uint32_t* scene_of) {
+    Tile const tl = tiles[blockIdx.x];
+    if (threadIdx.x < tl.count) { scene_of[tl.start + threadIdx.x] = tl.scene; }
+}
+
+void launch_scene_of(Tile const* tiles, uint32_t ntiles, uint32_t* scene_of, hipStream_t s) {
+    if (!ntiles) { return; }
+    hipLaunchKernelGGL(k_scene_of, dim3(ntiles), dim3(TILE), 0, s, tiles, scene_of);
+}
+
+// Moves the state of repacked scenes: for each segment {src, dst, n}, slots
+// [src, src + n) of the four state arrays go to [dst, dst + n) of the scratch
+// arrays (segments in increasing dst, contiguous from seg[0].dst).
+__global__ __launch_bounds__(256) void k_repack(uint3 const* seg, uint32_t nseg, uint32_t total,
+                                                V2 const* pos, V2 const* vel, float const* kv,
+                                                float const* kd, V2* pos_o, V2* vel_o,
+                                                float* kv_o, float* kd_o) {
+    uint32_t const base = seg[0].y;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += gridDim.x * blockDim.x) {
+        uint32_t const d = base + i;
+        uint32_t lo = 0, hi = nseg;  // last segment with dst <= d
+        while (hi - lo > 1) {
+            uint32_t const mid = (lo + hi) / 2;
+            if (seg[mid].y <= d) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        uint32_t const src = seg[lo].x + (d - seg[lo].y);
+        pos_o[d] = pos[src];
+        vel_o[d] = vel[src];
+        kv_o[d] = kv[src];
+        kd_o[d] = kd[src];
+    }
+}
+
+void launch_repack(uint3 const* seg, uint32_t nseg, uint32_t total, V2 const* pos, V2 const* vel,
+                   float const* kv, float const* kd, V2* pos_o, V2* vel_o, float* kv_o,
+                   float* kd_o, hipStream_t s) {
+    if (!total || !nseg) { return; }
+    uint32_t blocks = (total + 255) / 256;
+    if (blocks > 8192) { blocks = 8192; }
+    hipLaunchKernelGGL(k_repack, dim3(blocks), dim3(256), 0, s, seg, nseg, total, pos, vel, kv,
+                       kd, pos_o, vel_o, kv_o, kd_o);
+}
+
 void launch_scene_energy(Batch const& a, hipStream_t s) {
     uint32_t const blocks = a.nscenes < 4096 ? a.nscenes : 4096;
     hipLaunchKernelGGL(k_scene_energy, dim3(blocks ? blocks : 1), dim3(256), 0, s, a);

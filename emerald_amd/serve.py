@@ -81,8 +81,10 @@ def percentile(xs, q):
 
 
 class _PoolLoop:
-    def __init__(self, method: str, fluid_cap: int, solid_cap: int, max_scenes: int, cfl: float):
+    def __init__(self, method: str, fluid_cap: int, solid_cap: int, max_scenes: int, cfl: float,
+                 commit_interval: float):
         self.method = method
+        self.commit_interval = commit_interval
         self.pool = sb.Pool(fluid_cap, solid_cap, max_scenes, method)
         self.pool.set_cfl(cfl)
         self.ready = queue.Queue(maxsize=256)  # (Request, DamBreak, config) in arrival order
@@ -97,15 +99,19 @@ class _PoolLoop:
         self.live_particles = []
         self.finals = {}                       # rid -> final State (when keep_states)
         self.dirty = False                     # retirements staged since the last commit
+        self.commit_s = 0.0                    # wall time in commits
+        self.step_s = 0.0                      # wall time in sub-steps
 
     def run(self, t0: float, end: float, keep_states: bool) -> None:
         pool = self.pool
+        last_commit = -1e9
+        staged = False
         while True:
             now = time.perf_counter() - t0
             if now >= end:
                 break
-            # Admission, in arrival order, while the head has arrived and fits.
-            changed = self.dirty
+            # Admission, in arrival order, while the head has arrived and fits
+            # (staged now, live from the next commit).
             while True:
                 if self.pending is None:
                     try:
@@ -121,27 +127,38 @@ class _PoolLoop:
                 except EmsphError:
                     break  # does not fit until something retires
                 self.live[sid] = [req, 0, None, len(sc.positions)]
-                self.wait.append(now - req.arrival)
                 self.admitted += 1
                 self.pending = None
-                changed = True
-            if changed:
+                staged = True
+            # Commits (which repack the pool) at most every commit_interval;
+            # retired scenes keep stepping, unaccounted, until then.
+            idle = len(self.live) == 0 or not pool.live()
+            if (staged or self.dirty) and (idle or now - last_commit >= self.commit_interval):
+                tb = time.perf_counter()
                 pool.commit()
                 self.dirty = False
+                staged = False
                 tc = time.perf_counter()
+                last_commit = tc - t0
+                self.commit_s += tc - tb
                 for v in self.live.values():
                     if v[2] is None:
                         v[2] = tc
+                        self.wait.append(tc - t0 - v[0].arrival)
             ids = pool.live()
             if not ids:
                 time.sleep(0.0005)
                 continue
+            ts = time.perf_counter()
             pool.sub_step(with_stats=False)
             td = time.perf_counter()
+            self.step_s += td - ts
             self.ticks += 1
             n_live = 0
             for sid in ids:
-                v = self.live[sid]
+                v = self.live.get(sid)
+                if v is None:
+                    continue  # retired, removed at the next commit
                 self.latency.append(td - v[2])
                 v[2] = td
                 v[1] += 1
@@ -159,7 +176,7 @@ class _PoolLoop:
 
 def serve(requests: list, budget_bytes: float, window: float, cfl: float = 0.4,
           methods=("iisph", "dfsph_p"), max_scenes: int = 4096, solid_ratio: float = 0.25,
-          keep_states: bool = False, log=None) -> dict:
+          keep_states: bool = False, commit_interval: float = 0.2, log=None) -> dict:
     """Runs the request stream for window seconds; returns the served metrics
     (and the pools' final states of completed requests when keep_states)."""
     # Device bytes per fluid particle (with solid_ratio solids per fluid
@@ -169,7 +186,8 @@ def serve(requests: list, budget_bytes: float, window: float, cfl: float = 0.4,
     probe.close()
     share = budget_bytes / len(methods)
     fluid_cap = int(min(share / per_particle, (1 << 30) - 1))
-    loops = {m: _PoolLoop(m, fluid_cap, int(solid_ratio * fluid_cap), max_scenes, cfl)
+    loops = {m: _PoolLoop(m, fluid_cap, int(solid_ratio * fluid_cap), max_scenes, cfl,
+                          commit_interval)
              for m in methods}
     if log:
         log(f"served: {len(methods)} pools x {fluid_cap} particles "
@@ -227,6 +245,7 @@ def serve(requests: list, budget_bytes: float, window: float, cfl: float = 0.4,
                               "p99": 1e3 * percentile(wait, 99) if wait else None},
         "per_pool": {m: {"ticks": lp.ticks, "admitted": lp.admitted,
                          "completed": len(lp.completed),
+                         "commit_s": lp.commit_s, "sub_step_s": lp.step_s,
                          "mean_live_particles": float(np.mean(lp.live_particles))
                          if lp.live_particles else 0.0}
                      for m, lp in loops.items()},
