@@ -222,7 +222,8 @@ struct KernelTimer {
 
 // Pinned read-back block.
 struct Pinned {
-    KeyState keys;
+    KeyState keys[2];  // double-buffered read-backs of the solver loops
+    KeyState direct;   // synchronous read-back of the key range
 };
 
 struct SceneHost {
@@ -310,8 +311,15 @@ struct Engine {
     int last_div_iterations = 0;
     int64_t totals[3] = {0, 0, 0};  // sub-steps, pressure/density iterations, divergence iterations
 
+    hipEvent_t read_ev[2] = {nullptr, nullptr};
+    KeyState keys{};  // the last key state read back
+    int loop_expect[4] = {64, 64, 64, 64};  // iterations each solver loop needed last time
+
     ~Engine() {
         if (host) { (void)hipHostFree(host); }
+        for (hipEvent_t e : read_ev) {
+            if (e) { (void)hipEventDestroy(e); }
+        }
         for (hipEvent_t e : timer.ev) { (void)hipEventDestroy(e); }
         if (stream) { (void)hipStreamDestroy(stream); }
     }
@@ -413,6 +421,8 @@ emsph_status engine_init(Engine* g) {
     HIP_TRY(hipGetDevice(&g->device));
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g->host), sizeof(Pinned)));
+    HIP_TRY(hipEventCreateWithFlags(&g->read_ev[0], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&g->read_ev[1], hipEventDisableTiming));
     HIP_TRY(g->keystate.alloc(1));
     HIP_TRY(g->key_partials.alloc(2 * KEY_PARTS));
     return EMSPH_OK;
@@ -575,12 +585,12 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     a.key_partials = g->key_partials.p;
     if ((e = run("keys", [&] { launch_keys(a, s); }))) { return e; }
     if (passes_hint < 0) {
-        if ((e = hipMemcpyAsync(&g->host->keys, g->keystate.p, sizeof(KeyState),
+        if ((e = hipMemcpyAsync(&g->host->direct, g->keystate.p, sizeof(KeyState),
                                 hipMemcpyDeviceToHost, s))) {
             return e;
         }
         if ((e = hipStreamSynchronize(s))) { return e; }
-        si.key_passes = radix_pass_count(g->host->keys.kmin, g->host->keys.kmax);
+        si.key_passes = radix_pass_count(g->host->direct.kmin, g->host->direct.kmax);
     } else {
         si.key_passes = passes_hint;
     }
@@ -766,17 +776,91 @@ int pick_build_cap(unsigned int max_count) {
     return MAXN;
 }
 
-hipError_t read_keystate(Engine* g) {
-    hipError_t e = hipMemcpyAsync(&g->host->keys, g->keystate.p, sizeof(KeyState),
+// Propagates a HIP error out of a function or lambda returning hipError_t.
+#define HIP_PASS(x)                     \
+    do {                                \
+        hipError_t const e_ = (x);      \
+        if (e_ != hipSuccess) { return e_; } \
+    } while (0)
+
+// Key-state read-backs into pinned slot k: enqueue, then wait and take it in.
+hipError_t read_async(Engine* g, int k) {
+    hipError_t e = hipMemcpyAsync(&g->host->keys[k], g->keystate.p, sizeof(KeyState),
                                   hipMemcpyDeviceToHost, g->stream);
     if (e) { return e; }
-    if ((e = hipStreamSynchronize(g->stream))) { return e; }
-    g->build_cap = pick_build_cap(g->host->keys.max_count);
+    return hipEventRecord(g->read_ev[k], g->stream);
+}
+
+hipError_t read_wait(Engine* g, int k) {
+    hipError_t const e = hipEventSynchronize(g->read_ev[k]);
+    if (e) { return e; }
+    g->keys = g->host->keys[k];
+    g->build_cap = pick_build_cap(g->keys.max_count);
     if (!g->keys_checked) {
         g->keys_checked = true;
-        g->sort_short = g->host->keys.sort_short != 0u;
-        g->passes_hint = radix_pass_count(g->host->keys.kmin, g->host->keys.kmax);
+        g->sort_short = g->keys.sort_short != 0u;
+        g->passes_hint = radix_pass_count(g->keys.kmin, g->keys.kmax);
     }
+    return hipSuccess;
+}
+
+hipError_t read_keystate(Engine* g) {
+    hipError_t const e = read_async(g, 0);
+    return e ? e : read_wait(g, 0);
+}
+
+// A solver loop: launch(it) enqueues iteration it, whose kernels skip the
+// scenes that have converged. From iteration first_check on (and at the last
+// one) the key state is read back one iteration late: iteration it + 1 is
+// enqueued before the host waits for the read-back of iteration it, so the
+// device does not idle while the host decides; an iteration enqueued after
+// every scene converged does nothing. Iterations at or beyond the count the
+// same loop needed in the previous sub-step (kind indexes that memory) are not
+// enqueued ahead: there the loop most likely ends, and an empty iteration
+// costs more than the wait. *done receives the iterations until the last
+// scene converged; *short_sort is set when the sub-step's sort was short (the
+// caller then redoes the sub-step).
+template <class F>
+hipError_t run_loop(Engine* g, int kind, int max_it, int first_check, F&& launch, int* done,
+                    bool* short_sort) {
+    *done = max_it;
+    *short_sort = false;
+    int const expect = g->loop_expect[kind];
+    int pending = -1;  // iteration whose read-back is in flight
+    for (int it = 0; it < max_it; ++it) {
+        if (pending >= 0 && it >= expect) {
+            hipError_t e = read_wait(g, pending & 1);
+            if (e) { return e; }
+            if (g->sort_short || g->keys.active_count == 0) {
+                *short_sort = g->sort_short;
+                *done = pending + 1;
+                g->loop_expect[kind] = *done;
+                return hipSuccess;
+            }
+            pending = -1;
+        }
+        hipError_t e = launch(it);
+        if (e) { return e; }
+        bool const check = it >= first_check || it + 1 == max_it;
+        if (check && (e = read_async(g, it & 1))) { return e; }
+        if (pending >= 0) {
+            if ((e = read_wait(g, pending & 1))) { return e; }
+            if (g->sort_short || g->keys.active_count == 0) {
+                if (check && (e = read_wait(g, it & 1))) { return e; }  // drain the slot
+                *short_sort = g->sort_short;
+                *done = pending + 1;
+                g->loop_expect[kind] = *done;
+                return hipSuccess;
+            }
+        }
+        pending = check ? it : -1;
+    }
+    if (pending >= 0) {
+        hipError_t const e = read_wait(g, pending & 1);
+        if (e) { return e; }
+        *short_sort = g->sort_short;
+    }
+    g->loop_expect[kind] = *done;
     return hipSuccess;
 }
 
@@ -819,21 +903,16 @@ emsph_status iisph_sub_step(Engine* g) {
         max_it = std::max(max_it, std::min(h.cfg.params.iisph_max_pressure_iterations,
                                            MAX_RED_SLOTS));
     }
-    for (int it = 0; it < max_it; ++it) {
-        g->last_iterations = it + 1;
-        b.p_in = pa;
-        b.p_out = pb;
-        HIP_TRY(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
-        HIP_TRY(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, it, g->stream); }));
-        HIP_TRY(timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); }));
-        std::swap(pa, pb);
-        // iisph.cpp:126: no scene can exit before its third iteration.
-        if (it >= 2 || it + 1 == max_it) {
-            HIP_TRY(read_keystate(g));
-            if (g->sort_short) { return EMSPH_OK; }
-            if (g->host->keys.active_count == 0) { break; }
-        }
-    }
+    bool short_sort = false;
+    // iisph.cpp:126: no scene can exit before its third iteration.
+    HIP_TRY(run_loop(g, 0, max_it, 2, [&](int it) -> hipError_t {
+        b.p_in = (it & 1) ? pb : pa;
+        b.p_out = (it & 1) ? pa : pb;
+        HIP_PASS(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
+        HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, it, g->stream); }));
+        return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); });
+    }, &g->last_iterations, &short_sort));
+    if (short_sort) { return EMSPH_OK; }
     if (!g->keys_checked) {
         HIP_TRY(read_keystate(g));
         if (g->sort_short) { return EMSPH_OK; }
@@ -860,38 +939,29 @@ emsph_status dfsph_sub_step(Engine* g) {
     // Divergence solve (dfsph_p.cpp:136-212): warm start, then iterations.
     HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 1, g->stream); }));
     HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
-    for (int it = 0; it < max_it; ++it) {
-        g->last_div_iterations = it + 1;
-        HIP_TRY(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, it, g->stream); }));
-        HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
-        HIP_TRY(timed(g, "loop", [&] {
+    bool short_sort = false;
+    HIP_TRY(run_loop(g, 1, max_it, 1, [&](int it) -> hipError_t {
+        HIP_PASS(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, it, g->stream); }));
+        HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
+        return timed(g, "loop", [&] {
             launch_loop_decide(b, LOOP_DIVERGENCE, it, it, g->stream);
-        }));
-        if (it >= 1 || it + 1 == max_it) {
-            HIP_TRY(read_keystate(g));
-            if (g->sort_short) { return EMSPH_OK; }
-            if (g->host->keys.active_count == 0) { break; }
-        }
-    }
+        });
+    }, &g->last_div_iterations, &short_sort));
+    if (short_sort) { return EMSPH_OK; }
     // External forces, then the density solve (dfsph_p.cpp:350-366, :61-134).
     int const base = MAX_RED_SLOTS / 2;
     b.p_in = g->kds.p;
     b.kappa = g->kds.p;
     HIP_TRY(timed(g, "apply_gravity", [&] { launch_apply(b, 1, 0, 1, g->stream); }));
     HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
-    for (int it = 0; it < max_it; ++it) {
-        g->last_iterations = it + 1;
-        HIP_TRY(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, base + it, g->stream); }));
-        HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
-        HIP_TRY(timed(g, "loop", [&] {
+    HIP_TRY(run_loop(g, 2, max_it, 2, [&](int it) -> hipError_t {
+        HIP_PASS(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, base + it, g->stream); }));
+        HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
+        return timed(g, "loop", [&] {
             launch_loop_decide(b, LOOP_DENSITY, it, base + it, g->stream);
-        }));
-        if (it >= 2 || it + 1 == max_it) {
-            HIP_TRY(read_keystate(g));
-            if (g->sort_short) { return EMSPH_OK; }
-            if (g->host->keys.active_count == 0) { break; }
-        }
-    }
+        });
+    }, &g->last_iterations, &short_sort));
+    if (short_sort) { return EMSPH_OK; }
     if (!g->keys_checked) {
         HIP_TRY(read_keystate(g));
         if (g->sort_short) { return EMSPH_OK; }
@@ -914,20 +984,15 @@ emsph_status pcisph_sub_step(Engine* g) {
     b.vstar_out = vin;
     HIP_TRY(timed(g, "pci_init", [&] { launch_pci_init(b, g->stream); }));
     HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
-    for (int it = 0; it < 6; ++it) {
-        g->last_iterations = it + 1;
-        HIP_TRY(timed(g, "pci_density", [&] { launch_pci_density(b, it, g->stream); }));
-        b.vstar_in = vin;
-        b.vstar_out = vout;
-        HIP_TRY(timed(g, "pci_force", [&] { launch_pci_force(b, g->stream); }));
-        HIP_TRY(timed(g, "loop", [&] { launch_loop_decide(b, LOOP_PCISPH, it, it, g->stream); }));
-        std::swap(vin, vout);
-        if (it >= 2) {
-            HIP_TRY(read_keystate(g));
-            if (g->sort_short) { return EMSPH_OK; }
-            if (g->host->keys.active_count == 0) { break; }
-        }
-    }
+    bool short_sort = false;
+    HIP_TRY(run_loop(g, 3, 6, 2, [&](int it) -> hipError_t {
+        HIP_PASS(timed(g, "pci_density", [&] { launch_pci_density(b, it, g->stream); }));
+        b.vstar_in = (it & 1) ? vout : vin;
+        b.vstar_out = (it & 1) ? vin : vout;
+        HIP_PASS(timed(g, "pci_force", [&] { launch_pci_force(b, g->stream); }));
+        return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_PCISPH, it, it, g->stream); });
+    }, &g->last_iterations, &short_sort));
+    if (short_sort) { return EMSPH_OK; }
     if (!g->keys_checked) {
         HIP_TRY(read_keystate(g));
         if (g->sort_short) { return EMSPH_OK; }
@@ -990,8 +1055,8 @@ void to_public(SceneStats const& s, Engine const* g, emsph_step_stats* out) {
     out->divergence_error_max = s.divergence_error_max;
     std::memcpy(&out->max_velocity_sq, &s.vmax2_bits, 4);
     out->sort_passes = static_cast<uint32_t>(g->fidx.key_passes + g->fidx.scene_passes);
-    out->cell_count = g->host->keys.cells;
-    out->max_neighbors = g->host->keys.max_count;
+    out->cell_count = g->keys.cells;
+    out->max_neighbors = g->keys.max_count;
     out->dt = s.dt;
     out->kinetic_energy = s.kinetic_energy;
     out->potential_energy = s.potential_energy;
