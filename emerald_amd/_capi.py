@@ -136,6 +136,8 @@ SIGNATURES = {
     "emsph_frame_step": (_ST, [_P, _I, _I64, _P]),
     "emsph_run_sub_steps": (_ST, [_P, _I, _F, _I, _P, _P]),
     "emsph_set_cfl": (_ST, [_P, _F]),
+    "emsph_set_graphs": (_ST, [_P, _I]),
+    "emsph_batch_set_graphs": (_ST, [_P, _I]),
     "emsph_batch_set_cfl": (_ST, [_P, _F]),
     "emsph_set_kernel_timing": (_ST, [_P, _I]),
     "emsph_get_kernel_timing": (_I, [_P, _P, _P, _P, _I]),

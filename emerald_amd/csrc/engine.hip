@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -314,12 +315,18 @@ struct Engine {
     hipEvent_t read_ev[2] = {nullptr, nullptr};
     KeyState keys{};  // the last key state read back
     int loop_expect[4] = {64, 64, 64, 64};  // iterations each solver loop needed last time
+    // Captured graphs of the fixed segments of a sub-step, by what their launch
+    // arguments depend on; layout_gen changes whenever buffers or sizes do.
+    bool use_graphs = true;
+    uint64_t layout_gen = 1;
+    std::map<std::array<uint64_t, 8>, hipGraphExec_t> graphs;
 
     ~Engine() {
         if (host) { (void)hipHostFree(host); }
         for (hipEvent_t e : read_ev) {
             if (e) { (void)hipEventDestroy(e); }
         }
+        for (auto& kv : graphs) { (void)hipGraphExecDestroy(kv.second); }
         for (hipEvent_t e : timer.ev) { (void)hipEventDestroy(e); }
         if (stream) { (void)hipStreamDestroy(stream); }
     }
@@ -476,6 +483,7 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
 // Scene table and tiles from the host scene list; fluid slots are packed at
 // each scene's offset (offsets are prefix sums of the scene capacities).
 emsph_status upload_layout(Engine* g) {
+    ++g->layout_gen;
     std::vector<SceneDev> sd(g->M);
     std::vector<Tile> tl;
     for (uint32_t s = 0; s < g->M; ++s) {
@@ -545,6 +553,7 @@ hipError_t alloc_fluid(Engine* g, uint32_t cap) {
 hipError_t ensure_geometry(Engine* g) {
     hipError_t e = hipSuccess;
     size_t const m = static_cast<size_t>(MAXN) * g->stride;
+    if (!g->fl_dist.p) { ++g->layout_gen; }
     if (!g->fl_dist.p && (e = g->fl_dist.alloc(m))) { return e; }
     if (!g->fl_vec.p && (e = g->fl_vec.alloc(m))) { return e; }
     if (!g->sl_dist.p && (e = g->sl_dist.alloc(m))) { return e; }
@@ -864,8 +873,18 @@ hipError_t run_loop(Engine* g, int kind, int max_it, int first_check, F&& launch
     return hipSuccess;
 }
 
-// Steps 1-4 of the sub-step.
-emsph_status prepare(Engine* g, int build_mode, bool dfsph) {
+// Host-side state at the start of a sub-step attempt.
+void begin_attempt(Engine* g) {
+    g->keys_checked = g->passes_hint < 0;
+    g->sort_short = false;
+    g->last_iterations = 0;
+    g->last_div_iterations = 0;
+}
+
+// Steps 1-4 of the sub-step. With a known sort pass count this only enqueues
+// work (it can be captured into a graph); otherwise the key range is read back
+// first.
+hipError_t enqueue_prepare(Engine* g, int build_mode, bool dfsph) {
     IndexArgs ga{};
     ga.vel_in = g->vel.p;
     ga.pos_out = g->spos.p;
@@ -876,28 +895,78 @@ emsph_status prepare(Engine* g, int build_mode, bool dfsph) {
         ga.kv_out = g->kvs.p;
         ga.kd_out = g->kds.p;
     }
-    g->keys_checked = g->passes_hint < 0;
-    g->sort_short = false;
-    g->last_iterations = 0;
-    g->last_div_iterations = 0;
     Batch b = fluid_batch(g);
-    HIP_TRY(timed(g, "step_begin", [&] { launch_step_begin(b, g->stream); }));
-    HIP_TRY(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->passes_hint, ga,
-                         g->timer.enabled, g->NS ? &g->sidx : nullptr, g->swin.p));
-    if (g->passes_hint < 0) { g->passes_hint = g->fidx.key_passes; }
-    HIP_TRY(timed(g, "build", [&] { launch_build(build_mode, g->build_cap, b, g->stream); }));
-    g->have_lists = true;
-    return EMSPH_OK;
+    HIP_PASS(timed(g, "step_begin", [&] { launch_step_begin(b, g->stream); }));
+    HIP_PASS(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->passes_hint, ga,
+                          g->timer.enabled, g->NS ? &g->sidx : nullptr, g->swin.p));
+    return timed(g, "build", [&] { launch_build(build_mode, g->build_cap, b, g->stream); });
 }
+
+// Host-side fields of the fluid index that a sub-step's sort leaves (also
+// after a graph replay, which runs none of the host code).
+void finish_prepare(Engine* g) {
+    SpatialIndex& si = g->fidx;
+    if (g->passes_hint < 0) { g->passes_hint = si.key_passes; }
+    si.key_passes = g->passes_hint;
+    si.scene_passes = scene_pass_count(g->M);
+    bool const odd = ((si.key_passes + si.scene_passes) & 1) != 0;
+    si.sorted_keys = odd ? si.keys1.p : si.keys0.p;
+    si.sorted_vals = odd ? si.vals1.p : si.vals0.p;
+    g->have_lists = true;
+}
+
+// Runs one fixed segment of the sub-step (enqueue() launches its kernels):
+// replayed from a captured graph when the launch arguments are known in
+// advance (sort pass count known, no kernel timing), else enqueued directly.
+template <class F>
+hipError_t segment(Engine* g, int method, int seg, F&& enqueue) {
+    if (!g->use_graphs || g->timer.enabled || g->passes_hint < 0) { return enqueue(); }
+    std::array<uint64_t, 8> const key{
+      static_cast<uint64_t>(method), static_cast<uint64_t>(seg),
+      static_cast<uint64_t>(g->passes_hint), static_cast<uint64_t>(g->build_cap), g->layout_gen,
+      static_cast<uint64_t>(g->store_geometry), g->N, g->ntiles};
+    auto it = g->graphs.find(key);
+    if (it == g->graphs.end()) {
+        // Graphs of an older layout can never be replayed again.
+        for (auto jt = g->graphs.begin(); jt != g->graphs.end();) {
+            if (jt->first[4] != g->layout_gen) {
+                (void)hipGraphExecDestroy(jt->second);
+                jt = g->graphs.erase(jt);
+            } else {
+                ++jt;
+            }
+        }
+        HIP_PASS(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t const e = enqueue();
+        hipGraph_t graph = nullptr;
+        hipError_t const e2 = hipStreamEndCapture(g->stream, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) { (void)hipGraphDestroy(graph); }
+            return e != hipSuccess ? e : e2;
+        }
+        hipGraphExec_t exec = nullptr;
+        hipError_t const e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_PASS(e3);
+        it = g->graphs.emplace(key, exec).first;
+    }
+    return hipGraphLaunch(it->second, g->stream);
+}
+
+enum Segment : int { SEG_PREFIX = 0, SEG_MIDDLE = 1, SEG_SUFFIX = 2 };
 
 // iisph.cpp:23-151 for every scene of the batch.
 emsph_status iisph_sub_step(Engine* g) {
-    ST_TRY(prepare(g, BUILD_IISPH, false));
+    begin_attempt(g);
     float* pa = g->p0.p;
     float* pb = g->p1.p;
     Batch b = fluid_batch(g, nullptr, pa);
-    HIP_TRY(timed(g, "iisph_aii", [&] { launch_iisph_aii(b, g->stream); }));
-    HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
+    HIP_TRY(segment(g, EMSPH_IISPH, SEG_PREFIX, [&]() -> hipError_t {
+        HIP_PASS(enqueue_prepare(g, BUILD_IISPH, false));
+        HIP_PASS(timed(g, "iisph_aii", [&] { launch_iisph_aii(b, g->stream); }));
+        return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
+    }));
+    finish_prepare(g);
     int max_it = 0;
     for (auto const& h : g->sh) {
         max_it = std::max(max_it, std::min(h.cfg.params.iisph_max_pressure_iterations,
@@ -919,16 +988,20 @@ emsph_status iisph_sub_step(Engine* g) {
     }
     // Each scene's last pressure buffer follows its own iteration count.
     b.p_in = g->p0.p;
+    b.p_out = g->p0.p;
     b.p_alt = g->p1.p;
-    HIP_TRY(timed(g, "apply_integrate", [&] { launch_apply(b, 0, 1, 1, g->stream); }));
-    launch_scene_energy(b, g->stream);
+    HIP_TRY(segment(g, EMSPH_IISPH, SEG_SUFFIX, [&]() -> hipError_t {
+        HIP_PASS(timed(g, "apply_integrate", [&] { launch_apply(b, 0, 1, 1, g->stream); }));
+        launch_scene_energy(b, g->stream);
+        return hipGetLastError();
+    }));
     g->cur_p = g->p0.p;
     return EMSPH_OK;
 }
 
 // dfsph_p.cpp:265-374 (with the correction loops of :61-212).
 emsph_status dfsph_sub_step(Engine* g) {
-    ST_TRY(prepare(g, BUILD_DFSPH, true));
+    begin_attempt(g);
     Batch b = fluid_batch(g, g->kvs.p, nullptr);
     b.kappa = g->kvs.p;
     int max_it = 0;
@@ -937,8 +1010,12 @@ emsph_status dfsph_sub_step(Engine* g) {
                                            MAX_RED_SLOTS / 2));
     }
     // Divergence solve (dfsph_p.cpp:136-212): warm start, then iterations.
-    HIP_TRY(timed(g, "apply", [&] { launch_apply(b, 0, 0, 1, g->stream); }));
-    HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
+    HIP_TRY(segment(g, EMSPH_DFSPH_P, SEG_PREFIX, [&]() -> hipError_t {
+        HIP_PASS(enqueue_prepare(g, BUILD_DFSPH, true));
+        HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 1, g->stream); }));
+        return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
+    }));
+    finish_prepare(g);
     bool short_sort = false;
     HIP_TRY(run_loop(g, 1, max_it, 1, [&](int it) -> hipError_t {
         HIP_PASS(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, it, g->stream); }));
@@ -952,8 +1029,10 @@ emsph_status dfsph_sub_step(Engine* g) {
     int const base = MAX_RED_SLOTS / 2;
     b.p_in = g->kds.p;
     b.kappa = g->kds.p;
-    HIP_TRY(timed(g, "apply_gravity", [&] { launch_apply(b, 1, 0, 1, g->stream); }));
-    HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
+    HIP_TRY(segment(g, EMSPH_DFSPH_P, SEG_MIDDLE, [&]() -> hipError_t {
+        HIP_PASS(timed(g, "apply_gravity", [&] { launch_apply(b, 1, 0, 1, g->stream); }));
+        return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
+    }));
     HIP_TRY(run_loop(g, 2, max_it, 2, [&](int it) -> hipError_t {
         HIP_PASS(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, base + it, g->stream); }));
         HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
@@ -966,8 +1045,11 @@ emsph_status dfsph_sub_step(Engine* g) {
         HIP_TRY(read_keystate(g));
         if (g->sort_short) { return EMSPH_OK; }
     }
-    HIP_TRY(timed(g, "finalize", [&] { launch_finalize(b, 1, g->stream); }));
-    launch_scene_energy(b, g->stream);
+    HIP_TRY(segment(g, EMSPH_DFSPH_P, SEG_SUFFIX, [&]() -> hipError_t {
+        HIP_PASS(timed(g, "finalize", [&] { launch_finalize(b, 1, g->stream); }));
+        launch_scene_energy(b, g->stream);
+        return hipGetLastError();
+    }));
     g->cur_p = g->kds.p;
     return EMSPH_OK;
 }
@@ -975,15 +1057,19 @@ emsph_status dfsph_sub_step(Engine* g) {
 // pcisph.cpp:36-164 for every scene of the batch.
 emsph_status pcisph_sub_step(Engine* g) {
     HIP_TRY(ensure_geometry(g));
-    ST_TRY(prepare(g, BUILD_LISTS, false));
+    begin_attempt(g);
     Batch b = fluid_batch(g, nullptr, g->p0.p);
     b.fl.dist = g->fl_dist.p;
     b.fl.vec = g->fl_vec.p;
     V2* vin = g->vstar0.p;
     V2* vout = g->vstar1.p;
     b.vstar_out = vin;
-    HIP_TRY(timed(g, "pci_init", [&] { launch_pci_init(b, g->stream); }));
-    HIP_TRY(timed(g, "loop", [&] { launch_loop_begin(b, g->stream); }));
+    HIP_TRY(segment(g, EMSPH_PCISPH, SEG_PREFIX, [&]() -> hipError_t {
+        HIP_PASS(enqueue_prepare(g, BUILD_LISTS, false));
+        HIP_PASS(timed(g, "pci_init", [&] { launch_pci_init(b, g->stream); }));
+        return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
+    }));
+    finish_prepare(g);
     bool short_sort = false;
     HIP_TRY(run_loop(g, 3, 6, 2, [&](int it) -> hipError_t {
         HIP_PASS(timed(g, "pci_density", [&] { launch_pci_density(b, it, g->stream); }));
@@ -999,9 +1085,13 @@ emsph_status pcisph_sub_step(Engine* g) {
     }
     // Each scene's last predictor follows its own iteration count.
     b.vstar_in = g->vstar0.p;
+    b.vstar_out = g->vstar1.p;
     b.vstar_alt = g->vstar1.p;
-    HIP_TRY(timed(g, "pci_finalize", [&] { launch_pci_finalize(b, g->stream); }));
-    launch_scene_energy(b, g->stream);
+    HIP_TRY(segment(g, EMSPH_PCISPH, SEG_SUFFIX, [&]() -> hipError_t {
+        HIP_PASS(timed(g, "pci_finalize", [&] { launch_pci_finalize(b, g->stream); }));
+        launch_scene_energy(b, g->stream);
+        return hipGetLastError();
+    }));
     g->cur_p = g->p0.p;
     return EMSPH_OK;
 }
@@ -1587,6 +1677,18 @@ emsph_status emsph_get_iteration_totals(emsph_context* c, int reset, int64_t* ou
     if (reset) {
         for (int k = 0; k < 3; ++k) { c->e.totals[k] = 0; }
     }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_set_graphs(emsph_context* c, int enable) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    c->e.use_graphs = enable != 0;
+    return EMSPH_OK;
+}
+
+emsph_status emsph_batch_set_graphs(emsph_batch* b, int enable) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    b->e.use_graphs = enable != 0;
     return EMSPH_OK;
 }
 

@@ -288,6 +288,10 @@ class Context:
         """Per-sub-step dt = min(dt, factor * h / max |v|) on the device (0: off)."""
         check(lib().emsph_set_cfl(self._h, float(factor)), "set_cfl")
 
+    def set_graphs(self, enable: bool) -> None:
+        """Replay the fixed segments of each sub-step from captured HIP graphs."""
+        check(lib().emsph_set_graphs(self._h, 1 if enable else 0), "set_graphs")
+
     def iteration_totals(self, reset: bool = False) -> tuple:
         """(sub-steps, pressure or density iterations, divergence iterations)
         since creation or the last reset."""
@@ -420,6 +424,9 @@ class Batch:
     def set_cfl(self, factor: float) -> None:
         """Per-scene, per-sub-step dt = min(config dt, factor * h / max |v|) (0: off)."""
         check(lib().emsph_batch_set_cfl(self._h, float(factor)), "batch_set_cfl")
+
+    def set_graphs(self, enable: bool) -> None:
+        check(lib().emsph_batch_set_graphs(self._h, 1 if enable else 0), "batch_set_graphs")
 
     def frame_init(self, method: str) -> None:
         check(lib().emsph_batch_frame_init(self._h, _capi.METHODS[method]), "batch_frame_init")

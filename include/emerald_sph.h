@@ -215,6 +215,11 @@ emsph_status emsph_run_sub_steps(emsph_context* ctx, int method, float dt, int n
  * per-frame quantisation (adaptive_time_step.h:35-122, which emsph_frame_step
  * keeps). */
 emsph_status emsph_set_cfl(emsph_context* ctx, float factor);
+/* Replay the fixed segments of each sub-step (sort, cells, neighbour build,
+ * the first and last sweeps) from captured HIP graphs (on by default; used
+ * once the sort pass count is known and kernel timing is off). Results are
+ * identical either way. */
+emsph_status emsph_set_graphs(emsph_context* ctx, int enable);
 emsph_status emsph_set_kernel_timing(emsph_context* ctx, int enable);
 int emsph_get_kernel_timing(emsph_context* ctx, const char** names, double* total_ms,
                             int64_t* calls, int max);
@@ -242,6 +247,7 @@ emsph_status emsph_batch_set_state(emsph_batch* b, int scene, const float* pos_x
 emsph_status emsph_batch_get_state(emsph_batch* b, int scene, float* pos_xy, float* vel_xy);
 emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* out);
 emsph_status emsph_batch_set_cfl(emsph_batch* b, float factor);  /* as emsph_set_cfl, per scene */
+emsph_status emsph_batch_set_graphs(emsph_batch* b, int enable);  /* as emsph_set_graphs */
 emsph_status emsph_batch_frame_init(emsph_batch* b, int method);
 /* One sub-step of every scene; stats (array of n_scenes, may be NULL). */
 emsph_status emsph_batch_sub_step(emsph_batch* b, int method, emsph_step_stats* stats);
