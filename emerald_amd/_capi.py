@@ -92,6 +92,12 @@ class StepStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    """emsph_tuning."""
+    _fields_ = [("persistent_loop", ctypes.c_int32), ("graphs", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 6)]
+
+
 class FrameStats(ctypes.Structure):
     _fields_ = [
         ("sub_steps", ctypes.c_int32),
@@ -137,6 +143,12 @@ SIGNATURES = {
     "emsph_run_sub_steps": (_ST, [_P, _I, _F, _I, _P, _P]),
     "emsph_set_cfl": (_ST, [_P, _F]),
     "emsph_set_graphs": (_ST, [_P, _I]),
+    "emsph_get_tuning": (_ST, [_P, _P]),
+    "emsph_set_tuning": (_ST, [_P, _P]),
+    "emsph_autotune": (_ST, [_P, _I, _F, _I, _P, _P, _P]),
+    "emsph_batch_set_tuning": (_ST, [_P, _P]),
+    "emsph_batch_autotune": (_ST, [_P, _I, _I, _P, _P, _P]),
+    "emsph_pool_set_tuning": (_ST, [_P, _P]),
     "emsph_batch_set_graphs": (_ST, [_P, _I]),
     "emsph_batch_set_cfl": (_ST, [_P, _F]),
     "emsph_set_kernel_timing": (_ST, [_P, _I]),

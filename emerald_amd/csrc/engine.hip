@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1456,6 +1457,132 @@ emsph_status pool_commit(emsph_pool* P) {
     return EMSPH_OK;
 }
 
+// A copy of everything a sub-step changes (original-order state, DFSPH warm
+// starts, per-scene statistics, the host's sort and loop hints), so that trial
+// sub-steps can be undone.
+struct Snapshot {
+    DBuf<V2> pos, vel;
+    DBuf<float> kv, kd;
+    std::vector<SceneStats> stats;
+    int passes_hint = -1;
+    int build_cap = 32;
+    int loop_expect[4] = {64, 64, 64, 64};
+    int64_t totals[3] = {0, 0, 0};
+};
+
+hipError_t snapshot_take(Engine* g, Snapshot& sn) {
+    uint32_t const n = std::max<uint32_t>(g->N, 1);
+    HIP_PASS(sn.pos.ensure(n));
+    HIP_PASS(sn.vel.ensure(n));
+    HIP_PASS(sn.kv.ensure(n));
+    HIP_PASS(sn.kd.ensure(n));
+    if (g->N) {
+        HIP_PASS(hipMemcpyAsync(sn.pos.p, g->pos.p, g->N * sizeof(V2), hipMemcpyDeviceToDevice,
+                                g->stream));
+        HIP_PASS(hipMemcpyAsync(sn.vel.p, g->vel.p, g->N * sizeof(V2), hipMemcpyDeviceToDevice,
+                                g->stream));
+        HIP_PASS(hipMemcpyAsync(sn.kv.p, g->kv.p, g->N * sizeof(float), hipMemcpyDeviceToDevice,
+                                g->stream));
+        HIP_PASS(hipMemcpyAsync(sn.kd.p, g->kd.p, g->N * sizeof(float), hipMemcpyDeviceToDevice,
+                                g->stream));
+    }
+    sn.stats.resize(g->M);
+    if (g->M) {
+        HIP_PASS(hipMemcpyAsync(sn.stats.data(), g->stats.p, g->M * sizeof(SceneStats),
+                                hipMemcpyDeviceToHost, g->stream));
+    }
+    HIP_PASS(hipStreamSynchronize(g->stream));
+    sn.passes_hint = g->passes_hint;
+    sn.build_cap = g->build_cap;
+    std::copy(g->loop_expect, g->loop_expect + 4, sn.loop_expect);
+    std::copy(g->totals, g->totals + 3, sn.totals);
+    return hipSuccess;
+}
+
+hipError_t snapshot_restore(Engine* g, Snapshot const& sn) {
+    if (g->N) {
+        HIP_PASS(hipMemcpyAsync(g->pos.p, sn.pos.p, g->N * sizeof(V2), hipMemcpyDeviceToDevice,
+                                g->stream));
+        HIP_PASS(hipMemcpyAsync(g->vel.p, sn.vel.p, g->N * sizeof(V2), hipMemcpyDeviceToDevice,
+                                g->stream));
+        HIP_PASS(hipMemcpyAsync(g->kv.p, sn.kv.p, g->N * sizeof(float), hipMemcpyDeviceToDevice,
+                                g->stream));
+        HIP_PASS(hipMemcpyAsync(g->kd.p, sn.kd.p, g->N * sizeof(float), hipMemcpyDeviceToDevice,
+                                g->stream));
+    }
+    if (g->M) {
+        HIP_PASS(hipMemcpyAsync(g->stats.p, sn.stats.data(), g->M * sizeof(SceneStats),
+                                hipMemcpyHostToDevice, g->stream));
+    }
+    HIP_PASS(hipStreamSynchronize(g->stream));
+    g->passes_hint = sn.passes_hint;
+    g->build_cap = sn.build_cap;
+    std::copy(sn.loop_expect, sn.loop_expect + 4, g->loop_expect);
+    std::copy(sn.totals, sn.totals + 3, g->totals);
+    g->have_lists = false;
+    return hipSuccess;
+}
+
+void tuning_get(Engine const* g, emsph_tuning* t) {
+    std::memset(t, 0, sizeof(*t));
+    t->graphs = g->use_graphs ? 1 : 0;
+}
+
+// persistent_loop is refused: a cooperative launch of the whole IISPH solve
+// (grid-wide barriers between sweeps) did not finish at 1,792 workgroups on
+// the MI355X while it passed at 8, and the cause was not established, so the
+// mode is off.
+emsph_status tuning_set(Engine* g, emsph_tuning const* t) {
+    if (t->persistent_loop) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "persistent_loop is not available");
+    }
+    g->use_graphs = t->graphs != 0;
+    return EMSPH_OK;
+}
+
+// Times each candidate tuning on trial_sub_steps sub-steps of the engine's own
+// workload, all from the same state (restored after each trial and at the
+// end, so the simulation does not advance), and keeps the fastest. Every
+// candidate gives identical results; only the time differs.
+emsph_status engine_autotune(Engine* g, int method, int trials, emsph_tuning* chosen,
+                             double* times_ms, int* n_candidates) {
+    if (trials < 1) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "trial_sub_steps < 1"); }
+    std::vector<emsph_tuning> cands;
+    for (int graphs = 1; graphs >= 0; --graphs) {
+        emsph_tuning t{};
+        t.graphs = graphs;
+        cands.push_back(t);
+    }
+    Snapshot sn;
+    HIP_TRY(snapshot_take(g, sn));
+    double best = 0.0;
+    size_t best_k = 0;
+    for (size_t k = 0; k < cands.size(); ++k) {
+        ST_TRY(tuning_set(g, &cands[k]));
+        // Warm-up (graph capture, first-pass sort), then the timed trial from
+        // the same state.
+        ST_TRY(engine_sub_step(g, method));
+        HIP_TRY(snapshot_restore(g, sn));
+        ST_TRY(engine_sub_step(g, method));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        auto const t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < trials; ++i) { ST_TRY(engine_sub_step(g, method)); }
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        double const ms =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        HIP_TRY(snapshot_restore(g, sn));
+        if (times_ms && k < 8) { times_ms[k] = ms; }
+        if (k == 0 || ms < best) {
+            best = ms;
+            best_k = k;
+        }
+    }
+    ST_TRY(tuning_set(g, &cands[best_k]));
+    if (chosen) { *chosen = cands[best_k]; }
+    if (n_candidates) { *n_candidates = static_cast<int>(cands.size()); }
+    return EMSPH_OK;
+}
+
 int pool_index(emsph_pool const* P, int id) {
     for (size_t k = 0; k < P->live.size(); ++k) {
         if (P->live[k].id == id) { return static_cast<int>(k); }
@@ -1678,6 +1805,42 @@ emsph_status emsph_get_iteration_totals(emsph_context* c, int reset, int64_t* ou
         for (int k = 0; k < 3; ++k) { c->e.totals[k] = 0; }
     }
     return EMSPH_OK;
+}
+
+emsph_status emsph_get_tuning(emsph_context* c, emsph_tuning* out) {
+    if (!c || !out) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
+    tuning_get(&c->e, out);
+    return EMSPH_OK;
+}
+
+emsph_status emsph_set_tuning(emsph_context* c, const emsph_tuning* t) {
+    if (!c || !t) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
+    return tuning_set(&c->e, t);
+}
+
+emsph_status emsph_autotune(emsph_context* c, int method, float dt, int trial_sub_steps,
+                            emsph_tuning* chosen, double* times_ms, int* n_candidates) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    ST_TRY(check_method(method));
+    ST_TRY(engine_set_dt(&c->e, 0, dt));
+    return engine_autotune(&c->e, method, trial_sub_steps, chosen, times_ms, n_candidates);
+}
+
+emsph_status emsph_batch_autotune(emsph_batch* b, int method, int trial_sub_steps,
+                                  emsph_tuning* chosen, double* times_ms, int* n_candidates) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    ST_TRY(check_method(method));
+    return engine_autotune(&b->e, method, trial_sub_steps, chosen, times_ms, n_candidates);
+}
+
+emsph_status emsph_batch_set_tuning(emsph_batch* b, const emsph_tuning* t) {
+    if (!b || !t) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
+    return tuning_set(&b->e, t);
+}
+
+emsph_status emsph_pool_set_tuning(emsph_pool* P, const emsph_tuning* t) {
+    if (!P || !t) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
+    return tuning_set(&P->e, t);
 }
 
 emsph_status emsph_set_graphs(emsph_context* c, int enable) {

@@ -340,9 +340,8 @@ void launch_loop_begin(Batch const& b, hipStream_t s) {
 // dfsph_p.cpp:186-210 (from the second), DFSPH density dfsph_p.cpp:112-132
 // (from the third), PCISPH pcisph.cpp:144-148 (from the third). The average is
 // the reference's (double) isum / (1000 n) narrowed to float.
-__global__ __launch_bounds__(256) void k_loop_decide(Batch b, int kind, int it, int slot) {
-    uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= b.nscenes) { return; }
+__device__ __forceinline__ void loop_decide_scene(Batch const& b, int kind, int it, int slot,
+                                                  uint32_t s) {
     SceneStats& st = b.stats[s];
     if (!st.active) { return; }
     SceneDev const& sc = b.scenes[s];
@@ -396,6 +395,11 @@ __global__ __launch_bounds__(256) void k_loop_decide(Batch b, int kind, int it, 
         }
         atomicSub(&b.keystate->active_count, 1u);
     }
+}
+
+__global__ __launch_bounds__(256) void k_loop_decide(Batch b, int kind, int it, int slot) {
+    uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < b.nscenes) { loop_decide_scene(b, kind, it, slot, s); }
 }
 
 void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t s) {
@@ -932,12 +936,11 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
     a.p_out[t] = 0.0f;
 }
 
-// :225-258 sum_j d_ij p_j (fluid only). Converged scenes skip the sweep.
-__global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
-    TileRef const r = tile_ref(a, TILE);
-    if (!r.valid || !a.stats[r.scene].active) { return; }
-    uint32_t const t = r.t;
-    StepConsts const c = a.scenes[r.scene].c;
+// :225-258 sum_j d_ij p_j (fluid only) of one tile's particles.
+__device__ __forceinline__ void iisph_sdp_tile(Batch const& a, Tile const& tl) {
+    if (threadIdx.x >= tl.count) { return; }
+    uint32_t const t = tl.start + threadIdx.x;
+    StepConsts const c = a.scenes[tl.scene].c;
     float const kf = -(c.dt * c.dt) * (c.fluid_volume * c.rho0);
     int const fc = a.fl.count[t];
     V2 s{0.0f, 0.0f};
@@ -966,17 +969,22 @@ __global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
     a.sdp[t] = s;
 }
 
+// Converged scenes skip the sweep.
+__global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
+    Tile const tl = a.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    if (!a.stats[tl.scene].active) { return; }
+    iisph_sdp_tile(a, tl);
+}
+
 // :261-353 beta (into the new-pressure buffer) fused with :357-425 new
-// pressure, quantised error sum and error max. A converged scene skips the
-// sweep; its result stays in the buffer of its last iteration.
-__global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, int slot) {
-    TileRef const r = tile_ref(a, TILE);
-    if (!a.stats[r.scene].active) { return; }  // the whole workgroup
+// pressure, quantised error sum and error max, for one tile (every thread of
+// the workgroup calls it).
+__device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl, int slot) {
     unsigned long long qsum = 0;
     unsigned int emax = 0;
-    if (r.valid) {
-        uint32_t const t = r.t;
-        StepConsts const c = a.scenes[r.scene].c;
+    if (threadIdx.x < tl.count) {
+        uint32_t const t = tl.start + threadIdx.x;
+        StepConsts const c = a.scenes[tl.scene].c;
         float const mf = c.rho0 * c.fluid_volume;
         float const kf = -(c.dt * c.dt) * mf;
         float const pi = a.p_in[t];
@@ -1045,7 +1053,15 @@ __global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, int slot) {
         }
         a.p_out[t] = pn;
     }
-    block_reduce_residual<TILE>(qsum, emax, &a.red[r.scene * MAX_RED_SLOTS + slot]);
+    block_reduce_residual<TILE>(qsum, emax, &a.red[tl.scene * MAX_RED_SLOTS + slot]);
+}
+
+// A converged scene skips the sweep; its result stays in the buffer of its
+// last iteration.
+__global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, int slot) {
+    Tile const tl = a.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
+    iisph_beta_p_tile(a, tl, slot);
 }
 
 //------------------------------------------------------------------------------

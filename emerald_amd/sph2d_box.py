@@ -292,6 +292,26 @@ class Context:
         """Replay the fixed segments of each sub-step from captured HIP graphs."""
         check(lib().emsph_set_graphs(self._h, 1 if enable else 0), "set_graphs")
 
+    def tuning(self) -> dict:
+        t = _capi.Tuning()
+        check(lib().emsph_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
+        return {"persistent_loop": int(t.persistent_loop), "graphs": int(t.graphs)}
+
+    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True) -> None:
+        t = _capi.Tuning(int(persistent_loop), int(graphs))
+        check(lib().emsph_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
+
+    def autotune(self, method: str, dt: float, trial_sub_steps: int = 5):
+        """Times each candidate tuning on this state (which it leaves unchanged)
+        and keeps the fastest; returns (chosen, [ms per candidate])."""
+        t = _capi.Tuning()
+        times = (ctypes.c_double * 8)()
+        n = ctypes.c_int()
+        check(lib().emsph_autotune(self._h, _capi.METHODS[method], float(dt), int(trial_sub_steps),
+                                   ctypes.byref(t), times, ctypes.byref(n)), "autotune")
+        return ({"persistent_loop": int(t.persistent_loop), "graphs": int(t.graphs)},
+                list(times[:n.value]))
+
     def iteration_totals(self, reset: bool = False) -> tuple:
         """(sub-steps, pressure or density iterations, divergence iterations)
         since creation or the last reset."""
@@ -428,6 +448,20 @@ class Batch:
     def set_graphs(self, enable: bool) -> None:
         check(lib().emsph_batch_set_graphs(self._h, 1 if enable else 0), "batch_set_graphs")
 
+    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True) -> None:
+        t = _capi.Tuning(int(persistent_loop), int(graphs))
+        check(lib().emsph_batch_set_tuning(self._h, ctypes.byref(t)), "batch_set_tuning")
+
+    def autotune(self, method: str, trial_sub_steps: int = 3):
+        t = _capi.Tuning()
+        times = (ctypes.c_double * 8)()
+        n = ctypes.c_int()
+        check(lib().emsph_batch_autotune(self._h, _capi.METHODS[method], int(trial_sub_steps),
+                                         ctypes.byref(t), times, ctypes.byref(n)),
+              "batch_autotune")
+        return ({"persistent_loop": int(t.persistent_loop), "graphs": int(t.graphs)},
+                list(times[:n.value]))
+
     def frame_init(self, method: str) -> None:
         check(lib().emsph_batch_frame_init(self._h, _capi.METHODS[method]), "batch_frame_init")
 
@@ -497,6 +531,10 @@ class Pool:
 
     def set_cfl(self, factor: float) -> None:
         check(lib().emsph_pool_set_cfl(self._h, float(factor)), "pool_set_cfl")
+
+    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True) -> None:
+        t = _capi.Tuning(int(persistent_loop), int(graphs))
+        check(lib().emsph_pool_set_tuning(self._h, ctypes.byref(t)), "pool_set_tuning")
 
     def sub_step(self, with_stats: bool = True):
         """One sub-step of every live scene; stats in live() order."""

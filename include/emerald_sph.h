@@ -220,6 +220,24 @@ emsph_status emsph_set_cfl(emsph_context* ctx, float factor);
  * once the sort pass count is known and kernel timing is off). Results are
  * identical either way. */
 emsph_status emsph_set_graphs(emsph_context* ctx, int enable);
+
+/* Execution choices that do not change any result. */
+typedef struct emsph_tuning {
+    int32_t persistent_loop; /* reserved: must be 0 (a one-launch cooperative pressure
+                                solve is not available; setting it returns
+                                EMSPH_ERR_INVALID_ARGUMENT) */
+    int32_t graphs;          /* as emsph_set_graphs */
+    int32_t reserved[6];
+} emsph_tuning;
+emsph_status emsph_get_tuning(emsph_context* ctx, emsph_tuning* out);
+emsph_status emsph_set_tuning(emsph_context* ctx, const emsph_tuning* tuning);
+/* Autotuner: times every candidate tuning on trial_sub_steps sub-steps of
+ * this context's own state (restored after each trial and at the end, so the
+ * simulation does not advance) and keeps the fastest. times_ms (up to 8
+ * entries, may be NULL) receives each candidate's time and n_candidates their
+ * count. No counterpart in the reference. */
+emsph_status emsph_autotune(emsph_context* ctx, int method, float dt, int trial_sub_steps,
+                            emsph_tuning* chosen, double* times_ms, int* n_candidates);
 emsph_status emsph_set_kernel_timing(emsph_context* ctx, int enable);
 int emsph_get_kernel_timing(emsph_context* ctx, const char** names, double* total_ms,
                             int64_t* calls, int max);
@@ -248,6 +266,9 @@ emsph_status emsph_batch_get_state(emsph_batch* b, int scene, float* pos_xy, flo
 emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* out);
 emsph_status emsph_batch_set_cfl(emsph_batch* b, float factor);  /* as emsph_set_cfl, per scene */
 emsph_status emsph_batch_set_graphs(emsph_batch* b, int enable);  /* as emsph_set_graphs */
+emsph_status emsph_batch_set_tuning(emsph_batch* b, const emsph_tuning* tuning);
+emsph_status emsph_batch_autotune(emsph_batch* b, int method, int trial_sub_steps,
+                                  emsph_tuning* chosen, double* times_ms, int* n_candidates);
 emsph_status emsph_batch_frame_init(emsph_batch* b, int method);
 /* One sub-step of every scene; stats (array of n_scenes, may be NULL). */
 emsph_status emsph_batch_sub_step(emsph_batch* b, int method, emsph_step_stats* stats);
@@ -281,6 +302,7 @@ emsph_status emsph_pool_commit(emsph_pool* pool);
  * returns the live count. ids may be NULL. */
 int emsph_pool_live(const emsph_pool* pool, int* ids, int max);
 emsph_status emsph_pool_set_cfl(emsph_pool* pool, float factor);
+emsph_status emsph_pool_set_tuning(emsph_pool* pool, const emsph_tuning* tuning);
 /* One sub-step of every live scene; stats: one per live scene (may be NULL). */
 emsph_status emsph_pool_sub_step(emsph_pool* pool, emsph_step_stats* stats);
 emsph_status emsph_pool_get_state(emsph_pool* pool, int id, float* pos_xy, float* vel_xy);

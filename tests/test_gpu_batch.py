@@ -112,3 +112,52 @@ def test_batch_cfl_dt_and_energy(gpu):
     assert cfl_bound > 0, "the CFL bound never went below the largest dt"
     for i, ctx in enumerate(alone):
         assert_bits_equal(batch.get_state(i).positions, ctx.get_state().positions, f"scene {i}")
+
+
+@pytest.mark.skip(reason="persistent cooperative solve disabled: a 1,792-workgroup cooperative "
+                         "launch of it did not finish on the MI355X and the cause was not "
+                         "established; set_tuning(persistent_loop=True) now raises")
+def test_persistent_solve_equals_launches(gpu):
+    # The IISPH pressure solve in one cooperative launch against one launch per
+    # sweep, on a mixed batch whose scenes converge at different iterations.
+    method = "iisph"
+    scs = _mixed_scenes()
+    cfgs = [scenes.config_for(sc, method) for sc in scs]
+    out = []
+    for persistent in (True, False):
+        batch = sb.Batch(cfgs, [sb.Solid_state(s.solids) for s in scs],
+                         [sb.State(s.positions, s.velocities) for s in scs])
+        batch.set_tuning(persistent_loop=persistent)
+        batch.set_cfl(0.4)
+        batch.frame_init(method)
+        its = [[st.iterations for st in batch.sub_step(method)] for _ in range(15)]
+        out.append((its, [batch.get_state(i) for i in range(len(scs))]))
+        batch.close()
+    assert out[0][0] == out[1][0]
+    assert len({tuple(x) for x in out[0][0]}) > 1 or len(set(out[0][0][0])) > 1
+    for i, (a, b) in enumerate(zip(out[0][1], out[1][1])):
+        assert_bits_equal(a.positions, b.positions, f"scene {i} positions")
+        assert_bits_equal(a.velocities, b.velocities, f"scene {i} velocities")
+
+
+def test_autotune_leaves_state(gpu):
+    method = "iisph"
+    sc = scenes.dam_break(30, 40, 0.005, 2e-4, seed=31)
+    cfg = scenes.config_for(sc, method)
+    runs = []
+    for tune in (True, False):
+        ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(sc.positions))
+        ctx.set_state(sb.State(sc.positions, sc.velocities))
+        ctx.frame_init(method)
+        for k in range(3):
+            ctx.sub_step(method, 0.0, sc.dt)
+        before = ctx.get_state()
+        if tune:
+            chosen, times = ctx.autotune(method, sc.dt, 3)
+            assert len(times) == 2 and chosen == ctx.tuning()
+            assert_bits_equal(ctx.get_state().positions, before.positions, "state after autotune")
+        its = [ctx.sub_step(method, 0.0, sc.dt).iterations for _ in range(6)]
+        runs.append((its, ctx.get_state()))
+        ctx.close()
+    assert runs[0][0] == runs[1][0]
+    assert_bits_equal(runs[0][1].positions, runs[1][1].positions, "positions")
