@@ -130,6 +130,8 @@ def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict
     batch.set_cfl(0.4)
     batch.frame_init(method)
     batch.run_sub_steps(method, warmup)
+    log("batched form: autotune")
+    chosen, cand_ms = batch.autotune(method, 2)
     t0 = time.perf_counter()
     batch.run_sub_steps(method, steps)
     t_batch = time.perf_counter() - t0
@@ -166,6 +168,7 @@ def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict
         "speedup_vs_plain": v_batch / v_plain, "bit_identical_to_plain": bool(same),
         "iterations_min_mean_max": [min(iters), float(np.mean(iters)), max(iters)],
         "dt": "per scene, CFL 0.4 on the device",
+        "tuning": {"chosen": chosen, "candidate_ms_per_2_sub_steps": cand_ms},
     }
 
 
@@ -265,6 +268,8 @@ def main() -> int:
 
     log(f"warmup {args.warmup} sub-steps")
     ctx.run_sub_steps(method, scene.dt, args.warmup)
+    log("autotune")
+    chosen, cand_ms = ctx.autotune(method, scene.dt, 5)
 
     log(f"timing {args.steps} sub-steps")
     ctx.iteration_totals(reset=True)
@@ -365,6 +370,9 @@ def main() -> int:
         },
         "roofline": roofline,
         "whole_step": whole_step,
+        "tuning": {"chosen": chosen, "candidate_ms_per_5_sub_steps": cand_ms,
+                   "candidates": "neighbour batch 2/4/8 x XCD remap on/off with graphs, "
+                                 "then the best without graphs"},
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,

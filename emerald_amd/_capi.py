@@ -95,7 +95,12 @@ class StepStats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """emsph_tuning."""
     _fields_ = [("persistent_loop", ctypes.c_int32), ("graphs", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 6)]
+                ("neighbor_batch", ctypes.c_int32), ("xcd_remap", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
+
+    def as_dict(self) -> dict:
+        return {"graphs": int(self.graphs), "neighbor_batch": int(self.neighbor_batch),
+                "xcd_remap": int(self.xcd_remap)}
 
 
 class FrameStats(ctypes.Structure):

@@ -319,8 +319,10 @@ struct Engine {
     // Captured graphs of the fixed segments of a sub-step, by what their launch
     // arguments depend on; layout_gen changes whenever buffers or sizes do.
     bool use_graphs = true;
+    int nb = 4;              // neighbour batch depth of the sweeps
+    bool xcd_remap = true;   // XCD-aware workgroup -> tile order
     uint64_t layout_gen = 1;
-    std::map<std::array<uint64_t, 8>, hipGraphExec_t> graphs;
+    std::map<std::array<uint64_t, 10>, hipGraphExec_t> graphs;
 
     ~Engine() {
         if (host) { (void)hipHostFree(host); }
@@ -478,6 +480,8 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     b.xstar = g->xstar.p;
     b.force = g->force.p;
     b.store_geometry = g->store_geometry ? 1 : 0;
+    b.nb = g->nb;
+    b.xcd_remap = g->xcd_remap ? 1 : 0;
     return b;
 }
 
@@ -922,10 +926,11 @@ void finish_prepare(Engine* g) {
 template <class F>
 hipError_t segment(Engine* g, int method, int seg, F&& enqueue) {
     if (!g->use_graphs || g->timer.enabled || g->passes_hint < 0) { return enqueue(); }
-    std::array<uint64_t, 8> const key{
+    std::array<uint64_t, 10> const key{
       static_cast<uint64_t>(method), static_cast<uint64_t>(seg),
       static_cast<uint64_t>(g->passes_hint), static_cast<uint64_t>(g->build_cap), g->layout_gen,
-      static_cast<uint64_t>(g->store_geometry), g->N, g->ntiles};
+      static_cast<uint64_t>(g->store_geometry), g->N, g->ntiles, static_cast<uint64_t>(g->nb),
+      static_cast<uint64_t>(g->xcd_remap)};
     auto it = g->graphs.find(key);
     if (it == g->graphs.end()) {
         // Graphs of an older layout can never be replayed again.
@@ -1526,6 +1531,8 @@ hipError_t snapshot_restore(Engine* g, Snapshot const& sn) {
 void tuning_get(Engine const* g, emsph_tuning* t) {
     std::memset(t, 0, sizeof(*t));
     t->graphs = g->use_graphs ? 1 : 0;
+    t->neighbor_batch = g->nb;
+    t->xcd_remap = g->xcd_remap ? 1 : 0;
 }
 
 // persistent_loop is refused: a cooperative launch of the whole IISPH solve
@@ -1536,7 +1543,12 @@ emsph_status tuning_set(Engine* g, emsph_tuning const* t) {
     if (t->persistent_loop) {
         return fail(EMSPH_ERR_INVALID_ARGUMENT, "persistent_loop is not available");
     }
+    if (t->neighbor_batch != 2 && t->neighbor_batch != 4 && t->neighbor_batch != 8) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "neighbor_batch must be 2, 4 or 8");
+    }
     g->use_graphs = t->graphs != 0;
+    g->nb = t->neighbor_batch;
+    g->xcd_remap = t->xcd_remap != 0;
     return EMSPH_OK;
 }
 
@@ -1547,12 +1559,19 @@ emsph_status tuning_set(Engine* g, emsph_tuning const* t) {
 emsph_status engine_autotune(Engine* g, int method, int trials, emsph_tuning* chosen,
                              double* times_ms, int* n_candidates) {
     if (trials < 1) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "trial_sub_steps < 1"); }
+    // Shapes first (neighbour batch depth x XCD remap, with graphs), then the
+    // best shape without graphs.
     std::vector<emsph_tuning> cands;
-    for (int graphs = 1; graphs >= 0; --graphs) {
-        emsph_tuning t{};
-        t.graphs = graphs;
-        cands.push_back(t);
+    for (int nb : {2, 4, 8}) {
+        for (int xcd = 1; xcd >= 0; --xcd) {
+            emsph_tuning t{};
+            t.graphs = 1;
+            t.neighbor_batch = nb;
+            t.xcd_remap = xcd;
+            cands.push_back(t);
+        }
     }
+    size_t const n_shapes = cands.size();
     Snapshot sn;
     HIP_TRY(snapshot_take(g, sn));
     double best = 0.0;
@@ -1575,6 +1594,11 @@ emsph_status engine_autotune(Engine* g, int method, int trials, emsph_tuning* ch
         if (k == 0 || ms < best) {
             best = ms;
             best_k = k;
+        }
+        if (k + 1 == n_shapes) {
+            emsph_tuning t = cands[best_k];
+            t.graphs = 0;
+            cands.push_back(t);
         }
     }
     ST_TRY(tuning_set(g, &cands[best_k]));

@@ -200,6 +200,10 @@ struct Batch {
     V2 const* vstar_alt;
     V2* force;
     int store_geometry;
+    // Tuning choices (results do not depend on them): neighbour batch depth of
+    // the sweeps (2, 4 or 8) and the XCD-aware workgroup -> tile remap.
+    int nb;
+    int xcd_remap;
 };
 
 enum BuildMode : int { BUILD_LISTS = 0, BUILD_IISPH = 1, BUILD_DFSPH = 2 };

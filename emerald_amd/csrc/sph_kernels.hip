@@ -32,8 +32,14 @@ struct TileRef {
     uint32_t tile;
     bool valid;
 };
+// This workgroup's tile: remapped so that each XCD walks one contiguous run of
+// tiles (Batch::xcd_remap, a tuning choice), or in launch order.
+__device__ __forceinline__ uint32_t tile_index(Batch const& b) {
+    return b.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+}
+
 __device__ __forceinline__ TileRef tile_ref(Batch const& b, uint32_t threads) {
-    uint32_t const k = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t const k = tile_index(b);
     Tile const tl = b.tiles[k];
     (void)threads;
     TileRef r;
@@ -743,7 +749,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
     __shared__ uint64_t ent[CAP][BUILD_THREADS];
     uint32_t const lane = threadIdx.x;
     // Build tiles are the fluid tiles split in quarters (64 threads).
-    uint32_t const k = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t const k = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     Tile const tl = a.tiles[k >> 2];
     uint32_t const sub_off = (k & 3u) * BUILD_THREADS + lane;
     if ((k & 3u) * BUILD_THREADS >= tl.count) { return; }  // the whole wave
@@ -834,22 +840,23 @@ void launch_solid_volumes(Batch const& b, uint32_t n, hipStream_t s) {
 }
 
 //------------------------------------------------------------------------------
-// Neighbour loops of the sweeps walk a list in batches of NB_BATCH entries:
+// Neighbour loops of the sweeps walk a list in batches of NB entries:
 // the batch's indices and gradients are loaded together, then the gathers of
 // the neighbours' fields, then the entries are summed in list order. This
 // keeps several dependent load chains in flight per thread without changing
 // the order of any floating-point sum.
-constexpr int NB_BATCH = 4;
-
+// The batch depth NB is a tuning choice (2, 4 or 8; Batch::nb).
+template <int NB>
 struct NbrBatch {
-    uint32_t j[NB_BATCH];
-    V2 g[NB_BATCH];
+    uint32_t j[NB];
+    V2 g[NB];
 };
 
+template <int NB>
 __device__ __forceinline__ void load_batch(Lists const& l, uint32_t stride, uint32_t t, int k0,
-                                           int cnt, NbrBatch& nb) {
+                                           int cnt, NbrBatch<NB>& nb) {
 #pragma unroll
-    for (int u = 0; u < NB_BATCH; ++u) {
+    for (int u = 0; u < NB; ++u) {
         nb.j[u] = 0u;
         nb.g[u] = V2{0.0f, 0.0f};
         if (k0 + u < cnt) {
@@ -864,6 +871,7 @@ __device__ __forceinline__ void load_batch(Lists const& l, uint32_t stride, uint
 // IISPH sweeps: sph2d_box/iisph_ops.cpp.
 
 // :94-221 a_ii and density stars; also zeroes the pressures (iisph.cpp:83).
+template <int NB>
 __global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     if (!r.valid) { return; }
@@ -884,13 +892,13 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
     } else {
         float acc = 0.0f;
         float dd = 0.0f;
-        for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
-            NbrBatch nb;
-            load_batch(a.fl, a.stride, t, k0, fc, nb);
-            float rj[NB_BATCH];
-            V2 vj[NB_BATCH];
+        for (int k0 = 0; k0 < fc; k0 += NB) {
+            NbrBatch<NB> nb;
+            load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+            float rj[NB];
+            V2 vj[NB];
 #pragma unroll
-            for (int u = 0; u < NB_BATCH; ++u) {
+            for (int u = 0; u < NB; ++u) {
                 rj[u] = 1.0f;
                 vj[u] = V2{0.0f, 0.0f};
                 if (k0 + u < fc) {
@@ -899,7 +907,7 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
                 }
             }
 #pragma unroll
-            for (int u = 0; u < NB_BATCH; ++u) {
+            for (int u = 0; u < NB; ++u) {
                 if (k0 + u < fc) {
                     V2 const gw = nb.g[u];
                     V2 const numer = scale(kf, gw);
@@ -937,6 +945,7 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
 }
 
 // :225-258 sum_j d_ij p_j (fluid only) of one tile's particles.
+template <int NB>
 __device__ __forceinline__ void iisph_sdp_tile(Batch const& a, Tile const& tl) {
     if (threadIdx.x >= tl.count) { return; }
     uint32_t const t = tl.start + threadIdx.x;
@@ -944,12 +953,12 @@ __device__ __forceinline__ void iisph_sdp_tile(Batch const& a, Tile const& tl) {
     float const kf = -(c.dt * c.dt) * (c.fluid_volume * c.rho0);
     int const fc = a.fl.count[t];
     V2 s{0.0f, 0.0f};
-    for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
-        NbrBatch nb;
-        load_batch(a.fl, a.stride, t, k0, fc, nb);
-        float rj[NB_BATCH], pj[NB_BATCH];
+    for (int k0 = 0; k0 < fc; k0 += NB) {
+        NbrBatch<NB> nb;
+        load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+        float rj[NB], pj[NB];
 #pragma unroll
-        for (int u = 0; u < NB_BATCH; ++u) {
+        for (int u = 0; u < NB; ++u) {
             rj[u] = 1.0f;
             pj[u] = 0.0f;
             if (k0 + u < fc) {
@@ -958,7 +967,7 @@ __device__ __forceinline__ void iisph_sdp_tile(Batch const& a, Tile const& tl) {
             }
         }
 #pragma unroll
-        for (int u = 0; u < NB_BATCH; ++u) {
+        for (int u = 0; u < NB; ++u) {
             if (k0 + u < fc) {
                 V2 const numer = scale(kf * pj[u], nb.g[u]);
                 float const denom = rj[u] * rj[u];
@@ -970,15 +979,17 @@ __device__ __forceinline__ void iisph_sdp_tile(Batch const& a, Tile const& tl) {
 }
 
 // Converged scenes skip the sweep.
+template <int NB>
 __global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
-    Tile const tl = a.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    Tile const tl = a.tiles[tile_index(a)];
     if (!a.stats[tl.scene].active) { return; }
-    iisph_sdp_tile(a, tl);
+    iisph_sdp_tile<NB>(a, tl);
 }
 
 // :261-353 beta (into the new-pressure buffer) fused with :357-425 new
 // pressure, quantised error sum and error max, for one tile (every thread of
 // the workgroup calls it).
+template <int NB>
 __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl, int slot) {
     unsigned long long qsum = 0;
     unsigned int emax = 0;
@@ -991,13 +1002,13 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
         V2 const si = a.sdp[t];
         int const fc = a.fl.count[t];
         float beta = 0.0f;
-        for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
-            NbrBatch nb;
-            load_batch(a.fl, a.stride, t, k0, fc, nb);
-            float rj[NB_BATCH], pj[NB_BATCH];
-            V2 sj[NB_BATCH], djj[NB_BATCH];
+        for (int k0 = 0; k0 < fc; k0 += NB) {
+            NbrBatch<NB> nb;
+            load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+            float rj[NB], pj[NB];
+            V2 sj[NB], djj[NB];
 #pragma unroll
-            for (int u = 0; u < NB_BATCH; ++u) {
+            for (int u = 0; u < NB; ++u) {
                 rj[u] = 1.0f;
                 pj[u] = 0.0f;
                 sj[u] = V2{0.0f, 0.0f};
@@ -1011,7 +1022,7 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
                 }
             }
 #pragma unroll
-            for (int u = 0; u < NB_BATCH; ++u) {
+            for (int u = 0; u < NB; ++u) {
                 if (k0 + u < fc) {
                     V2 const gw = nb.g[u];
                     V2 const numer = scale(kf, gw);
@@ -1058,15 +1069,16 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
 
 // A converged scene skips the sweep; its result stays in the buffer of its
 // last iteration.
+template <int NB>
 __global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, int slot) {
-    Tile const tl = a.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    Tile const tl = a.tiles[tile_index(a)];
     if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
-    iisph_beta_p_tile(a, tl, slot);
+    iisph_beta_p_tile<NB>(a, tl, slot);
 }
 
 //------------------------------------------------------------------------------
 // pressure.cpp:9-80 apply_pressures_in_place (fluid pass, then solid pass).
-template <int GRAVITY, int FINAL>
+template <int GRAVITY, int FINAL, int NB>
 __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
     TileRef const r = tile_ref(a, TILE);
     if (!all_scenes && !a.stats[r.scene].active) { return; }  // whole workgroup
@@ -1100,12 +1112,12 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
             if (fc) {
                 float const mf = c.rho0 * c.fluid_volume;
                 V2 acc{0.0f, 0.0f};
-                for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
-                    NbrBatch nb;
-                    load_batch(a.fl, a.stride, t, k0, fc, nb);
-                    float rj[NB_BATCH], pj[NB_BATCH];
+                for (int k0 = 0; k0 < fc; k0 += NB) {
+                    NbrBatch<NB> nb;
+                    load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+                    float rj[NB], pj[NB];
 #pragma unroll
-                    for (int u = 0; u < NB_BATCH; ++u) {
+                    for (int u = 0; u < NB; ++u) {
                         rj[u] = 1.0f;
                         pj[u] = 0.0f;
                         if (k0 + u < fc) {
@@ -1114,7 +1126,7 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
                         }
                     }
 #pragma unroll
-                    for (int u = 0; u < NB_BATCH; ++u) {
+                    for (int u = 0; u < NB; ++u) {
                         if (k0 + u < fc) {
                             float const rj2 = rj[u] * rj[u];
                             float const porj = is_safe_div(pj[u], rj2) ? pj[u] / rj2 : 0.0f;
@@ -1182,22 +1194,23 @@ __global__ __launch_bounds__(TILE) void k_finalize(Batch a) {
 //------------------------------------------------------------------------------
 // DFSPH_P sweeps: divergence.cpp:5-74 and dfsph_p_ops.cpp:87-126, with the
 // residual of common.h:209-232 fused in.
+template <int NB>
 __device__ __forceinline__ float divergence_of(Batch const& a, StepConsts const& c, uint32_t t,
                                                V2 vi) {
     float const mf = c.rho0 * c.fluid_volume;
     int const fc = a.fl.count[t];
     float div = 0.0f;
-    for (int k0 = 0; k0 < fc; k0 += NB_BATCH) {
-        NbrBatch nb;
-        load_batch(a.fl, a.stride, t, k0, fc, nb);
-        V2 vj[NB_BATCH];
+    for (int k0 = 0; k0 < fc; k0 += NB) {
+        NbrBatch<NB> nb;
+        load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+        V2 vj[NB];
 #pragma unroll
-        for (int u = 0; u < NB_BATCH; ++u) {
+        for (int u = 0; u < NB; ++u) {
             vj[u] = V2{0.0f, 0.0f};
             if (k0 + u < fc) { vj[u] = a.vel[nb.j[u]]; }
         }
 #pragma unroll
-        for (int u = 0; u < NB_BATCH; ++u) {
+        for (int u = 0; u < NB; ++u) {
             if (k0 + u < fc) { div += mf * dot(sub(vi, vj[u]), nb.g[u]); }
         }
     }
@@ -1217,6 +1230,7 @@ __device__ __forceinline__ float divergence_of(Batch const& a, StepConsts const&
 
 // One divergence-solve iteration: divergence, kappa_v, residual of
 // max(0, dt * div / rho0) (dfsph_p.cpp:157-197).
+template <int NB>
 __global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a, int slot) {
     TileRef const r = tile_ref(a, TILE);
     if (!a.stats[r.scene].active) { return; }
@@ -1225,7 +1239,7 @@ __global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a, int slot) {
     if (r.valid) {
         uint32_t const t = r.t;
         StepConsts const c = a.scenes[r.scene].c;
-        float const div = divergence_of(a, c, t, a.vel[t]);
+        float const div = divergence_of<NB>(a, c, t, a.vel[t]);
         a.rho_star[t] = div;
         float kv = 0.0f;
         if (!(div <= 0.0f)) {
@@ -1242,6 +1256,7 @@ __global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a, int slot) {
 
 // One density-solve iteration: density star (density.cpp:111-145), kappa_rho,
 // residual of max(0, (rho* - rho0) / rho0) (dfsph_p.cpp:82-122).
+template <int NB>
 __global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a, int slot) {
     TileRef const r = tile_ref(a, TILE);
     if (!a.stats[r.scene].active) { return; }
@@ -1250,7 +1265,7 @@ __global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a, int slot) {
     if (r.valid) {
         uint32_t const t = r.t;
         StepConsts const c = a.scenes[r.scene].c;
-        float const div = divergence_of(a, c, t, a.vel[t]);
+        float const div = divergence_of<NB>(a, c, t, a.vel[t]);
         float rs = a.rho[t] + (c.dt * div);
         rs = (0.0f < rs) ? rs : 0.0f;  // std::max(0.0f, ...)
         a.rho_star[t] = rs;
@@ -1499,25 +1514,49 @@ __global__ __launch_bounds__(256) void k_export_lists(Lists l, uint32_t n, uint3
 //------------------------------------------------------------------------------
 static inline dim3 tile_grid(Batch const& b) { return dim3(b.ntiles ? b.ntiles : 1); }
 
+// Kernel instantiation for the batch depth Batch::nb (2, 4 or 8).
+#define NB_DISPATCH(nb, KERNEL, ...)                                                  \
+    do {                                                                              \
+        if ((nb) <= 2) {                                                              \
+            hipLaunchKernelGGL((KERNEL<2>), __VA_ARGS__);                             \
+        } else if ((nb) <= 4) {                                                       \
+            hipLaunchKernelGGL((KERNEL<4>), __VA_ARGS__);                             \
+        } else {                                                                      \
+            hipLaunchKernelGGL((KERNEL<8>), __VA_ARGS__);                             \
+        }                                                                             \
+    } while (0)
+
 void launch_iisph_aii(Batch const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_iisph_aii, tile_grid(a), dim3(TILE), 0, s, a);
+    NB_DISPATCH(a.nb, k_iisph_aii, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_iisph_sdp(Batch const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_iisph_sdp, tile_grid(a), dim3(TILE), 0, s, a);
+    NB_DISPATCH(a.nb, k_iisph_sdp, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_iisph_beta_p(Batch const& a, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a, slot);
+    NB_DISPATCH(a.nb, k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a, slot);
 }
+
+template <int G, int F>
+void launch_apply_nb(Batch const& a, int all_scenes, hipStream_t s) {
+    if (a.nb <= 2) {
+        hipLaunchKernelGGL((k_apply<G, F, 2>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+    } else if (a.nb <= 4) {
+        hipLaunchKernelGGL((k_apply<G, F, 4>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+    } else {
+        hipLaunchKernelGGL((k_apply<G, F, 8>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+    }
+}
+
 void launch_apply(Batch const& a, int with_gravity, int finalize, int all_scenes,
                   hipStream_t s) {
     if (with_gravity && finalize) {
-        hipLaunchKernelGGL((k_apply<1, 1>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+        launch_apply_nb<1, 1>(a, all_scenes, s);
     } else if (with_gravity) {
-        hipLaunchKernelGGL((k_apply<1, 0>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+        launch_apply_nb<1, 0>(a, all_scenes, s);
     } else if (finalize) {
-        hipLaunchKernelGGL((k_apply<0, 1>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+        launch_apply_nb<0, 1>(a, all_scenes, s);
     } else {
-        hipLaunchKernelGGL((k_apply<0, 0>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+        launch_apply_nb<0, 0>(a, all_scenes, s);
     }
 }
 void launch_finalize(Batch const& a, int dfsph, hipStream_t s) {
@@ -1528,10 +1567,10 @@ void launch_finalize(Batch const& a, int dfsph, hipStream_t s) {
     }
 }
 void launch_dfsph_div(Batch const& a, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_dfsph_div, tile_grid(a), dim3(TILE), 0, s, a, slot);
+    NB_DISPATCH(a.nb, k_dfsph_div, tile_grid(a), dim3(TILE), 0, s, a, slot);
 }
 void launch_dfsph_dens(Batch const& a, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_dfsph_dens, tile_grid(a), dim3(TILE), 0, s, a, slot);
+    NB_DISPATCH(a.nb, k_dfsph_dens, tile_grid(a), dim3(TILE), 0, s, a, slot);
 }
 void launch_pci_init(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_init, tile_grid(a), dim3(TILE), 0, s, a);

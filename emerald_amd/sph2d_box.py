@@ -295,10 +295,11 @@ class Context:
     def tuning(self) -> dict:
         t = _capi.Tuning()
         check(lib().emsph_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
-        return {"persistent_loop": int(t.persistent_loop), "graphs": int(t.graphs)}
+        return t.as_dict()
 
-    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True) -> None:
-        t = _capi.Tuning(int(persistent_loop), int(graphs))
+    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True,
+                   neighbor_batch: int = 4, xcd_remap: bool = True) -> None:
+        t = _capi.Tuning(int(persistent_loop), int(graphs), int(neighbor_batch), int(xcd_remap))
         check(lib().emsph_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
     def autotune(self, method: str, dt: float, trial_sub_steps: int = 5):
@@ -309,8 +310,7 @@ class Context:
         n = ctypes.c_int()
         check(lib().emsph_autotune(self._h, _capi.METHODS[method], float(dt), int(trial_sub_steps),
                                    ctypes.byref(t), times, ctypes.byref(n)), "autotune")
-        return ({"persistent_loop": int(t.persistent_loop), "graphs": int(t.graphs)},
-                list(times[:n.value]))
+        return (t.as_dict(), list(times[:n.value]))
 
     def iteration_totals(self, reset: bool = False) -> tuple:
         """(sub-steps, pressure or density iterations, divergence iterations)
@@ -448,8 +448,9 @@ class Batch:
     def set_graphs(self, enable: bool) -> None:
         check(lib().emsph_batch_set_graphs(self._h, 1 if enable else 0), "batch_set_graphs")
 
-    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True) -> None:
-        t = _capi.Tuning(int(persistent_loop), int(graphs))
+    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True,
+                   neighbor_batch: int = 4, xcd_remap: bool = True) -> None:
+        t = _capi.Tuning(int(persistent_loop), int(graphs), int(neighbor_batch), int(xcd_remap))
         check(lib().emsph_batch_set_tuning(self._h, ctypes.byref(t)), "batch_set_tuning")
 
     def autotune(self, method: str, trial_sub_steps: int = 3):
@@ -459,8 +460,7 @@ class Batch:
         check(lib().emsph_batch_autotune(self._h, _capi.METHODS[method], int(trial_sub_steps),
                                          ctypes.byref(t), times, ctypes.byref(n)),
               "batch_autotune")
-        return ({"persistent_loop": int(t.persistent_loop), "graphs": int(t.graphs)},
-                list(times[:n.value]))
+        return (t.as_dict(), list(times[:n.value]))
 
     def frame_init(self, method: str) -> None:
         check(lib().emsph_batch_frame_init(self._h, _capi.METHODS[method]), "batch_frame_init")
@@ -532,8 +532,9 @@ class Pool:
     def set_cfl(self, factor: float) -> None:
         check(lib().emsph_pool_set_cfl(self._h, float(factor)), "pool_set_cfl")
 
-    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True) -> None:
-        t = _capi.Tuning(int(persistent_loop), int(graphs))
+    def set_tuning(self, persistent_loop: bool = False, graphs: bool = True,
+                   neighbor_batch: int = 4, xcd_remap: bool = True) -> None:
+        t = _capi.Tuning(int(persistent_loop), int(graphs), int(neighbor_batch), int(xcd_remap))
         check(lib().emsph_pool_set_tuning(self._h, ctypes.byref(t)), "pool_set_tuning")
 
     def sub_step(self, with_stats: bool = True):

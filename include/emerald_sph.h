@@ -227,15 +227,20 @@ typedef struct emsph_tuning {
                                 solve is not available; setting it returns
                                 EMSPH_ERR_INVALID_ARGUMENT) */
     int32_t graphs;          /* as emsph_set_graphs */
-    int32_t reserved[6];
+    int32_t neighbor_batch;  /* neighbour-list entries each thread of a sweep has in flight
+                                (2, 4 or 8; sums stay in list order) */
+    int32_t xcd_remap;       /* workgroups take tiles so that each of the 8 XCDs walks one
+                                contiguous run of Morton-sorted space (1) or in launch order */
+    int32_t reserved[4];
 } emsph_tuning;
 emsph_status emsph_get_tuning(emsph_context* ctx, emsph_tuning* out);
 emsph_status emsph_set_tuning(emsph_context* ctx, const emsph_tuning* tuning);
-/* Autotuner: times every candidate tuning on trial_sub_steps sub-steps of
- * this context's own state (restored after each trial and at the end, so the
- * simulation does not advance) and keeps the fastest. times_ms (up to 8
- * entries, may be NULL) receives each candidate's time and n_candidates their
- * count. No counterpart in the reference. */
+/* Autotuner: times every candidate tuning (neighbour batch depth x XCD remap
+ * with graphs, then the best of those without graphs) on trial_sub_steps
+ * sub-steps of this context's own state (restored after each trial and at the
+ * end, so the simulation does not advance) and keeps the fastest. times_ms (up
+ * to 8 entries, may be NULL) receives each candidate's time and n_candidates
+ * their count. No counterpart in the reference. */
 emsph_status emsph_autotune(emsph_context* ctx, int method, float dt, int trial_sub_steps,
                             emsph_tuning* chosen, double* times_ms, int* n_candidates);
 emsph_status emsph_set_kernel_timing(emsph_context* ctx, int enable);

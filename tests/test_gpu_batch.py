@@ -154,10 +154,34 @@ def test_autotune_leaves_state(gpu):
         before = ctx.get_state()
         if tune:
             chosen, times = ctx.autotune(method, sc.dt, 3)
-            assert len(times) == 2 and chosen == ctx.tuning()
+            assert len(times) == 7 and chosen == ctx.tuning()
             assert_bits_equal(ctx.get_state().positions, before.positions, "state after autotune")
         its = [ctx.sub_step(method, 0.0, sc.dt).iterations for _ in range(6)]
         runs.append((its, ctx.get_state()))
         ctx.close()
     assert runs[0][0] == runs[1][0]
     assert_bits_equal(runs[0][1].positions, runs[1][1].positions, "positions")
+
+
+@pytest.mark.parametrize("method", ["iisph", "dfsph_p"])
+def test_tuning_shapes_equal(gpu, method):
+    # Every tuning shape (neighbour batch depth, XCD remap) gives the same bits.
+    scs = _mixed_scenes()
+    cfgs = [scenes.config_for(sc, method) for sc in scs]
+    ref = None
+    for nb in (2, 4, 8):
+        for xcd in (True, False):
+            batch = sb.Batch(cfgs, [sb.Solid_state(s.solids) for s in scs],
+                             [sb.State(s.positions, s.velocities) for s in scs])
+            batch.set_tuning(neighbor_batch=nb, xcd_remap=xcd)
+            batch.frame_init(method)
+            its = [[st.iterations for st in batch.sub_step(method)] for _ in range(8)]
+            got = (its, [batch.get_state(i).positions for i in range(len(scs))])
+            batch.close()
+            if ref is None:
+                ref = got
+                continue
+            assert got[0] == ref[0], (nb, xcd)
+            for i, (a, b) in enumerate(zip(got[1], ref[1])):
+                assert_bits_equal(a, b, f"nb {nb} xcd {xcd} scene {i}")
+
