@@ -277,7 +277,7 @@ def main() -> int:
     dev_ms, st = ctx.run_sub_steps(method, scene.dt, args.steps)
     wall = time.perf_counter() - t0
     value = n * args.steps / wall
-    _, iters, div_iters = ctx.iteration_totals(reset=True)
+    _, iters, div_iters, _ = ctx.iteration_totals(reset=True)
     k_mean = iters / args.steps
     kdiv_mean = div_iters / args.steps
     bytes_step = step_bytes(method, k_mean, kdiv_mean)

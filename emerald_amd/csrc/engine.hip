@@ -293,6 +293,8 @@ struct Engine {
     DBuf<uint3> repack_segments;
     DBuf<uint2> overflow;   // particles deferred by the capped neighbour build
     DBuf<double2> tile_energy;
+    DBuf<Reduction> tile_red;
+    DBuf<unsigned int> tile_vmax;
     int build_cap = 32;     // LDS list capacity of the next build (from the last max count)
 
     SpatialIndex sidx;
@@ -311,7 +313,9 @@ struct Engine {
     // (the maximum over the batch's scenes), and running totals.
     int last_iterations = 0;
     int last_div_iterations = 0;
-    int64_t totals[3] = {0, 0, 0};  // sub-steps, pressure/density iterations, divergence iterations
+    // sub-steps, pressure/density iterations, divergence iterations, sub-steps
+    // redone after a short sort
+    int64_t totals[4] = {0, 0, 0, 0};
 
     hipEvent_t read_ev[2] = {nullptr, nullptr};
     KeyState keys{};  // the last key state read back
@@ -457,6 +461,8 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     b.swin = g->swin.p;
     b.overflow = g->overflow.p;
     b.tile_energy = g->tile_energy.p;
+    b.tile_red = g->tile_red.p;
+    b.tile_vmax = g->tile_vmax.p;
     if (g->NS) {
         b.solids = SolidView{g->s_pos.p, g->s_vel.p, g->s_vol.p, g->sidx.rank.p, g->sidx.orig.p};
     }
@@ -485,6 +491,22 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     return b;
 }
 
+// Device scene entry of one host scene whose tiles start at first_tile.
+SceneDev scene_dev(SceneHost const& h, uint32_t first_tile) {
+    SceneDev sd{};
+    sd.c = step_consts(h.cfg, h.dt);
+    sd.thr = thresholds(h.cfg.params);
+    sd.offset = h.offset;
+    sd.n = h.n;
+    sd.soffset = h.soffset;
+    sd.ns = h.ns;
+    sd.first_tile = first_tile;
+    sd.ntiles = (h.n + TILE - 1) / TILE;
+    sd.cfl = h.cfl;
+    sd.dt_max = h.dt;
+    return sd;
+}
+
 // Scene table and tiles from the host scene list; fluid slots are packed at
 // each scene's offset (offsets are prefix sums of the scene capacities).
 emsph_status upload_layout(Engine* g) {
@@ -493,16 +515,7 @@ emsph_status upload_layout(Engine* g) {
     std::vector<Tile> tl;
     for (uint32_t s = 0; s < g->M; ++s) {
         SceneHost const& h = g->sh[s];
-        sd[s].c = step_consts(h.cfg, h.dt);
-        sd[s].thr = thresholds(h.cfg.params);
-        sd[s].offset = h.offset;
-        sd[s].n = h.n;
-        sd[s].soffset = h.soffset;
-        sd[s].ns = h.ns;
-        sd[s].first_tile = static_cast<uint32_t>(tl.size());
-        sd[s].ntiles = (h.n + TILE - 1) / TILE;
-        sd[s].cfl = h.cfl;
-        sd[s].dt_max = h.dt;
+        sd[s] = scene_dev(h, static_cast<uint32_t>(tl.size()));
         for (uint32_t k = 0; k < h.n; k += TILE) {
             tl.push_back(Tile{s, h.offset + k, std::min<uint32_t>(TILE, h.n - k), 0});
         }
@@ -511,6 +524,8 @@ emsph_status upload_layout(Engine* g) {
     HIP_TRY(g->scenes.ensure(std::max<uint32_t>(g->M, 1)));
     HIP_TRY(g->tiles.ensure(std::max<size_t>(tl.size(), 1)));
     HIP_TRY(g->tile_energy.ensure(std::max<size_t>(tl.size(), 1)));
+    HIP_TRY(g->tile_red.ensure(std::max<size_t>(tl.size(), 1)));
+    HIP_TRY(g->tile_vmax.ensure(std::max<size_t>(tl.size(), 1)));
     HIP_TRY(g->scene_of.ensure(std::max<uint32_t>(g->cap, 1)));
     HIP_TRY(g->stats.ensure(std::max<uint32_t>(g->M, 1)));
     HIP_TRY(g->red.ensure(static_cast<size_t>(std::max<uint32_t>(g->M, 1)) * MAX_RED_SLOTS));
@@ -984,7 +999,7 @@ emsph_status iisph_sub_step(Engine* g) {
         b.p_in = (it & 1) ? pb : pa;
         b.p_out = (it & 1) ? pa : pb;
         HIP_PASS(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
-        HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, it, g->stream); }));
+        HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, g->stream); }));
         return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); });
     }, &g->last_iterations, &short_sort));
     if (short_sort) { return EMSPH_OK; }
@@ -998,7 +1013,7 @@ emsph_status iisph_sub_step(Engine* g) {
     b.p_alt = g->p1.p;
     HIP_TRY(segment(g, EMSPH_IISPH, SEG_SUFFIX, [&]() -> hipError_t {
         HIP_PASS(timed(g, "apply_integrate", [&] { launch_apply(b, 0, 1, 1, g->stream); }));
-        launch_scene_energy(b, g->stream);
+        launch_scene_totals(b, 1, g->stream);
         return hipGetLastError();
     }));
     g->cur_p = g->p0.p;
@@ -1024,7 +1039,7 @@ emsph_status dfsph_sub_step(Engine* g) {
     finish_prepare(g);
     bool short_sort = false;
     HIP_TRY(run_loop(g, 1, max_it, 1, [&](int it) -> hipError_t {
-        HIP_PASS(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, it, g->stream); }));
+        HIP_PASS(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, g->stream); }));
         HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
         return timed(g, "loop", [&] {
             launch_loop_decide(b, LOOP_DIVERGENCE, it, it, g->stream);
@@ -1040,7 +1055,7 @@ emsph_status dfsph_sub_step(Engine* g) {
         return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
     }));
     HIP_TRY(run_loop(g, 2, max_it, 2, [&](int it) -> hipError_t {
-        HIP_PASS(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, base + it, g->stream); }));
+        HIP_PASS(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, g->stream); }));
         HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
         return timed(g, "loop", [&] {
             launch_loop_decide(b, LOOP_DENSITY, it, base + it, g->stream);
@@ -1053,7 +1068,7 @@ emsph_status dfsph_sub_step(Engine* g) {
     }
     HIP_TRY(segment(g, EMSPH_DFSPH_P, SEG_SUFFIX, [&]() -> hipError_t {
         HIP_PASS(timed(g, "finalize", [&] { launch_finalize(b, 1, g->stream); }));
-        launch_scene_energy(b, g->stream);
+        launch_scene_totals(b, 1, g->stream);
         return hipGetLastError();
     }));
     g->cur_p = g->kds.p;
@@ -1078,7 +1093,7 @@ emsph_status pcisph_sub_step(Engine* g) {
     finish_prepare(g);
     bool short_sort = false;
     HIP_TRY(run_loop(g, 3, 6, 2, [&](int it) -> hipError_t {
-        HIP_PASS(timed(g, "pci_density", [&] { launch_pci_density(b, it, g->stream); }));
+        HIP_PASS(timed(g, "pci_density", [&] { launch_pci_density(b, g->stream); }));
         b.vstar_in = (it & 1) ? vout : vin;
         b.vstar_out = (it & 1) ? vin : vout;
         HIP_PASS(timed(g, "pci_force", [&] { launch_pci_force(b, g->stream); }));
@@ -1095,7 +1110,7 @@ emsph_status pcisph_sub_step(Engine* g) {
     b.vstar_alt = g->vstar1.p;
     HIP_TRY(segment(g, EMSPH_PCISPH, SEG_SUFFIX, [&]() -> hipError_t {
         HIP_PASS(timed(g, "pci_finalize", [&] { launch_pci_finalize(b, g->stream); }));
-        launch_scene_energy(b, g->stream);
+        launch_scene_totals(b, 1, g->stream);
         return hipGetLastError();
     }));
     g->cur_p = g->p0.p;
@@ -1118,7 +1133,10 @@ emsph_status engine_sub_step(Engine* g, int method) {
     // first read-back; when it was short the sub-step is redone with the exact
     // count (the second attempt sorts exactly and cannot be short).
     for (int attempt = 0; attempt < 2; ++attempt) {
-        if (attempt == 1) { g->passes_hint = -1; }
+        if (attempt == 1) {
+            g->passes_hint = -1;
+            g->totals[3] += 1;
+        }
         switch (method) {
         case EMSPH_IISPH: ST_TRY(iisph_sub_step(g)); break;
         case EMSPH_DFSPH_P: ST_TRY(dfsph_sub_step(g)); break;
@@ -1319,14 +1337,9 @@ emsph_status engine_solid_volumes(Engine* g, uint32_t s, float* out) {
 emsph_status engine_set_dt(Engine* g, uint32_t s, float dt) {
     if (g->sh[s].dt != dt) {
         g->sh[s].dt = dt;
-        SceneDev sd{};
-        SceneHost const& h = g->sh[s];
-        sd.c = step_consts(h.cfg, h.dt);
-        sd.thr = thresholds(h.cfg.params);
-        sd.offset = h.offset;
-        sd.n = h.n;
-        sd.soffset = h.soffset;
-        sd.ns = h.ns;
+        uint32_t first_tile = 0;
+        for (uint32_t k = 0; k < s; ++k) { first_tile += (g->sh[k].n + TILE - 1) / TILE; }
+        SceneDev const sd = scene_dev(g->sh[s], first_tile);
         HIP_TRY(hipMemcpyAsync(g->scenes.p + s, &sd, sizeof(SceneDev), hipMemcpyHostToDevice,
                                g->stream));
     }
@@ -1472,7 +1485,7 @@ struct Snapshot {
     int passes_hint = -1;
     int build_cap = 32;
     int loop_expect[4] = {64, 64, 64, 64};
-    int64_t totals[3] = {0, 0, 0};
+    int64_t totals[4] = {0, 0, 0, 0};
 };
 
 hipError_t snapshot_take(Engine* g, Snapshot& sn) {
@@ -1500,7 +1513,7 @@ hipError_t snapshot_take(Engine* g, Snapshot& sn) {
     sn.passes_hint = g->passes_hint;
     sn.build_cap = g->build_cap;
     std::copy(g->loop_expect, g->loop_expect + 4, sn.loop_expect);
-    std::copy(g->totals, g->totals + 3, sn.totals);
+    std::copy(g->totals, g->totals + 4, sn.totals);
     return hipSuccess;
 }
 
@@ -1523,7 +1536,7 @@ hipError_t snapshot_restore(Engine* g, Snapshot const& sn) {
     g->passes_hint = sn.passes_hint;
     g->build_cap = sn.build_cap;
     std::copy(sn.loop_expect, sn.loop_expect + 4, g->loop_expect);
-    std::copy(sn.totals, sn.totals + 3, g->totals);
+    std::copy(sn.totals, sn.totals + 4, g->totals);
     g->have_lists = false;
     return hipSuccess;
 }
@@ -1823,10 +1836,10 @@ emsph_status emsph_run_sub_steps(emsph_context* c, int method, float dt, int n_s
 emsph_status emsph_get_iteration_totals(emsph_context* c, int reset, int64_t* out) {
     if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
     if (out) {
-        for (int k = 0; k < 3; ++k) { out[k] = c->e.totals[k]; }
+        for (int k = 0; k < 4; ++k) { out[k] = c->e.totals[k]; }
     }
     if (reset) {
-        for (int k = 0; k < 3; ++k) { c->e.totals[k] = 0; }
+        for (int k = 0; k < 4; ++k) { c->e.totals[k] = 0; }
     }
     return EMSPH_OK;
 }

@@ -170,6 +170,8 @@ struct Batch {
     uint2 const* swin;
     uint2* overflow;        // {slot, scene} of particles deferred to the full-cap pass
     double2* tile_energy;   // per fluid tile: FP64 kinetic, potential energy
+    Reduction* tile_red;    // per fluid tile: residual partial of the last sweep
+    unsigned int* tile_vmax;  // per fluid tile: max |v|^2 bits of the final kernel
     Lists fl;
     Lists sl;
     float* rho;
@@ -281,7 +283,7 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 // of scenes that use one.
 void launch_step_begin(Batch const& b, hipStream_t s);
 // Per-scene energy sums over the tile partials written by the final kernel.
-void launch_scene_energy(Batch const& b, hipStream_t s);
+void launch_scene_totals(Batch const& b, int energy, hipStream_t s);
 // scene_of of every slot covered by the tiles.
 void launch_scene_of(Tile const* tiles, uint32_t ntiles, uint32_t* scene_of, hipStream_t s);
 // Pool repacking: segments {src, dst, n} of the state arrays into the scratch
@@ -300,17 +302,17 @@ void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t 
 void launch_build(int mode, int cap, Batch const& b, hipStream_t s);
 void launch_iisph_aii(Batch const& b, hipStream_t s);
 void launch_iisph_sdp(Batch const& b, hipStream_t s);
-void launch_iisph_beta_p(Batch const& b, int slot, hipStream_t s);
+void launch_iisph_beta_p(Batch const& b, hipStream_t s);
 // Apply pressure accelerations (pressure.cpp:9-80) with p_in as the pressure,
 // for active scenes (or all scenes when all_scenes != 0).
 // with_gravity: first integrate the external gravity force (dynamics.cpp:24-37).
 // finalize: then x += dt v and scatter state to original order (dynamics.cpp:46-51).
 void launch_apply(Batch const& b, int with_gravity, int finalize, int all_scenes, hipStream_t s);
 void launch_finalize(Batch const& b, int dfsph, hipStream_t s);
-void launch_dfsph_div(Batch const& b, int slot, hipStream_t s);
-void launch_dfsph_dens(Batch const& b, int slot, hipStream_t s);
+void launch_dfsph_div(Batch const& b, hipStream_t s);
+void launch_dfsph_dens(Batch const& b, hipStream_t s);
 void launch_pci_init(Batch const& b, hipStream_t s);
-void launch_pci_density(Batch const& b, int slot, hipStream_t s);
+void launch_pci_density(Batch const& b, hipStream_t s);
 void launch_pci_force(Batch const& b, hipStream_t s);
 void launch_pci_finalize(Batch const& b, hipStream_t s);
 void launch_vmax2(Batch const& b, hipStream_t s);

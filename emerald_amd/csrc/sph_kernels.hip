@@ -50,11 +50,14 @@ __device__ __forceinline__ TileRef tile_ref(Batch const& b, uint32_t threads) {
     return r;
 }
 
-// Quantised residual accumulation over a workgroup followed by one atomic per
-// workgroup. Every thread of the block must call it.
+// Workgroup partials, written to the workgroup's own tile slot (no atomics:
+// one address taken by every workgroup serialises at the device-coherent level
+// across the XCDs). The consumer reduces a scene's tile slots: the loop
+// decision (residuals) and the end-of-sub-step scene totals (max |v|^2).
+// Quantised residual: integer sum and max of the (non-negative) float bits.
 template <int THREADS>
-__device__ __forceinline__ void block_reduce_residual(unsigned long long v, unsigned int m,
-                                                      Reduction* r) {
+__device__ __forceinline__ void block_residual(unsigned long long v, unsigned int m,
+                                               Reduction* out) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         v += __shfl_xor(v, off);
@@ -75,13 +78,14 @@ __device__ __forceinline__ void block_reduce_residual(unsigned long long v, unsi
             tv += sv[w];
             tm = sm[w] > tm ? sm[w] : tm;
         }
-        if (tv) { atomicAdd(&r->isum, tv); }
-        if (tm) { atomicMax(&r->max_bits, tm); }
+        out->isum = tv;
+        out->max_bits = tm;
+        out->pad = 0;
     }
 }
 
 template <int THREADS>
-__device__ __forceinline__ void block_reduce_max(unsigned int m, unsigned int* out) {
+__device__ __forceinline__ void block_max(unsigned int m, unsigned int* out) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         unsigned int const o = __shfl_xor(m, off);
@@ -93,7 +97,7 @@ __device__ __forceinline__ void block_reduce_max(unsigned int m, unsigned int* o
     if (threadIdx.x == 0) {
         unsigned int tm = 0;
         for (int w = 0; w < THREADS / 64; ++w) { tm = sm[w] > tm ? sm[w] : tm; }
-        if (tm) { atomicMax(out, tm); }
+        *out = tm;
     }
 }
 
@@ -303,8 +307,11 @@ __global__ __launch_bounds__(256) void k_step_begin(Batch b) {
             }
             sd->c.dt = dt;
         }
+        // max |v|^2 stays until the end of this sub-step replaces it: a
+        // sub-step redone after a short sort recomputes the same dt.
         SceneStats st{};
         st.dt = sd->c.dt;
+        st.vmax2_bits = b.stats[s].vmax2_bits;
         b.stats[s] = st;
         for (int k = 0; k < MAX_RED_SLOTS; ++k) {
             b.red[s * MAX_RED_SLOTS + k].isum = 0;
@@ -403,14 +410,30 @@ __device__ __forceinline__ void loop_decide_scene(Batch const& b, int kind, int 
     }
 }
 
+// One workgroup per active scene: the sweep's tile partials of the scene
+// (integer sum and max, so the order does not matter) into the scene's slot,
+// then the exit test.
 __global__ __launch_bounds__(256) void k_loop_decide(Batch b, int kind, int it, int slot) {
-    uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < b.nscenes) { loop_decide_scene(b, kind, it, slot, s); }
+    for (uint32_t s = blockIdx.x; s < b.nscenes; s += gridDim.x) {
+        if (!b.stats[s].active) { continue; }  // the same for the whole workgroup
+        SceneDev const& sd = b.scenes[s];
+        unsigned long long v = 0;
+        unsigned int m = 0;
+        for (uint32_t k = threadIdx.x; k < sd.ntiles; k += blockDim.x) {
+            Reduction const r = b.tile_red[sd.first_tile + k];
+            v += r.isum;
+            m = r.max_bits > m ? r.max_bits : m;
+        }
+        __syncthreads();  // the previous scene's use of the block's LDS is over
+        block_residual<256>(v, m, &b.red[s * MAX_RED_SLOTS + slot]);
+        if (threadIdx.x == 0) { loop_decide_scene(b, kind, it, slot, s); }
+    }
 }
 
 void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_loop_decide, dim3((b.nscenes + 255) / 256), dim3(256), 0, s, b, kind,
-                       it, slot);
+    uint32_t const blocks = b.nscenes < 4096 ? b.nscenes : 4096;
+    hipLaunchKernelGGL(k_loop_decide, dim3(blocks ? blocks : 1), dim3(256), 0, s, b, kind, it,
+                       slot);
 }
 
 //------------------------------------------------------------------------------
@@ -564,34 +587,38 @@ __device__ __forceinline__ int collect_window(uint64_t (*ent)[BUILD_THREADS], ui
                                               uint2 const* __restrict__ win,
                                               V2 const* __restrict__ opos,
                                               uint32_t const* __restrict__ oorig, float maxd) {
+    // Candidates are loaded CHUNK at a time (their loads in flight together,
+    // their distances computed independently), then taken in scan order.
+    constexpr int CHUNK = 4;
     int cnt = 0;
     uint2 cur = win[0];
     for (int k = 0; k < 9; ++k) {
         uint2 const nxt = win[k < 8 ? k + 1 : 8];  // next range in flight
         uint32_t const e = cur.y;
-        if (cur.x < e) {
-            V2 q = opos[cur.x];
-            for (uint32_t s = cur.x; s < e; ++s) {
-                V2 const qn = opos[s + 1 < e ? s + 1 : s];  // next candidate in flight
-                if (!(same && s == self)) {
-                    V2 const v = sub(q, p);
-                    float const d = imath_length(v);
-                    if (d < maxd) {
-                        if (CAP < MAXN && cnt == CAP) { return -1; }
-                        uint64_t const key = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | s;
-                        int kk = cnt;
-                        while (kk > 0) {
-                            uint64_t const prev = ent[kk - 1][lane];
-                            if (!entry_after(prev, key, oorig)) { break; }
-                            ent[kk][lane] = prev;
-                            --kk;
-                        }
-                        ent[kk][lane] = key;
-                        ++cnt;
-                        if (CAP == MAXN && cnt == MAXN) { return cnt; }
+        for (uint32_t s0 = cur.x; s0 < e; s0 += CHUNK) {
+            V2 q[CHUNK];
+#pragma unroll
+            for (int u = 0; u < CHUNK; ++u) { q[u] = opos[s0 + u < e ? s0 + u : e - 1]; }
+            float d[CHUNK];
+#pragma unroll
+            for (int u = 0; u < CHUNK; ++u) { d[u] = imath_length(sub(q[u], p)); }
+#pragma unroll
+            for (int u = 0; u < CHUNK; ++u) {
+                uint32_t const s = s0 + u;
+                if (s < e && !(same && s == self) && d[u] < maxd) {
+                    if (CAP < MAXN && cnt == CAP) { return -1; }
+                    uint64_t const key = (static_cast<uint64_t>(__float_as_uint(d[u])) << 32) | s;
+                    int kk = cnt;
+                    while (kk > 0) {
+                        uint64_t const prev = ent[kk - 1][lane];
+                        if (!entry_after(prev, key, oorig)) { break; }
+                        ent[kk][lane] = prev;
+                        --kk;
                     }
+                    ent[kk][lane] = key;
+                    ++cnt;
+                    if (CAP == MAXN && cnt == MAXN) { return cnt; }
                 }
-                q = qn;
             }
         }
         cur = nxt;
@@ -990,7 +1017,7 @@ __global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
 // pressure, quantised error sum and error max, for one tile (every thread of
 // the workgroup calls it).
 template <int NB>
-__device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl, int slot) {
+__device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl, uint32_t tile) {
     unsigned long long qsum = 0;
     unsigned int emax = 0;
     if (threadIdx.x < tl.count) {
@@ -1064,16 +1091,17 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
         }
         a.p_out[t] = pn;
     }
-    block_reduce_residual<TILE>(qsum, emax, &a.red[tl.scene * MAX_RED_SLOTS + slot]);
+    block_residual<TILE>(qsum, emax, &a.tile_red[tile]);
 }
 
 // A converged scene skips the sweep; its result stays in the buffer of its
 // last iteration.
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, int slot) {
-    Tile const tl = a.tiles[tile_index(a)];
+__global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a) {
+    uint32_t const tile = tile_index(a);
+    Tile const tl = a.tiles[tile];
     if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
-    iisph_beta_p_tile<NB>(a, tl, slot);
+    iisph_beta_p_tile<NB>(a, tl, tile);
 }
 
 //------------------------------------------------------------------------------
@@ -1160,7 +1188,7 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
         }
     }
     if (FINAL) {
-        block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
+        block_max<TILE>(vm, a.tile_vmax + r.tile);
         block_energy<TILE>(ke, pe, a.tile_energy + r.tile);
     }
 }
@@ -1187,7 +1215,7 @@ __global__ __launch_bounds__(TILE) void k_finalize(Batch a) {
         vm = __float_as_uint(dot(v, v));
         particle_energy(c, x, v, ke, pe);
     }
-    block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
+    block_max<TILE>(vm, a.tile_vmax + r.tile);
     block_energy<TILE>(ke, pe, a.tile_energy + r.tile);
 }
 
@@ -1231,7 +1259,7 @@ __device__ __forceinline__ float divergence_of(Batch const& a, StepConsts const&
 // One divergence-solve iteration: divergence, kappa_v, residual of
 // max(0, dt * div / rho0) (dfsph_p.cpp:157-197).
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a, int slot) {
+__global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     if (!a.stats[r.scene].active) { return; }
     unsigned long long qsum = 0;
@@ -1251,13 +1279,13 @@ __global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a, int slot) {
         emax = __float_as_uint(val);
         qsum = static_cast<unsigned long long>(static_cast<long long>(val * 1000.0f));
     }
-    block_reduce_residual<TILE>(qsum, emax, &a.red[r.scene * MAX_RED_SLOTS + slot]);
+    block_residual<TILE>(qsum, emax, &a.tile_red[r.tile]);
 }
 
 // One density-solve iteration: density star (density.cpp:111-145), kappa_rho,
 // residual of max(0, (rho* - rho0) / rho0) (dfsph_p.cpp:82-122).
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a, int slot) {
+__global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     if (!a.stats[r.scene].active) { return; }
     unsigned long long qsum = 0;
@@ -1282,7 +1310,7 @@ __global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a, int slot) {
         emax = __float_as_uint(val);
         qsum = static_cast<unsigned long long>(static_cast<long long>(val * 1000.0f));
     }
-    block_reduce_residual<TILE>(qsum, emax, &a.red[r.scene * MAX_RED_SLOTS + slot]);
+    block_residual<TILE>(qsum, emax, &a.tile_red[r.tile]);
 }
 
 //------------------------------------------------------------------------------
@@ -1307,7 +1335,7 @@ __global__ __launch_bounds__(TILE) void k_pci_init(Batch a) {
 // :90-115 geometry at x* (neighborhoods_from_state.cpp:103-128), densities
 // (pcisph_ops.cpp:18-63), pressure update (:65-76) and the max density error
 // (common.cpp:78-85). PCISPH densities are kept in rho.
-__global__ __launch_bounds__(TILE) void k_pci_density(Batch a, int slot) {
+__global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     if (!a.stats[r.scene].active) { return; }
     unsigned int emax = 0;
@@ -1347,7 +1375,7 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a, int slot) {
         a.p_out[t] = a.p_out[t] + e / c.pci_denom;
         emax = __float_as_uint(e);
     }
-    block_reduce_max<TILE>(emax, &a.red[r.scene * MAX_RED_SLOTS + slot].max_bits);
+    block_residual<TILE>(0ull, emax, &a.tile_red[r.tile]);
 }
 
 // :117-142 pressure + artificial viscosity forces, solid forces, then the next
@@ -1425,37 +1453,52 @@ __global__ __launch_bounds__(TILE) void k_pci_finalize(Batch a) {
         vm = __float_as_uint(dot(v, v));
         particle_energy(a.scenes[r.scene].c, x, v, ke, pe);
     }
-    block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
+    block_max<TILE>(vm, a.tile_vmax + r.tile);
     block_energy<TILE>(ke, pe, a.tile_energy + r.tile);
 }
 
-// Per-scene energies from the scene's tile partials: one workgroup per scene;
-// thread i sums tiles i, i + 256, ... of the scene in order, then a fixed
+// Per-scene totals from the scene's tile partials: max |v|^2 and, when the
+// final kernel computed them, the FP64 energies. One workgroup per scene;
+// thread i takes tiles i, i + 256, ... of the scene in order, then a fixed
 // tree. The order depends only on the scene's own tiles, so the result is the
 // same alone and in any batch.
-__global__ __launch_bounds__(256) void k_scene_energy(Batch a) {
+__global__ __launch_bounds__(256) void k_scene_totals(Batch a, int energy) {
     for (uint32_t s = blockIdx.x; s < a.nscenes; s += gridDim.x) {
         SceneDev const& sd = a.scenes[s];
         double ke = 0.0, pe = 0.0;
+        unsigned int vm = 0;
         for (uint32_t k = threadIdx.x; k < sd.ntiles; k += blockDim.x) {
-            double2 const e = a.tile_energy[sd.first_tile + k];
-            ke += e.x;
-            pe += e.y;
+            unsigned int const m = a.tile_vmax[sd.first_tile + k];
+            vm = m > vm ? m : vm;
+            if (energy) {
+                double2 const e = a.tile_energy[sd.first_tile + k];
+                ke += e.x;
+                pe += e.y;
+            }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             ke += __shfl_xor(ke, off);
             pe += __shfl_xor(pe, off);
+            unsigned int const o = __shfl_xor(vm, off);
+            vm = o > vm ? o : vm;
         }
         __shared__ double ske[4], spe[4];
+        __shared__ unsigned int svm[4];
         if (__lane_id() == 0) {
             ske[threadIdx.x / 64] = ke;
             spe[threadIdx.x / 64] = pe;
+            svm[threadIdx.x / 64] = vm;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            a.stats[s].kinetic_energy = (ske[0] + ske[1]) + (ske[2] + ske[3]);
-            a.stats[s].potential_energy = (spe[0] + spe[1]) + (spe[2] + spe[3]);
+            unsigned int m = svm[0];
+            for (int w = 1; w < 4; ++w) { m = svm[w] > m ? svm[w] : m; }
+            a.stats[s].vmax2_bits = m;
+            if (energy) {
+                a.stats[s].kinetic_energy = (ske[0] + ske[1]) + (ske[2] + ske[3]);
+                a.stats[s].potential_energy = (spe[0] + spe[1]) + (spe[2] + spe[3]);
+            }
         }
         __syncthreads();
     }
@@ -1468,7 +1511,7 @@ __global__ __launch_bounds__(TILE) void k_vmax2(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     unsigned int vm = 0;
     if (r.valid) { vm = __float_as_uint(dot(a.vel_orig[r.t], a.vel_orig[r.t])); }
-    block_reduce_max<TILE>(vm, &a.stats[r.scene].vmax2_bits);
+    block_max<TILE>(vm, a.tile_vmax + r.tile);
 }
 
 __global__ __launch_bounds__(256) void k_scatter_f32(float const* src, uint32_t const* orig,
@@ -1532,8 +1575,8 @@ void launch_iisph_aii(Batch const& a, hipStream_t s) {
 void launch_iisph_sdp(Batch const& a, hipStream_t s) {
     NB_DISPATCH(a.nb, k_iisph_sdp, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_iisph_beta_p(Batch const& a, int slot, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a, slot);
+void launch_iisph_beta_p(Batch const& a, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a);
 }
 
 template <int G, int F>
@@ -1566,17 +1609,17 @@ void launch_finalize(Batch const& a, int dfsph, hipStream_t s) {
         hipLaunchKernelGGL(k_finalize<0>, tile_grid(a), dim3(TILE), 0, s, a);
     }
 }
-void launch_dfsph_div(Batch const& a, int slot, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_dfsph_div, tile_grid(a), dim3(TILE), 0, s, a, slot);
+void launch_dfsph_div(Batch const& a, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_dfsph_div, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_dfsph_dens(Batch const& a, int slot, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_dfsph_dens, tile_grid(a), dim3(TILE), 0, s, a, slot);
+void launch_dfsph_dens(Batch const& a, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_dfsph_dens, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_pci_init(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_init, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_pci_density(Batch const& a, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_pci_density, tile_grid(a), dim3(TILE), 0, s, a, slot);
+void launch_pci_density(Batch const& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pci_density, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_pci_force(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_force, tile_grid(a), dim3(TILE), 0, s, a);
@@ -1633,12 +1676,13 @@ void launch_repack(uint3 const* seg, uint32_t nseg, uint32_t total, V2 const* po
                        kd, pos_o, vel_o, kv_o, kd_o);
 }
 
-void launch_scene_energy(Batch const& a, hipStream_t s) {
+void launch_scene_totals(Batch const& a, int energy, hipStream_t s) {
     uint32_t const blocks = a.nscenes < 4096 ? a.nscenes : 4096;
-    hipLaunchKernelGGL(k_scene_energy, dim3(blocks ? blocks : 1), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_scene_totals, dim3(blocks ? blocks : 1), dim3(256), 0, s, a, energy);
 }
 void launch_vmax2(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_vmax2, tile_grid(a), dim3(TILE), 0, s, a);
+    launch_scene_totals(a, 0, s);
 }
 void launch_scatter_f32(float const* src, uint32_t const* orig, float* dst, uint32_t n,
                         hipStream_t s) {

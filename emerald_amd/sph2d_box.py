@@ -313,12 +313,12 @@ class Context:
         return (t.as_dict(), list(times[:n.value]))
 
     def iteration_totals(self, reset: bool = False) -> tuple:
-        """(sub-steps, pressure or density iterations, divergence iterations)
-        since creation or the last reset."""
-        out = (ctypes.c_int64 * 3)()
+        """(sub-steps, pressure or density iterations, divergence iterations,
+        sub-steps redone after a short sort) since creation or the last reset."""
+        out = (ctypes.c_int64 * 4)()
         check(lib().emsph_get_iteration_totals(self._h, 1 if reset else 0, out),
               "get_iteration_totals")
-        return int(out[0]), int(out[1]), int(out[2])
+        return tuple(int(x) for x in out)
 
     def set_kernel_timing(self, enable: bool) -> None:
         check(lib().emsph_set_kernel_timing(self._h, 1 if enable else 0), "set_kernel_timing")

@@ -246,9 +246,11 @@ emsph_status emsph_autotune(emsph_context* ctx, int method, float dt, int trial_
 emsph_status emsph_set_kernel_timing(emsph_context* ctx, int enable);
 int emsph_get_kernel_timing(emsph_context* ctx, const char** names, double* total_ms,
                             int64_t* calls, int max);
-/* Running totals since creation or the last reset: out[0] sub-steps,
- * out[1] pressure (IISPH/PCISPH) or density (DFSPH) iterations, out[2] DFSPH
- * divergence iterations. Measurement only; no counterpart in the reference. */
+/* Running totals since creation or the last reset, into out[4]: out[0]
+ * sub-steps, out[1] pressure (IISPH/PCISPH) or density (DFSPH) iterations,
+ * out[2] DFSPH divergence iterations, out[3] sub-steps redone because the sort
+ * run with the previous sub-step's digit count was short (the key range grew).
+ * Measurement only; no counterpart in the reference. */
 emsph_status emsph_get_iteration_totals(emsph_context* ctx, int reset, int64_t* out);
 
 /* Batches: many independent scenes (own parameters, solids, particle count and

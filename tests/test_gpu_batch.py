@@ -114,6 +114,70 @@ def test_batch_cfl_dt_and_energy(gpu):
         assert_bits_equal(batch.get_state(i).positions, ctx.get_state().positions, f"scene {i}")
 
 
+def test_cfl_sub_step_redone_after_short_sort(gpu):
+    # A sub-step sorts with the previous sub-step's digit count and is redone
+    # when the key range has grown past it. The redone sub-step must take the
+    # same CFL dt. A 12 x 12 block near the origin has keys below 256 (one
+    # digit); one particle just left of grid column 16 moving in +x crosses it
+    # in the first sub-step, so the second needs two digits and is redone.
+    # Reference for every sub-step: the same state stepped with a fresh
+    # upload before each sub-step (exact digit count, max |v|^2 recomputed).
+    method = "iisph"
+    factor = 0.4
+    sc = scenes.dam_break(12, 12, 0.005, 2e-3, seed=21)
+    cfg = scenes.config_for(sc, method)
+    cell = 2.0 * float(np.float32(sc.support))
+    pos = sc.positions.astype(np.float32).copy()
+    assert np.max(np.floor(pos / cell)) < 16
+    u = scenes.uniform01(91, 2 * len(pos)).reshape(-1, 2)
+    vel = (np.array([1.5, -0.5]) + 0.2 * (u - 0.5)).astype(np.float32)
+    pos[0] = (np.float32(16 * cell - 0.0005), np.float32(0.05))
+    vel[0] = (np.float32(1.5), np.float32(0.0))
+    st0 = sb.State(pos, vel)
+    a = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
+    b = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
+    for ctx in (a, b):
+        ctx.set_cfl(factor)
+        ctx.set_state(st0)
+        ctx.frame_init(method)
+        ctx.iteration_totals(reset=True)
+    for k in range(12):
+        sa = a.sub_step(method, 0.0, cfg.seconds_per_sub_step)
+        sb_ = b.sub_step(method, 0.0, cfg.seconds_per_sub_step)
+        assert np.float32(sa.dt) == np.float32(sb_.dt), (k, sa.dt, sb_.dt)
+        ga, gb = a.get_state(), b.get_state()
+        assert_bits_equal(ga.positions, gb.positions, f"sub-step {k} positions")
+        assert_bits_equal(ga.velocities, gb.velocities, f"sub-step {k} velocities")
+        b.set_state(gb)
+    redone = a.iteration_totals()[3]
+    assert redone > 0, "no sub-step was redone: the scene did not grow its key range"
+    assert sa.dt < cfg.seconds_per_sub_step
+
+
+def test_changed_dt_keeps_scene_totals_and_cfl(gpu):
+    # A sub-step with another dt than the last re-uploads the scene's
+    # constants; its tiles (max |v|^2, energies) and CFL factor stay.
+    method = "iisph"
+    sc = scenes.dam_break(16, 20, 0.005, 2e-4, seed=23)
+    cfg = scenes.config_for(sc, method)
+    ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(sc.positions))
+    ctx.set_state(sb.State(sc.positions, sc.velocities))
+    ctx.frame_init(method)
+    m = np.float64(np.float32(cfg.mass_per_particle))
+    for k, dt in enumerate([2e-4, 1e-4, 1.5e-4]):
+        st = ctx.sub_step(method, 0.0, dt)
+        s = ctx.get_state()
+        v = s.velocities.astype(np.float64)
+        ke = 0.5 * m * np.sum(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1])
+        vmax2 = np.max(np.sum(s.velocities.astype(np.float32) ** 2, axis=1, dtype=np.float32))
+        assert abs(st.kinetic_energy - ke) <= 1e-12 * abs(ke), (k, st.kinetic_energy, ke)
+        assert np.float32(st.max_velocity_sq) == np.float32(vmax2), (k, st.max_velocity_sq, vmax2)
+    # CFL with a dt change: the bound applies to the new dt.
+    ctx.set_cfl(1e-4)
+    st = ctx.sub_step(method, 0.0, 3e-4)
+    assert np.float32(st.dt) < np.float32(3e-4)
+
+
 @pytest.mark.skip(reason="persistent cooperative solve disabled: a 1,792-workgroup cooperative "
                          "launch of it did not finish on the MI355X and the cause was not "
                          "established; set_tuning(persistent_loop=True) now raises")
