@@ -35,8 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # written once per sweep (neighbour gathers hit cache after the Morton sort);
 # the uniform fluid volume costs nothing; neighbour lists and solid-field reads
 # are not counted. Neighbour build NB = keys 16 + sort 16 + cells 8 = 40.
-# Per sub-step the tables sum to IISPH 184 + 68k, DFSPH_P 172 + 68k_v + 72k_rho,
-# PCISPH 116 + 108k.
+# Per sub-step the tables sum to IISPH 160 + 68k (the first iteration's
+# sum_j d_ij p_j is known to be zero: pressures start at zero), DFSPH_P
+# 172 + 68k_v + 72k_rho, PCISPH 116 + 108k.
 NB = {"keys": 16, "radix_sort": 16, "cells": 8, "step_begin": 0, "loop": 0}
 ALGO_BYTES = {
     "iisph": {**NB,
@@ -64,7 +65,7 @@ ALGO_BYTES = {
 def step_bytes(method: str, k: float, k_div: float) -> float:
     """Algorithmic bytes per particle-step at the measured mean iteration counts."""
     if method == "iisph":
-        return 184 + 68 * k
+        return 160 + 68 * k
     if method == "dfsph_p":
         return 172 + 68 * k_div + 72 * k
     return 116 + 108 * k
@@ -367,6 +368,7 @@ def main() -> int:
             "last_iterations": int(st.iterations),
             "sort_passes": int(st.sort_passes),
             "cells": int(st.cell_count),
+            "max_neighbors": int(st.max_neighbors),
         },
         "roofline": roofline,
         "whole_step": whole_step,

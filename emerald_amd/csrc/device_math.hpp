@@ -74,34 +74,31 @@ struct KernelConsts {
     float w0;      // W(0, h)
 };
 
-// sph_common/kernels.h:16-35 W(r, h)
+// sph_common/kernels.h:16-35 W(r, h). The two pieces' final divides by k are
+// one divide of the selected numerator: the same operations per lane, with
+// no divergent branches across a wave (lanes at q <= 1 and q > 1 are common).
 __device__ __forceinline__ float kernel_W(float r, KernelConsts const& c) {
     float const q = r / c.h;
-    if (q <= 1.0f) {
-        float const wa = 1.0f + q * q * (q * 0.75f - 1.5f);
-        return wa / c.k;
-    }
-    if (q <= 2.0f) {
-        float const t = 2.0f - q;
-        float const wb = (t * t * t) / 4.0f;
-        return wb / c.k;
-    }
-    return 0.0f;
+    float const wa = 1.0f + q * q * (q * 0.75f - 1.5f);
+    float const t = 2.0f - q;
+    float const wb = (t * t * t) / 4.0f;
+    float const w = (q <= 1.0f ? wa : wb) / c.k;
+    return (q <= 2.0f) ? w : 0.0f;
 }
 
-// sph_common/kernels.h:37-55 dW(r, h)
+// sph_common/kernels.h:37-55 dW(r, h), branch-free in the same way: the first
+// divide of either piece is one divide of selected operands
+// (-3 r / h^2 or -3 (2 - r/h)^2 / 4h), and the final divide by k is shared.
 __device__ __forceinline__ float kernel_dW(float r, KernelConsts const& c) {
     float const q = r / c.h;
-    if (q <= 1.0f) {
-        float const dwa = -3.0f * r / c.hh + 2.25f * r * r / c.hhh;
-        return dwa / c.k;
-    }
-    if (q <= 2.0f) {
-        float const t = 2.0f - (r / c.h);
-        float const dwb = -3.0f * (t * t) / c.four_h;
-        return dwb / c.k;
-    }
-    return 0.0f;
+    bool const inner = q <= 1.0f;
+    float const t = 2.0f - q;
+    float const num = inner ? -3.0f * r : -3.0f * (t * t);
+    float const den = inner ? c.hh : c.four_h;
+    float x = num / den;
+    if (inner) { x = x + 2.25f * r * r / c.hhh; }
+    float const dw = x / c.k;
+    return (q <= 2.0f) ? dw : 0.0f;
 }
 
 // sph_common/kernels.h:57-64 GradW(dP, h)

@@ -83,11 +83,23 @@ KernelConsts make_kernel_consts(float h) {
     return c;
 }
 
+// StepConsts::cut2 of a cut-off (host sqrtf is the correctly rounded IEEE
+// square root, as on the device). For cut >= 1e-18 every |v|^2 below 2 FLT_MIN
+// has Imath length < 2.2e-19 < cut and is also below cut2 (>= 1e-36).
+float cut_l2(float cut) {
+    if (!(cut >= 1e-18f) || !std::isfinite(cut)) { return 0.0f; }
+    float x = cut * cut;
+    while (x > 0.0f && std::sqrt(x) >= cut) { x = std::nextafter(x, 0.0f); }
+    while (std::sqrt(x) < cut) { x = std::nextafter(x, std::numeric_limits<float>::infinity()); }
+    return x;
+}
+
 StepConsts step_consts(emsph_config const& cfg, float dt) {
     StepConsts k{};
     emsph_parameters const& p = cfg.params;
     k.kc = make_kernel_consts(p.support);
     k.cell = p.support * 2.0f;
+    k.cut2 = cut_l2(k.cell);
     k.rho0 = p.target_density;
     k.fluid_volume = cfg.mass_per_particle / p.target_density;
     k.mass = cfg.mass_per_particle;
@@ -998,8 +1010,13 @@ emsph_status iisph_sub_step(Engine* g) {
     HIP_TRY(run_loop(g, 0, max_it, 2, [&](int it) -> hipError_t {
         b.p_in = (it & 1) ? pb : pa;
         b.p_out = (it & 1) ? pa : pb;
-        HIP_PASS(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
-        HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, g->stream); }));
+        // iisph.cpp:85-119: sum_j d_ij p_j, then beta and the new pressures.
+        // The pressures start at +0 (:83), so the first sums are +0 (written
+        // by the first beta sweep instead).
+        if (it > 0) {
+            HIP_PASS(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
+        }
+        HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, it == 0, g->stream); }));
         return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); });
     }, &g->last_iterations, &short_sort));
     if (short_sort) { return EMSPH_OK; }

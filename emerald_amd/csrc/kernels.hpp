@@ -24,6 +24,11 @@ constexpr int TILE = 256;          // fluid slots per tile (one workgroup)
 struct StepConsts {
     KernelConsts kc;
     float cell;          // 2h: grid cell and neighbour cut-off
+    // Smallest float x with sqrtf(x) >= cell: since the correctly rounded
+    // square root is monotone, |v| < cell exactly when |v|^2 < cut2 (also in
+    // Imath's tiny-length branch, |v|^2 < 2 FLT_MIN, when cell >= 1e-18);
+    // 0 when cell is below that, and lengths are compared instead.
+    float cut2;
     float rho0;          // target density
     float fluid_volume;  // m / rho0 (sph_common/volume.cpp:7-14)
     float mass;          // mass per particle
@@ -302,7 +307,9 @@ void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t 
 void launch_build(int mode, int cap, Batch const& b, hipStream_t s);
 void launch_iisph_aii(Batch const& b, hipStream_t s);
 void launch_iisph_sdp(Batch const& b, hipStream_t s);
-void launch_iisph_beta_p(Batch const& b, hipStream_t s);
+// first: the loop's first iteration, with every pressure +0 and the
+// sum_j d_ij p_j sweep skipped (written as +0 here).
+void launch_iisph_beta_p(Batch const& b, int first, hipStream_t s);
 // Apply pressure accelerations (pressure.cpp:9-80) with p_in as the pressure,
 // for active scenes (or all scenes when all_scenes != 0).
 // with_gravity: first integrate the external gravity force (dynamics.cpp:24-37).

@@ -21,12 +21,12 @@
 namespace emsph {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = 4;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_WAVES = RS_THREADS / 64;
 
 constexpr int SC_THREADS = 256;
-constexpr int SC_ITEMS = 16;
+constexpr int SC_ITEMS = 4;
 constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
 
 __device__ __forceinline__ uint64_t lanemask_lt() {

@@ -413,7 +413,10 @@ __device__ __forceinline__ void loop_decide_scene(Batch const& b, int kind, int 
 // One workgroup per active scene: the sweep's tile partials of the scene
 // (integer sum and max, so the order does not matter) into the scene's slot,
 // then the exit test.
-__global__ __launch_bounds__(256) void k_loop_decide(Batch b, int kind, int it, int slot) {
+constexpr int DECIDE_THREADS = 1024;
+
+__global__ __launch_bounds__(DECIDE_THREADS) void k_loop_decide(Batch b, int kind, int it,
+                                                                 int slot) {
     for (uint32_t s = blockIdx.x; s < b.nscenes; s += gridDim.x) {
         if (!b.stats[s].active) { continue; }  // the same for the whole workgroup
         SceneDev const& sd = b.scenes[s];
@@ -425,15 +428,15 @@ __global__ __launch_bounds__(256) void k_loop_decide(Batch b, int kind, int it, 
             m = r.max_bits > m ? r.max_bits : m;
         }
         __syncthreads();  // the previous scene's use of the block's LDS is over
-        block_residual<256>(v, m, &b.red[s * MAX_RED_SLOTS + slot]);
+        block_residual<DECIDE_THREADS>(v, m, &b.red[s * MAX_RED_SLOTS + slot]);
         if (threadIdx.x == 0) { loop_decide_scene(b, kind, it, slot, s); }
     }
 }
 
 void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t s) {
     uint32_t const blocks = b.nscenes < 4096 ? b.nscenes : 4096;
-    hipLaunchKernelGGL(k_loop_decide, dim3(blocks ? blocks : 1), dim3(256), 0, s, b, kind, it,
-                       slot);
+    hipLaunchKernelGGL(k_loop_decide, dim3(blocks ? blocks : 1), dim3(DECIDE_THREADS), 0, s, b,
+                       kind, it, slot);
 }
 
 //------------------------------------------------------------------------------
@@ -579,46 +582,83 @@ __device__ __forceinline__ bool entry_after(uint64_t a, uint64_t b,
     return oorig[static_cast<uint32_t>(a)] > oorig[static_cast<uint32_t>(b)];
 }
 
+// Positions of candidates s .. s + C - 1 of a range ending at e (clamped to
+// its last slot; nothing is loaded for an empty range).
+template <int C>
+__device__ __forceinline__ void load_chunk(V2 (&q)[C], V2 const* __restrict__ opos, uint32_t s,
+                                           uint32_t e) {
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        if (s < e) { q[u] = opos[s + u < e ? s + u : e - 1]; }
+    }
+}
+
+// Acceptance and sorted insertion of one loaded chunk, in scan order. Returns
+// false when the scan stops: a capped list overflowed (cnt = -1) or a
+// full-capacity list is full.
+template <int CAP, int C>
+__device__ __forceinline__ bool take_chunk(uint64_t (*ent)[BUILD_THREADS], uint32_t lane, V2 p,
+                                           uint32_t self, bool same, V2 const (&q)[C],
+                                           uint32_t s0, uint32_t e,
+                                           uint32_t const* __restrict__ oorig, float maxd,
+                                           float cut2, int& cnt) {
+    bool in[C];
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        V2 const v = sub(q[u], p);
+        float const l2 = v.x * v.x + v.y * v.y;  // as in imath_length
+        in[u] = (cut2 > 0.0f) ? (l2 < cut2) : (imath_length(v) < maxd);
+    }
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        uint32_t const s = s0 + u;
+        if (s < e && !(same && s == self) && in[u]) {
+            if (CAP < MAXN && cnt == CAP) {
+                cnt = -1;
+                return false;
+            }
+            float const d = imath_length(sub(q[u], p));
+            uint64_t const key = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | s;
+            int kk = cnt;
+            while (kk > 0) {
+                uint64_t const prev = ent[kk - 1][lane];
+                if (!entry_after(prev, key, oorig)) { break; }
+                ent[kk][lane] = prev;
+                --kk;
+            }
+            ent[kk][lane] = key;
+            ++cnt;
+            if (CAP == MAXN && cnt == MAXN) { return false; }
+        }
+    }
+    return true;
+}
+
 // Accepted candidates of one particle over its 3x3 cell window. Returns the
 // count, or -1 when CAP < 64 and more than CAP candidates are accepted.
+// A candidate is accepted when |v| < maxd; with cut2 > 0 (StepConsts::cut2)
+// that is decided as |v|^2 < cut2, the same answer without the square root,
+// which is then taken for accepted candidates only. (Loading each range's
+// first chunk while the previous range is scanned, with the nine ranges
+// unrolled, measured slower: 217 vs 174 us at 1M.)
 template <int CAP>
 __device__ __forceinline__ int collect_window(uint64_t (*ent)[BUILD_THREADS], uint32_t lane,
                                               V2 p, uint32_t self, bool same,
                                               uint2 const* __restrict__ win,
                                               V2 const* __restrict__ opos,
-                                              uint32_t const* __restrict__ oorig, float maxd) {
-    // Candidates are loaded CHUNK at a time (their loads in flight together,
-    // their distances computed independently), then taken in scan order.
-    constexpr int CHUNK = 4;
+                                              uint32_t const* __restrict__ oorig, float maxd,
+                                              float cut2) {
+    constexpr int C = 4;
     int cnt = 0;
     uint2 cur = win[0];
     for (int k = 0; k < 9; ++k) {
         uint2 const nxt = win[k < 8 ? k + 1 : 8];  // next range in flight
         uint32_t const e = cur.y;
-        for (uint32_t s0 = cur.x; s0 < e; s0 += CHUNK) {
-            V2 q[CHUNK];
-#pragma unroll
-            for (int u = 0; u < CHUNK; ++u) { q[u] = opos[s0 + u < e ? s0 + u : e - 1]; }
-            float d[CHUNK];
-#pragma unroll
-            for (int u = 0; u < CHUNK; ++u) { d[u] = imath_length(sub(q[u], p)); }
-#pragma unroll
-            for (int u = 0; u < CHUNK; ++u) {
-                uint32_t const s = s0 + u;
-                if (s < e && !(same && s == self) && d[u] < maxd) {
-                    if (CAP < MAXN && cnt == CAP) { return -1; }
-                    uint64_t const key = (static_cast<uint64_t>(__float_as_uint(d[u])) << 32) | s;
-                    int kk = cnt;
-                    while (kk > 0) {
-                        uint64_t const prev = ent[kk - 1][lane];
-                        if (!entry_after(prev, key, oorig)) { break; }
-                        ent[kk][lane] = prev;
-                        --kk;
-                    }
-                    ent[kk][lane] = key;
-                    ++cnt;
-                    if (CAP == MAXN && cnt == MAXN) { return cnt; }
-                }
+        for (uint32_t s0 = cur.x; s0 < e; s0 += C) {
+            V2 q[C];
+            load_chunk<C>(q, opos, s0, e);
+            if (!take_chunk<CAP, C>(ent, lane, p, self, same, q, s0, e, oorig, maxd, cut2, cnt)) {
+                return cnt;
             }
         }
         cur = nxt;
@@ -639,7 +679,7 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
 
     // Fluid neighbours.
     int const fc = collect_window<CAP>(ent, lane, p, t, true, a.win + static_cast<size_t>(cell) * 9,
-                                       a.pos, a.orig, c.cell);
+                                       a.pos, a.orig, c.cell, c.cut2);
     if (fc < 0) {
         max_count = CAP + 1;
         return false;
@@ -648,11 +688,27 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
     V2 msum_f{0.0f, 0.0f};
     V2 sum_mg{0.0f, 0.0f};    // DFSPH: sum of m grad W over fluid and solid
     float sum_mg2 = 0.0f;     // DFSPH: sum of |m grad W|^2 over fluid
-    for (int k2 = 0; k2 < fc; ++k2) {
-        uint64_t const e = ent[k2][lane];
+    // Entries are taken four at a time (their position loads in flight
+    // together) and summed in list order.
+    for (int k0 = 0; k0 < fc; k0 += 4) {
+      uint64_t e4[4];
+      V2 q4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k0 + u < fc) { e4[u] = ent[k0 + u][lane]; }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k0 + u < fc) { q4[u] = a.pos[static_cast<uint32_t>(e4[u])]; }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k0 + u >= fc) { break; }
+        int const k2 = k0 + u;
+        uint64_t const e = e4[u];
         uint32_t const s = static_cast<uint32_t>(e);
         float const d = __uint_as_float(static_cast<uint32_t>(e >> 32));
-        V2 const vec = sub(a.pos[s], p);
+        V2 const vec = sub(q4[u], p);
         size_t const o = static_cast<size_t>(k2) * a.stride + t;
         a.fl.idx[o] = s;
         if (a.store_geometry) {
@@ -671,6 +727,7 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
                 sum_mg2 += dot(mg, mg);
             }
         }
+      }
     }
 
     // Solid neighbours (no self exclusion: different arrays). A batch without
@@ -684,7 +741,8 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
     int sc = 0;
     if (a.swin && sv.pos) {
         sc = collect_window<CAP>(ent, lane, p, 0xFFFFFFFFu, false,
-                                 a.swin + static_cast<size_t>(cell) * 9, sv.pos, sv.orig, c.cell);
+                                 a.swin + static_cast<size_t>(cell) * 9, sv.pos, sv.orig, c.cell,
+                                 c.cut2);
         if (sc < 0) {
             max_count = CAP + 1;
             return false;
@@ -695,11 +753,25 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
     max_count = fc > sc ? fc : sc;
     float dens_s = 0.0f;
     V2 msum_s{0.0f, 0.0f};
-    for (int k2 = 0; k2 < sc; ++k2) {
-        uint64_t const e = ent[k2][lane];
+    for (int k0 = 0; k0 < sc; k0 += 4) {
+      uint64_t e4[4];
+      V2 q4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k0 + u < sc) { e4[u] = ent[k0 + u][lane]; }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k0 + u < sc) { q4[u] = sv.pos[static_cast<uint32_t>(e4[u])]; }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k0 + u >= sc) { break; }
+        int const k2 = k0 + u;
+        uint64_t const e = e4[u];
         uint32_t const b = static_cast<uint32_t>(e);
         float const d = __uint_as_float(static_cast<uint32_t>(e >> 32));
-        V2 const vec = sub(sv.pos[b], p);
+        V2 const vec = sub(q4[u], p);
         size_t const o = static_cast<size_t>(k2) * a.stride + t;
         a.sl.idx[o] = b;
         if (a.store_geometry) {
@@ -714,6 +786,7 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
             if (MODE == BUILD_IISPH) { msum_s = add(msum_s, scale(ms, gw)); }
             if (MODE == BUILD_DFSPH) { sum_mg = add(sum_mg, scale(ms, gw)); }
         }
+      }
     }
     if (MODE == BUILD_LISTS) { return true; }
 
@@ -1016,7 +1089,14 @@ __global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
 // :261-353 beta (into the new-pressure buffer) fused with :357-425 new
 // pressure, quantised error sum and error max, for one tile (every thread of
 // the workgroup calls it).
-template <int NB>
+//
+// FIRST: the first iteration. Every pressure is then +0 (k_iisph_aii set
+// them), so every sum_j d_ij p_j is exactly (+0, +0): each term is
+// (kf * +0) g / rho_j^2 = +-0 or skipped as an unsafe divide, and +0 plus
+// +-0 stays +0. The sweep of the sums is skipped; this kernel writes them and
+// takes p_i, p_j and the sums as those constants (the same IEEE operations
+// on the same values, so the same bits).
+template <int NB, int FIRST>
 __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl, uint32_t tile) {
     unsigned long long qsum = 0;
     unsigned int emax = 0;
@@ -1025,8 +1105,9 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
         StepConsts const c = a.scenes[tl.scene].c;
         float const mf = c.rho0 * c.fluid_volume;
         float const kf = -(c.dt * c.dt) * mf;
-        float const pi = a.p_in[t];
-        V2 const si = a.sdp[t];
+        float const pi = FIRST ? 0.0f : a.p_in[t];
+        V2 const si = FIRST ? V2{0.0f, 0.0f} : a.sdp[t];
+        if (FIRST) { a.sdp[t] = si; }
         int const fc = a.fl.count[t];
         float beta = 0.0f;
         for (int k0 = 0; k0 < fc; k0 += NB) {
@@ -1043,8 +1124,10 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
                 if (k0 + u < fc) {
                     uint32_t const j = nb.j[u];
                     rj[u] = a.rho[j];
-                    pj[u] = a.p_in[j];
-                    sj[u] = a.sdp[j];
+                    if (!FIRST) {
+                        pj[u] = a.p_in[j];
+                        sj[u] = a.sdp[j];
+                    }
                     djj[u] = a.dii[j];
                 }
             }
@@ -1096,12 +1179,12 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
 
 // A converged scene skips the sweep; its result stays in the buffer of its
 // last iteration.
-template <int NB>
+template <int NB, int FIRST>
 __global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a) {
     uint32_t const tile = tile_index(a);
     Tile const tl = a.tiles[tile];
     if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
-    iisph_beta_p_tile<NB>(a, tl, tile);
+    iisph_beta_p_tile<NB, FIRST>(a, tl, tile);
 }
 
 //------------------------------------------------------------------------------
@@ -1575,8 +1658,26 @@ void launch_iisph_aii(Batch const& a, hipStream_t s) {
 void launch_iisph_sdp(Batch const& a, hipStream_t s) {
     NB_DISPATCH(a.nb, k_iisph_sdp, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_iisph_beta_p(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a);
+void launch_iisph_beta_p(Batch const& a, int first, hipStream_t s) {
+    if (a.nb <= 2) {
+        if (first) {
+            hipLaunchKernelGGL((k_iisph_beta_p<2, 1>), tile_grid(a), dim3(TILE), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_iisph_beta_p<2, 0>), tile_grid(a), dim3(TILE), 0, s, a);
+        }
+    } else if (a.nb <= 4) {
+        if (first) {
+            hipLaunchKernelGGL((k_iisph_beta_p<4, 1>), tile_grid(a), dim3(TILE), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_iisph_beta_p<4, 0>), tile_grid(a), dim3(TILE), 0, s, a);
+        }
+    } else {
+        if (first) {
+            hipLaunchKernelGGL((k_iisph_beta_p<8, 1>), tile_grid(a), dim3(TILE), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_iisph_beta_p<8, 0>), tile_grid(a), dim3(TILE), 0, s, a);
+        }
+    }
 }
 
 template <int G, int F>
