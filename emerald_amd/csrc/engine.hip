@@ -928,7 +928,9 @@ hipError_t enqueue_prepare(Engine* g, int build_mode, bool dfsph) {
         ga.kd_out = g->kds.p;
     }
     Batch b = fluid_batch(g);
-    HIP_PASS(timed(g, "step_begin", [&] { launch_step_begin(b, g->stream); }));
+    ga.step_scenes = g->scenes.p;
+    ga.step_stats = g->stats.p;
+    ga.step_nscenes = g->M;
     HIP_PASS(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->passes_hint, ga,
                           g->timer.enabled, g->NS ? &g->sidx : nullptr, g->swin.p));
     return timed(g, "build", [&] { launch_build(build_mode, g->build_cap, b, g->stream); });
@@ -996,8 +998,8 @@ emsph_status iisph_sub_step(Engine* g) {
     Batch b = fluid_batch(g, nullptr, pa);
     HIP_TRY(segment(g, EMSPH_IISPH, SEG_PREFIX, [&]() -> hipError_t {
         HIP_PASS(enqueue_prepare(g, BUILD_IISPH, false));
-        HIP_PASS(timed(g, "iisph_aii", [&] { launch_iisph_aii(b, g->stream); }));
-        return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
+        // The a_ii sweep also starts the solver loop.
+        return timed(g, "iisph_aii", [&] { launch_iisph_aii(b, g->stream); });
     }));
     finish_prepare(g);
     int max_it = 0;

@@ -242,6 +242,11 @@ struct IndexArgs {
     int2* grid;                // optional
     KeyState* keystate;
     unsigned long long* key_partials;  // 2 * KEY_PARTS: per-workgroup key min, max
+    // Optional start of a sub-step (per-scene CFL dt, cleared statistics),
+    // done by the key-range kernel when step_stats is set.
+    SceneDev* step_scenes;
+    SceneStats* step_stats;
+    uint32_t step_nscenes;
     // cells
     int key_passes;
     int check_keys;

@@ -28,6 +28,9 @@ constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 4;
 constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+// Up to this many scan blocks each block sums the block sums before it
+// itself (at most ~8M block-sum reads), instead of a separate top-level scan.
+constexpr uint32_t SCAN_DIRECT_BLOCKS = 4096;
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t const lane = __lane_id();
@@ -202,10 +205,32 @@ __global__ __launch_bounds__(1024) void scan_top(uint32_t* __restrict__ bsum, ui
     if (total && threadIdx.x == 0) { *total = tot; }
 }
 
+// Downsweep. With nb > 0 the block's offset is the sum of the block sums
+// before it (read here, so no separate top-level scan runs; used while nb is
+// small), and the last block writes the grand total to *total (if not null);
+// with nb == 0, bsum already holds the scanned block offsets.
 __global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t const* __restrict__ in,
                                                         uint32_t* __restrict__ out, uint32_t n,
-                                                        uint32_t const* __restrict__ bsum) {
+                                                        uint32_t const* __restrict__ bsum,
+                                                        uint32_t nb, uint32_t* __restrict__ total) {
     __shared__ uint32_t tile[SC_TILE];
+    __shared__ uint32_t block_offset;
+    if (nb) {
+        uint32_t part = 0;
+        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += SC_THREADS) { part += bsum[i]; }
+        for (int off = 32; off > 0; off >>= 1) { part += __shfl_xor(part, off); }
+        __shared__ uint32_t ws[SC_THREADS / 64];
+        if (__lane_id() == 0) { ws[threadIdx.x / 64] = part; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t o = 0;
+            for (int w = 0; w < SC_THREADS / 64; ++w) { o += ws[w]; }
+            block_offset = o;
+            if (total && blockIdx.x == nb - 1) { *total = o + bsum[blockIdx.x]; }
+        }
+    } else if (threadIdx.x == 0) {
+        block_offset = bsum[blockIdx.x];
+    }
     uint32_t const base = blockIdx.x * SC_TILE;
     for (int r = 0; r < SC_ITEMS; ++r) {
         uint32_t const i = base + r * SC_THREADS + threadIdx.x;
@@ -217,7 +242,8 @@ __global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t const* __restri
     uint32_t s = 0;
     for (int r = 0; r < SC_ITEMS; ++r) { s += tile[t0 + r]; }
     uint32_t tot = 0;
-    uint32_t run = block_exclusive_scan<SC_THREADS>(s, tot) + bsum[blockIdx.x];
+    __syncthreads();  // block_offset
+    uint32_t run = block_exclusive_scan<SC_THREADS>(s, tot) + block_offset;
     for (int r = 0; r < SC_ITEMS; ++r) {
         uint32_t const v = tile[t0 + r];
         tile[t0 + r] = run;
@@ -240,8 +266,14 @@ void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32
     if (n == 0) { return; }
     uint32_t const nb = (n + SC_TILE - 1) / SC_TILE;
     hipLaunchKernelGGL(scan_reduce, dim3(nb), dim3(SC_THREADS), 0, s, in, n, scratch);
-    hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-    hipLaunchKernelGGL(scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, scratch);
+    if (nb <= SCAN_DIRECT_BLOCKS) {
+        hipLaunchKernelGGL(scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, scratch, nb,
+                           total);
+    } else {
+        hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, s, scratch, nb, total);
+        hipLaunchKernelGGL(scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, scratch, 0u,
+                           nullptr);
+    }
 }
 
 size_t radix_counts_words(size_t n) {

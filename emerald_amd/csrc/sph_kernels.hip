@@ -179,7 +179,37 @@ __global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
 
 // One workgroup: the key range over the partials; clears the short-sort flag
 // and the neighbour-build counters of this indexing run.
+// Start of a sub-step for one scene: the CFL dt and cleared statistics.
+__device__ __forceinline__ void step_begin_scene(SceneDev* sd, SceneStats* stats) {
+    if (sd->cfl > 0.0f) {
+        // cfl.cpp:12-21: factor * h / sqrt(max |v|^2) in double, over the
+        // velocities the previous sub-step left (dt_max while at rest).
+        float const vmax2 = __uint_as_float(stats->vmax2_bits);
+        float dt = sd->dt_max;
+        if (vmax2 > 0.0f) {
+            double const d = static_cast<double>(sd->cfl) * sd->c.kc.h /
+                             sqrt(static_cast<double>(vmax2));
+            if (d < static_cast<double>(dt)) { dt = static_cast<float>(d); }
+        }
+        sd->c.dt = dt;
+    }
+    // max |v|^2 stays until the end of this sub-step replaces it: a sub-step
+    // redone after a short sort recomputes the same dt.
+    SceneStats st{};
+    st.dt = sd->c.dt;
+    st.vmax2_bits = stats->vmax2_bits;
+    *stats = st;
+}
+
+// Key range of the index from the per-workgroup partials; with
+// IndexArgs::step_stats also the start of the sub-step for every scene (no
+// kernel reads a scene's dt or statistics before the sort is done).
 __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) {
+    if (a.step_stats) {
+        for (uint32_t s = threadIdx.x; s < a.step_nscenes; s += blockDim.x) {
+            step_begin_scene(a.step_scenes + s, a.step_stats + s);
+        }
+    }
     unsigned long long key_min = ~0ull;
     unsigned long long key_max = 0ull;
     for (uint32_t k = threadIdx.x; k < parts; k += blockDim.x) {
@@ -294,30 +324,7 @@ void launch_cells_finalize(IndexArgs const& a, hipStream_t s) {
 __global__ __launch_bounds__(256) void k_step_begin(Batch b) {
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < b.nscenes;
          s += gridDim.x * blockDim.x) {
-        SceneDev* sd = const_cast<SceneDev*>(b.scenes) + s;
-        if (sd->cfl > 0.0f) {
-            // cfl.cpp:12-21: factor * h / sqrt(max |v|^2) in double, over the
-            // velocities the previous sub-step left (dt_max while at rest).
-            float const vmax2 = __uint_as_float(b.stats[s].vmax2_bits);
-            float dt = sd->dt_max;
-            if (vmax2 > 0.0f) {
-                double const d = static_cast<double>(sd->cfl) * sd->c.kc.h /
-                                 sqrt(static_cast<double>(vmax2));
-                if (d < static_cast<double>(dt)) { dt = static_cast<float>(d); }
-            }
-            sd->c.dt = dt;
-        }
-        // max |v|^2 stays until the end of this sub-step replaces it: a
-        // sub-step redone after a short sort recomputes the same dt.
-        SceneStats st{};
-        st.dt = sd->c.dt;
-        st.vmax2_bits = b.stats[s].vmax2_bits;
-        b.stats[s] = st;
-        for (int k = 0; k < MAX_RED_SLOTS; ++k) {
-            b.red[s * MAX_RED_SLOTS + k].isum = 0;
-            b.red[s * MAX_RED_SLOTS + k].max_bits = 0;
-            b.red[s * MAX_RED_SLOTS + k].pad = 0;
-        }
+        step_begin_scene(const_cast<SceneDev*>(b.scenes) + s, b.stats + s);
     }
 }
 
@@ -326,7 +333,9 @@ void launch_step_begin(Batch const& b, hipStream_t s) {
     hipLaunchKernelGGL(k_step_begin, dim3(blocks ? blocks : 1), dim3(256), 0, s, b);
 }
 
-__global__ __launch_bounds__(1024) void k_loop_begin(Batch b) {
+// Every scene with particles enters the solver loop (one workgroup).
+template <int THREADS>
+__device__ __forceinline__ void loop_begin_block(Batch const& b) {
     uint32_t cnt = 0;
     for (uint32_t s = threadIdx.x; s < b.nscenes; s += blockDim.x) {
         unsigned int const act = b.scenes[s].n > 0 ? 1u : 0u;
@@ -334,15 +343,17 @@ __global__ __launch_bounds__(1024) void k_loop_begin(Batch b) {
         cnt += act;
     }
     for (int off = 32; off > 0; off >>= 1) { cnt += __shfl_xor(cnt, off); }
-    __shared__ uint32_t ws[16];
+    __shared__ uint32_t ws[THREADS / 64];
     if (__lane_id() == 0) { ws[threadIdx.x / 64] = cnt; }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (int w = 0; w < 16; ++w) { tot += ws[w]; }
+        for (int w = 0; w < THREADS / 64; ++w) { tot += ws[w]; }
         b.keystate->active_count = tot;
     }
 }
+
+__global__ __launch_bounds__(1024) void k_loop_begin(Batch b) { loop_begin_block<1024>(b); }
 
 void launch_loop_begin(Batch const& b, hipStream_t s) {
     hipLaunchKernelGGL(k_loop_begin, dim3(1), dim3(1024), 0, s, b);
@@ -582,6 +593,87 @@ __device__ __forceinline__ bool entry_after(uint64_t a, uint64_t b,
     return oorig[static_cast<uint32_t>(a)] > oorig[static_cast<uint32_t>(b)];
 }
 
+// Batcher's odd-even merge sorting network on N inputs (N a power of two),
+// as a compile-time comparator list (63 comparators for N = 16).
+struct Comparator {
+    int a;
+    int b;
+};
+template <int N>
+struct OddEvenMergeNet {
+    Comparator ce[N * N];
+    int count = 0;
+    constexpr OddEvenMergeNet() : ce{} {
+        for (int p = 1; p < N; p <<= 1) {
+            for (int k = p; k >= 1; k >>= 1) {
+                for (int j = k % p; j <= N - 1 - k; j += 2 * k) {
+                    int const last = (k - 1 < N - j - k - 1) ? k - 1 : N - j - k - 1;
+                    for (int i = 0; i <= last; ++i) {
+                        if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+                            ce[count].a = i + j;
+                            ce[count].b = i + j + k;
+                            ++count;
+                        }
+                    }
+                }
+            }
+        }
+    }
+};
+constexpr OddEvenMergeNet<16> NET16{};
+
+// Capped lists of this size are collected unsorted and sorted once by the
+// network (no per-candidate insertion chains through LDS).
+template <int CAP>
+constexpr bool net_sorted() {
+    return CAP == 16;
+}
+
+// Sorts one lane's cnt <= 16 collected entries (distance bits << 32 | slot)
+// into the reference order: by distance, then by original index. The network
+// orders by (distance, slot); equal distances (rare) are then put in original
+// index order by an insertion pass with the exact comparator.
+__device__ __forceinline__ void net_sort16(uint64_t (*ent)[BUILD_THREADS], uint32_t lane, int cnt,
+                                           uint32_t const* __restrict__ oorig) {
+    constexpr int N = 16;
+    uint64_t e[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        uint64_t const v = ent[k][lane];
+        e[k] = (k < cnt) ? v : ~0ull;
+    }
+#pragma unroll
+    for (int c = 0; c < NET16.count; ++c) {
+        uint64_t const x = e[NET16.ce[c].a];
+        uint64_t const y = e[NET16.ce[c].b];
+        bool const swap = y < x;
+        e[NET16.ce[c].a] = swap ? y : x;
+        e[NET16.ce[c].b] = swap ? x : y;
+    }
+    bool tie = false;
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+        tie = tie || (k < cnt && (e[k - 1] >> 32) == (e[k] >> 32));
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (k < cnt) { ent[k][lane] = e[k]; }
+    }
+    if (tie) {
+        for (int k = 1; k < cnt; ++k) {
+            uint64_t const key = ent[k][lane];
+            int kk = k;
+            while (kk > 0) {
+                uint64_t const prev = ent[kk - 1][lane];
+                if (!entry_after(prev, key, oorig)) { break; }
+                ent[kk][lane] = prev;
+                --kk;
+            }
+            ent[kk][lane] = key;
+        }
+    }
+}
+
 // Positions of candidates s .. s + C - 1 of a range ending at e (clamped to
 // its last slot; nothing is loaded for an empty range).
 template <int C>
@@ -619,14 +711,18 @@ __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[BUILD_THREADS], uint3
             }
             float const d = imath_length(sub(q[u], p));
             uint64_t const key = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | s;
-            int kk = cnt;
-            while (kk > 0) {
-                uint64_t const prev = ent[kk - 1][lane];
-                if (!entry_after(prev, key, oorig)) { break; }
-                ent[kk][lane] = prev;
-                --kk;
+            if (net_sorted<CAP>()) {
+                ent[cnt][lane] = key;  // sorted after the scan
+            } else {
+                int kk = cnt;
+                while (kk > 0) {
+                    uint64_t const prev = ent[kk - 1][lane];
+                    if (!entry_after(prev, key, oorig)) { break; }
+                    ent[kk][lane] = prev;
+                    --kk;
+                }
+                ent[kk][lane] = key;
             }
-            ent[kk][lane] = key;
             ++cnt;
             if (CAP == MAXN && cnt == MAXN) { return false; }
         }
@@ -658,11 +754,12 @@ __device__ __forceinline__ int collect_window(uint64_t (*ent)[BUILD_THREADS], ui
             V2 q[C];
             load_chunk<C>(q, opos, s0, e);
             if (!take_chunk<CAP, C>(ent, lane, p, self, same, q, s0, e, oorig, maxd, cut2, cnt)) {
-                return cnt;
+                return cnt;  // overflow (-1) or a full 64-entry list (insertion-sorted)
             }
         }
         cur = nxt;
     }
+    if (net_sorted<CAP>() && cnt > 1) { net_sort16(ent, lane, cnt, oorig); }
     return cnt;
 }
 
@@ -971,8 +1068,10 @@ __device__ __forceinline__ void load_batch(Lists const& l, uint32_t stride, uint
 // IISPH sweeps: sph2d_box/iisph_ops.cpp.
 
 // :94-221 a_ii and density stars; also zeroes the pressures (iisph.cpp:83).
+// Workgroup 0 also starts the solver loop (every scene with particles active).
 template <int NB>
 __global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
+    if (blockIdx.x == 0) { loop_begin_block<TILE>(a); }
     TileRef const r = tile_ref(a, TILE);
     if (!r.valid) { return; }
     uint32_t const t = r.t;
