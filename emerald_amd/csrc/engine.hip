@@ -292,7 +292,9 @@ struct Engine {
     DBuf<V2> pos, vel;
     DBuf<float> kv, kd;
     SpatialIndex fidx;
-    DBuf<V2> spos, svel, dii, sdp, vext, xstar, vstar0, vstar1, force;
+    DBuf<V2> spos, svel, dii, vext, xstar, vstar0, vstar1, force;
+    DBuf<V2> pr0, pr1;  // IISPH {p, rho} ping-pong
+    DBuf<SdDii> sd;     // IISPH {sum_j d_ij p_j, d_ii}
     DBuf<float> rho, rho_star, aii, p0, p1, alpha, kvs, kds;
     float* cur_p = nullptr;
     DBuf<uint8_t> fcount, scount;
@@ -484,9 +486,9 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     b.rho_star = g->rho_star.p;
     b.aii = g->aii.p;
     b.dii = g->dii.p;
-    b.sdp = g->sdp.p;
     b.p_in = p_in;
     b.p_out = p_out;
+    b.sd = g->sd.p;
     b.alpha = g->alpha.p;
     b.pos_orig = g->pos.p;
     b.vel_orig = g->vel.p;
@@ -565,12 +567,13 @@ hipError_t alloc_fluid(Engine* g, uint32_t cap) {
     hipError_t e;
     if ((e = g->pos.alloc(cap)) || (e = g->vel.alloc(cap)) || (e = g->kv.alloc(cap)) ||
         (e = g->kd.alloc(cap)) || (e = g->fidx.alloc(cap, false)) || (e = g->spos.alloc(cap)) ||
-        (e = g->svel.alloc(cap)) || (e = g->dii.alloc(cap)) || (e = g->sdp.alloc(cap)) ||
+        (e = g->svel.alloc(cap)) || (e = g->dii.alloc(cap)) ||
         (e = g->vext.alloc(cap)) || (e = g->xstar.alloc(cap)) || (e = g->vstar0.alloc(cap)) ||
         (e = g->vstar1.alloc(cap)) || (e = g->force.alloc(cap)) || (e = g->rho.alloc(cap)) ||
         (e = g->rho_star.alloc(cap)) || (e = g->aii.alloc(cap)) || (e = g->p0.alloc(cap)) ||
         (e = g->p1.alloc(cap)) || (e = g->alpha.alloc(cap)) || (e = g->kvs.alloc(cap)) ||
-        (e = g->kds.alloc(cap)) || (e = g->fcount.alloc(g->stride)) ||
+        (e = g->kds.alloc(cap)) || (e = g->pr0.alloc(cap)) || (e = g->pr1.alloc(cap)) ||
+        (e = g->sd.alloc(cap)) || (e = g->fcount.alloc(g->stride)) ||
         (e = g->scount.alloc(g->stride)) || (e = g->fl_idx.alloc(L)) ||
         (e = g->sl_idx.alloc(L)) || (e = g->fl_gw.alloc(L)) || (e = g->sl_gw.alloc(L)) ||
         (e = g->swin.alloc(static_cast<size_t>(cap) * 9)) || (e = g->overflow.alloc(cap))) {
@@ -993,9 +996,10 @@ enum Segment : int { SEG_PREFIX = 0, SEG_MIDDLE = 1, SEG_SUFFIX = 2 };
 // iisph.cpp:23-151 for every scene of the batch.
 emsph_status iisph_sub_step(Engine* g) {
     begin_attempt(g);
-    float* pa = g->p0.p;
-    float* pb = g->p1.p;
-    Batch b = fluid_batch(g, nullptr, pa);
+    V2* pa = g->pr0.p;
+    V2* pb = g->pr1.p;
+    Batch b = fluid_batch(g);
+    b.pr_out = pa;  // the a_ii sweep starts the pressures at +0
     HIP_TRY(segment(g, EMSPH_IISPH, SEG_PREFIX, [&]() -> hipError_t {
         HIP_PASS(enqueue_prepare(g, BUILD_IISPH, false));
         // The a_ii sweep also starts the solver loop.
@@ -1010,8 +1014,8 @@ emsph_status iisph_sub_step(Engine* g) {
     bool short_sort = false;
     // iisph.cpp:126: no scene can exit before its third iteration.
     HIP_TRY(run_loop(g, 0, max_it, 2, [&](int it) -> hipError_t {
-        b.p_in = (it & 1) ? pb : pa;
-        b.p_out = (it & 1) ? pa : pb;
+        b.pr_in = (it & 1) ? pb : pa;
+        b.pr_out = (it & 1) ? pa : pb;
         // iisph.cpp:85-119: sum_j d_ij p_j, then beta and the new pressures.
         // The pressures start at +0 (:83), so the first sums are +0 (written
         // by the first beta sweep instead).
@@ -1026,10 +1030,12 @@ emsph_status iisph_sub_step(Engine* g) {
         HIP_TRY(read_keystate(g));
         if (g->sort_short) { return EMSPH_OK; }
     }
-    // Each scene's last pressure buffer follows its own iteration count.
-    b.p_in = g->p0.p;
+    // Each scene's last pressure buffer follows its own iteration count; the
+    // final pressures are also written to p0 for the field read-back.
+    b.pr_in = pa;
+    b.pr_out = nullptr;
+    b.pr_alt = pb;
     b.p_out = g->p0.p;
-    b.p_alt = g->p1.p;
     HIP_TRY(segment(g, EMSPH_IISPH, SEG_SUFFIX, [&]() -> hipError_t {
         HIP_PASS(timed(g, "apply_integrate", [&] { launch_apply(b, 0, 1, 1, g->stream); }));
         launch_scene_totals(b, 1, g->stream);
@@ -1216,7 +1222,7 @@ emsph_status engine_get_field(Engine* g, uint32_t s, int field, float* out) {
     case EMSPH_FIELD_PRESSURE: f1 = g->cur_p ? g->cur_p : g->p0.p; break;
     case EMSPH_FIELD_AII: f1 = g->aii.p; break;
     case EMSPH_FIELD_DII: f2 = g->dii.p; break;
-    case EMSPH_FIELD_SUM_DIJ_PJ: f2 = g->sdp.p; break;
+    case EMSPH_FIELD_SUM_DIJ_PJ: break;  // the sdp half of the packed {sdp, d_ii}
     case EMSPH_FIELD_ALPHA: f1 = g->alpha.p; break;
     case EMSPH_FIELD_KAPPA_DENSITY: f1 = g->kds.p; break;
     case EMSPH_FIELD_KAPPA_DIVERGENCE: f1 = g->kvs.p; break;
@@ -1224,6 +1230,11 @@ emsph_status engine_get_field(Engine* g, uint32_t s, int field, float* out) {
     default: return fail(EMSPH_ERR_INVALID_ARGUMENT, "unknown field");
     }
     SceneHost const& h = g->sh[s];
+    uint32_t f2_stride = 1;
+    if (field == EMSPH_FIELD_SUM_DIJ_PJ) {
+        f2 = &g->sd.p[0].sdp;
+        f2_stride = sizeof(SdDii) / sizeof(V2);
+    }
     if (f1) {
         DBuf<float> tmp;
         HIP_TRY(tmp.alloc(g->N));
@@ -1234,7 +1245,7 @@ emsph_status engine_get_field(Engine* g, uint32_t s, int field, float* out) {
     } else {
         DBuf<V2> tmp;
         HIP_TRY(tmp.alloc(g->N));
-        launch_scatter_v2(f2, g->fidx.orig.p, tmp.p, g->N, g->stream);
+        launch_scatter_v2(f2, f2_stride, g->fidx.orig.p, tmp.p, g->N, g->stream);
         HIP_TRY(hipMemcpyAsync(out, tmp.p + h.offset, h.n * sizeof(V2), hipMemcpyDeviceToHost,
                                g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));

@@ -68,6 +68,13 @@ struct Tile {
     uint32_t pad;
 };
 
+// sum_j d_ij p_j and d_ii of one particle, gathered together by its
+// neighbours in the IISPH beta sweep.
+struct __align__(16) SdDii {
+    V2 sdp;
+    V2 dii;
+};
+
 // One residual reduction: quantised integer sum and float max (as bits; every
 // reduced value is >= 0 so its bit pattern orders like the float).
 struct Reduction {
@@ -183,13 +190,19 @@ struct Batch {
     float* rho_star;
     float* aii;
     V2* dii;
-    V2* sdp;
     float const* p_in;
     float* p_out;
     // Second buffer of a solver loop's ping-pong: a scene that ran an odd number
     // of iterations left its result there (converged scenes stop writing, so
     // scenes of one batch end in different buffers). Read by the final kernels.
     float* p_alt;
+    // IISPH gathers: {p, rho} of every particle (ping-ponged like the
+    // pressures; pr_alt as p_alt) and {sum_j d_ij p_j, d_ii}, so a neighbour's
+    // fields are two loads.
+    V2 const* pr_in;
+    V2* pr_out;
+    V2 const* pr_alt;
+    SdDii* sd;
     float* alpha;
     float* kappa;
     // original-order state (gathered in by the cells stage, scattered back at the end)
@@ -330,8 +343,9 @@ void launch_pci_finalize(Batch const& b, hipStream_t s);
 void launch_vmax2(Batch const& b, hipStream_t s);
 void launch_scatter_f32(float const* src_sorted, uint32_t const* orig, float* dst, uint32_t n,
                         hipStream_t s);
-void launch_scatter_v2(V2 const* src_sorted, uint32_t const* orig, V2* dst, uint32_t n,
-                       hipStream_t s);
+// src_sorted[t * src_stride] -> dst[orig[t]] (src_stride in V2 units).
+void launch_scatter_v2(V2 const* src_sorted, uint32_t src_stride, uint32_t const* orig, V2* dst,
+                       uint32_t n, hipStream_t s);
 // Solid volumes (solids.cpp:52-74) from the solid-solid lists (sorted solid slots).
 void launch_solid_volumes(Batch const& b, uint32_t n, hipStream_t s);
 // Neighbour lists exported in original order, particle-major [n][64]; indices

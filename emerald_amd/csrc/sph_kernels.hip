@@ -1140,7 +1140,7 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
     }
     a.aii[t] = aii;
     a.rho_star[t] = rs;
-    a.p_out[t] = 0.0f;
+    a.pr_out[t] = V2{0.0f, a.rho[t]};  // p = +0 (iisph.cpp:83), rho
 }
 
 // :225-258 sum_j d_ij p_j (fluid only) of one tile's particles.
@@ -1155,26 +1155,22 @@ __device__ __forceinline__ void iisph_sdp_tile(Batch const& a, Tile const& tl) {
     for (int k0 = 0; k0 < fc; k0 += NB) {
         NbrBatch<NB> nb;
         load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
-        float rj[NB], pj[NB];
+        V2 prj[NB];  // {p_j, rho_j}
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
-            rj[u] = 1.0f;
-            pj[u] = 0.0f;
-            if (k0 + u < fc) {
-                rj[u] = a.rho[nb.j[u]];
-                pj[u] = a.p_in[nb.j[u]];
-            }
+            prj[u] = V2{0.0f, 1.0f};
+            if (k0 + u < fc) { prj[u] = a.pr_in[nb.j[u]]; }
         }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             if (k0 + u < fc) {
-                V2 const numer = scale(kf * pj[u], nb.g[u]);
-                float const denom = rj[u] * rj[u];
+                V2 const numer = scale(kf * prj[u].x, nb.g[u]);
+                float const denom = prj[u].y * prj[u].y;
                 if (is_safe_div(numer, denom)) { s = add(s, divs(numer, denom)); }
             }
         }
     }
-    a.sdp[t] = s;
+    a.sd[t] = SdDii{s, a.dii[t]};
 }
 
 // Converged scenes skip the sweep.
@@ -1204,9 +1200,10 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
         StepConsts const c = a.scenes[tl.scene].c;
         float const mf = c.rho0 * c.fluid_volume;
         float const kf = -(c.dt * c.dt) * mf;
-        float const pi = FIRST ? 0.0f : a.p_in[t];
-        V2 const si = FIRST ? V2{0.0f, 0.0f} : a.sdp[t];
-        if (FIRST) { a.sdp[t] = si; }
+        V2 const pri = a.pr_in[t];  // {p_i, rho_i}
+        float const pi = FIRST ? 0.0f : pri.x;
+        V2 const si = FIRST ? V2{0.0f, 0.0f} : a.sd[t].sdp;
+        if (FIRST) { a.sd[t] = SdDii{si, a.dii[t]}; }
         int const fc = a.fl.count[t];
         float beta = 0.0f;
         for (int k0 = 0; k0 < fc; k0 += NB) {
@@ -1222,12 +1219,16 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
                 djj[u] = V2{0.0f, 0.0f};
                 if (k0 + u < fc) {
                     uint32_t const j = nb.j[u];
-                    rj[u] = a.rho[j];
-                    if (!FIRST) {
-                        pj[u] = a.p_in[j];
-                        sj[u] = a.sdp[j];
+                    V2 const prj = a.pr_in[j];
+                    rj[u] = prj.y;
+                    if (FIRST) {
+                        djj[u] = a.dii[j];  // {sdp, dii} of j is written by this sweep
+                    } else {
+                        SdDii const sdj = a.sd[j];
+                        pj[u] = prj.x;
+                        sj[u] = sdj.sdp;
+                        djj[u] = sdj.dii;
                     }
-                    djj[u] = a.dii[j];
                 }
             }
 #pragma unroll
@@ -1271,7 +1272,7 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
                 pn = (0.0f < mixed) ? mixed : 0.0f;
             }
         }
-        a.p_out[t] = pn;
+        a.pr_out[t] = V2{pn, pri.y};
     }
     block_residual<TILE>(qsum, emax, &a.tile_red[tile]);
 }
@@ -1306,18 +1307,20 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
         }
         int const fc = a.fl.count[t];
         int const sc = a.sl.count[t];
-        // The scene's last pressure buffer (odd iteration count: p_alt); it is
-        // moved into p_in for the field read-back, which no thread of this
-        // scene reads in this kernel.
+        // FINAL (the IISPH end of the sub-step): pressures and densities from
+        // the scene's last {p, rho} buffer (odd iteration count: pr_alt); the
+        // pressure is also written to p_out for the field read-back.
+        // Otherwise (DFSPH): the stiffnesses p_in and densities rho.
         float const* P = a.p_in;
-        if (a.p_alt && (a.stats[r.scene].iterations & 1)) {
-            P = a.p_alt;
-            const_cast<float*>(a.p_in)[t] = P[t];
+        V2 const* PR = nullptr;
+        if (FINAL) {
+            PR = (a.pr_alt && (a.stats[r.scene].iterations & 1)) ? a.pr_alt : a.pr_in;
+            a.p_out[t] = PR[t].x;
         }
         if (fc || sc) {
-            float const ri = a.rho[t];
+            float const ri = FINAL ? PR[t].y : a.rho[t];
             float const ri2 = ri * ri;
-            float const pi = P[t];
+            float const pi = FINAL ? PR[t].x : P[t];
             float const pori = is_safe_div(pi, ri2) ? pi / ri2 : 0.0f;
             if (fc) {
                 float const mf = c.rho0 * c.fluid_volume;
@@ -1331,8 +1334,14 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
                         rj[u] = 1.0f;
                         pj[u] = 0.0f;
                         if (k0 + u < fc) {
-                            rj[u] = a.rho[nb.j[u]];
-                            pj[u] = P[nb.j[u]];
+                            if (FINAL) {
+                                V2 const prj = PR[nb.j[u]];
+                                rj[u] = prj.y;
+                                pj[u] = prj.x;
+                            } else {
+                                rj[u] = a.rho[nb.j[u]];
+                                pj[u] = P[nb.j[u]];
+                            }
                         }
                     }
 #pragma unroll
@@ -1701,10 +1710,10 @@ __global__ __launch_bounds__(256) void k_scatter_f32(float const* src, uint32_t 
     uint32_t const t = blockIdx.x * 256 + threadIdx.x;
     if (t < n) { dst[orig[t]] = src[t]; }
 }
-__global__ __launch_bounds__(256) void k_scatter_v2(V2 const* src, uint32_t const* orig, V2* dst,
-                                                    uint32_t n) {
+__global__ __launch_bounds__(256) void k_scatter_v2(V2 const* src, uint32_t src_stride,
+                                                    uint32_t const* orig, V2* dst, uint32_t n) {
     uint32_t const t = blockIdx.x * 256 + threadIdx.x;
-    if (t < n) { dst[orig[t]] = src[t]; }
+    if (t < n) { dst[orig[t]] = src[static_cast<size_t>(t) * src_stride]; }
 }
 
 __global__ __launch_bounds__(256) void k_export_lists(Lists l, uint32_t n, uint32_t stride,
@@ -1889,10 +1898,11 @@ void launch_scatter_f32(float const* src, uint32_t const* orig, float* dst, uint
     if (!n) { return; }
     hipLaunchKernelGGL(k_scatter_f32, dim3((n + 255) / 256), dim3(256), 0, s, src, orig, dst, n);
 }
-void launch_scatter_v2(V2 const* src, uint32_t const* orig, V2* dst, uint32_t n,
-                       hipStream_t s) {
+void launch_scatter_v2(V2 const* src, uint32_t src_stride, uint32_t const* orig, V2* dst,
+                       uint32_t n, hipStream_t s) {
     if (!n) { return; }
-    hipLaunchKernelGGL(k_scatter_v2, dim3((n + 255) / 256), dim3(256), 0, s, src, orig, dst, n);
+    hipLaunchKernelGGL(k_scatter_v2, dim3((n + 255) / 256), dim3(256), 0, s, src, src_stride,
+                       orig, dst, n);
 }
 void launch_export_lists(Lists const& l, uint32_t n, uint32_t stride, uint32_t const* orig_self,
                          uint32_t const* orig_other, uint32_t const* other_scene_of,
