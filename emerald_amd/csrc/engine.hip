@@ -812,10 +812,14 @@ emsph_status engine_configure(Engine* g, std::vector<SceneHost> scenes,
 //------------------------------------------------------------------------------
 // KeyState read-backs.
 // The smallest LDS list capacity that held the last build's longest list with
-// a margin of two (a longer list in the next build is deferred, not lost).
-int pick_build_cap(unsigned int max_count) {
+// a margin of two; or 16 while the lists above 16 entries are at most 1/64 of
+// the particles: those are deferred to the full-capacity pass, which costs
+// less than halving the occupancy of the whole build. (A list longer than the
+// capacity is always deferred, never lost.)
+int pick_build_cap(KeyState const& k, uint32_t n) {
+    if (k.max_count + 2u <= 16u || static_cast<uint64_t>(k.long_lists) * 64u <= n) { return 16; }
     for (int c : BUILD_CAPS) {
-        if (max_count + 2u <= static_cast<unsigned int>(c)) { return c; }
+        if (k.max_count + 2u <= static_cast<unsigned int>(c)) { return c; }
     }
     return MAXN;
 }
@@ -839,7 +843,7 @@ hipError_t read_wait(Engine* g, int k) {
     hipError_t const e = hipEventSynchronize(g->read_ev[k]);
     if (e) { return e; }
     g->keys = g->host->keys[k];
-    g->build_cap = pick_build_cap(g->keys.max_count);
+    g->build_cap = pick_build_cap(g->keys, g->N);
     if (!g->keys_checked) {
         g->keys_checked = true;
         g->sort_short = g->keys.sort_short != 0u;

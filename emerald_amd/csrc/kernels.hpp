@@ -113,7 +113,7 @@ struct KeyState {
     unsigned int overflow_count;  // particles the last build deferred to its full-cap pass
     unsigned int max_count;       // largest fluid or solid list of the last build
     unsigned int cells;           // cells of the last indexing run
-    unsigned int pad;
+    unsigned int long_lists;      // particles of the last build with a list above 16
 };
 
 // Neighbour-list capacities of the build kernel's LDS list (the full one is

@@ -22,7 +22,8 @@
 
 namespace emsph {
 
-constexpr int BUILD_THREADS = 64;
+constexpr int BUILD_THREADS = 64;  // the full-capacity build and deferred pass
+constexpr int BUILD_WG = 64;        // the capped builds (16 or 32 entries)
 
 // The tile of this workgroup (XCD-aware order: each XCD walks one contiguous
 // run of tiles, i.e. of Morton-ordered space, and keeps its gathers in its L2).
@@ -241,6 +242,7 @@ __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) 
         a.keystate->sort_short = 0u;
         a.keystate->overflow_count = 0u;
         a.keystate->max_count = 0u;
+        a.keystate->long_lists = 0u;
     }
 }
 
@@ -633,7 +635,8 @@ constexpr bool net_sorted() {
 // into the reference order: by distance, then by original index. The network
 // orders by (distance, slot); equal distances (rare) are then put in original
 // index order by an insertion pass with the exact comparator.
-__device__ __forceinline__ void net_sort16(uint64_t (*ent)[BUILD_THREADS], uint32_t lane, int cnt,
+template <int W>
+__device__ __forceinline__ void net_sort16(uint64_t (*ent)[W], uint32_t lane, int cnt,
                                            uint32_t const* __restrict__ oorig) {
     constexpr int N = 16;
     uint64_t e[N];
@@ -688,8 +691,8 @@ __device__ __forceinline__ void load_chunk(V2 (&q)[C], V2 const* __restrict__ op
 // Acceptance and sorted insertion of one loaded chunk, in scan order. Returns
 // false when the scan stops: a capped list overflowed (cnt = -1) or a
 // full-capacity list is full.
-template <int CAP, int C>
-__device__ __forceinline__ bool take_chunk(uint64_t (*ent)[BUILD_THREADS], uint32_t lane, V2 p,
+template <int CAP, int C, int W>
+__device__ __forceinline__ bool take_chunk(uint64_t (*ent)[W], uint32_t lane, V2 p,
                                            uint32_t self, bool same, V2 const (&q)[C],
                                            uint32_t s0, uint32_t e,
                                            uint32_t const* __restrict__ oorig, float maxd,
@@ -737,8 +740,8 @@ __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[BUILD_THREADS], uint3
 // which is then taken for accepted candidates only. (Loading each range's
 // first chunk while the previous range is scanned, with the nine ranges
 // unrolled, measured slower: 217 vs 174 us at 1M.)
-template <int CAP>
-__device__ __forceinline__ int collect_window(uint64_t (*ent)[BUILD_THREADS], uint32_t lane,
+template <int CAP, int W>
+__device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
                                               V2 p, uint32_t self, bool same,
                                               uint2 const* __restrict__ win,
                                               V2 const* __restrict__ opos,
@@ -753,20 +756,21 @@ __device__ __forceinline__ int collect_window(uint64_t (*ent)[BUILD_THREADS], ui
         for (uint32_t s0 = cur.x; s0 < e; s0 += C) {
             V2 q[C];
             load_chunk<C>(q, opos, s0, e);
-            if (!take_chunk<CAP, C>(ent, lane, p, self, same, q, s0, e, oorig, maxd, cut2, cnt)) {
+            if (!take_chunk<CAP, C, W>(ent, lane, p, self, same, q, s0, e, oorig, maxd, cut2,
+                                       cnt)) {
                 return cnt;  // overflow (-1) or a full 64-entry list (insertion-sorted)
             }
         }
         cur = nxt;
     }
-    if (net_sorted<CAP>() && cnt > 1) { net_sort16(ent, lane, cnt, oorig); }
+    if (net_sorted<CAP>() && cnt > 1) { net_sort16<W>(ent, lane, cnt, oorig); }
     return cnt;
 }
 
 // Lists and the fused per-particle sums of slot t. Returns false (nothing of
 // the particle's state changed) when a list would exceed CAP.
-template <int MODE, int CAP>
-__device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_THREADS],
+template <int MODE, int CAP, int W>
+__device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
                                           uint32_t lane, uint32_t t, uint32_t scene,
                                           int& max_count) {
     StepConsts const c = a.scenes[scene].c;
@@ -775,7 +779,7 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
     float const mf = c.rho0 * c.fluid_volume;  // rho0 * V_j (uniform fluid volume)
 
     // Fluid neighbours.
-    int const fc = collect_window<CAP>(ent, lane, p, t, true, a.win + static_cast<size_t>(cell) * 9,
+    int const fc = collect_window<CAP, W>(ent, lane, p, t, true, a.win + static_cast<size_t>(cell) * 9,
                                        a.pos, a.orig, c.cell, c.cut2);
     if (fc < 0) {
         max_count = CAP + 1;
@@ -837,7 +841,7 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
     SolidView const& sv = a.solids;
     int sc = 0;
     if (a.swin && sv.pos) {
-        sc = collect_window<CAP>(ent, lane, p, 0xFFFFFFFFu, false,
+        sc = collect_window<CAP, W>(ent, lane, p, 0xFFFFFFFFu, false,
                                  a.swin + static_cast<size_t>(cell) * 9, sv.pos, sv.orig, c.cell,
                                  c.cut2);
         if (sc < 0) {
@@ -926,8 +930,14 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[BUILD_
 }
 
 // Wave maximum of the list counts into KeyState::max_count (read first: the
-// value rarely grows, so few waves take the atomic).
+// value rarely grows, so few waves take the atomic), and the wave's count of
+// lists above 16 entries into KeyState::long_lists (overflowed lists count:
+// m is then CAP + 1).
 __device__ __forceinline__ void note_max_count(Batch const& a, int m) {
+    unsigned long long const longs = __ballot(m > 16);
+    if (longs && __lane_id() == 0) {
+        atomicAdd(&a.keystate->long_lists, static_cast<unsigned int>(__popcll(longs)));
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         int const o = __shfl_xor(m, off);
@@ -941,20 +951,22 @@ __device__ __forceinline__ void note_max_count(Batch const& a, int m) {
     }
 }
 
-template <int MODE, int CAP>
-__global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
-    __shared__ uint64_t ent[CAP][BUILD_THREADS];
+template <int MODE, int CAP, int WG>
+__global__ __launch_bounds__(WG) void k_build(Batch a) {
+    __shared__ uint64_t ent[CAP][WG];
     uint32_t const lane = threadIdx.x;
-    // Build tiles are the fluid tiles split in quarters (64 threads).
+    // Build workgroups cover TILE / WG parts of a fluid tile each.
+    constexpr uint32_t PARTS = TILE / WG;
     uint32_t const k = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
-    Tile const tl = a.tiles[k >> 2];
-    uint32_t const sub_off = (k & 3u) * BUILD_THREADS + lane;
-    if ((k & 3u) * BUILD_THREADS >= tl.count) { return; }  // the whole wave
+    Tile const tl = a.tiles[k / PARTS];
+    uint32_t const part_off = (k % PARTS) * WG;
+    if (part_off >= tl.count) { return; }  // the whole workgroup
+    uint32_t const sub_off = part_off + lane;
     // Every lane stays for the cross-lane reductions below.
     bool const valid = sub_off < tl.count;
     uint32_t const t = tl.start + sub_off;
     int m = 0;
-    bool const ok = valid ? build_one<MODE, CAP>(a, ent, lane, t, tl.scene, m) : true;
+    bool const ok = valid ? build_one<MODE, CAP, WG>(a, ent, lane, t, tl.scene, m) : true;
     note_max_count(a, m);
     if (CAP < MAXN) {
         // Deferred particles, appended with one atomic per wave.
@@ -962,14 +974,15 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build(Batch a) {
         if (mask) {
             int const leader = __ffsll(mask) - 1;
             unsigned int base = 0;
-            if (static_cast<int>(lane) == leader) {
+            uint32_t const wl = __lane_id();
+            if (static_cast<int>(wl) == leader) {
                 base = atomicAdd(&a.keystate->overflow_count,
                                  static_cast<unsigned int>(__popcll(mask)));
             }
             base = __shfl(base, leader);
             if (!ok) {
                 unsigned int const r =
-                  static_cast<unsigned int>(__popcll(mask & ((1ull << lane) - 1ull)));
+                  static_cast<unsigned int>(__popcll(mask & ((1ull << wl) - 1ull)));
                 a.overflow[base + r] = make_uint2(t, tl.scene);
             }
         }
@@ -985,23 +998,25 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build_deferred(Batch a) {
          i += gridDim.x * BUILD_THREADS) {
         uint2 const e = a.overflow[i];
         int m = 0;
-        (void)build_one<MODE, MAXN>(a, ent, threadIdx.x, e.x, e.y, m);
+        (void)build_one<MODE, MAXN, BUILD_THREADS>(a, ent, threadIdx.x, e.x, e.y, m);
         if (m > 0) { atomicMax(&a.keystate->max_count, static_cast<unsigned int>(m)); }
     }
 }
 
 template <int MODE>
 void launch_build_mode(int cap, Batch const& a, hipStream_t s) {
-    uint32_t const blocks = a.ntiles * 4;
+    uint32_t const blocks = a.ntiles * (TILE / BUILD_WG);
     if (cap <= 16) {
-        hipLaunchKernelGGL((k_build<MODE, 16>), dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_build<MODE, 16, BUILD_WG>), dim3(blocks), dim3(BUILD_WG), 0, s, a);
     } else if (cap <= 32) {
-        hipLaunchKernelGGL((k_build<MODE, 32>), dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_build<MODE, 32, BUILD_WG>), dim3(blocks), dim3(BUILD_WG), 0, s, a);
     } else {
-        hipLaunchKernelGGL((k_build<MODE, MAXN>), dim3(blocks), dim3(BUILD_THREADS), 0, s, a);
+        // 64 entries of 64 lanes: 32 KB of LDS per workgroup.
+        hipLaunchKernelGGL((k_build<MODE, MAXN, BUILD_THREADS>),
+                           dim3(a.ntiles * (TILE / BUILD_THREADS)), dim3(BUILD_THREADS), 0, s, a);
         return;
     }
-    uint32_t const dblocks = blocks < 1024 ? blocks : 1024;
+    uint32_t const dblocks = a.ntiles * 4 < 1024 ? a.ntiles * 4 : 1024;
     hipLaunchKernelGGL(k_build_deferred<MODE>, dim3(dblocks), dim3(BUILD_THREADS), 0, s, a);
 }
 

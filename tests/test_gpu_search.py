@@ -80,6 +80,22 @@ def test_list_capacity_switches(gpu):
         s.close()
 
 
+def test_few_long_lists_deferred_at_cap_16(gpu):
+    # Sparse points with a small tight cluster: the cluster's lists (above 16
+    # entries) are under 1/64 of the particles, so the second run keeps the
+    # 16-entry build and defers them to the full-capacity pass.
+    rng = np.random.default_rng(12)
+    cluster = (0.3 + 0.0005 * rng.standard_normal((200, 2))).astype(np.float32)
+    xy = np.concatenate([rng.random((20000, 2)).astype(np.float32), cluster])
+    s = NeighborSearch(len(xy))
+    try:
+        for _ in range(3):
+            g = _compare(xy, 1.0 / 100.0, s=s)
+        assert g["counts"].max() > 16
+    finally:
+        s.close()
+
+
 def test_lattice_distance_ties(gpu):
     # An unjittered lattice has exact distance ties, broken by original index.
     i, j = np.meshgrid(np.arange(60), np.arange(50), indexing="ij")
