@@ -912,6 +912,22 @@ hipError_t run_loop(Engine* g, int kind, int max_it, int first_check, F&& launch
     return hipSuccess;
 }
 
+// The first iteration of a solver loop whose exit test can stop some scene:
+// the method's first exit check, or an earlier last iteration (the scene's
+// maximum). Before it the exit tests only keep every scene iterating, so they
+// are not launched.
+int first_decision(Engine const* g, int kind, int first_check) {
+    int least_max = first_check + 1;
+    for (auto const& h : g->sh) {
+        emsph_parameters const& p = h.cfg.params;
+        int const mx = kind == 0   ? std::min(p.iisph_max_pressure_iterations, MAX_RED_SLOTS)
+                       : kind == 3 ? 6
+                                   : std::min(p.dfsph_max_correction_iterations, MAX_RED_SLOTS / 2);
+        least_max = std::min(least_max, mx);
+    }
+    return std::min(first_check, least_max - 1);
+}
+
 // Host-side state at the start of a sub-step attempt.
 void begin_attempt(Engine* g) {
     g->keys_checked = g->passes_hint < 0;
@@ -1017,6 +1033,7 @@ emsph_status iisph_sub_step(Engine* g) {
     }
     bool short_sort = false;
     // iisph.cpp:126: no scene can exit before its third iteration.
+    int const decide_from = first_decision(g, 0, 2);
     HIP_TRY(run_loop(g, 0, max_it, 2, [&](int it) -> hipError_t {
         b.pr_in = (it & 1) ? pb : pa;
         b.pr_out = (it & 1) ? pa : pb;
@@ -1027,6 +1044,7 @@ emsph_status iisph_sub_step(Engine* g) {
             HIP_PASS(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
         }
         HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, it == 0, g->stream); }));
+        if (it < decide_from) { return hipSuccess; }
         return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); });
     }, &g->last_iterations, &short_sort));
     if (short_sort) { return EMSPH_OK; }
@@ -1067,9 +1085,11 @@ emsph_status dfsph_sub_step(Engine* g) {
     }));
     finish_prepare(g);
     bool short_sort = false;
+    int const div_from = first_decision(g, 1, 1);
     HIP_TRY(run_loop(g, 1, max_it, 1, [&](int it) -> hipError_t {
         HIP_PASS(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, g->stream); }));
         HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
+        if (it < div_from) { return hipSuccess; }
         return timed(g, "loop", [&] {
             launch_loop_decide(b, LOOP_DIVERGENCE, it, it, g->stream);
         });
@@ -1083,9 +1103,11 @@ emsph_status dfsph_sub_step(Engine* g) {
         HIP_PASS(timed(g, "apply_gravity", [&] { launch_apply(b, 1, 0, 1, g->stream); }));
         return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
     }));
+    int const dens_from = first_decision(g, 2, 2);
     HIP_TRY(run_loop(g, 2, max_it, 2, [&](int it) -> hipError_t {
         HIP_PASS(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, g->stream); }));
         HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
+        if (it < dens_from) { return hipSuccess; }
         return timed(g, "loop", [&] {
             launch_loop_decide(b, LOOP_DENSITY, it, base + it, g->stream);
         });
@@ -1121,11 +1143,13 @@ emsph_status pcisph_sub_step(Engine* g) {
     }));
     finish_prepare(g);
     bool short_sort = false;
+    int const pci_from = first_decision(g, 3, 2);
     HIP_TRY(run_loop(g, 3, 6, 2, [&](int it) -> hipError_t {
         HIP_PASS(timed(g, "pci_density", [&] { launch_pci_density(b, g->stream); }));
         b.vstar_in = (it & 1) ? vout : vin;
         b.vstar_out = (it & 1) ? vin : vout;
         HIP_PASS(timed(g, "pci_force", [&] { launch_pci_force(b, g->stream); }));
+        if (it < pci_from) { return hipSuccess; }
         return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_PCISPH, it, it, g->stream); });
     }, &g->last_iterations, &short_sort));
     if (short_sort) { return EMSPH_OK; }
