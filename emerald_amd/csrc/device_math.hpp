@@ -63,8 +63,25 @@ __device__ __forceinline__ bool is_safe_div(V2 n, float d) {
     return (mr > __builtin_fabsf(n.x)) && (mr > __builtin_fabsf(n.y));
 }
 
+// Correctly rounded single-precision a / b for a divisor b whose reciprocal
+// inv = RN64(1 / b) is known. When |a / b| >= 2^-126 (a normal float result,
+// or an overflow), RN32(RN64(a * inv)) == RN32(a / b). Proof sketch: in the
+// binade [2^E, 2^(E+1)) the float midpoints are M 2^(E-24) with M odd and
+// above 2^24, so a / b is never one (M b would need 25 significant bits) and
+// a - M 2^(E-24) b is a nonzero multiple of 2^(E-24+e_b): a / b lies more
+// than 2^-49 (relative) from every midpoint, while RN64(a * inv) is within
+// 2^-52 (relative) of a / b, on the same side of every midpoint. No double
+// involved is subnormal. Subnormal and zero results (where a / b can be an
+// exact midpoint, e.g. 3 2^-149 / 6) and NaN take the IEEE divide.
+__device__ __forceinline__ float div_by_recip(float a, float b, double inv) {
+    double const q = static_cast<double>(a) * inv;
+    if (!(__builtin_fabs(q) >= 0x1p-126)) { return a / b; }
+    return static_cast<float>(q);
+}
+
 // Constants of the cubic-spline kernel for one support h, evaluated on the
-// host in the same float operation order as sph_common/kernels.h:25-55.
+// host in the same float operation order as sph_common/kernels.h:25-55, with
+// the double reciprocals of the divisors (for div_by_recip).
 struct KernelConsts {
     float h;       // support (the reference's params.support)
     float k;       // h * h * h * pi
@@ -72,17 +89,18 @@ struct KernelConsts {
     float hhh;     // h * h * h
     float four_h;  // 4 * h
     float w0;      // W(0, h)
+    double inv_h, inv_k, inv_hh, inv_hhh, inv_four_h;  // RN64(1 / x)
 };
 
 // sph_common/kernels.h:16-35 W(r, h). The two pieces' final divides by k are
 // one divide of the selected numerator: the same operations per lane, with
 // no divergent branches across a wave (lanes at q <= 1 and q > 1 are common).
 __device__ __forceinline__ float kernel_W(float r, KernelConsts const& c) {
-    float const q = r / c.h;
+    float const q = div_by_recip(r, c.h, c.inv_h);  // r / h
     float const wa = 1.0f + q * q * (q * 0.75f - 1.5f);
     float const t = 2.0f - q;
     float const wb = (t * t * t) / 4.0f;
-    float const w = (q <= 1.0f ? wa : wb) / c.k;
+    float const w = div_by_recip(q <= 1.0f ? wa : wb, c.k, c.inv_k);  // / k
     return (q <= 2.0f) ? w : 0.0f;
 }
 
@@ -90,14 +108,14 @@ __device__ __forceinline__ float kernel_W(float r, KernelConsts const& c) {
 // divide of either piece is one divide of selected operands
 // (-3 r / h^2 or -3 (2 - r/h)^2 / 4h), and the final divide by k is shared.
 __device__ __forceinline__ float kernel_dW(float r, KernelConsts const& c) {
-    float const q = r / c.h;
+    float const q = div_by_recip(r, c.h, c.inv_h);  // r / h
     bool const inner = q <= 1.0f;
     float const t = 2.0f - q;
     float const num = inner ? -3.0f * r : -3.0f * (t * t);
-    float const den = inner ? c.hh : c.four_h;
-    float x = num / den;
-    if (inner) { x = x + 2.25f * r * r / c.hhh; }
-    float const dw = x / c.k;
+    float x = inner ? div_by_recip(num, c.hh, c.inv_hh)           // / h^2
+                    : div_by_recip(num, c.four_h, c.inv_four_h);  // / 4h
+    if (inner) { x = x + div_by_recip(2.25f * r * r, c.hhh, c.inv_hhh); }  // + 2.25 r^2 / h^3
+    float const dw = div_by_recip(x, c.k, c.inv_k);  // / k
     return (q <= 2.0f) ? dw : 0.0f;
 }
 

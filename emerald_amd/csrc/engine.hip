@@ -80,6 +80,12 @@ KernelConsts make_kernel_consts(float h) {
     // W(0, h) = _wa(0) / k (kernels.h:25-35)
     float const q = 0.0f / h;
     c.w0 = (1.0f + q * q * (q * 0.75f - 1.5f)) / c.k;
+    // Correctly rounded double reciprocals (the host divide is IEEE).
+    c.inv_h = 1.0 / static_cast<double>(c.h);
+    c.inv_k = 1.0 / static_cast<double>(c.k);
+    c.inv_hh = 1.0 / static_cast<double>(c.hh);
+    c.inv_hhh = 1.0 / static_cast<double>(c.hhh);
+    c.inv_four_h = 1.0 / static_cast<double>(c.four_h);
     return c;
 }
 
