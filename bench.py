@@ -42,9 +42,9 @@ NB = {"keys": 16, "radix_sort": 16, "cells": 8, "step_begin": 0, "loop": 0}
 ALGO_BYTES = {
     "iisph": {**NB,
               "build": 48,            # rho 12 + (v += g) 16 + d_ii 20
-              "iisph_aii": 40,        # (a_ii, rho*) 36 + (p = 0) 4
-              "iisph_sdp": 24,        # sum_j d_ij p_j, per iteration
-              "iisph_beta_p": 44,     # beta, p', residual, per iteration
+              "iisph_aii_beta": 84,   # (a_ii, rho*) 36 + (p = 0) 4 + first beta/p'/residual 44
+              "iisph_sdp": 24,        # sum_j d_ij p_j, iterations 2..k
+              "iisph_beta_p": 44,     # beta, p', residual, iterations 2..k
               "apply_integrate": 56},  # apply 32 + x 24
     "dfsph_p": {**NB,
                 "build": 28,          # rho 12 + alpha 16

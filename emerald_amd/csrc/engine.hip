@@ -1025,11 +1025,13 @@ emsph_status iisph_sub_step(Engine* g) {
     V2* pa = g->pr0.p;
     V2* pb = g->pr1.p;
     Batch b = fluid_batch(g);
-    b.pr_out = pa;  // the a_ii sweep starts the pressures at +0
+    // The a_ii sweep also starts the solver loop and runs its first
+    // iteration (pressures +0, iisph.cpp:83): its pressures go where
+    // iteration 0's would.
+    b.pr_out = pb;
     HIP_TRY(segment(g, EMSPH_IISPH, SEG_PREFIX, [&]() -> hipError_t {
         HIP_PASS(enqueue_prepare(g, BUILD_IISPH, false));
-        // The a_ii sweep also starts the solver loop.
-        return timed(g, "iisph_aii", [&] { launch_iisph_aii(b, g->stream); });
+        return timed(g, "iisph_aii_beta", [&] { launch_iisph_aii_beta0(b, g->stream); });
     }));
     finish_prepare(g);
     int max_it = 0;
@@ -1043,13 +1045,12 @@ emsph_status iisph_sub_step(Engine* g) {
     HIP_TRY(run_loop(g, 0, max_it, 2, [&](int it) -> hipError_t {
         b.pr_in = (it & 1) ? pb : pa;
         b.pr_out = (it & 1) ? pa : pb;
-        // iisph.cpp:85-119: sum_j d_ij p_j, then beta and the new pressures.
-        // The pressures start at +0 (:83), so the first sums are +0 (written
-        // by the first beta sweep instead).
+        // iisph.cpp:85-119: sum_j d_ij p_j, then beta and the new pressures
+        // (iteration 0 ran in the a_ii sweep).
         if (it > 0) {
             HIP_PASS(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
+            HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, g->stream); }));
         }
-        HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, it == 0, g->stream); }));
         if (it < decide_from) { return hipSuccess; }
         return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); });
     }, &g->last_iterations, &short_sort));

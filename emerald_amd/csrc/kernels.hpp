@@ -323,11 +323,11 @@ void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t 
 // cap: LDS list capacity (one of BUILD_CAPS). Below 64 a particle whose list
 // would exceed it is deferred and rebuilt by a full-capacity pass.
 void launch_build(int mode, int cap, Batch const& b, hipStream_t s);
-void launch_iisph_aii(Batch const& b, hipStream_t s);
+// a_ii, rho* and the pressure loop's first iteration (pressures +0) in one
+// sweep, into pr_out.
+void launch_iisph_aii_beta0(Batch const& b, hipStream_t s);
 void launch_iisph_sdp(Batch const& b, hipStream_t s);
-// first: the loop's first iteration, with every pressure +0 and the
-// sum_j d_ij p_j sweep skipped (written as +0 here).
-void launch_iisph_beta_p(Batch const& b, int first, hipStream_t s);
+void launch_iisph_beta_p(Batch const& b, hipStream_t s);
 // Apply pressure accelerations (pressure.cpp:9-80) with p_in as the pressure,
 // for active scenes (or all scenes when all_scenes != 0).
 // with_gravity: first integrate the external gravity force (dynamics.cpp:24-37).

@@ -1082,80 +1082,121 @@ __device__ __forceinline__ void load_batch(Lists const& l, uint32_t stride, uint
 //------------------------------------------------------------------------------
 // IISPH sweeps: sph2d_box/iisph_ops.cpp.
 
-// :94-221 a_ii and density stars; also zeroes the pressures (iisph.cpp:83).
-// Workgroup 0 also starts the solver loop (every scene with particles active).
+// :94-221 a_ii and density stars fused with the solver loop's first
+// iteration (:261-425 beta, new pressure and residual). The first iteration's
+// inputs are all known here: the pressures start at +0 (iisph.cpp:83), so
+// every sum_j d_ij p_j is exactly (+0, +0) (each term is (kf * +0) grad W /
+// rho_j^2 = +-0, or an unsafe divide that is skipped, and +0 + +-0 = +0), and
+// beta needs only rho_j and d_jj of the neighbours and the particle's own
+// a_ii and rho*. One pass over the list computes both, with the same IEEE
+// operations in the same order per accumulator (d_ij is the same value in
+// both), and skips the first sum sweep. Workgroup 0 also starts the loop.
+// Writes a_ii, rho*, {sum = (+0, +0), d_ii} and the first pressures
+// {p', rho} into pr_out.
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_iisph_aii(Batch a) {
+__global__ __launch_bounds__(TILE) void k_iisph_aii_beta0(Batch a) {
     if (blockIdx.x == 0) { loop_begin_block<TILE>(a); }
-    TileRef const r = tile_ref(a, TILE);
-    if (!r.valid) { return; }
-    uint32_t const t = r.t;
-    StepConsts const c = a.scenes[r.scene].c;
-    float const mf = c.fluid_volume * c.rho0;
-    float const neg_dt2 = -(c.dt * c.dt);
-    float const kf = neg_dt2 * mf;
-    float const dtm = c.dt * mf;
-    int const fc = a.fl.count[t];
-    V2 const d = a.dii[t];
-    V2 const vi = a.vel[t];
-    float aii;
-    float rs;
-    if (!fc) {
-        rs = a.rho[t];
-        aii = 0.0f;
-    } else {
-        float acc = 0.0f;
-        float dd = 0.0f;
-        for (int k0 = 0; k0 < fc; k0 += NB) {
-            NbrBatch<NB> nb;
-            load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
-            float rj[NB];
-            V2 vj[NB];
+    uint32_t const tile = tile_index(a);
+    Tile const tl = a.tiles[tile];
+    unsigned long long qsum = 0;
+    unsigned int emax = 0;
+    if (threadIdx.x < tl.count) {
+        uint32_t const t = tl.start + threadIdx.x;
+        StepConsts const c = a.scenes[tl.scene].c;
+        float const mf = c.fluid_volume * c.rho0;  // = rho0 * V (commutative)
+        float const neg_dt2 = -(c.dt * c.dt);
+        float const kf = neg_dt2 * mf;
+        float const dtm = c.dt * mf;
+        int const fc = a.fl.count[t];
+        V2 const d = a.dii[t];
+        V2 const vi = a.vel[t];
+        float const ri = a.rho[t];
+        float const pi = 0.0f;
+        V2 const si{0.0f, 0.0f};
+        a.sd[t] = SdDii{si, d};
+        float aii;
+        float rs;
+        float beta = 0.0f;
+        if (!fc) {
+            rs = ri;
+            aii = 0.0f;
+        } else {
+            float acc = 0.0f;
+            float dd = 0.0f;
+            for (int k0 = 0; k0 < fc; k0 += NB) {
+                NbrBatch<NB> nb;
+                load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+                float rj[NB];
+                V2 vj[NB], djj[NB];
 #pragma unroll
-            for (int u = 0; u < NB; ++u) {
-                rj[u] = 1.0f;
-                vj[u] = V2{0.0f, 0.0f};
-                if (k0 + u < fc) {
-                    rj[u] = a.rho[nb.j[u]];
-                    vj[u] = a.vel[nb.j[u]];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < NB; ++u) {
-                if (k0 + u < fc) {
-                    V2 const gw = nb.g[u];
-                    V2 const numer = scale(kf, gw);
-                    float const denom = rj[u] * rj[u];
-                    if (is_safe_div(numer, denom)) {
-                        V2 const dij = divs(numer, denom);
-                        V2 const dji = neg(dij);
-                        acc += mf * dot(sub(d, dji), gw);
+                for (int u = 0; u < NB; ++u) {
+                    rj[u] = 1.0f;
+                    vj[u] = V2{0.0f, 0.0f};
+                    djj[u] = V2{0.0f, 0.0f};
+                    if (k0 + u < fc) {
+                        uint32_t const j = nb.j[u];
+                        rj[u] = a.rho[j];
+                        vj[u] = a.vel[j];
+                        djj[u] = a.dii[j];
                     }
-                    dd += dtm * dot(sub(vi, vj[u]), gw);
+                }
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    if (k0 + u < fc) {
+                        V2 const gw = nb.g[u];
+                        V2 const numer = scale(kf, gw);
+                        float const denom = rj[u] * rj[u];
+                        bool const safe = is_safe_div(numer, denom);
+                        V2 const dij = safe ? divs(numer, denom) : V2{0.0f, 0.0f};
+                        V2 const dji = neg(dij);
+                        if (safe) { acc += mf * dot(sub(d, dji), gw); }  // a_ii
+                        dd += dtm * dot(sub(vi, vj[u]), gw);             // rho*
+                        V2 const sjk = sub(V2{0.0f, 0.0f}, scale(pi, dji));  // beta
+                        V2 const djj_pj = scale(0.0f, djj[u]);
+                        beta += mf * dot(sub(sub(si, djj_pj), sjk), gw);
+                    }
                 }
             }
+            aii = acc;
+            rs = ri + dd;
         }
-        aii = acc;
-        rs = a.rho[t] + dd;
-    }
-    int const sc = a.sl.count[t];
-    if (sc) {
-        float acc = 0.0f;
-        float dd = 0.0f;
-        for (int k = 0; k < sc; ++k) {
-            size_t const o = static_cast<size_t>(k) * a.stride + t;
-            uint32_t const b = a.sl.idx[o];
-            V2 const gw = a.sl.gw[o];
-            float const ms = a.solids.vol[b] * c.rho0;
-            acc += ms * dot(d, gw);
-            dd += c.dt * ms * dot(sub(vi, a.solids.vel[b]), gw);
+        int const sc = a.sl.count[t];
+        if (sc) {
+            float acc = 0.0f;
+            float dd = 0.0f;
+            float bs = 0.0f;
+            for (int k = 0; k < sc; ++k) {
+                size_t const o = static_cast<size_t>(k) * a.stride + t;
+                uint32_t const b = a.sl.idx[o];
+                V2 const gw = a.sl.gw[o];
+                float const ms = a.solids.vol[b] * c.rho0;
+                acc += ms * dot(d, gw);
+                dd += c.dt * ms * dot(sub(vi, a.solids.vel[b]), gw);
+                bs += ms * dot(si, gw);
+            }
+            aii += acc;
+            rs += dd;
+            beta += bs;
         }
-        aii += acc;
-        rs += dd;
+        a.aii[t] = aii;
+        a.rho_star[t] = rs;
+        // iisph_compute_new_pressures for p = +0
+        float pn = pi;
+        if (aii != 0.0f) {
+            float const numer = c.rho0 - rs - beta;
+            if (is_safe_div(numer, aii)) {
+                float err = -(numer - (aii * pi)) / c.rho0;
+                err = (0.0f < err) ? err : 0.0f;  // std::max(0.0f, err)
+                emax = __float_as_uint(err);
+                qsum = static_cast<unsigned long long>(err * 1000.0f);
+                float const np = numer / aii;
+                float const mixed = (pi * (1.0f - c.omega)) + (np * c.omega);
+                pn = (0.0f < mixed) ? mixed : 0.0f;
+            }
+        }
+        a.pr_out[t] = V2{pn, ri};
     }
-    a.aii[t] = aii;
-    a.rho_star[t] = rs;
-    a.pr_out[t] = V2{0.0f, a.rho[t]};  // p = +0 (iisph.cpp:83), rho
+    block_residual<TILE>(qsum, emax, &a.tile_red[tile]);
 }
 
 // :225-258 sum_j d_ij p_j (fluid only) of one tile's particles.
@@ -1199,14 +1240,7 @@ __global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
 // :261-353 beta (into the new-pressure buffer) fused with :357-425 new
 // pressure, quantised error sum and error max, for one tile (every thread of
 // the workgroup calls it).
-//
-// FIRST: the first iteration. Every pressure is then +0 (k_iisph_aii set
-// them), so every sum_j d_ij p_j is exactly (+0, +0): each term is
-// (kf * +0) g / rho_j^2 = +-0 or skipped as an unsafe divide, and +0 plus
-// +-0 stays +0. The sweep of the sums is skipped; this kernel writes them and
-// takes p_i, p_j and the sums as those constants (the same IEEE operations
-// on the same values, so the same bits).
-template <int NB, int FIRST>
+template <int NB>
 __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl, uint32_t tile) {
     unsigned long long qsum = 0;
     unsigned int emax = 0;
@@ -1216,9 +1250,8 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
         float const mf = c.rho0 * c.fluid_volume;
         float const kf = -(c.dt * c.dt) * mf;
         V2 const pri = a.pr_in[t];  // {p_i, rho_i}
-        float const pi = FIRST ? 0.0f : pri.x;
-        V2 const si = FIRST ? V2{0.0f, 0.0f} : a.sd[t].sdp;
-        if (FIRST) { a.sd[t] = SdDii{si, a.dii[t]}; }
+        float const pi = pri.x;
+        V2 const si = a.sd[t].sdp;
         int const fc = a.fl.count[t];
         float beta = 0.0f;
         for (int k0 = 0; k0 < fc; k0 += NB) {
@@ -1235,15 +1268,11 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
                 if (k0 + u < fc) {
                     uint32_t const j = nb.j[u];
                     V2 const prj = a.pr_in[j];
+                    SdDii const sdj = a.sd[j];
                     rj[u] = prj.y;
-                    if (FIRST) {
-                        djj[u] = a.dii[j];  // {sdp, dii} of j is written by this sweep
-                    } else {
-                        SdDii const sdj = a.sd[j];
-                        pj[u] = prj.x;
-                        sj[u] = sdj.sdp;
-                        djj[u] = sdj.dii;
-                    }
+                    pj[u] = prj.x;
+                    sj[u] = sdj.sdp;
+                    djj[u] = sdj.dii;
                 }
             }
 #pragma unroll
@@ -1294,12 +1323,12 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
 
 // A converged scene skips the sweep; its result stays in the buffer of its
 // last iteration.
-template <int NB, int FIRST>
+template <int NB>
 __global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a) {
     uint32_t const tile = tile_index(a);
     Tile const tl = a.tiles[tile];
     if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
-    iisph_beta_p_tile<NB, FIRST>(a, tl, tile);
+    iisph_beta_p_tile<NB>(a, tl, tile);
 }
 
 //------------------------------------------------------------------------------
@@ -1775,32 +1804,14 @@ static inline dim3 tile_grid(Batch const& b) { return dim3(b.ntiles ? b.ntiles :
         }                                                                             \
     } while (0)
 
-void launch_iisph_aii(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_iisph_aii, tile_grid(a), dim3(TILE), 0, s, a);
+void launch_iisph_aii_beta0(Batch const& a, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_iisph_aii_beta0, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_iisph_sdp(Batch const& a, hipStream_t s) {
     NB_DISPATCH(a.nb, k_iisph_sdp, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_iisph_beta_p(Batch const& a, int first, hipStream_t s) {
-    if (a.nb <= 2) {
-        if (first) {
-            hipLaunchKernelGGL((k_iisph_beta_p<2, 1>), tile_grid(a), dim3(TILE), 0, s, a);
-        } else {
-            hipLaunchKernelGGL((k_iisph_beta_p<2, 0>), tile_grid(a), dim3(TILE), 0, s, a);
-        }
-    } else if (a.nb <= 4) {
-        if (first) {
-            hipLaunchKernelGGL((k_iisph_beta_p<4, 1>), tile_grid(a), dim3(TILE), 0, s, a);
-        } else {
-            hipLaunchKernelGGL((k_iisph_beta_p<4, 0>), tile_grid(a), dim3(TILE), 0, s, a);
-        }
-    } else {
-        if (first) {
-            hipLaunchKernelGGL((k_iisph_beta_p<8, 1>), tile_grid(a), dim3(TILE), 0, s, a);
-        } else {
-            hipLaunchKernelGGL((k_iisph_beta_p<8, 0>), tile_grid(a), dim3(TILE), 0, s, a);
-        }
-    }
+void launch_iisph_beta_p(Batch const& a, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a);
 }
 
 template <int G, int F>
