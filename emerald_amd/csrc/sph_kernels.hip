@@ -1570,6 +1570,9 @@ __global__ __launch_bounds__(TILE) void k_pci_init(Batch a) {
 // :90-115 geometry at x* (neighborhoods_from_state.cpp:103-128), densities
 // (pcisph_ops.cpp:18-63), pressure update (:65-76) and the max density error
 // (common.cpp:78-85). PCISPH densities are kept in rho.
+// Neighbour entries are taken NB at a time (their loads in flight together)
+// and processed in list order.
+template <int NB>
 __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     if (!a.stats[r.scene].active) { return; }
@@ -1580,15 +1583,31 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
         V2 const xs = a.xstar[t];
         int const fc = a.fl.count[t];
         float w = c.kc.w0;
-        for (int k = 0; k < fc; ++k) {
-            size_t const o = static_cast<size_t>(k) * a.stride + t;
-            uint32_t const j = a.fl.idx[o];
-            V2 const vec = sub(a.xstar[j], xs);
-            float const d = imath_length(vec);
-            a.fl.vec[o] = vec;
-            a.fl.dist[o] = d;
-            a.fl.gw[o] = kernel_GradW_r(neg(vec), d, c.kc);
-            w += kernel_W(d, c.kc);
+        for (int k0 = 0; k0 < fc; k0 += NB) {
+            uint32_t j[NB];
+            V2 xj[NB];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                j[u] = 0u;
+                if (k0 + u < fc) { j[u] = a.fl.idx[static_cast<size_t>(k0 + u) * a.stride + t]; }
+            }
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                xj[u] = xs;
+                if (k0 + u < fc) { xj[u] = a.xstar[j[u]]; }
+            }
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                if (k0 + u < fc) {
+                    size_t const o = static_cast<size_t>(k0 + u) * a.stride + t;
+                    V2 const vec = sub(xj[u], xs);
+                    float const d = imath_length(vec);
+                    a.fl.vec[o] = vec;
+                    a.fl.dist[o] = d;
+                    a.fl.gw[o] = kernel_GradW_r(neg(vec), d, c.kc);
+                    w += kernel_W(d, c.kc);
+                }
+            }
         }
         float rho = c.mass * w;
         int const sc = a.sl.count[t];
@@ -1615,7 +1634,8 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
 
 // :117-142 pressure + artificial viscosity forces, solid forces, then the next
 // predictor (:73-88) from the new force. A converged scene carries its last
-// predictor forward.
+// predictor forward. Neighbour entries are taken NB at a time.
+template <int NB>
 __global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     if (!r.valid) { return; }
@@ -1634,21 +1654,49 @@ __global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
     if (fc) {
         V2 const v = a.vstar_in[t];
         V2 f{0.0f, 0.0f};
-        for (int k = 0; k < fc; ++k) {
-            size_t const o = static_cast<size_t>(k) * a.stride + t;
-            uint32_t const j = a.fl.idx[o];
-            V2 const rv = a.fl.vec[o];
-            V2 const gw = a.fl.gw[o];
-            float const od = rho[j];
-            f = add(f, scale(por + (p[j] / (od * od)), gw));
-            V2 const dv = sub(a.vstar_in[j], v);
-            float const dvr = dot(rv, dv);
-            if (dvr < 0.0f) {
-                float const dist = a.fl.dist[o];
-                float const mu = H * dvr / (dist + N2);
-                float const den_avg = (density + od) / 2.0f;
-                float const pi_ab = ((-1.0f * c.viscosity * mu) + (2.0f * (mu * mu))) / den_avg;
-                f = add(f, scale(pi_ab, gw));
+        for (int k0 = 0; k0 < fc; k0 += NB) {
+            uint32_t j[NB];
+            V2 rv[NB], gw[NB], vj[NB];
+            float od[NB], pj[NB], dist[NB];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                j[u] = 0u;
+                rv[u] = V2{0.0f, 0.0f};
+                gw[u] = V2{0.0f, 0.0f};
+                dist[u] = 1.0f;
+                if (k0 + u < fc) {
+                    size_t const o = static_cast<size_t>(k0 + u) * a.stride + t;
+                    j[u] = a.fl.idx[o];
+                    rv[u] = a.fl.vec[o];
+                    gw[u] = a.fl.gw[o];
+                    dist[u] = a.fl.dist[o];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                od[u] = 1.0f;
+                pj[u] = 0.0f;
+                vj[u] = v;
+                if (k0 + u < fc) {
+                    od[u] = rho[j[u]];
+                    pj[u] = p[j[u]];
+                    vj[u] = a.vstar_in[j[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                if (k0 + u < fc) {
+                    f = add(f, scale(por + (pj[u] / (od[u] * od[u])), gw[u]));
+                    V2 const dv = sub(vj[u], v);
+                    float const dvr = dot(rv[u], dv);
+                    if (dvr < 0.0f) {
+                        float const mu = H * dvr / (dist[u] + N2);
+                        float const den_avg = (density + od[u]) / 2.0f;
+                        float const pi_ab =
+                          ((-1.0f * c.viscosity * mu) + (2.0f * (mu * mu))) / den_avg;
+                        f = add(f, scale(pi_ab, gw[u]));
+                    }
+                }
             }
         }
         F = add(F, scale(-(M * M), f));
@@ -1854,10 +1902,10 @@ void launch_pci_init(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_init, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_pci_density(Batch const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_pci_density, tile_grid(a), dim3(TILE), 0, s, a);
+    NB_DISPATCH(a.nb, k_pci_density, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_pci_force(Batch const& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_pci_force, tile_grid(a), dim3(TILE), 0, s, a);
+    NB_DISPATCH(a.nb, k_pci_force, tile_grid(a), dim3(TILE), 0, s, a);
 }
 void launch_pci_finalize(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_finalize, tile_grid(a), dim3(TILE), 0, s, a);
