@@ -388,7 +388,7 @@ struct Temp {
     Neighborhood<T> nb;
     Neighborhood<T> snb;
     std::vector<V2<T>> external_forces, diis, sum_dij_pjs, velocity_external_forces,
-      position_stars, velocity_stars, pressure_forces;
+      position_stars, velocity_stars, pressure_forces, pressure_accelerations;
     std::vector<T> densities, density_stars, aiis, pressures, new_pressures,
       fluid_volumes, alphas, divergence_kappas, density_kappas;
 };
@@ -1158,6 +1158,220 @@ void dfsph_p_correct_divergence_error(T dt, Config<T> const& cfg, std::vector<V2
     }
 }
 
+//------------------------------------------------------------------------------
+// IISPH_AP (iisph_ap.cpp, iisph_ap_ops.cpp) and IISPH_PSEUDO_AP
+// (iisph_pseudo_ap.cpp, iisph_pseudo_ap_ops.cpp): the pressure-acceleration
+// form of IISPH; the pseudo variant solves for displacements with a
+// fictitious time step of 1 (no dt^2 in the diagonal or in A p).
+
+// iisph_ap_ops.cpp:19-106 iisph_ap_density_stars_and_diagonals (pseudo =
+// false) and iisph_pseudo_ap_ops.cpp:19-110 ..._pseudo_diagonals (pseudo =
+// true, whose diagonal has no -dt^2 factor).
+template <class T>
+void iisph_ap_density_stars_and_diagonals(size_t n, bool pseudo, T dt, T rho0, T* rho_star,
+                                          T* diag, T const* rho, T const* fvol,
+                                          V2<T> const* fvel, Neighborhood<T> const& nb,
+                                          T const* svol, V2<T> const* svel,
+                                          Neighborhood<T> const& snb) {
+#pragma omp parallel for schedule(static, 512)
+    for (size_t i = 0; i < n; ++i) {
+        int const fc = nb.counts[i];
+        int const sc = snb.counts[i];
+        if (!(fc + sc)) {
+            rho_star[i] = rho[i];
+            diag[i] = T(0);
+            continue;
+        }
+        T const self_mass = fvol[i] * rho0;
+        T const self_rho = rho[i];
+        V2<T> const vi = fvel[i];
+        V2<T> sum_m_gradw{T(0), T(0)};
+        T sum_m_gradw_dot_gradw = T(0);
+        T divergence = T(0);
+        for (int k = 0; k < fc; ++k) {
+            uint32_t const j = nb.indices[i * MAXN + k];
+            V2<T> const gw = nb.kernel_gradients[i * MAXN + k];
+            T const mass = fvol[j] * rho0;
+            sum_m_gradw += mass * gw;
+            sum_m_gradw_dot_gradw += mass * gw.dot(gw);
+            divergence += mass * (vi - fvel[j]).dot(gw);
+        }
+        for (int k = 0; k < sc; ++k) {
+            uint32_t const j = snb.indices[i * MAXN + k];
+            V2<T> const gw = snb.kernel_gradients[i * MAXN + k];
+            T const mass = svol[j] * rho0;
+            sum_m_gradw += mass * gw;
+            divergence += mass * (vi - svel[j]).dot(gw);
+        }
+        T const inner = sum_m_gradw.dot(sum_m_gradw) + self_mass * sum_m_gradw_dot_gradw;
+        T const numer = pseudo ? -inner : -sqr(dt) * inner;
+        T const denom = sqr(self_rho);
+        diag[i] = is_safe_divide(numer, denom) ? numer / denom : T(0);
+        rho_star[i] = self_rho + dt * divergence;
+    }
+}
+
+// sph_common/pressure.cpp:83-178 compute_pressure_accelerations; also
+// iisph_pseudo_ap_ops.cpp:113-225 (the displacements, the same arithmetic).
+// The fluid pass sets, the boundary pass adds.
+template <class T>
+void compute_pressure_accelerations(size_t n, T rho0, V2<T>* acc, T const* p, T const* rho,
+                                    T const* fvol, Neighborhood<T> const& nb, T const* svol,
+                                    Neighborhood<T> const& snb) {
+    auto por = [&](size_t i) {
+        T const d = sqr(rho[i]);
+        return is_safe_divide(p[i], d) ? p[i] / d : T(0);
+    };
+#pragma omp parallel for schedule(static, 512)
+    for (size_t i = 0; i < n; ++i) {
+        int const fc = nb.counts[i];
+        if (!fc) {
+            acc[i] = V2<T>{T(0), T(0)};
+            continue;
+        }
+        T const pi = por(i);
+        V2<T> a{T(0), T(0)};
+        for (int k = 0; k < fc; ++k) {
+            uint32_t const j = nb.indices[i * MAXN + k];
+            T const mass = rho0 * fvol[j];
+            V2<T> const gw = nb.kernel_gradients[i * MAXN + k];
+            T const coeff = pi + por(j);
+            a += (-mass * coeff) * gw;
+        }
+        acc[i] = a;
+    }
+#pragma omp parallel for schedule(static, 512)
+    for (size_t i = 0; i < n; ++i) {
+        int const sc = snb.counts[i];
+        if (!sc) { continue; }
+        T const coeff = por(i);
+        V2<T> a{T(0), T(0)};
+        for (int k = 0; k < sc; ++k) {
+            uint32_t const j = snb.indices[i * MAXN + k];
+            T const mass = rho0 * svol[j];
+            V2<T> const gw = snb.kernel_gradients[i * MAXN + k];
+            a += (-mass * coeff) * gw;
+        }
+        acc[i] += a;
+    }
+}
+
+// iisph_ap_ops.cpp:108-194 iisph_ap_iterate_pressures_in_place (pseudo =
+// false: A p scaled by dt^2) and iisph_pseudo_ap_ops.cpp:227-305 (pseudo =
+// true: a zero diagonal zeroes the pressure). Jacobi in place: each particle
+// reads only its own pressure.
+template <class T>
+std::pair<T, T> iisph_ap_iterate_pressures_in_place(size_t n, bool pseudo, T dt, T rho0,
+                                                    T omega, T* p, T const* rho_star,
+                                                    V2<T> const* acc, T const* diag,
+                                                    T const* fvol, Neighborhood<T> const& nb,
+                                                    T const* svol, Neighborhood<T> const& snb) {
+    if (!n) { return {T(0), T(0)}; }
+    uint64_t isum = 0;
+    T emax = T(0);
+#pragma omp parallel for schedule(static, 512) reduction(+ : isum) reduction(max : emax)
+    for (size_t i = 0; i < n; ++i) {
+        int const fc = nb.counts[i];
+        int const sc = snb.counts[i];
+        if (!fc && !sc) {
+            p[i] = T(0);
+            continue;
+        }
+        if (pseudo && diag[i] == T(0)) {
+            p[i] = T(0);
+            continue;
+        }
+        T api = T(0);
+        V2<T> const ai = acc[i];
+        for (int k = 0; k < fc; ++k) {
+            uint32_t const j = nb.indices[i * MAXN + k];
+            T const mass = rho0 * fvol[j];
+            V2<T> const gw = nb.kernel_gradients[i * MAXN + k];
+            api += mass * ((ai - acc[j]).dot(gw));
+        }
+        for (int k = 0; k < sc; ++k) {
+            uint32_t const j = snb.indices[i * MAXN + k];
+            T const mass = rho0 * svol[j];
+            V2<T> const gw = snb.kernel_gradients[i * MAXN + k];
+            api += mass * (ai.dot(gw));
+        }
+        if (!pseudo) { api *= sqr(dt); }
+        T const source = rho0 - rho_star[i];
+        T const q = api - source;
+        T const err = std::max(T(0), is_safe_divide(q, rho0) ? q / rho0 : T(0));
+        emax = std::max(emax, err);
+        isum += static_cast<uint64_t>(err * T(1000));
+        T const numer = omega * (source - api);
+        T const step = is_safe_divide(numer, diag[i]) ? numer / diag[i] : T(0);
+        p[i] = std::max(T(0), p[i] + step);
+    }
+    double const num = static_cast<double>(isum);
+    double const den = static_cast<double>(1000 * n);
+    return {static_cast<T>(num / den), emax};
+}
+
+// iisph_ap.cpp:44-138 (pseudo = false) and iisph_pseudo_ap.cpp:63-155
+// (pseudo = true; its integration, iisph_pseudo_ap_ops.cpp:307-341, moves by
+// the displacement and adds it over dt to the velocity).
+template <class T>
+void iisph_ap_sub_step(bool pseudo, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
+                       std::vector<V2<T>>& vel, Solids<T> const& s, Temp<T>& tp, Stats* st) {
+    size_t const n = pos.size();
+    T const rho0 = cfg.target_density;
+    compute_all_neighborhoods(cfg, pos, s, tp);
+    tp.external_forces.resize(n);
+    gravity_forces(n, cfg.mass_per_particle, cfg.gravity, tp.external_forces.data());
+    tp.densities.resize(n);
+    compute_densities(n, cfg.support, rho0, tp.densities.data(), tp.fluid_volumes.data(),
+                      tp.nb, s.volumes.data(), tp.snb);
+    integrate_velocities_forces(n, dt, rho0, vel.data(), tp.fluid_volumes.data(),
+                                tp.external_forces.data());
+    tp.density_stars.resize(n);
+    tp.aiis.resize(n);
+    iisph_ap_density_stars_and_diagonals(n, pseudo, dt, rho0, tp.density_stars.data(),
+                                         tp.aiis.data(), tp.densities.data(),
+                                         tp.fluid_volumes.data(), vel.data(), tp.nb,
+                                         s.volumes.data(), s.vel.data(), tp.snb);
+    tp.pressures.assign(n, T(0));
+    tp.pressure_accelerations.resize(n);
+    int iter = 0;
+    std::pair<T, T> err{T(0), T(0)};
+    for (; iter < cfg.iisph_max_iterations; ++iter) {
+        compute_pressure_accelerations(n, rho0, tp.pressure_accelerations.data(),
+                                       tp.pressures.data(), tp.densities.data(),
+                                       tp.fluid_volumes.data(), tp.nb, s.volumes.data(), tp.snb);
+        err = iisph_ap_iterate_pressures_in_place(
+          n, pseudo, dt, rho0, cfg.iisph_omega, tp.pressures.data(), tp.density_stars.data(),
+          tp.pressure_accelerations.data(), tp.aiis.data(), tp.fluid_volumes.data(), tp.nb,
+          s.volumes.data(), tp.snb);
+        if (iter > 1 && err.first <= cfg.iisph_error_average_threshold &&
+            err.second <= cfg.iisph_error_max_threshold) {
+            break;
+        }
+    }
+    if (st) {
+        st->iterations = (iter < cfg.iisph_max_iterations) ? iter + 1 : iter;
+        st->error_average = static_cast<float>(err.first);
+        st->error_max = static_cast<float>(err.second);
+    }
+    compute_pressure_accelerations(n, rho0, tp.pressure_accelerations.data(),
+                                   tp.pressures.data(), tp.densities.data(),
+                                   tp.fluid_volumes.data(), tp.nb, s.volumes.data(), tp.snb);
+    V2<T> const* a = tp.pressure_accelerations.data();
+#pragma omp parallel for schedule(static, 512)
+    for (size_t i = 0; i < n; ++i) {
+        if (pseudo) {
+            V2<T> const v = vel[i];
+            pos[i] = pos[i] + (dt * v) + a[i];
+            if (is_safe_divide(v, dt)) { vel[i] = v + (a[i] / dt); }
+        } else {
+            // dynamics.cpp:53-62 integrate_velocities_and_positions_in_place
+            vel[i] += dt * a[i];
+            pos[i] += dt * vel[i];
+        }
+    }
+}
+
 // dfsph_p.cpp:265-374 (active #else branch :325-373)
 template <class T>
 void dfsph_p_sub_step(T /*t*/, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
@@ -1215,6 +1429,10 @@ void sub_step(Scene<T>& sc, int method, T t, T dt) {
     case IISPH: iisph_sub_step(t, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st); break;
     case PCISPH: pcisph_sub_step(t, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st); break;
     case DFSPH_P: dfsph_p_sub_step(t, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st); break;
+    case IISPH_AP: iisph_ap_sub_step(false, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st); break;
+    case IISPH_PSEUDO_AP:
+        iisph_ap_sub_step(true, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st);
+        break;
     default: break;
     }
     sc.last = st;
@@ -1547,7 +1765,9 @@ void orc_scene_solid_volumes(void* p, double* out) {
 
 // Copy a named per-particle field (original particle order) into out (double).
 // Names: pos, vel (2 comps), density, density_star, pressure, aii, dii (2),
-// sum_dij_pj (2), alpha, kappa_density, kappa_divergence, pressure_force (2).
+// sum_dij_pj (2), alpha, kappa_density, kappa_divergence, pressure_force (2),
+// pressure_acceleration (2; IISPH_AP accelerations, IISPH_PSEUDO_AP
+// displacements).
 extern "C++" template <class T>
 int get_field(Scene<T>& sc, char const* name, double* out) {
     size_t const n = sc.pos.size();
@@ -1577,6 +1797,7 @@ int get_field(Scene<T>& sc, char const* name, double* out) {
     if (s == "kappa_density") { return sc1(sc.tmp.density_kappas); }
     if (s == "kappa_divergence") { return sc1(sc.tmp.divergence_kappas); }
     if (s == "pressure_force") { return v2(sc.tmp.pressure_forces); }
+    if (s == "pressure_acceleration") { return v2(sc.tmp.pressure_accelerations); }
     return -1;
 }
 int orc_scene_get_field(void* p, char const* name, double* out) {

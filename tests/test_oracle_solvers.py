@@ -1,8 +1,9 @@
 """Pins the oracle's solver arithmetic (no reference test covers it):
 
-* an independent FP64 all-pairs restatement of one IISPH, DFSPH_P and PCISPH
-  sub-step (iisph.cpp:23-151, dfsph_p.cpp:61-374, pcisph.cpp:36-164) against
-  the oracle's FP64 build, to 1e-9 relative;
+* an independent FP64 all-pairs restatement of one IISPH, IISPH_AP,
+  IISPH_PSEUDO_AP, DFSPH_P and PCISPH sub-step (iisph.cpp:23-151,
+  iisph_ap.cpp:44-138, iisph_pseudo_ap.cpp:63-155, dfsph_p.cpp:61-374,
+  pcisph.cpp:36-164) against the oracle's FP64 build, to 1e-9 relative;
 * invariants of the float oracle: a uniform lattice has uniform interior
   density, fluid-fluid pressure forces conserve momentum, v = 0 gives zero
   divergence, p = 0 gives sum_j d_ij p_j = 0.
@@ -138,6 +139,57 @@ class Scene64:
         return {"pos": x, "vel": v, "density": rho, "pressure": p, "aii": aii,
                 "density_star": rstar, "dii": dii, "iterations": it + 1}
 
+    def accel(self, p, rho, gw, gws, mf):
+        """pressure.cpp:83-178 pressure accelerations (fluid + boundary)."""
+        rho0 = self.c["target_density"]
+        por = p / rho ** 2
+        a = -mf * np.einsum("ij,ijk->ik", por[:, None] + por[None, :], gw)
+        return a - np.einsum("j,ijk->ik", rho0 * self.vs, gws) * por[:, None]
+
+    # iisph_ap.cpp:44-138 (pseudo False) and iisph_pseudo_ap.cpp:63-155 (True)
+    def iisph_ap(self, dt, pseudo):
+        c = self.c
+        rho0, m, g = c["target_density"], c["mass_per_particle"], c["gravity"]
+        V = m / rho0
+        mf = rho0 * V
+        _, r, nf, _, rs, ns = self.lists(self.x)
+        d, r, ds, rs, gw, gws = self.geometry(self.x, nf, ns)
+        rho = self.density(r, rs, nf, ns, mf, V)
+        v = self.v.copy()
+        v[:, 1] += dt * (-m * g) / (rho0 * V)
+        S = mf * np.sum(gw, 1) + np.einsum("j,ijk->ik", rho0 * self.vs, gws)
+        SS = mf * np.sum(np.sum(gw * gw, -1), 1)
+        inner = np.sum(S * S, 1) + mf * SS
+        diag = (-inner if pseudo else -dt * dt * inner) / rho ** 2
+        rstar = rho + dt * self.divergence(v, gw, gws, mf)
+        p = np.zeros(len(v))
+        thr_a, thr_m = c["iisph_error_average_threshold"], c["iisph_error_max_threshold"]
+        om = c["iisph_omega"]
+        it = 0
+        for it in range(c["iisph_max_iterations"]):
+            a = self.accel(p, rho, gw, gws, mf)
+            da = np.where(nf[..., None], a[:, None, :] - a[None, :, :], 0.0)
+            api = mf * np.einsum("ijk,ijk->i", da, gw) + \
+                np.einsum("j,ik,ijk->i", rho0 * self.vs, a, gws)
+            if not pseudo:
+                api = api * dt * dt
+            src = rho0 - rstar
+            err = np.maximum(0.0, (api - src) / rho0)
+            avg = float(np.sum(np.floor(err * 1000))) / (1000 * len(p))
+            mx = float(err.max())
+            p = np.maximum(0.0, p + om * (src - api) / diag)
+            if it > 1 and avg <= thr_a and mx <= thr_m:
+                break
+        a = self.accel(p, rho, gw, gws, mf)
+        if pseudo:
+            x = self.x + dt * v + a
+            v = v + a / dt
+        else:
+            v = v + dt * a
+            x = self.x + dt * v
+        return {"pos": x, "vel": v, "density": rho, "pressure": p, "aii": diag,
+                "density_star": rstar, "pressure_acceleration": a, "iterations": it + 1}
+
     # dfsph_p.cpp:265-374
     def dfsph(self, dt):
         c = self.c
@@ -235,12 +287,14 @@ def _scene(seed=4):
     return x, v, xs, cfg
 
 
-@pytest.mark.parametrize("method", ["iisph", "dfsph_p", "pcisph"])
+@pytest.mark.parametrize("method", ["iisph", "iisph_ap", "iisph_pseudo_ap", "dfsph_p", "pcisph"])
 def test_fp64_all_pairs_restatement(method):
     x, v, xs, cfg = _scene()
     ref = Scene64(x, v, xs, cfg)
     dt = 2e-4
-    exp = {"iisph": ref.iisph, "dfsph_p": ref.dfsph, "pcisph": ref.pcisph}[method](dt)
+    exp = {"iisph": ref.iisph, "dfsph_p": ref.dfsph, "pcisph": ref.pcisph,
+           "iisph_ap": lambda t: ref.iisph_ap(t, False),
+           "iisph_pseudo_ap": lambda t: ref.iisph_ap(t, True)}[method](dt)
     o = oracle.Scene(cfg, xs, x, v, fp64=True)
     o.frame_init(method)
     st = o.sub_step(method, 0.0, dt)
@@ -293,3 +347,23 @@ def test_zero_velocity_zero_divergence_and_zero_pressure_terms():
     o2.frame_init("iisph")
     o2.sub_step("iisph", 0.0, 2e-4)
     assert np.all(o2.field("sum_dij_pj") == 0.0)  # first iteration runs with p = 0
+
+
+def test_pseudo_ap_is_ap_with_unit_time_step():
+    # With pseudo pressures p' = dt^2 p, IISPH_PSEUDO_AP's iteration and
+    # integration are IISPH_AP's (the diagonal and A p each lose a dt^2), so in
+    # FP64 the two agree to rounding.
+    x, v, xs, cfg = _scene(5)
+    dt = 2e-4
+    out = {}
+    for m in ("iisph_ap", "iisph_pseudo_ap"):
+        o = oracle.Scene(cfg, xs, x, v, fp64=True)
+        o.frame_init(m)
+        st = o.sub_step(m, 0.0, dt)
+        out[m] = (st.iterations, o.field("pos"), o.field("vel"), o.field("pressure"))
+    (ia, xa, va, pa), (ip, xp, vp, pp) = out["iisph_ap"], out["iisph_pseudo_ap"]
+    assert ia == ip
+    assert np.abs(xa - xp).max() <= 1e-12 * np.abs(xa).max()
+    assert np.abs(va - vp).max() <= 1e-9 * np.abs(va).max()
+    assert np.abs(pp - dt * dt * pa).max() <= 1e-9 * np.abs(pp).max()
+    assert np.abs(pa).max() > 0
