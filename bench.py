@@ -12,7 +12,8 @@ events on the context's stream) and the CPU baseline (the oracle, a C++
 restatement of the reference's CPU path, timed on this host's cores on a
 bounded sample of the same workload).
 
-  python bench.py [--steps K] [--warmup W] [--method iisph|dfsph_p|pcisph]
+  python bench.py [--steps K] [--warmup W]
+                  [--method iisph|iisph_ap|iisph_pseudo_ap|dfsph_p|pcisph]
 """
 
 from __future__ import annotations
@@ -53,6 +54,15 @@ ALGO_BYTES = {
                 "dfsph_div": 36,      # divergence iteration 68 - apply 32
                 "dfsph_dens": 40,     # density iteration 72 - apply 32
                 "finalize": 24},      # x 24
+    # IISPH_AP / IISPH_PSEUDO_AP (the same sweeps, by the same conventions):
+    # build rho 12 + (v += g) 16; diagonal sweep (v, rho) 12 + (a_ii, rho*) 8
+    # + (p' of the first iteration) 4; per further iteration accelerations
+    # (p, rho) 8 + a 8 and the update (a, a_ii, rho*, p) 20 + p' 4; final
+    # (p, rho) 8 + a 8 + (x, v) 16 + (x, v) 16. Sum 100 + 40k.
+    "iisph_ap": {**NB, "build": 28, "ap_diag": 24, "ap_accel": 16, "ap_iterate": 24,
+                 "ap_final": 48},
+    "iisph_pseudo_ap": {**NB, "build": 28, "ap_diag": 24, "ap_accel": 16, "ap_iterate": 24,
+                        "ap_final": 48},
     "pcisph": {**NB,
                "build": 0,            # index lists only (not counted)
                "pci_init": 28,        # v_ext 16 + init 12
@@ -66,6 +76,8 @@ def step_bytes(method: str, k: float, k_div: float) -> float:
     """Algorithmic bytes per particle-step at the measured mean iteration counts."""
     if method == "iisph":
         return 160 + 68 * k
+    if method in ("iisph_ap", "iisph_pseudo_ap"):
+        return 100 + 40 * k
     if method == "dfsph_p":
         return 172 + 68 * k_div + 72 * k
     return 116 + 108 * k
@@ -220,7 +232,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--method", default="iisph", choices=["iisph", "dfsph_p", "pcisph"])
+    ap.add_argument("--method", default="iisph",
+                    choices=["iisph", "iisph_ap", "iisph_pseudo_ap", "dfsph_p", "pcisph"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batched", action="store_true")
@@ -232,9 +245,8 @@ def main() -> int:
     ap.add_argument("--served-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are "
-                                   "not part of it (see DESIGN.md for how the path would "
-                                   "shard)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU only; --gpus above 1 is "
+                                   "not supported", "n_gpus": args.gpus}))
         return 2
 
     from emerald_amd import lib, scenes

@@ -30,7 +30,7 @@ METHODS = {"pcisph": PCISPH, "iisph": IISPH, "iisph_ap": IISPH_AP,
 FIELDS = {
     "density": (0, 1), "density_star": (1, 1), "pressure": (2, 1), "aii": (3, 1),
     "dii": (4, 2), "sum_dij_pj": (5, 2), "alpha": (6, 1), "kappa_density": (7, 1),
-    "kappa_divergence": (8, 1), "pressure_force": (9, 2),
+    "kappa_divergence": (8, 1), "pressure_force": (9, 2), "pressure_acceleration": (10, 2),
 }
 
 

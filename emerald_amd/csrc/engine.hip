@@ -300,6 +300,7 @@ struct Engine {
     SpatialIndex fidx;
     DBuf<V2> spos, svel, dii, vext, xstar, vstar0, vstar1, force;
     DBuf<V2> pr0, pr1;  // IISPH {p, rho} ping-pong
+    DBuf<V2> pacc;      // IISPH_AP / IISPH_PSEUDO_AP pressure accelerations
     DBuf<SdDii> sd;     // IISPH {sum_j d_ij p_j, d_ii}
     DBuf<float> rho, rho_star, aii, p0, p1, alpha, kvs, kds;
     float* cur_p = nullptr;
@@ -495,6 +496,7 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     b.p_in = p_in;
     b.p_out = p_out;
     b.sd = g->sd.p;
+    b.pacc = g->pacc.p;
     b.alpha = g->alpha.p;
     b.pos_orig = g->pos.p;
     b.vel_orig = g->vel.p;
@@ -579,6 +581,7 @@ hipError_t alloc_fluid(Engine* g, uint32_t cap) {
         (e = g->rho_star.alloc(cap)) || (e = g->aii.alloc(cap)) || (e = g->p0.alloc(cap)) ||
         (e = g->p1.alloc(cap)) || (e = g->alpha.alloc(cap)) || (e = g->kvs.alloc(cap)) ||
         (e = g->kds.alloc(cap)) || (e = g->pr0.alloc(cap)) || (e = g->pr1.alloc(cap)) ||
+        (e = g->pacc.alloc(cap)) ||
         (e = g->sd.alloc(cap)) || (e = g->fcount.alloc(g->stride)) ||
         (e = g->scount.alloc(g->stride)) || (e = g->fl_idx.alloc(L)) ||
         (e = g->sl_idx.alloc(L)) || (e = g->fl_gw.alloc(L)) || (e = g->sl_gw.alloc(L)) ||
@@ -1074,6 +1077,48 @@ emsph_status iisph_sub_step(Engine* g) {
     return EMSPH_OK;
 }
 
+// iisph_ap.cpp:44-138 (pseudo = false) and iisph_pseudo_ap.cpp:63-155
+// (pseudo = true) for every scene of the batch: the same loop shape as IISPH
+// (exit tests from the third iteration, iisph thresholds), pressures updated in
+// place, the first iteration folded into the diagonal sweep.
+emsph_status iisph_ap_sub_step(Engine* g, bool pseudo) {
+    begin_attempt(g);
+    int const method = pseudo ? EMSPH_IISPH_PSEUDO_AP : EMSPH_IISPH_AP;
+    Batch b = fluid_batch(g, g->p0.p, g->p0.p);
+    HIP_TRY(segment(g, method, SEG_PREFIX, [&]() -> hipError_t {
+        HIP_PASS(enqueue_prepare(g, BUILD_AP, false));
+        return timed(g, "ap_diag", [&] { launch_ap_diag0(b, pseudo ? 1 : 0, g->stream); });
+    }));
+    finish_prepare(g);
+    int max_it = 0;
+    for (auto const& h : g->sh) {
+        max_it = std::max(max_it, std::min(h.cfg.params.iisph_max_pressure_iterations,
+                                           MAX_RED_SLOTS));
+    }
+    bool short_sort = false;
+    int const decide_from = first_decision(g, 0, 2);
+    HIP_TRY(run_loop(g, 0, max_it, 2, [&](int it) -> hipError_t {
+        if (it > 0) {
+            HIP_PASS(timed(g, "ap_accel", [&] { launch_ap_accel(b, g->stream); }));
+            HIP_PASS(timed(g, "ap_iterate", [&] { launch_ap_iterate(b, pseudo ? 1 : 0, g->stream); }));
+        }
+        if (it < decide_from) { return hipSuccess; }
+        return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); });
+    }, &g->last_iterations, &short_sort));
+    if (short_sort) { return EMSPH_OK; }
+    if (!g->keys_checked) {
+        HIP_TRY(read_keystate(g));
+        if (g->sort_short) { return EMSPH_OK; }
+    }
+    HIP_TRY(segment(g, method, SEG_SUFFIX, [&]() -> hipError_t {
+        HIP_PASS(timed(g, "ap_final", [&] { launch_ap_final(b, pseudo ? 1 : 0, g->stream); }));
+        launch_scene_totals(b, 1, g->stream);
+        return hipGetLastError();
+    }));
+    g->cur_p = g->p0.p;
+    return EMSPH_OK;
+}
+
 // dfsph_p.cpp:265-374 (with the correction loops of :61-212).
 emsph_status dfsph_sub_step(Engine* g) {
     begin_attempt(g);
@@ -1178,7 +1223,8 @@ emsph_status pcisph_sub_step(Engine* g) {
 }
 
 emsph_status check_method(int method) {
-    if (method == EMSPH_IISPH || method == EMSPH_PCISPH || method == EMSPH_DFSPH_P) {
+    if (method == EMSPH_IISPH || method == EMSPH_PCISPH || method == EMSPH_DFSPH_P ||
+        method == EMSPH_IISPH_AP || method == EMSPH_IISPH_PSEUDO_AP) {
         return EMSPH_OK;
     }
     return fail(EMSPH_ERR_UNKNOWN_METHOD,
@@ -1199,6 +1245,8 @@ emsph_status engine_sub_step(Engine* g, int method) {
         }
         switch (method) {
         case EMSPH_IISPH: ST_TRY(iisph_sub_step(g)); break;
+        case EMSPH_IISPH_AP: ST_TRY(iisph_ap_sub_step(g, false)); break;
+        case EMSPH_IISPH_PSEUDO_AP: ST_TRY(iisph_ap_sub_step(g, true)); break;
         case EMSPH_DFSPH_P: ST_TRY(dfsph_sub_step(g)); break;
         default: ST_TRY(pcisph_sub_step(g)); break;
         }
@@ -1262,6 +1310,7 @@ emsph_status engine_get_field(Engine* g, uint32_t s, int field, float* out) {
     case EMSPH_FIELD_KAPPA_DENSITY: f1 = g->kds.p; break;
     case EMSPH_FIELD_KAPPA_DIVERGENCE: f1 = g->kvs.p; break;
     case EMSPH_FIELD_PRESSURE_FORCE: f2 = g->force.p; break;
+    case EMSPH_FIELD_PRESSURE_ACCELERATION: f2 = g->pacc.p; break;
     default: return fail(EMSPH_ERR_INVALID_ARGUMENT, "unknown field");
     }
     SceneHost const& h = g->sh[s];

@@ -203,6 +203,8 @@ struct Batch {
     V2* pr_out;
     V2 const* pr_alt;
     SdDii* sd;
+    // IISPH_AP / IISPH_PSEUDO_AP pressure accelerations (displacements)
+    V2* pacc;
     float* alpha;
     float* kappa;
     // original-order state (gathered in by the cells stage, scattered back at the end)
@@ -226,7 +228,9 @@ struct Batch {
     int xcd_remap;
 };
 
-enum BuildMode : int { BUILD_LISTS = 0, BUILD_IISPH = 1, BUILD_DFSPH = 2 };
+// BUILD_AP: density and the external-force velocity update (IISPH_AP and
+// IISPH_PSEUDO_AP), without IISPH's d_ii.
+enum BuildMode : int { BUILD_LISTS = 0, BUILD_IISPH = 1, BUILD_DFSPH = 2, BUILD_AP = 3 };
 enum LoopKind : int { LOOP_IISPH = 0, LOOP_DIVERGENCE = 1, LOOP_DENSITY = 2, LOOP_PCISPH = 3 };
 
 //------------------------------------------------------------------------------
@@ -328,6 +332,15 @@ void launch_build(int mode, int cap, Batch const& b, hipStream_t s);
 void launch_iisph_aii_beta0(Batch const& b, hipStream_t s);
 void launch_iisph_sdp(Batch const& b, hipStream_t s);
 void launch_iisph_beta_p(Batch const& b, hipStream_t s);
+// IISPH_AP (pseudo = 0) and IISPH_PSEUDO_AP (pseudo = 1): diagonals and
+// density stars with the loop start and the first iteration (pressures +0);
+// pressure accelerations of active scenes; one in-place Jacobi iteration of
+// active scenes; and the final accelerations with the integration (every
+// scene), writing the state back in original order.
+void launch_ap_diag0(Batch const& b, int pseudo, hipStream_t s);
+void launch_ap_accel(Batch const& b, hipStream_t s);
+void launch_ap_iterate(Batch const& b, int pseudo, hipStream_t s);
+void launch_ap_final(Batch const& b, int pseudo, hipStream_t s);
 // Apply pressure accelerations (pressure.cpp:9-80) with p_in as the pressure,
 // for active scenes (or all scenes when all_scenes != 0).
 // with_gravity: first integrate the external gravity force (dynamics.cpp:24-37).

@@ -910,6 +910,8 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
             dii = add(dii, ds);
         }
         a.dii[t] = dii;
+    }
+    if (MODE == BUILD_IISPH || MODE == BUILD_AP) {
         // forces.cpp:11-83 (zeroed force, y -= m g) and dynamics.cpp:24-37.
         V2 const f{0.0f, 0.0f - c.mass * c.gravity};
         float const vden = c.rho0 * c.fluid_volume;
@@ -1025,6 +1027,7 @@ void launch_build(int mode, int cap, Batch const& a, hipStream_t s) {
     switch (mode) {
     case BUILD_IISPH: launch_build_mode<BUILD_IISPH>(cap, a, s); break;
     case BUILD_DFSPH: launch_build_mode<BUILD_DFSPH>(cap, a, s); break;
+    case BUILD_AP: launch_build_mode<BUILD_AP>(cap, a, s); break;
     default: launch_build_mode<BUILD_LISTS>(cap, a, s); break;
     }
 }
@@ -1329,6 +1332,247 @@ __global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a) {
     Tile const tl = a.tiles[tile];
     if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
     iisph_beta_p_tile<NB>(a, tl, tile);
+}
+
+//------------------------------------------------------------------------------
+// IISPH_AP (iisph_ap.cpp, iisph_ap_ops.cpp) and IISPH_PSEUDO_AP
+// (iisph_pseudo_ap.cpp, iisph_pseudo_ap_ops.cpp; PSEUDO: a fictitious time
+// step of 1 in the solve, so no dt^2 in the diagonal or A p). The pressures
+// are one buffer updated in place (each particle reads only its own).
+
+// p / rho^2, or 0 when the divide is unsafe (pressure.cpp:104-107, :134-137).
+__device__ __forceinline__ float p_over_rho2(float p, float rho) {
+    float const d = rho * rho;
+    return is_safe_div(p, d) ? p / d : 0.0f;
+}
+
+// One iteration's pressure update of a particle with at least one neighbour
+// (iisph_ap_ops.cpp:170-190, iisph_pseudo_ap_ops.cpp:281-301): A p in api
+// (before the dt^2 factor of IISPH_AP), the old pressure p; returns the new
+// one and the residual contribution.
+template <int PSEUDO>
+__device__ __forceinline__ float ap_update(StepConsts const& c, float api, float p,
+                                           float rho_star, float diag,
+                                           unsigned long long& qsum, unsigned int& emax) {
+    if (!PSEUDO) { api *= c.dt * c.dt; }
+    float const source = c.rho0 - rho_star;
+    float const q = api - source;
+    float err = is_safe_div(q, c.rho0) ? q / c.rho0 : 0.0f;
+    err = (0.0f < err) ? err : 0.0f;  // std::max(0.0f, ...)
+    emax = __float_as_uint(err);
+    qsum = static_cast<unsigned long long>(err * 1000.0f);
+    float const numer = c.omega * (source - api);
+    float const step = is_safe_div(numer, diag) ? numer / diag : 0.0f;
+    float const pn = p + step;
+    return (0.0f < pn) ? pn : 0.0f;
+}
+
+// iisph_ap_ops.cpp:19-106 / iisph_pseudo_ap_ops.cpp:19-110 diagonals and
+// density stars, fused with the loop's first iteration: the pressures start
+// at +0 (iisph_ap.cpp:95), so every pressure acceleration is exactly (+0, +0)
+// ((-m (+0 + +0)) grad W = +-0 terms added to +0) and so is A p; the first
+// update needs only the particle's own diagonal and density star. Workgroup 0
+// also starts the loop.
+template <int NB, int PSEUDO>
+__global__ __launch_bounds__(TILE) void k_ap_diag0(Batch a) {
+    if (blockIdx.x == 0) { loop_begin_block<TILE>(a); }
+    uint32_t const tile = tile_index(a);
+    Tile const tl = a.tiles[tile];
+    unsigned long long qsum = 0;
+    unsigned int emax = 0;
+    if (threadIdx.x < tl.count) {
+        uint32_t const t = tl.start + threadIdx.x;
+        StepConsts const c = a.scenes[tl.scene].c;
+        int const fc = a.fl.count[t];
+        int const sc = a.sl.count[t];
+        float const rho = a.rho[t];
+        float diag = 0.0f;
+        float rs = rho;
+        float pn = 0.0f;
+        if (fc + sc) {
+            float const mf = c.fluid_volume * c.rho0;  // fluid mass (uniform volume)
+            V2 const vi = a.vel[t];
+            V2 sm{0.0f, 0.0f};
+            float smgg = 0.0f;
+            float div = 0.0f;
+            for (int k0 = 0; k0 < fc; k0 += NB) {
+                NbrBatch<NB> nb;
+                load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+                V2 vj[NB];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    vj[u] = V2{0.0f, 0.0f};
+                    if (k0 + u < fc) { vj[u] = a.vel[nb.j[u]]; }
+                }
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    if (k0 + u < fc) {
+                        V2 const gw = nb.g[u];
+                        sm = add(sm, scale(mf, gw));
+                        smgg += mf * dot(gw, gw);
+                        div += mf * dot(sub(vi, vj[u]), gw);
+                    }
+                }
+            }
+            for (int k = 0; k < sc; ++k) {
+                size_t const o = static_cast<size_t>(k) * a.stride + t;
+                uint32_t const b = a.sl.idx[o];
+                V2 const gw = a.sl.gw[o];
+                float const ms = a.solids.vol[b] * c.rho0;
+                sm = add(sm, scale(ms, gw));
+                div += ms * dot(sub(vi, a.solids.vel[b]), gw);
+            }
+            float const inner = dot(sm, sm) + mf * smgg;
+            float const numer = PSEUDO ? -inner : -(c.dt * c.dt) * inner;
+            float const denom = rho * rho;
+            diag = is_safe_div(numer, denom) ? numer / denom : 0.0f;
+            rs = rho + c.dt * div;
+            if (!(PSEUDO && diag == 0.0f)) {
+                pn = ap_update<PSEUDO>(c, 0.0f, 0.0f, rs, diag, qsum, emax);
+            }
+        }
+        a.aii[t] = diag;
+        a.rho_star[t] = rs;
+        a.p_out[t] = pn;
+    }
+    block_residual<TILE>(qsum, emax, &a.tile_red[tile]);
+}
+
+// Pressure acceleration of slot t (pressure.cpp:83-178; also the pseudo
+// displacements, iisph_pseudo_ap_ops.cpp:113-225): the fluid pass sets it, the
+// boundary pass adds its own sum.
+template <int NB>
+__device__ __forceinline__ V2 ap_acceleration(Batch const& a, StepConsts const& c, uint32_t t) {
+    int const fc = a.fl.count[t];
+    int const sc = a.sl.count[t];
+    float const* P = a.p_in;
+    float const pi = p_over_rho2(P[t], a.rho[t]);
+    float const mf = c.rho0 * c.fluid_volume;
+    V2 acc{0.0f, 0.0f};
+    for (int k0 = 0; k0 < fc; k0 += NB) {
+        NbrBatch<NB> nb;
+        load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+        float pj[NB], rj[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            pj[u] = 0.0f;
+            rj[u] = 1.0f;
+            if (k0 + u < fc) {
+                pj[u] = P[nb.j[u]];
+                rj[u] = a.rho[nb.j[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            if (k0 + u < fc) {
+                float const coeff = pi + p_over_rho2(pj[u], rj[u]);
+                acc = add(acc, scale(-mf * coeff, nb.g[u]));
+            }
+        }
+    }
+    if (sc) {
+        V2 ab{0.0f, 0.0f};
+        for (int k = 0; k < sc; ++k) {
+            size_t const o = static_cast<size_t>(k) * a.stride + t;
+            float const ms = c.rho0 * a.solids.vol[a.sl.idx[o]];
+            ab = add(ab, scale(-ms * pi, a.sl.gw[o]));
+        }
+        acc = add(acc, ab);
+    }
+    return acc;
+}
+
+// Accelerations from the current pressures, for the scenes still iterating.
+template <int NB>
+__global__ __launch_bounds__(TILE) void k_ap_accel(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!a.stats[r.scene].active) { return; }  // the whole workgroup
+    if (!r.valid) { return; }
+    StepConsts const c = a.scenes[r.scene].c;
+    a.pacc[r.t] = ap_acceleration<NB>(a, c, r.t);
+}
+
+// iisph_ap_ops.cpp:108-194 / iisph_pseudo_ap_ops.cpp:227-305: A p from the
+// accelerations, the in-place pressure update and its residual.
+template <int NB, int PSEUDO>
+__global__ __launch_bounds__(TILE) void k_ap_iterate(Batch a) {
+    uint32_t const tile = tile_index(a);
+    Tile const tl = a.tiles[tile];
+    if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
+    unsigned long long qsum = 0;
+    unsigned int emax = 0;
+    if (threadIdx.x < tl.count) {
+        uint32_t const t = tl.start + threadIdx.x;
+        StepConsts const c = a.scenes[tl.scene].c;
+        int const fc = a.fl.count[t];
+        int const sc = a.sl.count[t];
+        float* const P = a.p_out;
+        float const diag = a.aii[t];
+        if (!fc && !sc) {
+            P[t] = 0.0f;
+        } else if (PSEUDO && diag == 0.0f) {
+            P[t] = 0.0f;
+        } else {
+            float const mf = c.rho0 * c.fluid_volume;
+            V2 const ai = a.pacc[t];
+            float api = 0.0f;
+            for (int k0 = 0; k0 < fc; k0 += NB) {
+                NbrBatch<NB> nb;
+                load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
+                V2 aj[NB];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    aj[u] = V2{0.0f, 0.0f};
+                    if (k0 + u < fc) { aj[u] = a.pacc[nb.j[u]]; }
+                }
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    if (k0 + u < fc) { api += mf * dot(sub(ai, aj[u]), nb.g[u]); }
+                }
+            }
+            for (int k = 0; k < sc; ++k) {
+                size_t const o = static_cast<size_t>(k) * a.stride + t;
+                float const ms = c.rho0 * a.solids.vol[a.sl.idx[o]];
+                api += ms * dot(ai, a.sl.gw[o]);
+            }
+            P[t] = ap_update<PSEUDO>(c, api, P[t], a.rho_star[t], diag, qsum, emax);
+        }
+    }
+    block_residual<TILE>(qsum, emax, &a.tile_red[tile]);
+}
+
+// The final accelerations (every scene) and the integration: IISPH_AP
+// dynamics.cpp:53-62 (v += dt a, x += dt v); IISPH_PSEUDO_AP
+// iisph_pseudo_ap_ops.cpp:307-341 (x += dt v + d, v += d / dt when safe);
+// state written back in original order, max |v|^2 and energies per tile.
+template <int NB, int PSEUDO>
+__global__ __launch_bounds__(TILE) void k_ap_final(Batch a) {
+    TileRef const r = tile_ref(a, TILE);
+    unsigned int vm = 0;
+    double ke = 0.0, pe = 0.0;
+    if (r.valid) {
+        uint32_t const t = r.t;
+        StepConsts const c = a.scenes[r.scene].c;
+        V2 const acc = ap_acceleration<NB>(a, c, t);
+        a.pacc[t] = acc;
+        V2 v = a.vel[t];
+        V2 x = a.pos[t];
+        if (PSEUDO) {
+            V2 const vs = v;
+            x = add(add(x, scale(c.dt, vs)), acc);
+            if (is_safe_div(vs, c.dt)) { v = add(vs, divs(acc, c.dt)); }
+        } else {
+            v = add(v, scale(c.dt, acc));
+            x = add(x, scale(c.dt, v));
+        }
+        uint32_t const o = a.orig[t];
+        a.pos_orig[o] = x;
+        a.vel_orig[o] = v;
+        vm = __float_as_uint(dot(v, v));
+        particle_energy(c, x, v, ke, pe);
+    }
+    block_max<TILE>(vm, a.tile_vmax + r.tile);
+    block_energy<TILE>(ke, pe, a.tile_energy + r.tile);
 }
 
 //------------------------------------------------------------------------------
@@ -1860,6 +2104,42 @@ void launch_iisph_sdp(Batch const& a, hipStream_t s) {
 }
 void launch_iisph_beta_p(Batch const& a, hipStream_t s) {
     NB_DISPATCH(a.nb, k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a);
+}
+
+#define AP_DISPATCH(KERNEL, PSEUDO)                                                    \
+    do {                                                                              \
+        if (a.nb <= 2) {                                                              \
+            hipLaunchKernelGGL((KERNEL<2, PSEUDO>), tile_grid(a), dim3(TILE), 0, s, a);   \
+        } else if (a.nb <= 4) {                                                       \
+            hipLaunchKernelGGL((KERNEL<4, PSEUDO>), tile_grid(a), dim3(TILE), 0, s, a);   \
+        } else {                                                                      \
+            hipLaunchKernelGGL((KERNEL<8, PSEUDO>), tile_grid(a), dim3(TILE), 0, s, a);   \
+        }                                                                             \
+    } while (0)
+
+void launch_ap_diag0(Batch const& a, int pseudo, hipStream_t s) {
+    if (pseudo) {
+        AP_DISPATCH(k_ap_diag0, 1);
+    } else {
+        AP_DISPATCH(k_ap_diag0, 0);
+    }
+}
+void launch_ap_accel(Batch const& a, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_ap_accel, tile_grid(a), dim3(TILE), 0, s, a);
+}
+void launch_ap_iterate(Batch const& a, int pseudo, hipStream_t s) {
+    if (pseudo) {
+        AP_DISPATCH(k_ap_iterate, 1);
+    } else {
+        AP_DISPATCH(k_ap_iterate, 0);
+    }
+}
+void launch_ap_final(Batch const& a, int pseudo, hipStream_t s) {
+    if (pseudo) {
+        AP_DISPATCH(k_ap_final, 1);
+    } else {
+        AP_DISPATCH(k_ap_final, 0);
+    }
 }
 
 template <int G, int F>

@@ -45,8 +45,9 @@ typedef enum emsph_status {
     EMSPH_ERR_NO_DEVICE = 6
 } emsph_status;
 
-/* EZ_EXAMPLE_SIM::Method (simulation.h:26-32). IISPH_AP and IISPH_PSEUDO_AP
- * are not implemented and return EMSPH_ERR_UNKNOWN_METHOD. */
+/* EZ_EXAMPLE_SIM::Method (simulation.h:26-32). IISPH_AP (iisph_ap.cpp) and
+ * IISPH_PSEUDO_AP (iisph_pseudo_ap.cpp, the reference's default method) use the
+ * iisph_* parameters; their frame step is (60, 4, CFL 0.4) like DFSPH_P. */
 typedef enum emsph_method {
     EMSPH_PCISPH = 0,
     EMSPH_IISPH = 1,
@@ -124,7 +125,11 @@ typedef enum emsph_field {
     EMSPH_FIELD_ALPHA = 6,
     EMSPH_FIELD_KAPPA_DENSITY = 7,
     EMSPH_FIELD_KAPPA_DIVERGENCE = 8,
-    EMSPH_FIELD_PRESSURE_FORCE = 9 /* 2 floats per particle */
+    EMSPH_FIELD_PRESSURE_FORCE = 9, /* 2 floats per particle (PCISPH) */
+    /* 2 floats per particle: IISPH_AP pressure accelerations (pressure.cpp:
+     * 83-178), IISPH_PSEUDO_AP pseudo-pressure displacements, of the final
+     * pressures */
+    EMSPH_FIELD_PRESSURE_ACCELERATION = 10
 } emsph_field;
 
 typedef struct emsph_context emsph_context;

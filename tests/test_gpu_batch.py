@@ -21,7 +21,7 @@ def _mixed_scenes():
     return out
 
 
-@pytest.mark.parametrize("method", ["iisph", "dfsph_p", "pcisph"])
+@pytest.mark.parametrize("method", ["iisph", "iisph_ap", "iisph_pseudo_ap", "dfsph_p", "pcisph"])
 def test_batch_equals_alone(gpu, method):
     scs = _mixed_scenes()
     cfgs = [scenes.config_for(sc, method) for sc in scs]

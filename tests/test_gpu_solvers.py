@@ -1,6 +1,8 @@
-"""GPU sub-steps of IISPH, PCISPH and DFSPH_P vs the float oracle, bit for bit.
+"""GPU sub-steps of IISPH, IISPH_AP, IISPH_PSEUDO_AP, PCISPH and DFSPH_P vs the
+float oracle, bit for bit.
 
-The oracle restates iisph.cpp:23-151, pcisph.cpp:36-164 and dfsph_p.cpp:265-374;
+The oracle restates iisph.cpp:23-151, iisph_ap.cpp:44-138,
+iisph_pseudo_ap.cpp:63-155, pcisph.cpp:36-164 and dfsph_p.cpp:265-374;
 every per-particle field, the iteration counts and the quantised residuals are
 compared exactly (the bar for this path: same bits as the reference CPU path).
 """
@@ -18,7 +20,10 @@ FIELDS = {
     "iisph": ["density", "density_star", "aii", "dii", "sum_dij_pj", "pressure"],
     "pcisph": ["density", "pressure", "pressure_force"],
     "dfsph_p": ["density", "alpha", "kappa_density", "kappa_divergence", "density_star"],
+    "iisph_ap": ["density", "density_star", "aii", "pressure", "pressure_acceleration"],
+    "iisph_pseudo_ap": ["density", "density_star", "aii", "pressure", "pressure_acceleration"],
 }
+ALL = ["iisph", "iisph_ap", "iisph_pseudo_ap", "pcisph", "dfsph_p"]
 ORACLE_NAME = {"kappa_density": "kappa_density", "kappa_divergence": "kappa_divergence"}
 
 
@@ -48,7 +53,7 @@ def test_solid_volumes(gpu, method):
     assert_bits_equal(ctx.solid_volumes(), orc.solid_volumes().astype(np.float32), "solid volumes")
 
 
-@pytest.mark.parametrize("method", ["iisph", "pcisph", "dfsph_p"])
+@pytest.mark.parametrize("method", ALL)
 def test_single_sub_step_fields(gpu, method):
     scene, cfg, ctx, orc = _make(method)
     ctx.set_store_geometry(True)
@@ -69,7 +74,7 @@ def test_single_sub_step_fields(gpu, method):
         assert np.array_equal(np.where(mask, gi, 0), np.where(mask, oi, 0))
 
 
-@pytest.mark.parametrize("method", ["iisph", "pcisph", "dfsph_p"])
+@pytest.mark.parametrize("method", ALL)
 def test_many_sub_steps_bit_exact(gpu, method):
     scene, cfg, ctx, orc = _make(method, nx=30, ny=50, seed=9)
     print(f"[solver parity] {method} n={len(scene.positions)} 60 sub-steps", flush=True)
@@ -81,7 +86,7 @@ def test_many_sub_steps_bit_exact(gpu, method):
     _check_step(method, ctx, orc, 60)
 
 
-@pytest.mark.parametrize("method", ["iisph", "dfsph_p", "pcisph"])
+@pytest.mark.parametrize("method", ALL)
 def test_frame_step_adaptive(gpu, method):
     """adaptive_time_step.h:35-122 (CFL-quantised sub-steps) and the PCISPH
     fixed-sub-step frame, two frames."""
@@ -99,18 +104,20 @@ def test_frame_step_adaptive(gpu, method):
     assert_bits_equal(g.velocities, vel, "frame velocities")
 
 
-def test_reference_scene_ez_example_sim(gpu):
+@pytest.mark.parametrize("method", ["iisph", "iisph_pseudo_ap"])
+def test_reference_scene_ez_example_sim(gpu, method):
     """EZ_EXAMPLE_SIM with the reference's own default scene inputs: world walls
     with random obstacles and the hex dam break (solids.cpp:139-213,
-    initial_state.cpp:14-97), support 0.025, IISPH, one frame."""
-    params = sb.Parameters(method="iisph")
+    initial_state.cpp:14-97), support 0.025, one frame; IISPH_PSEUDO_AP is the
+    reference's default method (parameters.h:40)."""
+    params = sb.Parameters(method=method)
     sim = sb.EZ_EXAMPLE_SIM(params)
     orc = oracle_scene(scenes.DamBreak(sim.state.positions, sim.state.velocities,
                                        sim.solid_state.positions, params.support, 0.0, 0.0,
                                        np.float32(sim.config.mass_per_particle)),
                        sim.config)
     fs = sim.step()
-    n, ost = orc.frame_step("iisph", 0)
+    n, ost = orc.frame_step(method, 0)
     assert fs.sub_steps == n
     pos, vel = orc.state_f32()
     st = sim.fetch_state()
@@ -118,7 +125,7 @@ def test_reference_scene_ez_example_sim(gpu):
     assert_bits_equal(st.velocities, vel, "EZ velocities")
 
 
-@pytest.mark.parametrize("method", ["iisph", "dfsph_p", "pcisph"])
+@pytest.mark.parametrize("method", ["iisph", "iisph_pseudo_ap", "dfsph_p", "pcisph"])
 def test_graph_replays_equal_eager(gpu, method):
     # Sub-steps replayed from captured graphs against the same sub-steps
     # enqueued directly, with eager work (state and field read-backs, a
