@@ -140,6 +140,51 @@ __device__ __forceinline__ V2 kernel_GradW_r(V2 dP, float r, KernelConsts const&
     return V2{0.0f, 0.0f};
 }
 
+// div_by_recip's fast path: RN32(RN64(a * inv)), clearing ok where
+// div_by_recip would take the IEEE divide. A zero quotient (a = 0, or an
+// infinite divisor) is exact and stays on the fast path: a / b is then the
+// same signed zero.
+__device__ __forceinline__ float div_recip_checked(float a, double inv, bool& ok) {
+    double const q = static_cast<double>(a) * inv;
+    ok = ok && (__builtin_fabs(q) >= 0x1p-126 || q == 0.0);
+    return static_cast<float>(q);
+}
+
+// W(r) and GradW(dP) of one neighbour, r = |dP| (r == imath_length(dP) bit
+// for bit): the values of kernel_W(r) and kernel_GradW_r(dP, r), bit for bit,
+// with q = r / h computed once, the pieces' operands selected instead of
+// branched on, and one test for the rare IEEE-divide cases of div_by_recip
+// (then both are evaluated by those functions).
+__device__ __forceinline__ void kernel_W_GradW_r(V2 dP, float r, KernelConsts const& c,
+                                                 float& w, V2& gw) {
+    bool ok = true;
+    float const q = div_recip_checked(r, c.inv_h, ok);  // r / h
+    bool const inner = q <= 1.0f;
+    float const t = 2.0f - q;
+    float const wa = 1.0f + q * q * (q * 0.75f - 1.5f);
+    float const wb = (t * t * t) / 4.0f;
+    float const wk = div_recip_checked(inner ? wa : wb, c.inv_k, ok);  // / k
+    float const num = inner ? -3.0f * r : -3.0f * (t * t);
+    float x = div_recip_checked(num, inner ? c.inv_hh : c.inv_four_h, ok);  // / h^2 or / 4h
+    float const x2 = div_recip_checked(2.25f * r * r, c.inv_hhh, ok);      // 2.25 r^2 / h^3
+    x = inner ? x + x2 : x;
+    float const dwk = div_recip_checked(x, c.inv_k, ok);  // / k
+    if (!ok) {
+        w = kernel_W(r, c);
+        gw = kernel_GradW_r(dP, r, c);
+        return;
+    }
+    bool const in_support = q <= 2.0f;
+    w = in_support ? wk : 0.0f;
+    float const dw = in_support ? dwk : 0.0f;
+    if (is_safe_div(dP, r)) {
+        V2 const u = divs(dP, r);
+        gw = V2{u.x * dw, u.y * dw};
+    } else {
+        gw = V2{0.0f, 0.0f};
+    }
+}
+
 // sph_common/common.cpp:43-67 grid coordinate: floor(p / cell) per axis with
 // the bounds minimum at the origin.
 __device__ __forceinline__ int2 grid_coord(V2 p, float cell) {

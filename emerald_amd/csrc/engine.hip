@@ -147,18 +147,23 @@ struct DBuf {
     void release() {
         if (p) {
             (void)hipFree(p);
-            g_device_bytes -= static_cast<int64_t>(n * sizeof(T));
+            g_device_bytes -= static_cast<int64_t>((n + TAIL_PAD) * sizeof(T));
         }
         p = nullptr;
         n = 0;
     }
+    // Every buffer carries TAIL_PAD readable elements past its count: the
+    // neighbour build loads candidate positions 16 bytes at a time, up to
+    // three elements past the end of a cell's slot range.
+    static constexpr size_t TAIL_PAD = 4;
     hipError_t alloc(size_t count) {
         release();
         if (count == 0) { count = 1; }
-        hipError_t const e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        hipError_t const e =
+          hipMalloc(reinterpret_cast<void**>(&p), (count + TAIL_PAD) * sizeof(T));
         if (e == hipSuccess) {
             n = count;
-            g_device_bytes += static_cast<int64_t>(n * sizeof(T));
+            g_device_bytes += static_cast<int64_t>((n + TAIL_PAD) * sizeof(T));
         } else {
             p = nullptr;
         }

@@ -135,6 +135,9 @@ struct CellList {
 // Built once on the host with a load factor of at most 1/4, so a lookup of a
 // cell that is not in the set mostly ends at its first slot.
 constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
+// Top bit of a fluid window range's end: the same neighbour cell holds solids
+// (slot counts stay below 2^31).
+constexpr uint32_t WIN_SOLID_NEAR = 0x80000000u;
 struct CellSet {
     uint4 const* slots;
     uint32_t mask;

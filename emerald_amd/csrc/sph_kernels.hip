@@ -561,11 +561,14 @@ __global__ __launch_bounds__(256) void k_cell_windows(WindowArgs w) {
         uint32_t const i = (w.self.key[hint] == nk && w.self.tag[hint] == tag)
                              ? hint
                              : cell_gallop(w.self, n, hint, tag, nk);
-        w.win[item] = cell_range(w.self, i, n, tag, nk);
+        uint2 r = cell_range(w.self, i, n, tag, nk);
         if (w.owin) {
-            w.owin[item] = w.other_set.slots ? cell_set_range(w.other_set, w.other, nk, tag)
-                                             : make_uint2(0u, 0u);
+            uint2 const o = w.other_set.slots ? cell_set_range(w.other_set, w.other, nk, tag)
+                                              : make_uint2(0u, 0u);
+            w.owin[item] = o;
+            if (o.y > o.x) { r.y |= WIN_SOLID_NEAR; }
         }
+        w.win[item] = r;
     }
 }
 
@@ -631,13 +634,19 @@ constexpr bool net_sorted() {
     return CAP == 16;
 }
 
-// Sorts one lane's cnt <= 16 collected entries (distance bits << 32 | slot)
-// into the reference order: by distance, then by original index. The network
-// orders by (distance, slot); equal distances (rare) are then put in original
-// index order by an insertion pass with the exact comparator.
+// Sorts one lane's cnt <= 16 collected entries into the reference order (by
+// distance, then by original index) and leaves them as (distance bits << 32 |
+// slot). They arrive as (|v|^2 bits << 32 | slot), v = candidate - particle,
+// and the network orders them by (|v|^2, slot). The distances are then taken
+// as imath_length takes them, sqrt(|v|^2), which is monotone in |v|^2, so the
+// order is by distance except among equal distances (rare), which an
+// insertion pass with the exact comparator puts in original index order. A
+// |v|^2 below 2 FLT_MIN, where imath_length scales instead, sends the list
+// to that pass with every distance taken again from the positions.
 template <int W>
 __device__ __forceinline__ void net_sort16(uint64_t (*ent)[W], uint32_t lane, int cnt,
-                                           uint32_t const* __restrict__ oorig) {
+                                           uint32_t const* __restrict__ oorig,
+                                           V2 const* __restrict__ opos, V2 p) {
     constexpr int N = 16;
     uint64_t e[N];
 #pragma unroll
@@ -653,6 +662,16 @@ __device__ __forceinline__ void net_sort16(uint64_t (*ent)[W], uint32_t lane, in
         e[NET16.ce[c].a] = swap ? y : x;
         e[NET16.ce[c].b] = swap ? x : y;
     }
+    bool tiny = false;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (k < cnt) {
+            float const l2 = __uint_as_float(static_cast<uint32_t>(e[k] >> 32));
+            tiny = tiny || !(l2 >= 2.0f * FLT_MIN);
+            float const d = __builtin_sqrtf(l2);
+            e[k] = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | static_cast<uint32_t>(e[k]);
+        }
+    }
     bool tie = false;
 #pragma unroll
     for (int k = 1; k < N; ++k) {
@@ -662,7 +681,14 @@ __device__ __forceinline__ void net_sort16(uint64_t (*ent)[W], uint32_t lane, in
     for (int k = 0; k < N; ++k) {
         if (k < cnt) { ent[k][lane] = e[k]; }
     }
-    if (tie) {
+    if (tiny) {
+        for (int k = 0; k < cnt; ++k) {
+            uint32_t const s = static_cast<uint32_t>(ent[k][lane]);
+            float const d = imath_length(sub(opos[s], p));
+            ent[k][lane] = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | s;
+        }
+    }
+    if (tie || tiny) {
         for (int k = 1; k < cnt; ++k) {
             uint64_t const key = ent[k][lane];
             int kk = k;
@@ -677,20 +703,29 @@ __device__ __forceinline__ void net_sort16(uint64_t (*ent)[W], uint32_t lane, in
     }
 }
 
-// Positions of candidates s .. s + C - 1 of a range ending at e (clamped to
-// its last slot; nothing is loaded for an empty range).
+// Positions of candidates s .. s + C - 1 (s inside the range), two per 16-byte
+// load; entries past the range's end are loaded but not used (position
+// buffers carry tail padding for the last range).
+struct alignas(8) V2Pair {
+    V2 a;
+    V2 b;
+};
 template <int C>
-__device__ __forceinline__ void load_chunk(V2 (&q)[C], V2 const* __restrict__ opos, uint32_t s,
-                                           uint32_t e) {
+__device__ __forceinline__ void load_chunk(V2 (&q)[C], V2 const* __restrict__ opos, uint32_t s) {
+    static_assert(C % 2 == 0, "chunks are loaded in pairs");
 #pragma unroll
-    for (int u = 0; u < C; ++u) {
-        if (s < e) { q[u] = opos[s + u < e ? s + u : e - 1]; }
+    for (int u = 0; u < C; u += 2) {
+        V2Pair v;
+        __builtin_memcpy(&v, opos + s + u, sizeof(V2Pair));
+        q[u] = v.a;
+        q[u + 1] = v.b;
     }
 }
 
-// Acceptance and sorted insertion of one loaded chunk, in scan order. Returns
-// false when the scan stops: a capped list overflowed (cnt = -1) or a
-// full-capacity list is full.
+// Acceptance of one loaded chunk, in scan order: appended for the 16-entry
+// build (sorted afterwards), inserted in order otherwise. Returns false when
+// the scan stops: a capped list overflowed (cnt = -1) or a full-capacity list
+// is full.
 template <int CAP, int C, int W>
 __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[W], uint32_t lane, V2 p,
                                            uint32_t self, bool same, V2 const (&q)[C],
@@ -698,11 +733,30 @@ __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[W], uint32_t lane, V2
                                            uint32_t const* __restrict__ oorig, float maxd,
                                            float cut2, int& cnt) {
     bool in[C];
+    float l2[C];
 #pragma unroll
     for (int u = 0; u < C; ++u) {
         V2 const v = sub(q[u], p);
-        float const l2 = v.x * v.x + v.y * v.y;  // as in imath_length
-        in[u] = (cut2 > 0.0f) ? (l2 < cut2) : (imath_length(v) < maxd);
+        l2[u] = v.x * v.x + v.y * v.y;  // as in imath_length
+        in[u] = (cut2 > 0.0f) ? (l2[u] < cut2) : (imath_length(v) < maxd);
+    }
+    if (net_sorted<CAP>()) {
+        // Unsorted append of (|v|^2 bits << 32 | slot); the distances are
+        // taken after the sort (net_sort16), once per accepted entry.
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            uint32_t const s = s0 + u;
+            bool const acc = s < e && !(same && s == self) && in[u];
+            if (acc && cnt < CAP) {
+                ent[cnt][lane] = (static_cast<uint64_t>(__float_as_uint(l2[u])) << 32) | s;
+            }
+            cnt += acc ? 1 : 0;
+        }
+        if (cnt > CAP) {
+            cnt = -1;
+            return false;
+        }
+        return true;
     }
 #pragma unroll
     for (int u = 0; u < C; ++u) {
@@ -714,18 +768,14 @@ __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[W], uint32_t lane, V2
             }
             float const d = imath_length(sub(q[u], p));
             uint64_t const key = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | s;
-            if (net_sorted<CAP>()) {
-                ent[cnt][lane] = key;  // sorted after the scan
-            } else {
-                int kk = cnt;
-                while (kk > 0) {
-                    uint64_t const prev = ent[kk - 1][lane];
-                    if (!entry_after(prev, key, oorig)) { break; }
-                    ent[kk][lane] = prev;
-                    --kk;
-                }
-                ent[kk][lane] = key;
+            int kk = cnt;
+            while (kk > 0) {
+                uint64_t const prev = ent[kk - 1][lane];
+                if (!entry_after(prev, key, oorig)) { break; }
+                ent[kk][lane] = prev;
+                --kk;
             }
+            ent[kk][lane] = key;
             ++cnt;
             if (CAP == MAXN && cnt == MAXN) { return false; }
         }
@@ -740,22 +790,26 @@ __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[W], uint32_t lane, V2
 // which is then taken for accepted candidates only. (Loading each range's
 // first chunk while the previous range is scanned, with the nine ranges
 // unrolled, measured slower: 217 vs 174 us at 1M.)
+// Window ranges carry WIN_SOLID_NEAR in their end's top bit when the same
+// neighbour cell holds solids (k_cell_windows); *near receives the OR.
 template <int CAP, int W>
 __device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
                                               V2 p, uint32_t self, bool same,
                                               uint2 const* __restrict__ win,
                                               V2 const* __restrict__ opos,
                                               uint32_t const* __restrict__ oorig, float maxd,
-                                              float cut2) {
+                                              float cut2, uint32_t* near = nullptr) {
     constexpr int C = 4;
     int cnt = 0;
+    uint32_t flags = 0;
     uint2 cur = win[0];
     for (int k = 0; k < 9; ++k) {
         uint2 const nxt = win[k < 8 ? k + 1 : 8];  // next range in flight
-        uint32_t const e = cur.y;
+        flags |= cur.y;
+        uint32_t const e = cur.y & ~WIN_SOLID_NEAR;
         for (uint32_t s0 = cur.x; s0 < e; s0 += C) {
             V2 q[C];
-            load_chunk<C>(q, opos, s0, e);
+            load_chunk<C>(q, opos, s0);
             if (!take_chunk<CAP, C, W>(ent, lane, p, self, same, q, s0, e, oorig, maxd, cut2,
                                        cnt)) {
                 return cnt;  // overflow (-1) or a full 64-entry list (insertion-sorted)
@@ -763,7 +817,8 @@ __device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
         }
         cur = nxt;
     }
-    if (net_sorted<CAP>() && cnt > 1) { net_sort16<W>(ent, lane, cnt, oorig); }
+    if (near) { *near = flags & WIN_SOLID_NEAR; }
+    if (net_sorted<CAP>() && cnt > 0) { net_sort16<W>(ent, lane, cnt, oorig, opos, p); }
     return cnt;
 }
 
@@ -779,8 +834,9 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
     float const mf = c.rho0 * c.fluid_volume;  // rho0 * V_j (uniform fluid volume)
 
     // Fluid neighbours.
+    uint32_t solid_near = 0;
     int const fc = collect_window<CAP, W>(ent, lane, p, t, true, a.win + static_cast<size_t>(cell) * 9,
-                                       a.pos, a.orig, c.cell, c.cut2);
+                                       a.pos, a.orig, c.cell, c.cut2, &solid_near);
     if (fc < 0) {
         max_count = CAP + 1;
         return false;
@@ -817,9 +873,10 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
             a.fl.vec[o] = vec;
         }
         if (MODE != BUILD_LISTS) {
-            V2 const gw = kernel_GradW_r(neg(vec), d, c.kc);
+            float w;
+            V2 gw;
+            kernel_W_GradW_r(neg(vec), d, c.kc, w, gw);
             a.fl.gw[o] = gw;
-            float const w = kernel_W(d, c.kc);
             dens += w * mf;                                  // density.cpp:69-74
             if (MODE == BUILD_IISPH) { msum_f = add(msum_f, scale(mf, gw)); }  // iisph_ops.cpp:42-47
             if (MODE == BUILD_DFSPH) {                        // dfsph_p_ops.cpp:47-57
@@ -840,7 +897,7 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
     }
     SolidView const& sv = a.solids;
     int sc = 0;
-    if (a.swin && sv.pos) {
+    if (a.swin && sv.pos && solid_near) {
         sc = collect_window<CAP, W>(ent, lane, p, 0xFFFFFFFFu, false,
                                  a.swin + static_cast<size_t>(cell) * 9, sv.pos, sv.orig, c.cell,
                                  c.cut2);
@@ -880,10 +937,12 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
             a.sl.vec[o] = vec;
         }
         if (MODE != BUILD_LISTS) {
-            V2 const gw = kernel_GradW_r(neg(vec), d, c.kc);
+            float w;
+            V2 gw;
+            kernel_W_GradW_r(neg(vec), d, c.kc, w, gw);
             a.sl.gw[o] = gw;
             float const ms = c.rho0 * sv.vol[b];
-            dens_s += kernel_W(d, c.kc) * ms;
+            dens_s += w * ms;
             if (MODE == BUILD_IISPH) { msum_s = add(msum_s, scale(ms, gw)); }
             if (MODE == BUILD_DFSPH) { sum_mg = add(sum_mg, scale(ms, gw)); }
         }
@@ -1848,8 +1907,11 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
                     float const d = imath_length(vec);
                     a.fl.vec[o] = vec;
                     a.fl.dist[o] = d;
-                    a.fl.gw[o] = kernel_GradW_r(neg(vec), d, c.kc);
-                    w += kernel_W(d, c.kc);
+                    float wk;
+                    V2 gw;
+                    kernel_W_GradW_r(neg(vec), d, c.kc, wk, gw);
+                    a.fl.gw[o] = gw;
+                    w += wk;
                 }
             }
         }
@@ -1862,8 +1924,11 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
                 uint32_t const b = a.sl.idx[o];
                 V2 const vec = sub(a.solids.pos[b], xs);
                 float const d = imath_length(vec);
-                a.sl.gw[o] = kernel_GradW_r(neg(vec), d, c.kc);
-                vf += a.solids.vol[b] * kernel_W(d, c.kc);
+                float wk;
+                V2 gw;
+                kernel_W_GradW_r(neg(vec), d, c.kc, wk, gw);
+                a.sl.gw[o] = gw;
+                vf += a.solids.vol[b] * wk;
             }
             rho += c.rho0 * vf;
         }
