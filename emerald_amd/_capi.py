@@ -31,6 +31,7 @@ FIELDS = {
     "density": (0, 1), "density_star": (1, 1), "pressure": (2, 1), "aii": (3, 1),
     "dii": (4, 2), "sum_dij_pj": (5, 2), "alpha": (6, 1), "kappa_density": (7, 1),
     "kappa_divergence": (8, 1), "pressure_force": (9, 2), "pressure_acceleration": (10, 2),
+    "external_force": (11, 2),
 }
 
 
@@ -103,6 +104,29 @@ class Tuning(ctypes.Structure):
                 "xcd_remap": int(self.xcd_remap)}
 
 
+FORCE_GRAVITY, FORCE_POLE, FORCE_DRAG, FORCE_ANTI_COUPLING = 1, 2, 4, 8
+
+
+class ForceModel(ctypes.Structure):
+    """emsph_force_model (sph2d_box/forces.cpp:62-149)."""
+    _fields_ = [("flags", ctypes.c_uint32), ("gravity", ctypes.c_float),
+                ("pole_magnitude", ctypes.c_float), ("pole_x", ctypes.c_float),
+                ("pole_y", ctypes.c_float), ("drag_magnitude", ctypes.c_float),
+                ("drag_diameter", ctypes.c_float),
+                ("anti_coupling_max_distance", ctypes.c_float),
+                ("anti_coupling_magnitude", ctypes.c_float)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# emsph_user_forces_fn: (user, count, t, dt, mass, radius, forces_xy, pos_xy, vel_xy).
+USER_FORCES_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_float))
+
+
 class FrameStats(ctypes.Structure):
     _fields_ = [
         ("sub_steps", ctypes.c_int32),
@@ -142,6 +166,11 @@ SIGNATURES = {
     "emsph_get_solid_volumes": (_ST, [_P, _P]),
     "emsph_get_field": (_ST, [_P, _I, _P]),
     "emsph_get_neighborhood": (_ST, [_P, _I, _P, _P, _P, _P]),
+    "emsph_default_force_model": (_ST, [_P, _P]),
+    "emsph_set_force_model": (_ST, [_P, _P]),
+    "emsph_get_force_model": (_ST, [_P, _P]),
+    "emsph_set_user_forces": (_ST, [_P, USER_FORCES_FN, _P]),
+    "emsph_batch_set_force_model": (_ST, [_P, _I, _P]),
     "emsph_frame_init": (_ST, [_P, _I]),
     "emsph_sub_step": (_ST, [_P, _I, _F, _F, _P]),
     "emsph_frame_step": (_ST, [_P, _I, _I64, _P]),

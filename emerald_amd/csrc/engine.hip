@@ -260,7 +260,19 @@ struct SceneHost {
     uint32_t soffset = 0; // first solid slot
     float dt = 0.0f;      // sub-step dt (the largest one when cfl > 0)
     float cfl = 0.0f;     // > 0: per-sub-step dt from the scene's CFL number
+    bool force_set = false;     // force model set explicitly (else gravity of the params)
+    emsph_force_model force{};
 };
+
+// The scene's force model: the one set explicitly, or the reference's
+// default_gravity_forces with the parameters' gravity.
+emsph_force_model force_model(SceneHost const& h) {
+    if (h.force_set) { return h.force; }
+    emsph_force_model m{};
+    m.flags = EMSPH_FORCE_GRAVITY;
+    m.gravity = h.cfg.params.gravity;
+    return m;
+}
 
 int scene_pass_count(uint32_t m) {
     if (m <= 1) { return 0; }
@@ -353,6 +365,18 @@ struct Engine {
     bool xcd_remap = true;   // XCD-aware workgroup -> tile order
     uint64_t layout_gen = 1;
     std::map<std::array<uint64_t, 10>, hipGraphExec_t> graphs;
+    // External forces: a host callback (contexts only) and the per-slot force
+    // buffer used whenever some scene's model is not the default gravity.
+    emsph_user_forces_fn user_fn = nullptr;
+    void* user_data = nullptr;
+    bool use_fext = false;
+    DBuf<V2> fext;
+    float step_time = 0.0f;  // time of the current sub-step (for the callback)
+    struct {
+        std::vector<V2> spos, svel, fs;
+        std::vector<uint32_t> orig;
+        std::vector<float> pos, vel, forces;
+    } fhost;
 
     ~Engine() {
         if (host) { (void)hipHostFree(host); }
@@ -512,6 +536,7 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     b.vext = g->vext.p;
     b.xstar = g->xstar.p;
     b.force = g->force.p;
+    b.fext = g->use_fext ? g->fext.p : nullptr;
     b.store_geometry = g->store_geometry ? 1 : 0;
     b.nb = g->nb;
     b.xcd_remap = g->xcd_remap ? 1 : 0;
@@ -531,6 +556,15 @@ SceneDev scene_dev(SceneHost const& h, uint32_t first_tile) {
     sd.ntiles = (h.n + TILE - 1) / TILE;
     sd.cfl = h.cfl;
     sd.dt_max = h.dt;
+    emsph_force_model const fm = force_model(h);
+    sd.c.gravity = fm.gravity;
+    sd.force.flags = fm.flags;
+    sd.force.pole_mag = fm.pole_magnitude;
+    sd.force.pole_x = fm.pole_x;
+    sd.force.pole_y = fm.pole_y;
+    sd.force.drag_mult = -fm.drag_magnitude * fm.drag_diameter;
+    sd.force.anti_maxd = fm.anti_coupling_max_distance;
+    sd.force.anti_mag = fm.anti_coupling_magnitude;
     return sd;
 }
 
@@ -991,7 +1025,9 @@ void finish_prepare(Engine* g) {
 // advance (sort pass count known, no kernel timing), else enqueued directly.
 template <class F>
 hipError_t segment(Engine* g, int method, int seg, F&& enqueue) {
-    if (!g->use_graphs || g->timer.enabled || g->passes_hint < 0) { return enqueue(); }
+    if (!g->use_graphs || g->timer.enabled || g->passes_hint < 0 || g->user_fn) {
+        return enqueue();
+    }
     std::array<uint64_t, 10> const key{
       static_cast<uint64_t>(method), static_cast<uint64_t>(seg),
       static_cast<uint64_t>(g->passes_hint), static_cast<uint64_t>(g->build_cap), g->layout_gen,
@@ -1027,6 +1063,72 @@ hipError_t segment(Engine* g, int method, int seg, F&& enqueue) {
 
 enum Segment : int { SEG_PREFIX = 0, SEG_MIDDLE = 1, SEG_SUFFIX = 2 };
 
+// The host callback's forces (forces.cpp:11-61 calling User_forces_function)
+// into g->fext in sorted order: the state at this point of the sub-step is
+// copied back in original order, the callback accumulates into zeroed forces,
+// and they are copied in. after_dt: the callback's time is t + dt (DFSPH_P,
+// dfsph_p.cpp:350). A short sort is detected first (the sub-step is then
+// redone and the callback is not called for the discarded attempt).
+hipError_t host_user_forces(Engine* g, bool after_dt) {
+    if (!g->keys_checked) {
+        HIP_PASS(read_keystate(g));
+        if (g->sort_short) { return hipSuccess; }
+    }
+    uint32_t const n = g->N;
+    auto& H = g->fhost;
+    H.spos.resize(n);
+    H.svel.resize(n);
+    H.orig.resize(n);
+    SceneDev sd{};
+    HIP_PASS(hipMemcpyAsync(H.spos.data(), g->spos.p, n * sizeof(V2), hipMemcpyDeviceToHost,
+                            g->stream));
+    HIP_PASS(hipMemcpyAsync(H.svel.data(), g->svel.p, n * sizeof(V2), hipMemcpyDeviceToHost,
+                            g->stream));
+    HIP_PASS(hipMemcpyAsync(H.orig.data(), g->fidx.orig.p, n * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, g->stream));
+    HIP_PASS(hipMemcpyAsync(&sd, g->scenes.p, sizeof(SceneDev), hipMemcpyDeviceToHost,
+                            g->stream));
+    HIP_PASS(hipStreamSynchronize(g->stream));
+    H.pos.resize(2 * static_cast<size_t>(n));
+    H.vel.resize(2 * static_cast<size_t>(n));
+    H.forces.assign(2 * static_cast<size_t>(n), 0.0f);
+    for (uint32_t t = 0; t < n; ++t) {
+        size_t const o = H.orig[t];
+        H.pos[2 * o] = H.spos[t].x;
+        H.pos[2 * o + 1] = H.spos[t].y;
+        H.vel[2 * o] = H.svel[t].x;
+        H.vel[2 * o + 1] = H.svel[t].y;
+    }
+    emsph_config const& cfg = g->sh[0].cfg;
+    float const dt = sd.c.dt;
+    float const t = after_dt ? g->step_time + dt : g->step_time;
+    g->user_fn(g->user_data, n, t, dt, cfg.mass_per_particle, cfg.draw_radius, H.forces.data(),
+               H.pos.data(), H.vel.data());
+    H.fs.resize(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        size_t const o = H.orig[k];
+        H.fs[k] = V2{H.forces[2 * o], H.forces[2 * o + 1]};
+    }
+    HIP_PASS(hipMemcpyAsync(g->fext.p, H.fs.data(), n * sizeof(V2), hipMemcpyHostToDevice,
+                            g->stream));
+    return hipStreamSynchronize(g->stream);
+}
+
+// External forces of a sub-step when some scene does not use the default
+// gravity model (otherwise the build or the apply kernel integrates gravity
+// itself): the host callback's forces, then the device models; apply = 1
+// integrates them into the velocities, 0 stores them for the kernel that does.
+hipError_t enqueue_forces(Engine* g, Batch const& b, int apply, bool after_dt) {
+    if (!b.fext) { return hipSuccess; }
+    int host = 0;
+    if (g->user_fn) {
+        HIP_PASS(host_user_forces(g, after_dt));
+        if (g->sort_short) { return hipSuccess; }
+        host = 1;
+    }
+    return timed(g, "external_forces", [&] { launch_external_forces(b, apply, host, g->stream); });
+}
+
 // iisph.cpp:23-151 for every scene of the batch.
 emsph_status iisph_sub_step(Engine* g) {
     begin_attempt(g);
@@ -1039,6 +1141,7 @@ emsph_status iisph_sub_step(Engine* g) {
     b.pr_out = pb;
     HIP_TRY(segment(g, EMSPH_IISPH, SEG_PREFIX, [&]() -> hipError_t {
         HIP_PASS(enqueue_prepare(g, BUILD_IISPH, false));
+        HIP_PASS(enqueue_forces(g, b, 1, false));
         return timed(g, "iisph_aii_beta", [&] { launch_iisph_aii_beta0(b, g->stream); });
     }));
     finish_prepare(g);
@@ -1092,6 +1195,7 @@ emsph_status iisph_ap_sub_step(Engine* g, bool pseudo) {
     Batch b = fluid_batch(g, g->p0.p, g->p0.p);
     HIP_TRY(segment(g, method, SEG_PREFIX, [&]() -> hipError_t {
         HIP_PASS(enqueue_prepare(g, BUILD_AP, false));
+        HIP_PASS(enqueue_forces(g, b, 1, false));
         return timed(g, "ap_diag", [&] { launch_ap_diag0(b, pseudo ? 1 : 0, g->stream); });
     }));
     finish_prepare(g);
@@ -1157,6 +1261,7 @@ emsph_status dfsph_sub_step(Engine* g) {
     b.p_in = g->kds.p;
     b.kappa = g->kds.p;
     HIP_TRY(segment(g, EMSPH_DFSPH_P, SEG_MIDDLE, [&]() -> hipError_t {
+        HIP_PASS(enqueue_forces(g, b, 0, true));
         HIP_PASS(timed(g, "apply_gravity", [&] { launch_apply(b, 1, 0, 1, g->stream); }));
         return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
     }));
@@ -1195,6 +1300,7 @@ emsph_status pcisph_sub_step(Engine* g) {
     b.vstar_out = vin;
     HIP_TRY(segment(g, EMSPH_PCISPH, SEG_PREFIX, [&]() -> hipError_t {
         HIP_PASS(enqueue_prepare(g, BUILD_LISTS, false));
+        HIP_PASS(enqueue_forces(g, b, 0, false));
         HIP_PASS(timed(g, "pci_init", [&] { launch_pci_init(b, g->stream); }));
         return timed(g, "loop", [&] { launch_loop_begin(b, g->stream); });
     }));
@@ -1316,9 +1422,19 @@ emsph_status engine_get_field(Engine* g, uint32_t s, int field, float* out) {
     case EMSPH_FIELD_KAPPA_DIVERGENCE: f1 = g->kvs.p; break;
     case EMSPH_FIELD_PRESSURE_FORCE: f2 = g->force.p; break;
     case EMSPH_FIELD_PRESSURE_ACCELERATION: f2 = g->pacc.p; break;
+    case EMSPH_FIELD_EXTERNAL_FORCE: f2 = g->fext.p; break;
     default: return fail(EMSPH_ERR_INVALID_ARGUMENT, "unknown field");
     }
     SceneHost const& h = g->sh[s];
+    if (field == EMSPH_FIELD_EXTERNAL_FORCE && !g->use_fext) {
+        // The fused default: gravity only (forces.cpp:62-83).
+        float const fy = 0.0f - h.cfg.mass_per_particle * force_model(h).gravity;
+        for (uint32_t i = 0; i < h.n; ++i) {
+            out[2 * i] = 0.0f;
+            out[2 * i + 1] = fy;
+        }
+        return EMSPH_OK;
+    }
     uint32_t f2_stride = 1;
     if (field == EMSPH_FIELD_SUM_DIJ_PJ) {
         f2 = &g->sd.p[0].sdp;
@@ -1848,7 +1964,7 @@ emsph_status emsph_frame_init(emsph_context* c, int method) {
 emsph_status emsph_sub_step(emsph_context* c, int method, float time_seconds, float dt,
                             emsph_step_stats* stats) {
     if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
-    (void)time_seconds;  // the built-in gravity force model does not depend on time
+    c->e.step_time = time_seconds;  // passed to a host force callback
     if (stats) { std::memset(stats, 0, sizeof(*stats)); }
     ST_TRY(check_method(method));
     ST_TRY(engine_set_dt(&c->e, 0, dt));
@@ -1931,6 +2047,7 @@ emsph_status emsph_run_sub_steps(emsph_context* c, int method, float dt, int n_s
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, c->e.stream));
     for (int k = 0; k < n_sub_steps; ++k) {
+        c->e.step_time = static_cast<float>(k) * dt;  // sub-step k starts at k dt
         emsph_status const s = engine_sub_step(&c->e, method);
         if (s) {
             (void)hipEventDestroy(e0);
@@ -1997,6 +2114,72 @@ emsph_status emsph_batch_set_tuning(emsph_batch* b, const emsph_tuning* t) {
 emsph_status emsph_pool_set_tuning(emsph_pool* P, const emsph_tuning* t) {
     if (!P || !t) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
     return tuning_set(&P->e, t);
+}
+
+//------------------------------------------------------------------------------
+// External forces.
+
+namespace {
+
+emsph_status check_force_model(emsph_force_model const* m) {
+    if (!m) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null force model"); }
+    uint32_t const known = EMSPH_FORCE_GRAVITY | EMSPH_FORCE_POLE | EMSPH_FORCE_DRAG |
+                           EMSPH_FORCE_ANTI_COUPLING;
+    if (m->flags & ~known) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "unknown force flags"); }
+    return EMSPH_OK;
+}
+
+// After a force model or callback change: whether the per-slot force buffer is
+// used, the buffer itself, and the scene table (a new layout generation, so
+// no captured graph of the old kernel sequence is replayed).
+emsph_status engine_forces_changed(Engine* g) {
+    bool use = g->user_fn != nullptr;
+    for (auto const& h : g->sh) { use = use || force_model(h).flags != EMSPH_FORCE_GRAVITY; }
+    if (use && !g->fext.p && g->cap) { HIP_TRY(g->fext.alloc(g->cap)); }
+    g->use_fext = use;
+    return upload_layout(g);
+}
+
+}  // namespace
+
+emsph_status emsph_default_force_model(const emsph_config* config, emsph_force_model* out) {
+    if (!config || !out) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    SceneHost h;
+    h.cfg = *config;
+    *out = force_model(h);
+    return EMSPH_OK;
+}
+
+emsph_status emsph_set_force_model(emsph_context* c, const emsph_force_model* model) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    ST_TRY(check_force_model(model));
+    c->e.sh[0].force = *model;
+    c->e.sh[0].force_set = true;
+    return engine_forces_changed(&c->e);
+}
+
+emsph_status emsph_get_force_model(emsph_context* c, emsph_force_model* out) {
+    if (!c || !out) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    *out = force_model(c->e.sh[0]);
+    return EMSPH_OK;
+}
+
+emsph_status emsph_set_user_forces(emsph_context* c, emsph_user_forces_fn fn, void* user) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    c->e.user_fn = fn;
+    c->e.user_data = fn ? user : nullptr;
+    return engine_forces_changed(&c->e);
+}
+
+emsph_status emsph_batch_set_force_model(emsph_batch* b, int scene,
+                                         const emsph_force_model* model) {
+    if (!b || scene < 0 || static_cast<uint32_t>(scene) >= b->e.M) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad batch or scene");
+    }
+    ST_TRY(check_force_model(model));
+    b->e.sh[scene].force = *model;
+    b->e.sh[scene].force_set = true;
+    return engine_forces_changed(&b->e);
 }
 
 emsph_status emsph_set_graphs(emsph_context* c, int enable) {

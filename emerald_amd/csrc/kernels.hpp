@@ -48,6 +48,24 @@ struct Thresholds {
     int32_t dfsph_max_iters;
 };
 
+// External force models of one scene (sph2d_box/forces.cpp:62-149), applied in
+// flag order: FORCE_GRAVITY (forces.y -= m g, with StepConsts::gravity),
+// FORCE_POLE, FORCE_DRAG, FORCE_ANTI_COUPLING. FORCE_GRAVITY alone with no
+// host forces is the default, fused into the kernel that integrates the force.
+constexpr uint32_t FORCE_GRAVITY = 1u;
+constexpr uint32_t FORCE_POLE = 2u;
+constexpr uint32_t FORCE_DRAG = 4u;
+constexpr uint32_t FORCE_ANTI_COUPLING = 8u;
+struct ForceDev {
+    uint32_t flags;
+    float pole_mag;
+    float pole_x, pole_y;
+    float drag_mult;    // -drag magnitude * drag diameter (forces.cpp:122)
+    float anti_maxd;    // anti-coupling distance limit and magnitude
+    float anti_mag;
+    uint32_t pad;
+};
+
 struct SceneDev {
     StepConsts c;
     Thresholds thr;
@@ -59,6 +77,7 @@ struct SceneDev {
     uint32_t ntiles;
     float cfl;            // > 0: c.dt = min(dt_max, cfl * h / max |v|) at each sub-step
     float dt_max;
+    ForceDev force;
 };
 
 struct Tile {
@@ -224,6 +243,10 @@ struct Batch {
     V2* vstar_out;
     V2 const* vstar_alt;
     V2* force;
+    // External forces per sorted slot when a scene's force model is not the
+    // default gravity (or a host callback supplies forces); nullptr otherwise,
+    // and the kernels that integrate the force use gravity directly.
+    V2* fext;
     int store_geometry;
     // Tuning choices (results do not depend on them): neighbour batch depth of
     // the sweeps (2, 4 or 8) and the XCD-aware workgroup -> tile remap.
@@ -353,6 +376,11 @@ void launch_finalize(Batch const& b, int dfsph, hipStream_t s);
 void launch_dfsph_div(Batch const& b, hipStream_t s);
 void launch_dfsph_dens(Batch const& b, hipStream_t s);
 void launch_pci_init(Batch const& b, hipStream_t s);
+// External forces of every fluid slot into b.fext (forces.cpp:11-149): the
+// host-supplied forces already in b.fext (host_forces != 0) or zero, then each
+// scene's force models. apply != 0 (IISPH, IISPH_AP, IISPH_PSEUDO_AP): instead
+// of only storing, also integrate them into the velocities (dynamics.cpp:24-37).
+void launch_external_forces(Batch const& b, int apply, int host_forces, hipStream_t s);
 void launch_pci_density(Batch const& b, hipStream_t s);
 void launch_pci_force(Batch const& b, hipStream_t s);
 void launch_pci_finalize(Batch const& b, hipStream_t s);

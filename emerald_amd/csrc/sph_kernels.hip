@@ -970,8 +970,9 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
         }
         a.dii[t] = dii;
     }
-    if (MODE == BUILD_IISPH || MODE == BUILD_AP) {
-        // forces.cpp:11-83 (zeroed force, y -= m g) and dynamics.cpp:24-37.
+    if ((MODE == BUILD_IISPH || MODE == BUILD_AP) && !a.fext) {
+        // forces.cpp:11-83 (zeroed force, y -= m g) and dynamics.cpp:24-37;
+        // other force models integrate in k_external_forces after the build.
         V2 const f{0.0f, 0.0f - c.mass * c.gravity};
         float const vden = c.rho0 * c.fluid_volume;
         V2 const numer = scale(c.dt, f);
@@ -1647,7 +1648,7 @@ __global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
         StepConsts const c = a.scenes[r.scene].c;
         V2 v = a.vel[t];
         if (GRAVITY) {
-            V2 const f{0.0f, 0.0f - c.mass * c.gravity};
+            V2 const f = a.fext ? a.fext[t] : V2{0.0f, 0.0f - c.mass * c.gravity};
             float const vden = c.rho0 * c.fluid_volume;
             V2 const numer = scale(c.dt, f);
             if (is_safe_div(numer, vden)) { v = add(v, divs(numer, vden)); }
@@ -1852,6 +1853,119 @@ __global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a) {
 }
 
 //------------------------------------------------------------------------------
+// External forces (sph2d_box/forces.cpp:11-149).
+
+// util/random.h:112-134 Lehmer_rand_gen_64 seeded with i, first draw: the
+// 128-bit state i is multiplied by 0xda942042e4dd58b5 three times (twice at
+// seeding, once per draw) and the draw is the state's high 64 bits. The state
+// is kept as two 64-bit halves: (hi, lo) * m = (hi m + mulhi(lo, m), lo m)
+// modulo 2^128.
+__device__ __forceinline__ uint64_t lehmer64_first_draw(uint64_t seed) {
+    constexpr uint64_t MUL = 0xda942042e4dd58b5ull;
+    uint64_t lo = seed;
+    uint64_t hi = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint64_t const h = hi * MUL + __umul64hi(lo, MUL);
+        lo = lo * MUL;
+        hi = h;
+    }
+    return hi;
+}
+
+// forces.cpp:134-141: std::uniform_real_distribution<float>{-pi, pi} over that
+// generator, as the host C++ library (libstdc++ bits/random.tcc) computes it:
+// generate_canonical<float, 24> takes one draw u of the 2^64-wide range,
+// s = float(u) / 2^64, clamped below 1 (to the largest float below 1), and the
+// distribution returns s * (b - a) + a in float.
+__device__ __forceinline__ float anti_coupling_angle(uint64_t i) {
+    float const a = -3.14159274101257324219f;  // float(-M_PI)
+    float const b = 3.14159274101257324219f;   // float(M_PI)
+    float s = static_cast<float>(lehmer64_first_draw(i)) * 5.42101086242752217004e-20f;  // 2^-64
+    if (s >= 1.0f) { s = 0.99999994039535522461f; }  // nextafter(1, 0)
+    return s * (b - a) + a;
+}
+
+// cos and sin of the angle rounded from double precision (the host library's
+// float cos/sin are accurate to well under one ulp, so the two agree but for
+// rare last-bit differences).
+__device__ __forceinline__ V2 unit_direction(float angle) {
+    double const d = static_cast<double>(angle);
+    return V2{static_cast<float>(cos(d)), static_cast<float>(sin(d))};
+}
+
+template <int APPLY>
+__global__ __launch_bounds__(TILE) void k_external_forces(Batch a, int host_forces) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!r.valid) { return; }
+    uint32_t const t = r.t;
+    SceneDev const& sd = a.scenes[r.scene];
+    StepConsts const c = sd.c;
+    ForceDev const fm = sd.force;
+    V2 f = host_forces ? a.fext[t] : V2{0.0f, 0.0f};
+    V2 const x = a.pos[t];
+    V2 v = a.vel[t];
+    if (fm.flags & FORCE_GRAVITY) {
+        // :62-69 accumulate_gravity_forces
+        f.y -= c.mass * c.gravity;
+    }
+    if (fm.flags & FORCE_POLE) {
+        // :85-114 accumulate_constant_pole_attraction_forces: r = x - pole,
+        // Imath normalized() (zero for a zero length), forces -= magnitude rN,
+        // then += the x, y components of the cross product of the torque
+        // (0, 0, 0.125 magnitude) with (rN, 0).
+        V2 const rv = sub(x, V2{fm.pole_x, fm.pole_y});
+        float const l = imath_length(rv);
+        V2 const rn = (l == 0.0f) ? V2{0.0f, 0.0f} : V2{rv.x / l, rv.y / l};
+        f = sub(f, scale(fm.pole_mag, rn));
+        float const tz = 0.125f * fm.pole_mag;
+        V2 const turn{0.0f * 0.0f - tz * rn.y, tz * rn.x - 0.0f * 0.0f};
+        f = add(f, turn);
+    }
+    if (fm.flags & FORCE_DRAG) {
+        // :116-124 accumulate_simple_drag_forces: forces += (-magnitude d) v
+        f = add(f, scale(fm.drag_mult, v));
+    }
+    if (fm.flags & FORCE_ANTI_COUPLING) {
+        // :126-149: one kick per fluid neighbour closer than the limit; the
+        // list is in (distance, index) order, so the walk stops at the first
+        // neighbour that is not. Every kick of particle i has the same angle
+        // (its generator is re-seeded with i for each neighbour).
+        int const cnt = a.fl.count[t];
+        bool have = false;
+        V2 kick{0.0f, 0.0f};
+        for (int k = 0; k < cnt; ++k) {
+            uint32_t const o = a.fl.idx[static_cast<size_t>(k) * a.stride + t];
+            float const dist = imath_length(sub(a.pos[o], x));
+            if (!(dist < fm.anti_maxd)) { break; }
+            if (!have) {
+                uint64_t const i = a.orig[t] - sd.offset;
+                kick = scale(fm.anti_mag, unit_direction(anti_coupling_angle(i)));
+                have = true;
+            }
+            f = add(f, kick);
+        }
+    }
+    a.fext[t] = f;
+    if (APPLY) {
+        // dynamics.cpp:24-37 integrate_velocities_in_place
+        float const vden = c.rho0 * c.fluid_volume;
+        V2 const numer = scale(c.dt, f);
+        if (is_safe_div(numer, vden)) { v = add(v, divs(numer, vden)); }
+        a.vel[t] = v;
+    }
+}
+
+void launch_external_forces(Batch const& b, int apply, int host_forces, hipStream_t s) {
+    if (!b.ntiles || !b.fext) { return; }
+    if (apply) {
+        hipLaunchKernelGGL(k_external_forces<1>, dim3(b.ntiles), dim3(TILE), 0, s, b, host_forces);
+    } else {
+        hipLaunchKernelGGL(k_external_forces<0>, dim3(b.ntiles), dim3(TILE), 0, s, b, host_forces);
+    }
+}
+
+//------------------------------------------------------------------------------
 // PCISPH: sph2d_box/pcisph.cpp:36-164 and pcisph_ops.cpp.
 
 // :54-88 first predictor: v_ext = v + (dt/m) F_ext; v* = v_ext; x* = x + dt v*;
@@ -1861,7 +1975,7 @@ __global__ __launch_bounds__(TILE) void k_pci_init(Batch a) {
     if (!r.valid) { return; }
     uint32_t const t = r.t;
     StepConsts const c = a.scenes[r.scene].c;
-    V2 const f{0.0f, 0.0f - c.mass * c.gravity};
+    V2 const f = a.fext ? a.fext[t] : V2{0.0f, 0.0f - c.mass * c.gravity};
     float const dtm = c.dt / c.mass;
     V2 const vext = add(a.vel[t], scale(dtm, f));
     a.vext[t] = vext;

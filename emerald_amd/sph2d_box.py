@@ -189,6 +189,70 @@ def dam_break_initial_state(params: Parameters, solid_state: Solid_state) -> Sta
     return State(pos, np.zeros_like(pos))
 
 
+@dataclasses.dataclass
+class ForceModel:
+    """Device external-force models (emsph_force_model; forces.cpp:62-149),
+    accumulated in this order after a host callback: gravity
+    (accumulate_gravity_forces), constant pole attraction, simple drag,
+    anti-coupling repulsion. gravity alone is default_gravity_forces."""
+    gravity_on: bool = True
+    gravity: float = 9.81
+    pole_magnitude: Optional[float] = None   # None: off
+    pole: tuple = (0.5, 0.5)
+    drag_magnitude: Optional[float] = None   # None: off
+    drag_diameter: float = 0.0
+    anti_coupling_max_distance: Optional[float] = None  # None: off
+    anti_coupling_magnitude: float = 0.0
+
+    def to_c(self) -> _capi.ForceModel:
+        m = _capi.ForceModel()
+        m.flags = ((_capi.FORCE_GRAVITY if self.gravity_on else 0)
+                   | (_capi.FORCE_POLE if self.pole_magnitude is not None else 0)
+                   | (_capi.FORCE_DRAG if self.drag_magnitude is not None else 0)
+                   | (_capi.FORCE_ANTI_COUPLING
+                      if self.anti_coupling_max_distance is not None else 0))
+        m.gravity = self.gravity
+        m.pole_magnitude = self.pole_magnitude or 0.0
+        m.pole_x, m.pole_y = self.pole
+        m.drag_magnitude = self.drag_magnitude or 0.0
+        m.drag_diameter = self.drag_diameter
+        m.anti_coupling_max_distance = self.anti_coupling_max_distance or 0.0
+        m.anti_coupling_magnitude = self.anti_coupling_magnitude
+        return m
+
+    @staticmethod
+    def from_c(m: _capi.ForceModel) -> "ForceModel":
+        f = int(m.flags)
+        return ForceModel(
+            gravity_on=bool(f & _capi.FORCE_GRAVITY), gravity=float(m.gravity),
+            pole_magnitude=float(m.pole_magnitude) if f & _capi.FORCE_POLE else None,
+            pole=(float(m.pole_x), float(m.pole_y)),
+            drag_magnitude=float(m.drag_magnitude) if f & _capi.FORCE_DRAG else None,
+            drag_diameter=float(m.drag_diameter),
+            anti_coupling_max_distance=(float(m.anti_coupling_max_distance)
+                                        if f & _capi.FORCE_ANTI_COUPLING else None),
+            anti_coupling_magnitude=float(m.anti_coupling_magnitude))
+
+
+def default_gravity_forces(gravity: float) -> ForceModel:
+    """forces.cpp:72-83: the default User_forces_function, as the device model."""
+    return ForceModel(gravity_on=True, gravity=gravity)
+
+
+def user_forces_callback(fn):
+    """A C callback (emsph_user_forces_fn) around a User_forces_function
+    (forces.h:13-21) written in Python: fn(count, t, dt, mass_per_particle,
+    radius_per_particle, forces, positions, velocities) with (count, 2)
+    float32 views; forces arrives zeroed and fn accumulates into it."""
+    def tramp(_user, count, t, dt, mass, radius, f, p, v):
+        n = int(count)
+        fa = np.ctypeslib.as_array(f, shape=(n, 2))
+        pa = np.ctypeslib.as_array(p, shape=(n, 2))
+        va = np.ctypeslib.as_array(v, shape=(n, 2))
+        fn(n, float(t), float(dt), float(mass), float(radius), fa, pa, va)
+    return _capi.USER_FORCES_FN(tramp)
+
+
 class Context:
     """Owns one device-resident scene (emsph_context)."""
 
@@ -261,6 +325,23 @@ class Context:
         check(lib().emsph_get_neighborhood(self._h, which, _ptr(counts), _ptr(idx), _ptr(dist),
                                            _ptr(vec)), "get_neighborhood")
         return counts, idx, dist, vec
+
+    def set_force_model(self, model: ForceModel) -> None:
+        """Device external-force models (emsph_set_force_model)."""
+        m = model.to_c()
+        check(lib().emsph_set_force_model(self._h, ctypes.byref(m)), "set_force_model")
+
+    def force_model(self) -> ForceModel:
+        m = _capi.ForceModel()
+        check(lib().emsph_get_force_model(self._h, ctypes.byref(m)), "get_force_model")
+        return ForceModel.from_c(m)
+
+    def set_user_forces(self, fn) -> None:
+        """A host User_forces_function (see user_forces_callback), called once
+        per sub-step before the device models; None removes it."""
+        self._user_cb = user_forces_callback(fn) if fn else None
+        check(lib().emsph_set_user_forces(self._h, self._user_cb or _capi.USER_FORCES_FN(),
+                                          None), "set_user_forces")
 
     def frame_init(self, method: str) -> None:
         check(lib().emsph_frame_init(self._h, _capi.METHODS[method]), "frame_init")
@@ -365,7 +446,8 @@ class EZ_EXAMPLE_SIM:  # noqa: N801 - the reference's type name
     """simulation.h:24-44: parameters -> solids -> fluid -> step(method)."""
 
     def __init__(self, params: Parameters, solid_initial_state=world_walls_initial_solid_state,
-                 fluid_initial_state=dam_break_initial_state, config: Simulation_config = None):
+                 fluid_initial_state=dam_break_initial_state, config: Simulation_config = None,
+                 user_forces=None):
         if params.method not in ("pcisph", "iisph", "iisph_ap", "iisph_pseudo_ap", "dfsph_p"):
             # simulation.cpp:37 EMLD_FAIL
             raise ValueError("Unknown simulation method: " + params.method)
@@ -377,6 +459,14 @@ class EZ_EXAMPLE_SIM:  # noqa: N801 - the reference's type name
         self.time = 0  # flicks
         self.ctx = Context(self.config, self.solid_state, max(len(self.state.positions), 1))
         self.ctx.set_state(self.state)
+        # user_forces: default_gravity_forces(params.gravity) when None
+        # (simulation.cpp:14-24); a ForceModel runs on the device, a callable
+        # is a host User_forces_function that replaces gravity.
+        if isinstance(user_forces, ForceModel):
+            self.ctx.set_force_model(user_forces)
+        elif user_forces is not None:
+            self.ctx.set_force_model(ForceModel(gravity_on=False, gravity=params.gravity))
+            self.ctx.set_user_forces(user_forces)
 
     def step(self, override_method: Optional[str] = None) -> _capi.FrameStats:
         fs = self.ctx.frame_step(override_method or self.method, self.time)
@@ -440,6 +530,11 @@ class Batch:
         out = np.empty((self.counts[i], comps), np.float32)
         check(lib().emsph_batch_get_field(self._h, i, fid, _ptr(out)), "batch_get_field")
         return out[:, 0].copy() if comps == 1 else out
+
+    def set_force_model(self, i: int, model: ForceModel) -> None:
+        m = model.to_c()
+        check(lib().emsph_batch_set_force_model(self._h, i, ctypes.byref(m)),
+              "batch_set_force_model")
 
     def set_cfl(self, factor: float) -> None:
         """Per-scene, per-sub-step dt = min(config dt, factor * h / max |v|) (0: off)."""

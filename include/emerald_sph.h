@@ -129,7 +129,10 @@ typedef enum emsph_field {
     /* 2 floats per particle: IISPH_AP pressure accelerations (pressure.cpp:
      * 83-178), IISPH_PSEUDO_AP pseudo-pressure displacements, of the final
      * pressures */
-    EMSPH_FIELD_PRESSURE_ACCELERATION = 10
+    EMSPH_FIELD_PRESSURE_ACCELERATION = 10,
+    /* 2 floats per particle: the external force of the sub-step
+     * (compute_all_external_forces, forces.cpp:11-61) */
+    EMSPH_FIELD_EXTERNAL_FORCE = 11
 } emsph_field;
 
 typedef struct emsph_context emsph_context;
@@ -194,6 +197,53 @@ emsph_status emsph_get_field(emsph_context* ctx, int field, float* out);
 emsph_status emsph_get_neighborhood(emsph_context* ctx, int which, uint8_t* counts,
                                     uint32_t* indices, float* distances, float* vectors_xy);
 
+/* External forces (forces.h:13-21, forces.cpp:11-149). Each sub-step the
+ * reference zeroes a per-particle force array, lets the User_forces_function
+ * accumulate into it (default_gravity_forces by default) and integrates it
+ * into the velocities (IISPH, IISPH_AP, IISPH_PSEUDO_AP: at time t from the
+ * state at the start of the sub-step; DFSPH_P: at t + dt after the divergence
+ * solve; PCISPH: at t into v_ext). Here the force of a particle is built in
+ * this order from zero:
+ *   1. the host callback set by emsph_set_user_forces, if any;
+ *   2. the device force models of emsph_force_model.flags, in flag order:
+ *      EMSPH_FORCE_GRAVITY          accumulate_gravity_forces (forces.cpp:62-69)
+ *      EMSPH_FORCE_POLE             accumulate_constant_pole_attraction_forces (:85-114)
+ *      EMSPH_FORCE_DRAG             accumulate_simple_drag_forces (:116-124)
+ *      EMSPH_FORCE_ANTI_COUPLING    accumulate_anti_coupling_repulsive_forces
+ *                                   (:126-149; over the fluid neighbour list)
+ * The default model is EMSPH_FORCE_GRAVITY with the parameters' gravity and no
+ * callback: the reference's default_gravity_forces. */
+enum {
+    EMSPH_FORCE_GRAVITY = 1,
+    EMSPH_FORCE_POLE = 2,
+    EMSPH_FORCE_DRAG = 4,
+    EMSPH_FORCE_ANTI_COUPLING = 8
+};
+typedef struct emsph_force_model {
+    uint32_t flags;
+    float gravity;                 /* forces.y -= mass * gravity */
+    float pole_magnitude;          /* attraction towards (pole_x, pole_y) plus the swirl */
+    float pole_x, pole_y;
+    float drag_magnitude;          /* forces += (-drag_magnitude * drag_diameter) * v */
+    float drag_diameter;
+    float anti_coupling_max_distance;  /* per fluid neighbour closer than this ... */
+    float anti_coupling_magnitude;     /* ... += magnitude * (cos a_i, sin a_i) */
+} emsph_force_model;
+/* The default force model of a config (gravity from its parameters). */
+emsph_status emsph_default_force_model(const emsph_config* config, emsph_force_model* out);
+emsph_status emsph_set_force_model(emsph_context* ctx, const emsph_force_model* model);
+emsph_status emsph_get_force_model(emsph_context* ctx, emsph_force_model* out);
+/* User_forces_function (forces.h:13-21): called once per sub-step on the
+ * calling thread with host arrays in original particle order; forces_xy is
+ * zeroed before the call and the callback accumulates into it. Honoured by a
+ * device-to-host copy of the state, the call, and a host-to-device copy of the
+ * forces at the point of the sub-step where the reference calls it (slow: two
+ * synchronisations per sub-step, and the sub-step's fixed segments are not
+ * replayed from graphs while a callback is set). fn = NULL removes it. */
+typedef void (*emsph_user_forces_fn)(void* user, size_t count, float time_seconds, float dt,
+                                     float mass_per_particle, float radius_per_particle,
+                                     float* forces_xy, const float* pos_xy, const float* vel_xy);
+emsph_status emsph_set_user_forces(emsph_context* ctx, emsph_user_forces_fn fn, void* user);
 /* Frame-start initialisation of the reference's std_adaptive_time_step and
  * DFSPH temps (constant volumes, zeroed DFSPH stiffness warm starts). */
 emsph_status emsph_frame_init(emsph_context* ctx, int method);
@@ -279,6 +329,10 @@ emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* 
 emsph_status emsph_batch_set_cfl(emsph_batch* b, float factor);  /* as emsph_set_cfl, per scene */
 emsph_status emsph_batch_set_graphs(emsph_batch* b, int enable);  /* as emsph_set_graphs */
 emsph_status emsph_batch_set_tuning(emsph_batch* b, const emsph_tuning* tuning);
+/* Per-scene device force model (as emsph_set_force_model; host callbacks are
+ * for contexts only). */
+emsph_status emsph_batch_set_force_model(emsph_batch* b, int scene,
+                                         const emsph_force_model* model);
 emsph_status emsph_batch_autotune(emsph_batch* b, int method, int trial_sub_steps,
                                   emsph_tuning* chosen, double* times_ms, int* n_candidates);
 emsph_status emsph_batch_frame_init(emsph_batch* b, int method);

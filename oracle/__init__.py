@@ -105,6 +105,38 @@ _U64 = ctypes.c_uint64
 _I64 = ctypes.c_int64
 
 
+# User_forces_function (forces.h:13-21) as a C callback:
+# (user, count, t, dt, mass, radius, forces_xy, pos_xy, vel_xy).
+USER_FORCES_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_float))
+
+FORCE_GRAVITY, FORCE_POLE, FORCE_DRAG, FORCE_ANTI_COUPLING = 1, 2, 4, 8
+
+
+def wrap_user_forces(fn):
+    """A C callback around fn(count, t, dt, mass, radius, forces, pos, vel),
+    where forces/pos/vel are (count, 2) float32 views (forces is written)."""
+    def tramp(_user, count, t, dt, mass, radius, f, p, v):
+        n = int(count)
+        fa = np.ctypeslib.as_array(f, shape=(n, 2))
+        pa = np.ctypeslib.as_array(p, shape=(n, 2))
+        va = np.ctypeslib.as_array(v, shape=(n, 2))
+        fn(n, float(t), float(dt), float(mass), float(radius), fa, pa, va)
+    return USER_FORCES_FN(tramp)
+
+
+def anti_coupling_angle(i: int) -> float:
+    """forces.cpp:134-141: the angle drawn for particle i (host C++ library)."""
+    return float(lib().orc_anti_coupling_angle(int(i)))
+
+
+def lehmer_first_draw(seed: int) -> int:
+    """util/random.h:112-134: the first draw of Lehmer_rand_gen_64{seed}."""
+    return int(lib().orc_lehmer_first_draw(int(seed)))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -139,6 +171,9 @@ def lib():
             "orc_scene_get_state_f32": (None, [_P, _P, _P]),
             "orc_scene_get_neighborhood": (None, [_P, _I, _P, _P]),
             "orc_fixture_points_mt19937_64": (None, [_U64, _SZ, _F, _F, _P]),
+            "orc_anti_coupling_angle": (_F, [_U64]),
+            "orc_lehmer_first_draw": (_U64, [_U64]),
+            "orc_scene_set_forces": (_I, [_P, _U32, _P, USER_FORCES_FN, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -261,6 +296,20 @@ class Scene:
         if h:
             lib().orc_scene_destroy(h)
             self._h = None
+
+    def set_forces(self, flags: int = FORCE_GRAVITY, gravity: float = 9.81,
+                   pole_magnitude: float = 0.0, pole=(0.0, 0.0), drag_magnitude: float = 0.0,
+                   drag_diameter: float = 0.0, anti_coupling_max_distance: float = 0.0,
+                   anti_coupling_magnitude: float = 0.0, draw_radius: float = 0.0,
+                   user_forces=None) -> None:
+        """External force model (forces.cpp:11-149): user_forces (a Python
+        callable, see wrap_user_forces) first, then the flagged models."""
+        v = np.array([gravity, pole_magnitude, pole[0], pole[1], drag_magnitude, drag_diameter,
+                      anti_coupling_max_distance, anti_coupling_magnitude, draw_radius],
+                     np.float64)
+        self._user_cb = wrap_user_forces(user_forces) if user_forces else USER_FORCES_FN()
+        if lib().orc_scene_set_forces(self._h, int(flags), _ptr(v), self._user_cb, None):
+            raise ValueError("user force callbacks need a float (not fp64) scene")
 
     def frame_init(self, method: str) -> None:
         lib().orc_scene_frame_init(self._h, METHODS[method])

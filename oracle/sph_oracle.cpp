@@ -352,6 +352,11 @@ void compute_neighbor_kernels(size_t n, T h, Neighborhood<T>& nb) {
 
 //------------------------------------------------------------------------------
 // Parameters (sph2d_box/parameters.h:12-43) and Simulation_config (config.h).
+// User_forces_function (forces.h:13-21) as a C callback on float arrays:
+// (user, count, t, dt, mass, radius, forces_xy, pos_xy, vel_xy).
+using UserForces = void (*)(void*, size_t, float, float, float, float, float*, float const*,
+                            float const*);
+
 template <class T>
 struct Config {
     T support = T(0.025);
@@ -372,6 +377,15 @@ struct Config {
     T seconds_per_sub_step = T(1);
     T mass_per_particle = T(1);
     T pressure_correction_denom = T(1);
+    T draw_radius = T(0);
+    // External forces (forces.cpp:11-149): an optional user callback, then the
+    // built-in models of `force_flags` in flag order (see external_forces).
+    uint32_t force_flags = 1;
+    T pole_magnitude = T(0), pole_x = T(0), pole_y = T(0);
+    T drag_magnitude = T(0), drag_diameter = T(0);
+    T anti_max_distance = T(0), anti_magnitude = T(0);
+    UserForces user_forces = nullptr;
+    void* user_data = nullptr;
 };
 
 template <class T>
@@ -582,13 +596,93 @@ void accumulate(size_t n, T mult, V2<T>* dst, V2<T> const* src) {
     for (size_t i = 0; i < n; ++i) { dst[i] += mult * src[i]; }
 }
 
-// forces.cpp:11-61 with default_gravity_forces (forces.cpp:72-83):
-// zeroed forces, then forces[i][1] -= m * g.
+// util/random.h:112-134 Lehmer_rand_gen_64: a 128-bit state multiplied by
+// 0xda942042e4dd58b5 twice at seeding and once per draw, which returns the high
+// 64 bits; range [0, 2^64 - 1].
+struct Lehmer64 {
+    using result_type = uint64_t;
+    static constexpr result_type min() { return 0; }
+    static constexpr result_type max() { return std::numeric_limits<uint64_t>::max(); }
+    explicit Lehmer64(uint64_t seed) : state(seed) {
+        state *= 0xda942042e4dd58b5ull;
+        state *= 0xda942042e4dd58b5ull;
+    }
+    uint64_t operator()() {
+        state *= 0xda942042e4dd58b5ull;
+        return static_cast<uint64_t>(state >> 64);
+    }
+    unsigned __int128 state;
+};
+
+// The angle forces.cpp:134-141 draws for particle i: a generator seeded with i
+// and std::uniform_real_distribution<float>{-pi, pi} (the host C++ library's,
+// as in the reference).
+inline float anti_coupling_angle(uint64_t i) {
+    std::uniform_real_distribution<float> angle_dist{static_cast<float>(-M_PI),
+                                                     static_cast<float>(M_PI)};
+    Lehmer64 gen{i};
+    return angle_dist(gen);
+}
+
+// forces.cpp:11-61 compute_all_external_forces: zeroed forces; the user
+// callback (default_gravity_forces, forces.cpp:72-83, unless replaced); then
+// the built-in models of cfg.force_flags in this order: gravity
+// (accumulate_gravity_forces :62-69), constant pole attraction (:85-114),
+// simple drag (:116-124), anti-coupling repulsion over the fluid neighbour
+// distances (:126-149). Flag 1 alone is the reference's default.
 template <class T>
-void gravity_forces(size_t n, T mass, T g, V2<T>* f) {
-    for (size_t i = 0; i < n; ++i) {
-        f[i] = V2<T>{T(0), T(0)};
-        f[i].y -= mass * g;
+void external_forces(size_t n, T t, T dt, Config<T> const& cfg, V2<T> const* pos,
+                     V2<T> const* vel, Neighborhood<T> const& nb, V2<T>* f) {
+    for (size_t i = 0; i < n; ++i) { f[i] = V2<T>{T(0), T(0)}; }
+    if (cfg.user_forces) {
+        std::vector<float> fx(2 * n, 0.0f), px(2 * n), vx(2 * n);
+        for (size_t i = 0; i < n; ++i) {
+            px[2 * i] = static_cast<float>(pos[i].x);
+            px[2 * i + 1] = static_cast<float>(pos[i].y);
+            vx[2 * i] = static_cast<float>(vel[i].x);
+            vx[2 * i + 1] = static_cast<float>(vel[i].y);
+        }
+        cfg.user_forces(cfg.user_data, n, static_cast<float>(t), static_cast<float>(dt),
+                        static_cast<float>(cfg.mass_per_particle),
+                        static_cast<float>(cfg.draw_radius), fx.data(), px.data(), vx.data());
+        for (size_t i = 0; i < n; ++i) { f[i] = V2<T>{T(fx[2 * i]), T(fx[2 * i + 1])}; }
+    }
+    uint32_t const flags = cfg.force_flags;
+    if (flags & 1u) {
+        T const mg = cfg.mass_per_particle * cfg.gravity;
+        for (size_t i = 0; i < n; ++i) { f[i].y -= mg; }
+    }
+    if (flags & 2u) {
+        T const mag = cfg.pole_magnitude;
+        V2<T> const pole{cfg.pole_x, cfg.pole_y};
+        T const tz = T(0.125f) * mag;  // torque (0, 0, 0.125 magnitude)
+        for (size_t i = 0; i < n; ++i) {
+            V2<T> const r = pos[i] - pole;
+            // Imath Vec2::normalized: zero when the length is zero.
+            T const l = r.length();
+            V2<T> const rn = (l == T(0)) ? V2<T>{T(0), T(0)} : V2<T>{r.x / l, r.y / l};
+            f[i] -= mag * rn;
+            // Imath Vec3::cross of the torque with (rn, 0), components x and y.
+            V2<T> const turn{T(0) * T(0) - tz * rn.y, tz * rn.x - T(0) * T(0)};
+            f[i] += turn;
+        }
+    }
+    if (flags & 4u) {
+        T const mult = -cfg.drag_magnitude * cfg.drag_diameter;
+        for (size_t i = 0; i < n; ++i) { f[i] += mult * vel[i]; }
+    }
+    if (flags & 8u) {
+        for (size_t i = 0; i < n; ++i) {
+            int const c = nb.counts[i];
+            if (!c) { continue; }
+            T const* d = &nb.distances[i * MAXN];
+            for (int j = 0; j < c; ++j) {
+                if (!(d[j] < cfg.anti_max_distance)) { break; }
+                float const angle = anti_coupling_angle(i);
+                f[i] += cfg.anti_magnitude * V2<T>{static_cast<T>(std::cos(angle)),
+                                                   static_cast<T>(std::sin(angle))};
+            }
+        }
     }
 }
 
@@ -812,13 +906,13 @@ std::pair<T, T> iisph_compute_new_pressures(size_t n, T omega, T rho0, T* beta_i
 //------------------------------------------------------------------------------
 // iisph.cpp:23-151 iisph_sub_step
 template <class T>
-void iisph_sub_step(T /*t*/, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
+void iisph_sub_step(T t, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
                     std::vector<V2<T>>& vel, Solids<T> const& s, Temp<T>& tp, Stats* st) {
     size_t const n = pos.size();
     T const rho0 = cfg.target_density;
     compute_all_neighborhoods(cfg, pos, s, tp);
     tp.external_forces.resize(n);
-    gravity_forces(n, cfg.mass_per_particle, cfg.gravity, tp.external_forces.data());
+    external_forces(n, t, dt, cfg, pos.data(), vel.data(), tp.nb, tp.external_forces.data());
     tp.densities.resize(n);
     compute_densities(n, cfg.support, rho0, tp.densities.data(), tp.fluid_volumes.data(),
                       tp.nb, s.volumes.data(), tp.snb);
@@ -960,13 +1054,13 @@ void pcisph_accumulate_pressure_forces_from_solids(size_t n, T mass, T rho0, V2<
 
 // pcisph.cpp:36-164 pcisph_sub_step
 template <class T>
-void pcisph_sub_step(T /*t*/, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
+void pcisph_sub_step(T t, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
                      std::vector<V2<T>>& vel, Solids<T> const& s, Temp<T>& tp, Stats* st) {
     size_t const n = pos.size();
     T const m = cfg.mass_per_particle;
     compute_all_neighborhoods(cfg, pos, s, tp);
     tp.external_forces.resize(n);
-    gravity_forces(n, m, cfg.gravity, tp.external_forces.data());
+    external_forces(n, t, dt, cfg, pos.data(), vel.data(), tp.nb, tp.external_forces.data());
     tp.velocity_external_forces = vel;
     accumulate(n, dt / m, tp.velocity_external_forces.data(), tp.external_forces.data());
     tp.pressures.assign(n, T(0));
@@ -1314,13 +1408,13 @@ std::pair<T, T> iisph_ap_iterate_pressures_in_place(size_t n, bool pseudo, T dt,
 // (pseudo = true; its integration, iisph_pseudo_ap_ops.cpp:307-341, moves by
 // the displacement and adds it over dt to the velocity).
 template <class T>
-void iisph_ap_sub_step(bool pseudo, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
+void iisph_ap_sub_step(bool pseudo, T t, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
                        std::vector<V2<T>>& vel, Solids<T> const& s, Temp<T>& tp, Stats* st) {
     size_t const n = pos.size();
     T const rho0 = cfg.target_density;
     compute_all_neighborhoods(cfg, pos, s, tp);
     tp.external_forces.resize(n);
-    gravity_forces(n, cfg.mass_per_particle, cfg.gravity, tp.external_forces.data());
+    external_forces(n, t, dt, cfg, pos.data(), vel.data(), tp.nb, tp.external_forces.data());
     tp.densities.resize(n);
     compute_densities(n, cfg.support, rho0, tp.densities.data(), tp.fluid_volumes.data(),
                       tp.nb, s.volumes.data(), tp.snb);
@@ -1374,7 +1468,7 @@ void iisph_ap_sub_step(bool pseudo, T dt, Config<T> const& cfg, std::vector<V2<T
 
 // dfsph_p.cpp:265-374 (active #else branch :325-373)
 template <class T>
-void dfsph_p_sub_step(T /*t*/, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
+void dfsph_p_sub_step(T t, T dt, Config<T> const& cfg, std::vector<V2<T>>& pos,
                       std::vector<V2<T>>& vel, Solids<T> const& s, Temp<T>& tp, Stats* st) {
     size_t const n = pos.size();
     T const rho0 = cfg.target_density;
@@ -1388,7 +1482,9 @@ void dfsph_p_sub_step(T /*t*/, T dt, Config<T> const& cfg, std::vector<V2<T>>& p
     tp.density_stars.resize(n);
     dfsph_p_correct_divergence_error(dt, cfg, vel, s, tp, st);
     tp.external_forces.resize(n);
-    gravity_forces(n, cfg.mass_per_particle, cfg.gravity, tp.external_forces.data());
+    // dfsph_p.cpp:350: the forces at t + dt.
+    external_forces(n, t + dt, dt, cfg, pos.data(), vel.data(), tp.nb,
+                    tp.external_forces.data());
     integrate_velocities_forces(n, dt, rho0, vel.data(), tp.fluid_volumes.data(),
                                 tp.external_forces.data());
     dfsph_p_correct_density_error(dt, cfg, vel, s, tp, st);
@@ -1429,9 +1525,11 @@ void sub_step(Scene<T>& sc, int method, T t, T dt) {
     case IISPH: iisph_sub_step(t, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st); break;
     case PCISPH: pcisph_sub_step(t, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st); break;
     case DFSPH_P: dfsph_p_sub_step(t, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st); break;
-    case IISPH_AP: iisph_ap_sub_step(false, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st); break;
+    case IISPH_AP:
+        iisph_ap_sub_step(false, t, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st);
+        break;
     case IISPH_PSEUDO_AP:
-        iisph_ap_sub_step(true, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st);
+        iisph_ap_sub_step(true, t, dt, sc.cfg, sc.pos, sc.vel, sc.solids, sc.tmp, &st);
         break;
     default: break;
     }
@@ -1567,6 +1665,8 @@ uint64_t orc_z_index(uint32_t x, uint32_t y) { return z_index(x, y); }
 uint64_t orc_single_axis_index_2(uint32_t i) { return single_axis_index_2(i); }
 
 float orc_length(float x, float y) { return V2<float>{x, y}.length(); }
+float orc_anti_coupling_angle(uint64_t i) { return anti_coupling_angle(i); }
+uint64_t orc_lehmer_first_draw(uint64_t seed) { Lehmer64 g{seed}; return g(); }
 float orc_W(float r, float h) { return K<float>::W(r, h); }
 float orc_dW(float r, float h) { return K<float>::dW(r, h); }
 void orc_GradW(float x, float y, float h, float* out) {
@@ -1721,6 +1821,36 @@ void* orc_scene_create(OrcConfig const* cfg, size_t n_solid, float const* solid_
     return s;
 }
 void orc_scene_destroy(void* p) { delete static_cast<OrcScene*>(p); }
+
+// External force model of a scene: flags (1 gravity, 2 pole, 4 drag, 8
+// anti-coupling), v = {gravity, pole magnitude, pole x, pole y, drag
+// magnitude, drag diameter, anti-coupling max distance, magnitude, draw
+// radius}, and an optional user callback (float scenes only).
+extern "C++" template <class T>
+void set_forces(Config<T>& c, uint32_t flags, double const* v, UserForces fn, void* user) {
+    c.force_flags = flags;
+    c.gravity = static_cast<T>(static_cast<float>(v[0]));
+    c.pole_magnitude = static_cast<T>(static_cast<float>(v[1]));
+    c.pole_x = static_cast<T>(static_cast<float>(v[2]));
+    c.pole_y = static_cast<T>(static_cast<float>(v[3]));
+    c.drag_magnitude = static_cast<T>(static_cast<float>(v[4]));
+    c.drag_diameter = static_cast<T>(static_cast<float>(v[5]));
+    c.anti_max_distance = static_cast<T>(static_cast<float>(v[6]));
+    c.anti_magnitude = static_cast<T>(static_cast<float>(v[7]));
+    c.draw_radius = static_cast<T>(static_cast<float>(v[8]));
+    c.user_forces = fn;
+    c.user_data = user;
+}
+int orc_scene_set_forces(void* p, uint32_t flags, double const* v, UserForces fn, void* user) {
+    auto* s = static_cast<OrcScene*>(p);
+    if (s->fp64) {
+        if (fn) { return 1; }
+        set_forces(s->d.cfg, flags, v, fn, user);
+    } else {
+        set_forces(s->f.cfg, flags, v, fn, user);
+    }
+    return 0;
+}
 void orc_set_threads(int n) { omp_set_num_threads(n); }
 int orc_get_threads() { return omp_get_max_threads(); }
 
@@ -1798,6 +1928,7 @@ int get_field(Scene<T>& sc, char const* name, double* out) {
     if (s == "kappa_divergence") { return sc1(sc.tmp.divergence_kappas); }
     if (s == "pressure_force") { return v2(sc.tmp.pressure_forces); }
     if (s == "pressure_acceleration") { return v2(sc.tmp.pressure_accelerations); }
+    if (s == "external_force") { return v2(sc.tmp.external_forces); }
     return -1;
 }
 int orc_scene_get_field(void* p, char const* name, double* out) {
