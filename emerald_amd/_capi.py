@@ -127,6 +127,20 @@ USER_FORCES_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes
                                   ctypes.POINTER(ctypes.c_float))
 
 
+# emsph_user_colors_fn: (user, count, t, mass, radius, colors_rgba, pos_xy, vel_xy, densities).
+USER_COLORS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_float))
+
+
+class SnapshotHeader(ctypes.Structure):
+    """emsph_snapshot_header."""
+    _fields_ = [("count", ctypes.c_uint64), ("time_flicks", ctypes.c_int64),
+                ("method", ctypes.c_int32), ("flags", ctypes.c_uint32),
+                ("bytes", ctypes.c_uint64)]
+
+
 class FrameStats(ctypes.Structure):
     _fields_ = [
         ("sub_steps", ctypes.c_int32),
@@ -171,6 +185,10 @@ SIGNATURES = {
     "emsph_get_force_model": (_ST, [_P, _P]),
     "emsph_set_user_forces": (_ST, [_P, USER_FORCES_FN, _P]),
     "emsph_batch_set_force_model": (_ST, [_P, _I, _P]),
+    "emsph_set_user_colors": (_ST, [_P, USER_COLORS_FN, _P]),
+    "emsph_compute_colors": (_ST, [_P, _F]),
+    "emsph_get_colors": (_ST, [_P, _P]),
+    "emsph_get_colors_u8": (_ST, [_P, _P]),
     "emsph_frame_init": (_ST, [_P, _I]),
     "emsph_sub_step": (_ST, [_P, _I, _F, _F, _P]),
     "emsph_frame_step": (_ST, [_P, _I, _I64, _P]),
@@ -207,6 +225,14 @@ SIGNATURES = {
     "emsph_pool_sub_step": (_ST, [_P, _P]),
     "emsph_pool_get_state": (_ST, [_P, _I, _P, _P]),
     "emsph_device_bytes": (_I64, []),
+    "emsph_snapshot_bytes": (_SZ, [_SZ, ctypes.c_uint32]),
+    "emsph_snapshot_encode": (_ST, [_P, _P, _P, _SZ, _I64, ctypes.c_int32, _P, _SZ, _P]),
+    "emsph_snapshot_info": (_ST, [_P, _SZ, _P]),
+    "emsph_snapshot_decode": (_ST, [_P, _SZ, _P, _P, _P]),
+    "emsph_snapshot_write_file": (_ST, [ctypes.c_char_p, _P, _SZ]),
+    "emsph_save_snapshot": (_ST, [_P, ctypes.c_char_p, _I64, ctypes.c_int32]),
+    "emsph_load_snapshot": (_ST, [_P, ctypes.c_char_p, _P, _P]),
+    "emsph_pool_save_snapshot": (_ST, [_P, _I, ctypes.c_char_p, _I64]),
     "emsph_search_create": (_ST, [_SZ, _P]),
     "emsph_search_destroy": (_ST, [_P]),
     "emsph_search_set_points": (_ST, [_P, _P, _SZ]),

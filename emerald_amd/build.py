@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libemerald_sph.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["engine.hip", "sph_kernels.hip", "sort.hip", "host_scene.cpp"]
+SOURCES = ["engine.hip", "sph_kernels.hip", "sort.hip", "host_scene.cpp", "snapshot.cpp"]
 HEADERS = ["device_math.hpp", "kernels.hpp"]
 
 FLAGS = [

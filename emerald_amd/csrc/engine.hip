@@ -46,6 +46,14 @@ emsph_status fail(emsph_status s, std::string const& msg) {
     return s;
 }
 
+}  // namespace
+
+// Error reporting for the library's host-only translation units.
+emsph_status emsph_set_error(emsph_status s, char const* msg) { return fail(s, msg); }
+bool emsph_snapshot_read_file(char const* path, std::vector<unsigned char>& out);
+
+namespace {
+
 #define HIP_TRY(expr)                                                                      \
     do {                                                                                   \
         hipError_t const e_ = (expr);                                                      \
@@ -372,6 +380,14 @@ struct Engine {
     bool use_fext = false;
     DBuf<V2> fext;
     float step_time = 0.0f;  // time of the current sub-step (for the callback)
+    // State::colors of a context: computed at the end of each frame step (or
+    // on request), RGBA in original order; a host User_colors_function
+    // replaces the default target-density colors.
+    emsph_user_colors_fn colors_fn = nullptr;
+    void* colors_data = nullptr;
+    DBuf<float4> colors;
+    DBuf<uchar4> colors8;
+    bool colors_valid = false;
     struct {
         std::vector<V2> spos, svel, fs;
         std::vector<uint32_t> orig;
@@ -2033,6 +2049,12 @@ emsph_status emsph_frame_step(emsph_context* c, int method, int64_t global_time_
         }
     }
     local.last = st;
+    // adaptive_time_step.h:105-110 / pcisph.cpp:186-192: colors at the frame's
+    // end time.
+    int64_t const end_time = global_time_flicks + (method == EMSPH_PCISPH
+        ? (P.time_per_step / P.sub_steps) * P.sub_steps
+        : (P.time_per_step / 60) * 60);
+    if (c->e.N) { ST_TRY(emsph_compute_colors(c, static_cast<float>(to_seconds(end_time)))); }
     if (fs) { *fs = local; }
     return EMSPH_OK;
 }
@@ -2114,6 +2136,81 @@ emsph_status emsph_batch_set_tuning(emsph_batch* b, const emsph_tuning* t) {
 emsph_status emsph_pool_set_tuning(emsph_pool* P, const emsph_tuning* t) {
     if (!P || !t) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "bad argument"); }
     return tuning_set(&P->e, t);
+}
+
+//------------------------------------------------------------------------------
+// Colors (colors.cpp:9-70; computed where adaptive_time_step.h:105-110 and
+// pcisph.cpp:186-192 call compute_colors).
+
+namespace {
+
+emsph_status engine_compute_colors(Engine* g, float time_seconds) {
+    SceneHost const& h = g->sh[0];
+    if (!h.n) {
+        g->colors_valid = true;
+        return EMSPH_OK;
+    }
+    if (!g->have_lists) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no sub-step yet"); }
+    if (!g->colors.p) { HIP_TRY(g->colors.alloc(g->cap)); }
+    if (g->colors_fn) {
+        // compute_colors (colors.cpp:22-40): the callback gets the state and the
+        // densities in original order on the host.
+        size_t const n = h.n;
+        std::vector<float> pos(2 * n), vel(2 * n), rho(n), rgba(4 * n, 0.0f);
+        ST_TRY(engine_get_state(g, 0, pos.data(), vel.data()));
+        ST_TRY(engine_get_field(g, 0, EMSPH_FIELD_DENSITY, rho.data()));
+        g->colors_fn(g->colors_data, n, time_seconds, h.cfg.mass_per_particle, h.cfg.draw_radius,
+                     rgba.data(), pos.data(), vel.data(), rho.data());
+        HIP_TRY(hipMemcpyAsync(g->colors.p, rgba.data(), n * sizeof(float4),
+                               hipMemcpyHostToDevice, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    } else {
+        launch_colors(fluid_batch(g), g->colors.p, g->stream);
+        HIP_TRY(hipGetLastError());
+    }
+    g->colors_valid = true;
+    return EMSPH_OK;
+}
+
+}  // namespace
+
+emsph_status emsph_set_user_colors(emsph_context* c, emsph_user_colors_fn fn, void* user) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    c->e.colors_fn = fn;
+    c->e.colors_data = fn ? user : nullptr;
+    return EMSPH_OK;
+}
+
+emsph_status emsph_compute_colors(emsph_context* c, float time_seconds) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    HIP_TRY(hipSetDevice(c->e.device));
+    return engine_compute_colors(&c->e, time_seconds);
+}
+
+emsph_status emsph_get_colors(emsph_context* c, float* rgba) {
+    if (!c || !rgba) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    Engine* g = &c->e;
+    if (!g->colors_valid) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no colors computed yet"); }
+    if (!g->sh[0].n) { return EMSPH_OK; }
+    HIP_TRY(hipMemcpyAsync(rgba, g->colors.p, g->sh[0].n * sizeof(float4), hipMemcpyDeviceToHost,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
+}
+
+emsph_status emsph_get_colors_u8(emsph_context* c, uint8_t* rgba) {
+    if (!c || !rgba) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    Engine* g = &c->e;
+    if (!g->colors_valid) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no colors computed yet"); }
+    uint32_t const n = g->sh[0].n;
+    if (!n) { return EMSPH_OK; }
+    if (!g->colors8.p) { HIP_TRY(g->colors8.alloc(g->cap)); }
+    launch_convert_colors(g->colors.p, g->colors8.p, n, g->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(rgba, g->colors8.p, n * sizeof(uchar4), hipMemcpyDeviceToHost,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return EMSPH_OK;
 }
 
 //------------------------------------------------------------------------------
@@ -2620,6 +2717,64 @@ emsph_status emsph_pool_get_state(emsph_pool* P, int id, float* pos_xy, float* v
     int const k = pool_index(P, id);
     if (k < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no such live scene"); }
     return engine_get_state(&P->e, static_cast<uint32_t>(k), pos_xy, vel_xy);
+}
+
+//------------------------------------------------------------------------------
+// Snapshots of device state (format and encoding in snapshot.cpp).
+
+namespace {
+
+emsph_status save_state(Engine* g, uint32_t s, const char* path, int64_t time_flicks,
+                        int32_t method, bool with_colors) {
+    if (!path) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null path"); }
+    size_t const n = g->sh[s].n;
+    std::vector<float> pos(2 * n), vel(2 * n), rgba;
+    ST_TRY(engine_get_state(g, s, pos.data(), vel.data()));
+    if (with_colors && n) {
+        rgba.resize(4 * n);
+        HIP_TRY(hipMemcpyAsync(rgba.data(), g->colors.p, n * sizeof(float4),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    std::vector<unsigned char> buf(emsph_snapshot_bytes(n, with_colors ? 3u : 1u));
+    size_t written = 0;
+    ST_TRY(emsph_snapshot_encode(pos.data(), vel.data(), with_colors ? rgba.data() : nullptr, n,
+                                 time_flicks, method, buf.data(), buf.size(), &written));
+    return emsph_snapshot_write_file(path, buf.data(), written);
+}
+
+}  // namespace
+
+emsph_status emsph_save_snapshot(emsph_context* c, const char* path, int64_t time_flicks,
+                                 int32_t method) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    return save_state(&c->e, 0, path, time_flicks, method, c->e.colors_valid);
+}
+
+emsph_status emsph_load_snapshot(emsph_context* c, const char* path, int64_t* time_flicks,
+                                 int32_t* method) {
+    if (!c || !path) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    std::vector<unsigned char> buf;
+    if (!emsph_snapshot_read_file(path, buf)) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, std::string("cannot read ") + path);
+    }
+    emsph_snapshot_header h{};
+    ST_TRY(emsph_snapshot_info(buf.data(), buf.size(), &h));
+    size_t const n = static_cast<size_t>(h.count);
+    std::vector<float> pos(2 * n), vel(2 * n);
+    ST_TRY(emsph_snapshot_decode(buf.data(), buf.size(), pos.data(), vel.data(), nullptr));
+    ST_TRY(engine_set_state(&c->e, 0, pos.data(), vel.data(), n));
+    if (time_flicks) { *time_flicks = h.time_flicks; }
+    if (method) { *method = h.method; }
+    return EMSPH_OK;
+}
+
+emsph_status emsph_pool_save_snapshot(emsph_pool* P, int id, const char* path,
+                                      int64_t time_flicks) {
+    if (!P) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null pool"); }
+    int const k = pool_index(P, id);
+    if (k < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no such live scene"); }
+    return save_state(&P->e, static_cast<uint32_t>(k), path, time_flicks, P->method, false);
 }
 
 }  // extern "C"

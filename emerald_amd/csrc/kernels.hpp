@@ -376,6 +376,11 @@ void launch_finalize(Batch const& b, int dfsph, hipStream_t s);
 void launch_dfsph_div(Batch const& b, hipStream_t s);
 void launch_dfsph_dens(Batch const& b, hipStream_t s);
 void launch_pci_init(Batch const& b, hipStream_t s);
+// colors.cpp:9-20 target_density_colors from the densities of the last
+// sub-step, RGBA per particle in original order.
+void launch_colors(Batch const& b, float4* out, hipStream_t s);
+// colors.cpp:57-70 convert_colors.
+void launch_convert_colors(float4 const* in, uchar4* out, uint32_t n, hipStream_t s);
 // External forces of every fluid slot into b.fext (forces.cpp:11-149): the
 // host-supplied forces already in b.fext (host_forces != 0) or zero, then each
 // scene's force models. apply != 0 (IISPH, IISPH_AP, IISPH_PSEUDO_AP): instead

@@ -1966,6 +1966,46 @@ void launch_external_forces(Batch const& b, int apply, int host_forces, hipStrea
 }
 
 //------------------------------------------------------------------------------
+// Colors (sph2d_box/colors.cpp:9-70).
+
+// std::clamp(v, lo, hi) as the host library writes it (NaN passes through).
+__device__ __forceinline__ float clamp01(float v) {
+    return (v < 0.0f) ? 0.0f : ((1.0f < v) ? 1.0f : v);
+}
+
+// :9-20 c = clamp(1 - rho / (1.1 rho0), 0, 1); RGBA (c, c, 1, 1).
+__global__ __launch_bounds__(TILE) void k_colors(Batch a, float4* out) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!r.valid) { return; }
+    uint32_t const t = r.t;
+    float const rho0 = a.scenes[r.scene].c.rho0;
+    float const norm_d = a.rho[t] / (1.1f * rho0);
+    float const c = clamp01(1.0f - norm_d);
+    out[a.orig[t]] = make_float4(c, c, 1.0f, 1.0f);
+}
+
+// :57-70 each channel static_cast<uint8_t>(255 * clamp(f, 0, 1)).
+__global__ void k_convert_colors(float4 const* in, uchar4* out, uint32_t n) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    float4 const c = in[i];
+    out[i] = make_uchar4(static_cast<unsigned char>(255.0f * clamp01(c.x)),
+                         static_cast<unsigned char>(255.0f * clamp01(c.y)),
+                         static_cast<unsigned char>(255.0f * clamp01(c.z)),
+                         static_cast<unsigned char>(255.0f * clamp01(c.w)));
+}
+
+void launch_colors(Batch const& b, float4* out, hipStream_t s) {
+    if (!b.ntiles) { return; }
+    hipLaunchKernelGGL(k_colors, dim3(b.ntiles), dim3(TILE), 0, s, b, out);
+}
+
+void launch_convert_colors(float4 const* in, uchar4* out, uint32_t n, hipStream_t s) {
+    if (!n) { return; }
+    hipLaunchKernelGGL(k_convert_colors, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n);
+}
+
+//------------------------------------------------------------------------------
 // PCISPH: sph2d_box/pcisph.cpp:36-164 and pcisph_ops.cpp.
 
 // :54-88 first predictor: v_ext = v + (dt/m) F_ext; v* = v_ext; x* = x + dt v*;

@@ -146,9 +146,10 @@ class Simulation_config:  # noqa: N801 - the reference's type name
 
 @dataclasses.dataclass
 class State:
-    """state.h:13-17 (colors are display-only and not carried)."""
+    """state.h:13-17."""
     positions: np.ndarray   # [N, 2] float32
     velocities: np.ndarray  # [N, 2] float32
+    colors: Optional[np.ndarray] = None  # [N, 4] float32 RGBA, set by a frame step
 
 
 @dataclasses.dataclass
@@ -253,6 +254,63 @@ def user_forces_callback(fn):
     return _capi.USER_FORCES_FN(tramp)
 
 
+def user_colors_callback(fn):
+    """A C callback (emsph_user_colors_fn) around a User_colors_function
+    (colors.h:13-21) written in Python: fn(count, t, mass_per_particle,
+    radius_per_particle, colors, positions, velocities, densities) with
+    (count, 4) float32 colors (written), (count, 2) state and (count,)
+    densities."""
+    def tramp(_user, count, t, mass, radius, c, p, v, d):
+        n = int(count)
+        fn(n, float(t), float(mass), float(radius), np.ctypeslib.as_array(c, shape=(n, 4)),
+           np.ctypeslib.as_array(p, shape=(n, 2)), np.ctypeslib.as_array(v, shape=(n, 2)),
+           np.ctypeslib.as_array(d, shape=(n,)))
+    return _capi.USER_COLORS_FN(tramp)
+
+
+def encode_snapshot(state: State, time_flicks: int = 0, method: Optional[str] = None) -> bytes:
+    """A State (and its colors, if set) in the snapshot format of
+    emsph_snapshot_encode. No counterpart in the reference."""
+    pos = np.ascontiguousarray(state.positions, np.float32).reshape(-1, 2)
+    vel = None if state.velocities is None else np.ascontiguousarray(
+        state.velocities, np.float32).reshape(-1, 2)
+    col = None if state.colors is None else np.ascontiguousarray(
+        state.colors, np.float32).reshape(-1, 4)
+    flags = (1 if vel is not None else 0) | (2 if col is not None else 0)
+    n = len(pos)
+    buf = ctypes.create_string_buffer(lib().emsph_snapshot_bytes(n, flags))
+    written = ctypes.c_size_t()
+    check(lib().emsph_snapshot_encode(_ptr(pos), _ptr(vel), _ptr(col), n, int(time_flicks),
+                                      _capi.METHODS.get(method, -1), buf, len(buf),
+                                      ctypes.byref(written)), "snapshot_encode")
+    return buf.raw[:written.value]
+
+
+def decode_snapshot(data: bytes):
+    """(State, time in flicks, method name or None) from snapshot bytes."""
+    h = _capi.SnapshotHeader()
+    check(lib().emsph_snapshot_info(data, len(data), ctypes.byref(h)), "snapshot_info")
+    n = int(h.count)
+    pos = np.empty((n, 2), np.float32)
+    vel = np.empty((n, 2), np.float32)
+    col = np.empty((n, 4), np.float32) if h.flags & 2 else None
+    check(lib().emsph_snapshot_decode(data, len(data), _ptr(pos), _ptr(vel), _ptr(col)),
+          "snapshot_decode")
+    names = {v: k for k, v in _capi.METHODS.items()}
+    return State(pos, vel, col), int(h.time_flicks), names.get(int(h.method))
+
+
+def save_state(path: str, state: State, time_flicks: int = 0,
+               method: Optional[str] = None) -> None:
+    data = encode_snapshot(state, time_flicks, method)
+    check(lib().emsph_snapshot_write_file(path.encode(), data, len(data)), "snapshot_write")
+
+
+def load_state(path: str):
+    with open(path, "rb") as f:
+        return decode_snapshot(f.read())
+
+
 class Context:
     """Owns one device-resident scene (emsph_context)."""
 
@@ -343,6 +401,42 @@ class Context:
         check(lib().emsph_set_user_forces(self._h, self._user_cb or _capi.USER_FORCES_FN(),
                                           None), "set_user_forces")
 
+    def save_snapshot(self, path: str, time_flicks: int = 0,
+                      method: Optional[str] = None) -> None:
+        check(lib().emsph_save_snapshot(self._h, path.encode(), int(time_flicks),
+                                        _capi.METHODS.get(method, -1)), "save_snapshot")
+
+    def load_snapshot(self, path: str):
+        """Loads the state; returns (time in flicks, method name or None)."""
+        t = ctypes.c_int64()
+        m = ctypes.c_int32()
+        check(lib().emsph_load_snapshot(self._h, path.encode(), ctypes.byref(t), ctypes.byref(m)),
+              "load_snapshot")
+        names = {v: k for k, v in _capi.METHODS.items()}
+        return int(t.value), names.get(int(m.value))
+
+    def set_user_colors(self, fn) -> None:
+        """A host User_colors_function (see user_colors_callback); None restores
+        target_density_colors on the device."""
+        self._colors_cb = user_colors_callback(fn) if fn else None
+        check(lib().emsph_set_user_colors(self._h, self._colors_cb or _capi.USER_COLORS_FN(),
+                                          None), "set_user_colors")
+
+    def compute_colors(self, time_seconds: float) -> None:
+        check(lib().emsph_compute_colors(self._h, float(time_seconds)), "compute_colors")
+
+    def colors(self) -> np.ndarray:
+        """State::colors of the last frame (RGBA float32, original order)."""
+        out = np.empty((self.n, 4), np.float32)
+        check(lib().emsph_get_colors(self._h, _ptr(out)), "get_colors")
+        return out
+
+    def colors_u8(self) -> np.ndarray:
+        """convert_colors (colors.cpp:57-70) of State::colors."""
+        out = np.empty((self.n, 4), np.uint8)
+        check(lib().emsph_get_colors_u8(self._h, _ptr(out)), "get_colors_u8")
+        return out
+
     def frame_init(self, method: str) -> None:
         check(lib().emsph_frame_init(self._h, _capi.METHODS[method]), "frame_init")
 
@@ -421,7 +515,9 @@ def _simulation_step(method: str, global_time: int, config: Simulation_config, s
     try:
         ctx.set_state(state)
         ctx.frame_step(method, global_time)
-        return ctx.get_state()
+        out = ctx.get_state()
+        out.colors = ctx.colors()
+        return out
     finally:
         if own:
             ctx.close()
@@ -475,6 +571,7 @@ class EZ_EXAMPLE_SIM:  # noqa: N801 - the reference's type name
 
     def fetch_state(self) -> State:
         self.state = self.ctx.get_state()
+        self.state.colors = self.ctx.colors()
         return self.state
 
 
@@ -641,6 +738,10 @@ class Pool:
         st = (_capi.StepStats * max(n, 1))()
         check(lib().emsph_pool_sub_step(self._h, st), "pool_sub_step")
         return list(st[:n])
+
+    def save_snapshot(self, sid: int, path: str, time_flicks: int = 0) -> None:
+        check(lib().emsph_pool_save_snapshot(self._h, int(sid), path.encode(), int(time_flicks)),
+              "pool_save_snapshot")
 
     def get_state(self, sid: int) -> State:
         n = self.counts[sid]

@@ -244,6 +244,25 @@ typedef void (*emsph_user_forces_fn)(void* user, size_t count, float time_second
                                      float mass_per_particle, float radius_per_particle,
                                      float* forces_xy, const float* pos_xy, const float* vel_xy);
 emsph_status emsph_set_user_forces(emsph_context* ctx, emsph_user_forces_fn fn, void* user);
+/* State::colors (state.h:13-17): computed at the end of every emsph_frame_step
+ * at the frame's end time, as compute_colors (colors.cpp:22-40) is called by
+ * adaptive_time_step.h:105-110 and pcisph.cpp:186-192, from the densities of
+ * the last sub-step. The default is target_density_colors (colors.cpp:9-20,
+ * on the device); a host User_colors_function (colors.h:13-21) replaces it:
+ * it gets the state and densities in original order and writes RGBA floats
+ * (the array arrives zeroed). fn = NULL restores the default.
+ * emsph_compute_colors computes them on request (callers of emsph_sub_step);
+ * emsph_get_colors returns RGBA floats [count][4] and emsph_get_colors_u8 the
+ * bytes of convert_colors (colors.cpp:57-70). */
+typedef void (*emsph_user_colors_fn)(void* user, size_t count, float time_seconds,
+                                     float mass_per_particle, float radius_per_particle,
+                                     float* colors_rgba, const float* pos_xy, const float* vel_xy,
+                                     const float* densities);
+emsph_status emsph_set_user_colors(emsph_context* ctx, emsph_user_colors_fn fn, void* user);
+emsph_status emsph_compute_colors(emsph_context* ctx, float time_seconds);
+emsph_status emsph_get_colors(emsph_context* ctx, float* rgba);
+emsph_status emsph_get_colors_u8(emsph_context* ctx, uint8_t* rgba);
+
 /* Frame-start initialisation of the reference's std_adaptive_time_step and
  * DFSPH temps (constant volumes, zeroed DFSPH stiffness warm starts). */
 emsph_status emsph_frame_init(emsph_context* ctx, int method);
@@ -374,6 +393,46 @@ emsph_status emsph_pool_sub_step(emsph_pool* pool, emsph_step_stats* stats);
 emsph_status emsph_pool_get_state(emsph_pool* pool, int id, float* pos_xy, float* vel_xy);
 /* Device bytes currently held by the library (all contexts, batches, pools). */
 int64_t emsph_device_bytes(void);
+
+/* State snapshots: a fluid State (state.h:13-17: positions, velocities and
+ * optionally colors) and its time in flicks in a self-describing binary
+ * format, for checkpoints and served results. No counterpart in the reference.
+ * Layout (little-endian): a 48-byte header - magic "EMSPHST\0", u32 version
+ * (1), u32 flags (bit 0 velocities, bit 1 colors), u64 count, i64 time in
+ * flicks, i32 method (or -1), u32 reserved, u64 FNV-1a hash of the payload -
+ * then positions [count][2], velocities [count][2] and colors [count][4] as
+ * IEEE float bit patterns (round trips are exact). Decoding checks the magic,
+ * version, size and hash and fails with EMSPH_ERR_INVALID_ARGUMENT. The
+ * encode/decode functions need no device. */
+typedef struct emsph_snapshot_header {
+    uint64_t count;
+    int64_t time_flicks;
+    int32_t method;
+    uint32_t flags;
+    uint64_t bytes;  /* size of the whole snapshot */
+} emsph_snapshot_header;
+/* Bytes of a snapshot of count particles with the given flags. */
+size_t emsph_snapshot_bytes(size_t count, uint32_t flags);
+/* vel_xy and colors_rgba may be NULL (not stored). */
+emsph_status emsph_snapshot_encode(const float* pos_xy, const float* vel_xy,
+                                   const float* colors_rgba, size_t count, int64_t time_flicks,
+                                   int32_t method, void* out, size_t capacity, size_t* written);
+emsph_status emsph_snapshot_info(const void* buf, size_t len, emsph_snapshot_header* info);
+/* Outputs may be NULL; absent velocities/colors are returned as zeros. */
+emsph_status emsph_snapshot_decode(const void* buf, size_t len, float* pos_xy, float* vel_xy,
+                                   float* colors_rgba);
+/* Writes buf to path atomically (a temporary file renamed into place). */
+emsph_status emsph_snapshot_write_file(const char* path, const void* buf, size_t len);
+/* A context's state (and its colors once a frame computed them) to a file,
+ * and back (the count must fit the context's capacity; colors are not
+ * restored: they are an output of the next frame). */
+emsph_status emsph_save_snapshot(emsph_context* ctx, const char* path, int64_t time_flicks,
+                                 int32_t method);
+emsph_status emsph_load_snapshot(emsph_context* ctx, const char* path, int64_t* time_flicks,
+                                 int32_t* method);
+/* A live pool scene's state to a file (the served form's result record). */
+emsph_status emsph_pool_save_snapshot(emsph_pool* pool, int id, const char* path,
+                                      int64_t time_flicks);
 
 /* Stand-alone z-index sort and neighbour search on one point set (no solids,
  * no dynamics): the reference's compute_all_neighbhorhoods fluid part. */

@@ -174,6 +174,8 @@ def lib():
             "orc_anti_coupling_angle": (_F, [_U64]),
             "orc_lehmer_first_draw": (_U64, [_U64]),
             "orc_scene_set_forces": (_I, [_P, _U32, _P, USER_FORCES_FN, _P]),
+            "orc_scene_get_colors": (_I, [_P, _P, _P]),
+            "orc_convert_colors": (None, [_SZ, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -332,6 +334,14 @@ class Scene:
             raise KeyError(name)
         return out.ravel()[:self.n].copy() if k == 1 else out
 
+    def colors(self):
+        """State::colors after the last frame step: (RGBA floats, bytes)."""
+        rgba = np.empty((self.n, 4), np.float32)
+        rgba8 = np.empty((self.n, 4), np.uint8)
+        if not lib().orc_scene_get_colors(self._h, _ptr(rgba), _ptr(rgba8)):
+            raise RuntimeError("no frame step has computed colors")
+        return rgba, rgba8
+
     def state_f32(self):
         pos = np.empty((self.n, 2), np.float32)
         vel = np.empty((self.n, 2), np.float32)
@@ -349,6 +359,14 @@ class Scene:
         out = np.empty(n, np.float64)
         lib().orc_scene_solid_volumes(self._h, _ptr(out))
         return out
+
+
+def convert_colors(rgba: np.ndarray) -> np.ndarray:
+    """colors.cpp:57-70 on float RGBA -> uint8 RGBA."""
+    rgba = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)
+    out = np.empty(rgba.shape, np.uint8)
+    lib().orc_convert_colors(len(rgba), _ptr(rgba), _ptr(out))
+    return out
 
 
 def set_threads(n: int) -> None:

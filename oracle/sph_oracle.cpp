@@ -1500,9 +1500,37 @@ struct Scene {
     Config<T> cfg;
     Solids<T> solids;
     std::vector<V2<T>> pos, vel;
+    std::vector<T> colors;  // State::colors (state.h:13-17), RGBA per particle
     Temp<T> tmp;
     Stats last{};
 };
+
+// colors.cpp:9-20 target_density_colors (the default User_colors_function,
+// colors.cpp:42-55): c = clamp(1 - rho / (1.1 rho0), 0, 1), RGBA (c, c, 1, 1),
+// from the densities of the last sub-step.
+template <class T>
+void compute_colors(Scene<T>& sc) {
+    size_t const n = sc.pos.size();
+    sc.colors.assign(4 * n, T(0));
+    if (sc.tmp.densities.size() < n) { return; }
+    T const denom = T(1.1f) * sc.cfg.target_density;
+    for (size_t i = 0; i < n; ++i) {
+        T const norm_d = sc.tmp.densities[i] / denom;
+        T const c = std::clamp(T(1) - norm_d, T(0), T(1));
+        sc.colors[4 * i] = c;
+        sc.colors[4 * i + 1] = c;
+        sc.colors[4 * i + 2] = T(1);
+        sc.colors[4 * i + 3] = T(1);
+    }
+}
+
+// colors.cpp:57-70 convert_colors: each channel to uint8 as
+// static_cast<uint8_t>(255 * clamp(f, 0, 1)).
+inline void convert_colors(size_t n, uint8_t* out, float const* rgba) {
+    for (size_t k = 0; k < 4 * n; ++k) {
+        out[k] = static_cast<uint8_t>(255.0f * std::clamp(rgba[k], 0.0f, 1.0f));
+    }
+}
 
 // Per-frame temp resize/init: iisph.cpp:153-166, dfsph_p.cpp:214-229 and the
 // constant volumes of adaptive_time_step.h:56-61.
@@ -1551,6 +1579,7 @@ int frame_step(Scene<T>& sc, int method, int64_t global_time) {
             time += per;
             ++sub_steps;
         }
+        compute_colors(sc);  // pcisph.cpp:186-192
         return sub_steps;
     }
     int const min_den = 60;
@@ -1571,6 +1600,7 @@ int frame_step(Scene<T>& sc, int method, int64_t global_time) {
         ++sub_steps;
         time += sub;
     }
+    compute_colors(sc);  // adaptive_time_step.h:105-110
     return sub_steps;
 }
 
@@ -1936,6 +1966,21 @@ int orc_scene_get_field(void* p, char const* name, double* out) {
     return s->fp64 ? get_field(s->d, name, out) : get_field(s->f, name, out);
 }
 // Raw float copy of positions/velocities (float precision only).
+// State::colors after the last frame step, RGBA floats and converted bytes
+// (colors.cpp:57-70); returns 0 when no frame has run.
+int orc_scene_get_colors(void* p, float* rgba, uint8_t* rgba8) {
+    auto* s = static_cast<OrcScene*>(p);
+    size_t const n = s->fp64 ? s->d.pos.size() : s->f.pos.size();
+    size_t const m = s->fp64 ? s->d.colors.size() : s->f.colors.size();
+    if (m != 4 * n) { return 0; }
+    for (size_t k = 0; k < 4 * n; ++k) {
+        rgba[k] = s->fp64 ? static_cast<float>(s->d.colors[k]) : s->f.colors[k];
+    }
+    if (rgba8) { convert_colors(n, rgba8, rgba); }
+    return 1;
+}
+void orc_convert_colors(size_t n, float const* rgba, uint8_t* out) { convert_colors(n, out, rgba); }
+
 void orc_scene_get_state_f32(void* p, float* pos, float* vel) {
     auto* s = static_cast<OrcScene*>(p);
     size_t const n = s->fp64 ? s->d.pos.size() : s->f.pos.size();
