@@ -380,6 +380,7 @@ struct Engine {
     bool use_fext = false;
     DBuf<V2> fext;
     float step_time = 0.0f;  // time of the current sub-step (for the callback)
+    bool frame_on = false;   // a device-side frame loop is running (its time is on the device)
     // State::colors of a context: computed at the end of each frame step (or
     // on request), RGBA in original order; a host User_colors_function
     // replaces the default target-density colors.
@@ -1117,7 +1118,8 @@ hipError_t host_user_forces(Engine* g, bool after_dt) {
     }
     emsph_config const& cfg = g->sh[0].cfg;
     float const dt = sd.c.dt;
-    float const t = after_dt ? g->step_time + dt : g->step_time;
+    float const t0 = sd.frame.on ? static_cast<float>(to_seconds(sd.frame.time)) : g->step_time;
+    float const t = after_dt ? t0 + dt : t0;
     g->user_fn(g->user_data, n, t, dt, cfg.mass_per_particle, cfg.draw_radius, H.forces.data(),
                H.pos.data(), H.vel.data());
     H.fs.resize(n);
@@ -1996,65 +1998,82 @@ emsph_status emsph_frame_step(emsph_context* c, int method, int64_t global_time_
                               emsph_frame_stats* fs) {
     if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
     ST_TRY(emsph_frame_init(c, method));
+    Engine* g = &c->e;
+    HIP_TRY(hipSetDevice(g->device));
     emsph_frame_stats local{};
-    emsph_step_stats st{};
     emsph_parameters const& P = c->cfg.params;
+    int64_t end_time = global_time_flicks;
     if (method == EMSPH_PCISPH) {
-        // pcisph.cpp:166-204: fixed sub-steps of the configured dt.
+        // pcisph.cpp:166-204: fixed sub-steps of the configured dt, enqueued
+        // back to back (no read-back between them).
         int64_t const per = P.time_per_step / P.sub_steps;
-        int64_t time = global_time_flicks;
+        ST_TRY(engine_set_dt(g, 0, c->cfg.seconds_per_sub_step));
         for (int k = 0; k < P.sub_steps; ++k) {
-            ST_TRY(emsph_sub_step(c, method, static_cast<float>(to_seconds(time)),
-                                  c->cfg.seconds_per_sub_step, &st));
-            time += per;
+            g->step_time = static_cast<float>(to_seconds(end_time));
+            ST_TRY(engine_sub_step(g, method));
+            end_time += per;
             local.sub_steps += 1;
-            local.total_iterations += st.iterations;
-            local.last_dt = c->cfg.seconds_per_sub_step;
+            local.total_iterations += g->last_iterations;
         }
-    } else {
+    } else if (g->N) {
         // adaptive_time_step.h:35-122 with (60, 6, 0.2) for IISPH
-        // (iisph.cpp:175-181) and (60, 4, 0.4) for DFSPH (dfsph_p.cpp:383-389).
+        // (iisph.cpp:175-181) and (60, 4, 0.4) for DFSPH_P, IISPH_AP and
+        // IISPH_PSEUDO_AP (dfsph_p.cpp:383-389): each sub-step's dt is picked
+        // on the device at its start from the max |v|^2 the previous one left
+        // (k_key_range, frame_begin); the host learns the quanta left from the
+        // key-state read-back that the solver loop makes anyway, so sub-steps
+        // follow each other without a read-back of their own.
         int const min_den = 60;
         int const max_den = (method == EMSPH_IISPH) ? 6 : 4;
         float const cfl = (method == EMSPH_IISPH) ? 0.2f : 0.4f;
-        int64_t const per = P.time_per_step / min_den;
-        int const divisor = min_den / max_den;
-        int remaining = min_den;
-        int64_t time = global_time_flicks;
-        float vmax2 = 0.0f;
-        if (c->e.N) { ST_TRY(engine_vmax2(&c->e, &vmax2)); }
-        while (remaining > 0) {
-            // cfl.cpp:12-21
-            double const numer = cfl * P.support;
-            double const denom = std::sqrt(static_cast<double>(vmax2));
-            bool safe = false;
-            if (denom != 0.0) {
-                safe = std::fabs(denom) >= 1.0 ||
-                       std::fabs(denom) / std::numeric_limits<double>::min() > std::fabs(numer);
-            }
-            int64_t const cfl_step = to_flicks(safe ? numer / denom : 1.0);
-            int ns = static_cast<int>(std::ceil(to_seconds(cfl_step) / to_seconds(per)));
-            ns = std::clamp(ns, 1, divisor);
-            ns = std::min(ns, remaining);
-            int64_t const sub = ns * per;
-            float const dt = static_cast<float>(to_seconds(sub));
-            ST_TRY(emsph_sub_step(c, method, static_cast<float>(to_seconds(time)), dt, &st));
-            vmax2 = st.max_velocity_sq;
-            remaining -= ns;
+        Batch b = fluid_batch(g);
+        launch_step_begin(b, g->stream);
+        launch_vmax2(b, g->stream);  // max |v|^2 of the state at the frame's start
+        HIP_TRY(hipGetLastError());
+        SceneDev sd = scene_dev(g->sh[0], 0);
+        sd.frame.on = 1;
+        sd.frame.per = P.time_per_step / min_den;
+        sd.frame.time = global_time_flicks;
+        sd.frame.next_time = global_time_flicks;
+        sd.frame.remaining = min_den;
+        sd.frame.next_remaining = min_den;
+        sd.frame.divisor = min_den / max_den;
+        sd.frame.cfl = cfl;
+        HIP_TRY(hipMemcpyAsync(g->scenes.p, &sd, sizeof(SceneDev), hipMemcpyHostToDevice,
+                               g->stream));
+        g->frame_on = true;
+        emsph_status st = EMSPH_OK;
+        while (local.sub_steps < min_den) {
+            st = engine_sub_step(g, method);
+            if (st != EMSPH_OK) { break; }
             local.sub_steps += 1;
-            local.total_iterations += st.iterations;
-            local.total_divergence_iterations += st.divergence_iterations;
-            local.last_dt = dt;
-            time += sub;
+            local.total_iterations += g->last_iterations;
+            local.total_divergence_iterations += g->last_div_iterations;
+            if (g->keys.frame_remaining <= 0) { break; }
         }
+        g->frame_on = false;
+        // Back to the host's sub-step dt for later emsph_sub_step calls.
+        SceneDev const plain = scene_dev(g->sh[0], 0);
+        HIP_TRY(hipMemcpyAsync(g->scenes.p, &plain, sizeof(SceneDev), hipMemcpyHostToDevice,
+                               g->stream));
+        ST_TRY(st);
+        end_time = global_time_flicks + sd.frame.per * min_den;
+    } else {
+        // No particles: max |v|^2 is 0, every sub-step takes the most quanta.
+        int const max_den = (method == EMSPH_IISPH) ? 6 : 4;
+        local.sub_steps = max_den;
+        end_time = global_time_flicks + (P.time_per_step / 60) * 60;
+    }
+    emsph_step_stats st{};
+    if (g->N) {
+        ST_TRY(read_stats(g));
+        to_public(g->host_stats[0], g, &st);
     }
     local.last = st;
+    local.last_dt = st.dt;
     // adaptive_time_step.h:105-110 / pcisph.cpp:186-192: colors at the frame's
     // end time.
-    int64_t const end_time = global_time_flicks + (method == EMSPH_PCISPH
-        ? (P.time_per_step / P.sub_steps) * P.sub_steps
-        : (P.time_per_step / 60) * 60);
-    if (c->e.N) { ST_TRY(emsph_compute_colors(c, static_cast<float>(to_seconds(end_time)))); }
+    if (g->N) { ST_TRY(emsph_compute_colors(c, static_cast<float>(to_seconds(end_time)))); }
     if (fs) { *fs = local; }
     return EMSPH_OK;
 }

@@ -66,6 +66,23 @@ struct ForceDev {
     uint32_t pad;
 };
 
+// Frame loop of the adaptive time step (adaptive_time_step.h:35-122) kept on
+// the device: the start of each sub-step picks its dt from the CFL condition
+// in 1/min_den quanta of the frame; the end of the sub-step commits the
+// remaining quanta and the time (so a sub-step redone after a short sort picks
+// the same dt).
+struct FrameDev {
+    int64_t per;        // flicks per quantum (time_per_step / min_den)
+    int64_t time;       // start of the current sub-step, flicks
+    int64_t next_time;  // its end (committed at the end of the sub-step)
+    int32_t remaining;  // quanta left before the current sub-step
+    int32_t next_remaining;
+    int32_t divisor;    // min_den / max_den: the most quanta one sub-step takes
+    float cfl;          // the method's CFL factor
+    int32_t on;
+    int32_t pad;
+};
+
 struct SceneDev {
     StepConsts c;
     Thresholds thr;
@@ -78,6 +95,7 @@ struct SceneDev {
     float cfl;            // > 0: c.dt = min(dt_max, cfl * h / max |v|) at each sub-step
     float dt_max;
     ForceDev force;
+    FrameDev frame;
 };
 
 struct Tile {
@@ -133,6 +151,8 @@ struct KeyState {
     unsigned int max_count;       // largest fluid or solid list of the last build
     unsigned int cells;           // cells of the last indexing run
     unsigned int long_lists;      // particles of the last build with a list above 16
+    int frame_remaining;          // scene 0's frame quanta left after this sub-step (-1: off)
+    unsigned int pad;
 };
 
 // Neighbour-list capacities of the build kernel's LDS list (the full one is

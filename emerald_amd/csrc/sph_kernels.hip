@@ -180,9 +180,51 @@ __global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
 
 // One workgroup: the key range over the partials; clears the short-sort flag
 // and the neighbour-build counters of this indexing run.
+// Float-to-integer conversions as the x86-64 host code of the frame loop
+// performs them (cvttsd2si): truncation, and the "integer indefinite" value
+// (the most negative integer) for NaN and out-of-range inputs, which the host
+// meets for velocities near rest (adaptive_time_step.h:73-78).
+__device__ __forceinline__ int64_t host_double_to_i64(double x) {
+    if (!(x > -9223372036854775808.0 && x < 9223372036854775808.0)) {
+        return static_cast<int64_t>(0x8000000000000000ull);
+    }
+    return static_cast<int64_t>(x);
+}
+__device__ __forceinline__ int32_t host_double_to_i32(double x) {
+    if (!(x > -2147483649.0 && x < 2147483648.0)) { return static_cast<int32_t>(0x80000000u); }
+    return static_cast<int32_t>(x);
+}
+__device__ __forceinline__ double flicks_to_seconds(int64_t f) {
+    return static_cast<double>(f) / 705600000.0;  // util/flicks.h:88-116
+}
+
+// adaptive_time_step.h:70-86 for one sub-step: cfl_maximum_time_step
+// (cfl.cpp:12-21: factor h / sqrt(max |v|^2) in double, 1 s when unsafe, to
+// flicks), then ceil(cfl step / quantum) clamped to [1, divisor] and to the
+// quanta left; dt = quanta * quantum.
+__device__ __forceinline__ void frame_begin(SceneDev* sd, float vmax2) {
+    FrameDev& fr = sd->frame;
+    double const numer = static_cast<double>(fr.cfl * sd->c.kc.h);
+    double const denom = sqrt(static_cast<double>(vmax2));
+    bool safe = false;
+    if (denom != 0.0) {
+        safe = fabs(denom) >= 1.0 || fabs(denom) / 2.2250738585072014e-308 > fabs(numer);
+    }
+    int64_t const cfl_step = host_double_to_i64((safe ? numer / denom : 1.0) * 705600000.0);
+    int32_t ns = host_double_to_i32(ceil(flicks_to_seconds(cfl_step) / flicks_to_seconds(fr.per)));
+    ns = ns < 1 ? 1 : (ns > fr.divisor ? fr.divisor : ns);
+    ns = ns < fr.remaining ? ns : fr.remaining;
+    int64_t const sub = static_cast<int64_t>(ns) * fr.per;
+    sd->c.dt = static_cast<float>(flicks_to_seconds(sub));
+    fr.next_remaining = fr.remaining - ns;
+    fr.next_time = fr.time + sub;
+}
+
 // Start of a sub-step for one scene: the CFL dt and cleared statistics.
 __device__ __forceinline__ void step_begin_scene(SceneDev* sd, SceneStats* stats) {
-    if (sd->cfl > 0.0f) {
+    if (sd->frame.on) {
+        frame_begin(sd, __uint_as_float(stats->vmax2_bits));
+    } else if (sd->cfl > 0.0f) {
         // cfl.cpp:12-21: factor * h / sqrt(max |v|^2) in double, over the
         // velocities the previous sub-step left (dt_max while at rest).
         float const vmax2 = __uint_as_float(stats->vmax2_bits);
@@ -239,6 +281,8 @@ __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) 
         }
         a.keystate->kmin = key_min;
         a.keystate->kmax = key_max;
+        a.keystate->frame_remaining =
+          (a.step_stats && a.step_scenes[0].frame.on) ? a.step_scenes[0].frame.next_remaining : -1;
         a.keystate->sort_short = 0u;
         a.keystate->overflow_count = 0u;
         a.keystate->max_count = 0u;
@@ -2244,6 +2288,12 @@ __global__ __launch_bounds__(256) void k_scene_totals(Batch a, int energy) {
             if (energy) {
                 a.stats[s].kinetic_energy = (ske[0] + ske[1]) + (ske[2] + ske[3]);
                 a.stats[s].potential_energy = (spe[0] + spe[1]) + (spe[2] + spe[3]);
+                if (sd.frame.on) {
+                    // End of the sub-step: commit the frame loop's progress.
+                    FrameDev& fr = const_cast<SceneDev&>(sd).frame;
+                    fr.remaining = fr.next_remaining;
+                    fr.time = fr.next_time;
+                }
             }
         }
         __syncthreads();

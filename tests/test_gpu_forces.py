@@ -191,3 +191,35 @@ def test_batch_mixed_force_models_equal_alone(gpu):
         assert_bits_equal(bb.velocities, a.velocities, f"scene {i} velocities")
         ctx.close()
     b.close()
+
+
+@pytest.mark.parametrize("method", ["iisph", "dfsph_p"])
+def test_host_callback_in_frame_steps(gpu, method):
+    """The callback's time inside a frame step comes from the device-side
+    frame loop (the sub-step start, t + dt for DFSPH_P): the same calls as the
+    oracle's frame step."""
+    scene, cfg, ctx, orc = _pair(method, nx=12, ny=18, seed=43)
+    g = np.float32(cfg.params.gravity)
+    logs = {"gpu": [], "orc": []}
+
+    def make(which):
+        def fn(n, t, dt, mass, radius, forces, pos, vel):
+            logs[which].append((n, np.float32(t), np.float32(dt)))
+            forces[:, 1] -= np.float32(mass) * g
+            forces[:, 0] += np.float32(mass) * np.float32(np.cos(np.float32(t)))
+        return fn
+
+    ctx.set_force_model(sb.ForceModel(gravity_on=False, gravity=float(g)))
+    ctx.set_user_forces(make("gpu"))
+    orc.set_forces(flags=0, gravity=float(g), draw_radius=cfg.draw_radius,
+                   user_forces=make("orc"))
+    T = cfg.params.time_per_step
+    for f in range(2):
+        fs = ctx.frame_step(method, 7 * T + f * T)
+        n, _ = orc.frame_step(method, 7 * T + f * T)
+        assert fs.sub_steps == n
+    assert logs["gpu"] == logs["orc"]
+    pos, vel = orc.state_f32()
+    gs = ctx.get_state()
+    assert_bits_equal(gs.positions, pos, f"{method} frame callback positions")
+    assert_bits_equal(gs.velocities, vel, f"{method} frame callback velocities")

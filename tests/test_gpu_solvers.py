@@ -152,3 +152,24 @@ def test_graph_replays_equal_eager(gpu, method):
         assert a[0] == b[0] and a[1] == b[1], k
         assert_bits_equal(a[2], b[2], f"positions after sub-step {k}")
         assert_bits_equal(a[3], b[3], f"density after sub-step {k}")
+
+
+@pytest.mark.parametrize("method", ["iisph", "dfsph_p"])
+@pytest.mark.parametrize("speed", [1e-7, 1e-12, 1e-16])
+def test_frame_step_near_rest(gpu, method, speed):
+    """The frame loop's dt choice (adaptive_time_step.h:70-86, on the device)
+    for velocities near rest, where the host's float-to-integer conversions of
+    the CFL step leave the integer range (they then take one quantum): the
+    sub-step count and the state equal the oracle's."""
+    scene, cfg, ctx, orc = _make(method, nx=12, ny=16, seed=41)
+    vel = np.full_like(scene.positions, np.float32(speed))
+    ctx.set_state(sb.State(scene.positions, vel))
+    orc2 = oracle_scene(scenes.DamBreak(scene.positions, vel, scene.solids, scene.support,
+                                        scene.spacing, scene.dt, scene.mass), cfg)
+    fs = ctx.frame_step(method, 0)
+    n, ost = orc2.frame_step(method, 0)
+    assert fs.sub_steps == n
+    pos, v = orc2.state_f32()
+    g = ctx.get_state()
+    assert_bits_equal(g.positions, pos, "near-rest positions")
+    assert_bits_equal(g.velocities, v, "near-rest velocities")
