@@ -67,6 +67,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_headless(verbose)
+    build_api_check(verbose)
     build_probe(verbose)
     return LIB
 
@@ -100,6 +101,24 @@ def build_headless(verbose: bool = False) -> str:
     subprocess.run(cmd, check=True)
     os.replace(HEADLESS + ".tmp", HEADLESS)
     return HEADLESS
+
+
+API_CHECK = os.path.join(os.path.dirname(HERE), "tests", "cpp", "api_check")
+
+
+def build_api_check(verbose: bool = False) -> str:
+    """The C++ mirror header's check program (tests/cpp/api_check.cpp), linked
+    against the C ABI only (plain g++), run by the GPU tests."""
+    src = API_CHECK + ".cpp"
+    if not os.path.exists(src):
+        return ""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", API_CHECK + ".tmp", src, "-L" + HERE,
+           "-lemerald_sph", "-Wl,-rpath,$ORIGIN/../../emerald_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(API_CHECK + ".tmp", API_CHECK)
+    return API_CHECK
 
 
 def _wait(item) -> None:

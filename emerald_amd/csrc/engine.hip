@@ -74,9 +74,6 @@ constexpr float kPi = 3.14159274101257324218750f;
 double to_seconds(int64_t f) {
     return static_cast<double>(f) / static_cast<double>(kFlicksPerSecond);
 }
-int64_t to_flicks(double s) {
-    return static_cast<int64_t>(s * static_cast<double>(kFlicksPerSecond));
-}
 
 KernelConsts make_kernel_consts(float h) {
     KernelConsts c;
@@ -1596,16 +1593,6 @@ emsph_status engine_set_dt(Engine* g, uint32_t s, float dt) {
         HIP_TRY(hipMemcpyAsync(g->scenes.p + s, &sd, sizeof(SceneDev), hipMemcpyHostToDevice,
                                g->stream));
     }
-    return EMSPH_OK;
-}
-
-emsph_status engine_vmax2(Engine* g, float* out) {
-    Batch b = fluid_batch(g);
-    launch_step_begin(b, g->stream);
-    launch_vmax2(b, g->stream);
-    HIP_TRY(hipGetLastError());
-    ST_TRY(read_stats(g));
-    std::memcpy(out, &g->host_stats[0].vmax2_bits, 4);
     return EMSPH_OK;
 }
 

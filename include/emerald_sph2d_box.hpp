@@ -9,16 +9,23 @@
 //   iisph/pcisph/dfsph_p_simulation_step  iisph.h:13-19, pcisph.h:12-18, dfsph_p.h:15-21
 //   EZ_EXAMPLE_SIM                        sph2d_box/simulation.h:24-44
 //
+//   User_forces_function, default_gravity_forces   sph2d_box/forces.h:13-21, forces.cpp:62-83
+//   User_colors_function, default_target_density_colors  sph2d_box/colors.h:13-21,
+//                                          colors.cpp:42-55
+//
 // Differences that follow from device residency: Temp_data owns the device
 // context (the reference's caller-owned scratch that persists across calls);
-// the built-in gravity force stands for default_gravity_forces; the colour
-// callback (display only) is not carried. Errors throw std::runtime_error
-// where the reference aborts (EMLD_FAIL).
+// default_gravity_forces and the other forces.cpp models are device force
+// models (Forces::device), while a host User_forces_function runs through a
+// per-sub-step copy round trip; default_target_density_colors is computed on
+// the device (an empty User_colors_function stands for it). Errors throw
+// std::runtime_error where the reference aborts (EMLD_FAIL).
 #pragma once
 
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -152,10 +159,67 @@ struct Simulation_config {
     }
 };
 
+struct C4f {
+    float r = 0.0f, g = 0.0f, b = 0.0f, a = 0.0f;
+};
+static_assert(sizeof(C4f) == 16, "C4f is four packed floats");
+
 struct State {
     std::vector<V2f> positions;
     std::vector<V2f> velocities;
+    std::vector<C4f> colors;
 };
+
+// forces.h:13-21: (count, time, dt, mass per particle, radius per particle,
+// forces, positions, velocities); the forces arrive zeroed.
+using User_forces_function =
+  std::function<void(size_t, float, float, float, float, V2f*, V2f const*, V2f const*)>;
+// colors.h:13-21: (count, time, mass, radius, colors, positions, velocities,
+// densities).
+using User_colors_function = std::function<void(size_t, float, float, float, C4f*, V2f const*,
+                                                V2f const*, float const*)>;
+
+// The forces of a step: device force models (flags 0: none) and an optional
+// host callback, accumulated callback first (emsph_set_force_model,
+// emsph_set_user_forces).
+struct Forces {
+    emsph_force_model device{};
+    User_forces_function host;
+};
+// forces.cpp:72-83 as the device gravity model.
+inline Forces default_gravity_forces(float gravity) {
+    Forces f;
+    f.device.flags = EMSPH_FORCE_GRAVITY;
+    f.device.gravity = gravity;
+    return f;
+}
+// A host User_forces_function in place of gravity (keeps the device's
+// gravity value for the energy diagnostic).
+inline Forces host_forces(User_forces_function fn, float gravity) {
+    Forces f;
+    f.device.gravity = gravity;
+    f.host = std::move(fn);
+    return f;
+}
+// colors.cpp:42-55: the default colours, computed on the device.
+inline User_colors_function default_target_density_colors(float /*target_density*/) {
+    return {};
+}
+
+namespace detail {
+inline void forces_trampoline(void* user, size_t n, float t, float dt, float m, float r,
+                              float* f, float const* p, float const* v) {
+    (*static_cast<User_forces_function const*>(user))(n, t, dt, m, r, reinterpret_cast<V2f*>(f),
+                                                      reinterpret_cast<V2f const*>(p),
+                                                      reinterpret_cast<V2f const*>(v));
+}
+inline void colors_trampoline(void* user, size_t n, float t, float m, float r, float* c,
+                              float const* p, float const* v, float const* d) {
+    (*static_cast<User_colors_function const*>(user))(n, t, m, r, reinterpret_cast<C4f*>(c),
+                                                      reinterpret_cast<V2f const*>(p),
+                                                      reinterpret_cast<V2f const*>(v), d);
+}
+}  // namespace detail
 
 struct Solid_state {
     std::vector<V2f> positions;
@@ -207,11 +271,26 @@ inline State dam_break_initial_state(Parameters const& params, Solid_state const
     return s;
 }
 
+// Installs a step's forces and colour callback on the context (they must
+// outlive the step: the callbacks are called during it).
+inline void set_callbacks(emsph_context* ctx, Forces const& forces,
+                          User_colors_function const& colors) {
+    check(emsph_set_force_model(ctx, &forces.device), "emsph_set_force_model");
+    check(emsph_set_user_forces(ctx, forces.host ? detail::forces_trampoline : nullptr,
+                                const_cast<User_forces_function*>(&forces.host)),
+          "emsph_set_user_forces");
+    check(emsph_set_user_colors(ctx, colors ? detail::colors_trampoline : nullptr,
+                                const_cast<User_colors_function*>(&colors)),
+          "emsph_set_user_colors");
+}
+
 inline State simulation_step(int method, flicks global_time, Simulation_config const& config,
                              State&& state, Solid_state const& solid_state, Temp_data& temp,
+                             Forces const& forces, User_colors_function const& colors,
                              emsph_frame_stats* stats = nullptr) {
     size_t const n = state.positions.size();
     temp.ensure(config, solid_state, n);
+    set_callbacks(temp.ctx.get(), forces, colors);
     check(emsph_set_state(temp.ctx.get(), reinterpret_cast<float const*>(state.positions.data()),
                           reinterpret_cast<float const*>(state.velocities.data()), n),
           "emsph_set_state");
@@ -219,21 +298,33 @@ inline State simulation_step(int method, flicks global_time, Simulation_config c
     check(emsph_get_state(temp.ctx.get(), reinterpret_cast<float*>(state.positions.data()),
                           reinterpret_cast<float*>(state.velocities.data())),
           "emsph_get_state");
+    state.colors.resize(n);  // compute_colors (colors.cpp:22-40) at the frame's end
+    if (n) {
+        check(emsph_get_colors(temp.ctx.get(), reinterpret_cast<float*>(state.colors.data())),
+              "emsph_get_colors");
+    }
+    set_callbacks(temp.ctx.get(), default_gravity_forces(forces.device.gravity), {});
     return std::move(state);
 }
 
-inline State iisph_simulation_step(flicks t, Simulation_config const& c, State&& s,
-                                   Solid_state const& ss, Temp_data& temp) {
-    return simulation_step(EMSPH_IISPH, t, c, std::move(s), ss, temp);
-}
-inline State pcisph_simulation_step(flicks t, Simulation_config const& c, State&& s,
-                                    Solid_state const& ss, Temp_data& temp) {
-    return simulation_step(EMSPH_PCISPH, t, c, std::move(s), ss, temp);
-}
-inline State dfsph_p_simulation_step(flicks t, Simulation_config const& c, State&& s,
-                                     Solid_state const& ss, Temp_data& temp) {
-    return simulation_step(EMSPH_DFSPH_P, t, c, std::move(s), ss, temp);
-}
+// The reference's signatures (iisph.h:13-19, pcisph.h:12-18, dfsph_p.h:15-21,
+// iisph_ap.h:11-17, iisph_pseudo_ap.h:14-20).
+#define EMSPH_SIMULATION_STEP(name, id)                                                     \
+    inline State name(flicks t, Simulation_config const& c, State&& s, Solid_state const& ss, \
+                      Temp_data& temp, Forces const& forces,                                \
+                      User_colors_function const& colors = {}) {                            \
+        return simulation_step(id, t, c, std::move(s), ss, temp, forces, colors);             \
+    }                                                                                        \
+    inline State name(flicks t, Simulation_config const& c, State&& s, Solid_state const& ss, \
+                      Temp_data& temp) {                                                     \
+        return name(t, c, std::move(s), ss, temp, default_gravity_forces(c.params.gravity));  \
+    }
+EMSPH_SIMULATION_STEP(iisph_simulation_step, EMSPH_IISPH)
+EMSPH_SIMULATION_STEP(pcisph_simulation_step, EMSPH_PCISPH)
+EMSPH_SIMULATION_STEP(dfsph_p_simulation_step, EMSPH_DFSPH_P)
+EMSPH_SIMULATION_STEP(iisph_ap_simulation_step, EMSPH_IISPH_AP)
+EMSPH_SIMULATION_STEP(iisph_pseudo_ap_simulation_step, EMSPH_IISPH_PSEUDO_AP)
+#undef EMSPH_SIMULATION_STEP
 
 // simulation.h:24-44. The state stays resident on the device between steps;
 // fetch_state() copies it back (the reference's GUI reads it every frame).
@@ -244,17 +335,27 @@ struct EZ_EXAMPLE_SIM {
     Solid_state solid_state;
     State state;
     Temp_data temp_data;
+    Forces user_forces;
+    User_colors_function user_colors;
     flicks time = 0;
     Method method = IISPH_PSEUDO_AP;
 
+    // simulation.cpp:14-39 (forces and colours default to
+    // default_gravity_forces / default_target_density_colors, as the
+    // reference's bins pass them).
     EZ_EXAMPLE_SIM(Parameters const& params,
                    Solid_state (*solid_initial_state)(Parameters const&, bool),
-                   State (*fluid_initial_state)(Parameters const&, Solid_state const&))
+                   State (*fluid_initial_state)(Parameters const&, Solid_state const&),
+                   std::optional<Forces> in_user_forces = std::nullopt,
+                   User_colors_function in_user_colors = {})
       : config(params),
         solid_state(solid_initial_state(params, true)),
-        state(fluid_initial_state(params, solid_state)) {
+        state(fluid_initial_state(params, solid_state)),
+        user_forces(in_user_forces.value_or(default_gravity_forces(params.gravity))),
+        user_colors(std::move(in_user_colors)) {
         method = static_cast<Method>(method_id(params.method));
         temp_data.ensure(config, solid_state, state.positions.size());
+        set_callbacks(temp_data.ctx.get(), user_forces, user_colors);
         check(emsph_set_state(temp_data.ctx.get(),
                               reinterpret_cast<float const*>(state.positions.data()),
                               reinterpret_cast<float const*>(state.velocities.data()),
@@ -275,6 +376,12 @@ struct EZ_EXAMPLE_SIM {
                               reinterpret_cast<float*>(state.positions.data()),
                               reinterpret_cast<float*>(state.velocities.data())),
               "emsph_get_state");
+        state.colors.resize(state.positions.size());
+        if (time > 0 && !state.colors.empty()) {
+            check(emsph_get_colors(temp_data.ctx.get(),
+                                   reinterpret_cast<float*>(state.colors.data())),
+                  "emsph_get_colors");
+        }
         return state;
     }
 };
