@@ -12,8 +12,17 @@ events on the context's stream) and the CPU baseline (the oracle, a C++
 restatement of the reference's CPU path, timed on this host's cores on a
 bounded sample of the same workload).
 
+The same line carries the other workloads as objects beside the headline:
+"batched" (256 ragged scenes, one launch per stage, against per-scene
+contexts), "served" (a Poisson request stream over HBM-budgeted pools, p50/p99
+step latency), "search" (4,194,304 points, keys, sort, cells and lists alone)
+and "small" (the 12,800-particle dam break, 1000 sub-steps, GPU against the
+oracle on the host cores, final states compared bit for bit).
+
   python bench.py [--steps K] [--warmup W]
                   [--method iisph|iisph_ap|iisph_pseudo_ap|dfsph_p|pcisph]
+                  [--no-cpu-baseline] [--no-batched] [--no-served] [--no-search]
+                  [--no-small]
 """
 
 from __future__ import annotations
@@ -185,6 +194,99 @@ def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict
     }
 
 
+def search_form(runs: int = 5, n: int = 4_194_304) -> dict:
+    """Z-index keys, sort, cells and 64-capped neighbour lists alone (no
+    dynamics) on 4,194,304 points, half uniform and half from 8 Gaussian
+    clusters (sigma 0.02), cell 1/2048 (h = 1/4096). Algorithmic bytes per point
+    40 + 1 + 4 n (key build 40, count 1, u32 indices; the lists are the
+    product)."""
+    from emerald_amd import scenes
+    from emerald_amd.search import NeighborSearch
+    log(f"search form: {n} points")
+    xy = scenes.search_points(n, seed=3)
+    s = NeighborSearch(n)
+    s.set_points(xy)
+    s.run(1.0 / 2048.0)  # first run: list capacity and pass count
+    ms = [s.run(1.0 / 2048.0) for _ in range(runs)]
+    r = s.results(lists=True)
+    s.close()
+    counts = r["counts"].astype(np.float64)
+    nbar = float(counts.mean())
+    best = min(ms)
+    per_point = 40.0 + 1.0 + 4.0 * nbar
+    return {
+        "workload": "4,194,304 points (half U[0,1)^2, half 8 Gaussian clusters sigma 0.02), "
+                    "cell 1/2048, 64-capped lists",
+        "points": n,
+        "ms_per_search": {"min": best, "median": float(np.median(ms))},
+        "points_per_s": n / (best / 1e3),
+        "mean_neighbors": nbar,
+        "max_neighbors": int(counts.max()),
+        "capped_lists": int((counts >= 64).sum()),
+        "algorithmic_bytes_per_point": per_point,
+        "achieved_GBps": per_point * n / (best / 1e3) / 1e9,
+        "cells": int(len(r["block_begin"])) if "block_begin" in r else None,
+    }
+
+
+def small_form(steps: int = 1000, cpu_budget: float = 20.0) -> dict:
+    """The reference's own CPU-runnable case: the 12,800-particle IISPH dam
+    break (80 x 160 lattice, spacing 0.005, dt 2e-4), 1000 sub-steps on the GPU
+    and the same sub-steps in the oracle on the host cores, with the final
+    states compared bit for bit."""
+    import oracle
+    from emerald_amd import scenes
+    from emerald_amd import sph2d_box as sb
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import oracle_scene
+    log(f"small form: 12,800-particle IISPH dam break, {steps} sub-steps")
+    scene = scenes.dam_break_12800()
+    cfg = scenes.config_for(scene, "iisph")
+    n = len(scene.positions)
+    ctx = sb.Context(cfg, sb.Solid_state(scene.solids), n)
+    ctx.set_state(sb.State(scene.positions, scene.velocities))
+    ctx.frame_init("iisph")
+    ctx.iteration_totals(reset=True)
+    t0 = time.perf_counter()
+    dev_ms, _ = ctx.run_sub_steps("iisph", scene.dt, steps)
+    wall = time.perf_counter() - t0
+    _, iters, _, _ = ctx.iteration_totals(reset=True)
+    gst = ctx.get_state()
+    ctx.close()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    o = oracle_scene(scene, cfg)
+    o.frame_init("iisph")
+    done = 0
+    t1 = time.perf_counter()
+    while done < steps:
+        o.sub_step("iisph", done * scene.dt, scene.dt)
+        done += 1
+        if time.perf_counter() - t1 > cpu_budget:
+            break
+    cwall = time.perf_counter() - t1
+    same = None
+    if done == steps:
+        pos, vel = o.state_f32()
+        same = bool(np.array_equal(pos.view(np.uint32), gst.positions.view(np.uint32)) and
+                    np.array_equal(vel.view(np.uint32), gst.velocities.view(np.uint32)))
+    return {
+        "workload": "iisph dam break, 80x160 lattice (12,800 particles), spacing 0.005, "
+                    "dt 2e-4, hex walls",
+        "sub_steps": steps,
+        "value": n * steps / wall,
+        "unit": "particle-steps/s",
+        "ms_per_sub_step": wall * 1e3 / steps,
+        "device_ms_per_sub_step": dev_ms / steps,
+        "mean_iterations": iters / steps,
+        "cpu_value": n * done / cwall,
+        "cpu_cores": threads,
+        "cpu_sub_steps": done,
+        "speedup_vs_cpu": (n * steps / wall) / (n * done / cwall),
+        "final_state_bit_identical_to_oracle": same,
+    }
+
+
 def served_form(budget_gb: float, window: float, rate: float = 200.0,
                 plain_steps: int = 2) -> dict:
     """Poisson stream of dam-break requests (200/s, 5k-500k particles, IISPH or
@@ -243,6 +345,8 @@ def main() -> int:
     ap.add_argument("--served-only", action="store_true")
     ap.add_argument("--served-window", type=float, default=10.0)
     ap.add_argument("--served-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-search", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark runs on one GPU only; --gpus above 1 is "
@@ -355,6 +459,14 @@ def main() -> int:
     if not args.no_served:
         ctx.close()
         served = served_form(args.served_budget_gb, args.served_window)
+    search = None
+    if not args.no_search:
+        ctx.close()
+        search = search_form()
+    small = None
+    if not args.no_small:
+        ctx.close()
+        small = small_form()
 
     out = {
         "metric": "IISPH particle-steps/s @1M (1 GPU)" if method == "iisph"
@@ -390,6 +502,8 @@ def main() -> int:
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "search": search,
+        "small": small,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

@@ -670,15 +670,29 @@ struct OddEvenMergeNet {
     }
 };
 constexpr OddEvenMergeNet<16> NET16{};
+constexpr OddEvenMergeNet<32> NET32{};
+constexpr OddEvenMergeNet<64> NET64{};
 
-// Capped lists of this size are collected unsorted and sorted once by the
-// network (no per-candidate insertion chains through LDS).
-template <int CAP>
-constexpr bool net_sorted() {
-    return CAP == 16;
+template <int N>
+__device__ constexpr OddEvenMergeNet<N> const& net_of() {
+    if constexpr (N == 16) {
+        return NET16;
+    } else if constexpr (N == 32) {
+        return NET32;
+    } else {
+        return NET64;
+    }
 }
 
-// Sorts one lane's cnt <= 16 collected entries into the reference order (by
+// Lists of every capacity are collected unsorted and sorted once by a network
+// (no per-candidate insertion chains through LDS: those run as long as the
+// longest lane's shift, for every accepted candidate of any lane).
+template <int CAP>
+constexpr bool net_sorted() {
+    return true;
+}
+
+// Sorts one lane's cnt <= N collected entries into the reference order (by
 // distance, then by original index) and leaves them as (distance bits << 32 |
 // slot). They arrive as (|v|^2 bits << 32 | slot), v = candidate - particle,
 // and the network orders them by (|v|^2, slot). The distances are then taken
@@ -687,11 +701,11 @@ constexpr bool net_sorted() {
 // insertion pass with the exact comparator puts in original index order. A
 // |v|^2 below 2 FLT_MIN, where imath_length scales instead, sends the list
 // to that pass with every distance taken again from the positions.
-template <int W>
-__device__ __forceinline__ void net_sort16(uint64_t (*ent)[W], uint32_t lane, int cnt,
-                                           uint32_t const* __restrict__ oorig,
-                                           V2 const* __restrict__ opos, V2 p) {
-    constexpr int N = 16;
+template <int N, int W>
+__device__ __forceinline__ void net_sort(uint64_t (*ent)[W], uint32_t lane, int cnt,
+                                         uint32_t const* __restrict__ oorig,
+                                         V2 const* __restrict__ opos, V2 p) {
+    constexpr OddEvenMergeNet<N> const& net = net_of<N>();
     uint64_t e[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
@@ -699,12 +713,12 @@ __device__ __forceinline__ void net_sort16(uint64_t (*ent)[W], uint32_t lane, in
         e[k] = (k < cnt) ? v : ~0ull;
     }
 #pragma unroll
-    for (int c = 0; c < NET16.count; ++c) {
-        uint64_t const x = e[NET16.ce[c].a];
-        uint64_t const y = e[NET16.ce[c].b];
+    for (int c = 0; c < net.count; ++c) {
+        uint64_t const x = e[net.ce[c].a];
+        uint64_t const y = e[net.ce[c].b];
         bool const swap = y < x;
-        e[NET16.ce[c].a] = swap ? y : x;
-        e[NET16.ce[c].b] = swap ? x : y;
+        e[net.ce[c].a] = swap ? y : x;
+        e[net.ce[c].b] = swap ? x : y;
     }
     bool tiny = false;
 #pragma unroll
@@ -786,7 +800,10 @@ __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[W], uint32_t lane, V2
     }
     if (net_sorted<CAP>()) {
         // Unsorted append of (|v|^2 bits << 32 | slot); the distances are
-        // taken after the sort (net_sort16), once per accepted entry.
+        // taken after the sort (net_sort), once per accepted entry. A capped
+        // list (CAP < 64) that overflows is deferred (cnt = -1); a full
+        // 64-entry list keeps the first 64 accepted in scan order
+        // (neighborhood.h:233-250) and ends the scan.
 #pragma unroll
         for (int u = 0; u < C; ++u) {
             uint32_t const s = s0 + u;
@@ -794,13 +811,17 @@ __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[W], uint32_t lane, V2
             if (acc && cnt < CAP) {
                 ent[cnt][lane] = (static_cast<uint64_t>(__float_as_uint(l2[u])) << 32) | s;
             }
-            cnt += acc ? 1 : 0;
+            if (CAP == MAXN) {
+                cnt += (acc && cnt < CAP) ? 1 : 0;
+            } else {
+                cnt += acc ? 1 : 0;
+            }
         }
-        if (cnt > CAP) {
+        if (CAP < MAXN && cnt > CAP) {
             cnt = -1;
             return false;
         }
-        return true;
+        return !(CAP == MAXN && cnt == CAP);
     }
 #pragma unroll
     for (int u = 0; u < C; ++u) {
@@ -847,7 +868,29 @@ __device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
     int cnt = 0;
     uint32_t flags = 0;
     uint2 cur = win[0];
-    for (int k = 0; k < 9; ++k) {
+    if constexpr (CAP <= 16) {
+        // The 16-entry build (the common case): a list either fits or is
+        // deferred, and every lane sorts with the 16-input network.
+        for (int k = 0; k < 9; ++k) {
+            uint2 const nxt = win[k < 8 ? k + 1 : 8];  // next range in flight
+            flags |= cur.y;
+            uint32_t const e = cur.y & ~WIN_SOLID_NEAR;
+            for (uint32_t s0 = cur.x; s0 < e; s0 += C) {
+                V2 q[C];
+                load_chunk<C>(q, opos, s0);
+                if (!take_chunk<CAP, C, W>(ent, lane, p, self, same, q, s0, e, oorig, maxd,
+                                           cut2, cnt)) {
+                    return cnt;  // overflow (-1)
+                }
+            }
+            cur = nxt;
+        }
+        if (near) { *near = flags & WIN_SOLID_NEAR; }
+        if (cnt > 0) { net_sort<16, W>(ent, lane, cnt, oorig, opos, p); }
+        return cnt;
+    }
+    bool go = true;
+    for (int k = 0; k < 9 && go; ++k) {
         uint2 const nxt = win[k < 8 ? k + 1 : 8];  // next range in flight
         flags |= cur.y;
         uint32_t const e = cur.y & ~WIN_SOLID_NEAR;
@@ -856,13 +899,33 @@ __device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
             load_chunk<C>(q, opos, s0);
             if (!take_chunk<CAP, C, W>(ent, lane, p, self, same, q, s0, e, oorig, maxd, cut2,
                                        cnt)) {
-                return cnt;  // overflow (-1) or a full 64-entry list (insertion-sorted)
+                go = false;  // overflow (-1) or a full 64-entry list
+                break;
             }
         }
         cur = nxt;
     }
+    if (cnt < 0) { return cnt; }
+    // A full list stops before the last ranges: their solid flags are taken
+    // from the window directly.
+    for (int k = 0; k < 9 && !go; ++k) { flags |= win[k].y; }
     if (near) { *near = flags & WIN_SOLID_NEAR; }
-    if (net_sorted<CAP>() && cnt > 0) { net_sort16<W>(ent, lane, cnt, oorig, opos, p); }
+    // The smallest network that covers the wave's longest list.
+    int wmax = cnt;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int const o = __shfl_xor(wmax, off);
+        wmax = o > wmax ? o : wmax;
+    }
+    if (wmax > 0) {  // also turns a single entry's |v|^2 into its distance
+        if (wmax <= 16) {
+            net_sort<16, W>(ent, lane, cnt, oorig, opos, p);
+        } else if (CAP <= 32 || wmax <= 32) {
+            if constexpr (CAP >= 32) { net_sort<32, W>(ent, lane, cnt, oorig, opos, p); }
+        } else {
+            if constexpr (CAP >= 64) { net_sort<64, W>(ent, lane, cnt, oorig, opos, p); }
+        }
+    }
     return cnt;
 }
 

@@ -173,3 +173,41 @@ def test_frame_step_near_rest(gpu, method, speed):
     g = ctx.get_state()
     assert_bits_equal(g.positions, pos, "near-rest positions")
     assert_bits_equal(g.velocities, v, "near-rest velocities")
+
+
+@pytest.mark.parametrize("method", ["iisph", "pcisph"])
+def test_full_fluid_list_near_solids(gpu, method):
+    """Particles whose fluid list holds 64 neighbours (the first 64 accepted in
+    scan order, neighborhood.h:233-250) next to a wall: their solid lists are
+    built as well (the reference's fluid->solid search is a separate pass), and
+    both lists equal the oracle's, in the first build and in a capped build
+    whose long lists go to the full-capacity pass."""
+    base = scenes.dam_break(8, 8, 0.005, 2e-4, seed=47)
+    h = base.support
+    rng = np.random.default_rng(47)
+    # A dense block (spacing h / 3) resting on the bottom wall's inner face.
+    g = np.stack(np.meshgrid(np.arange(36), np.arange(12)), -1).reshape(-1, 2)
+    pos = (g * (h / 3.0) + np.array([0.3, 0.5 * h])).astype(np.float32)
+    pos += rng.uniform(-1e-3 * h, 1e-3 * h, pos.shape).astype(np.float32)
+    sc = scenes.DamBreak(np.ascontiguousarray(pos), np.zeros_like(pos), base.solids,
+                         base.support, base.spacing, base.dt, base.mass)
+    cfg = scenes.config_for(sc, method)
+    ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
+    ctx.set_state(sb.State(sc.positions, sc.velocities))
+    orc = oracle_scene(sc, cfg)
+    ctx.frame_init(method)
+    orc.frame_init(method)
+    for step in range(2):
+        ctx.sub_step(method, 0.0, 1e-6)
+        orc.sub_step(method, 0.0, 1e-6)
+        fc, fi, _, _ = ctx.neighborhood(0)
+        oc, oi = orc.neighborhood(0)
+        assert np.array_equal(fc, oc), f"step {step} fluid counts"
+        mask = np.arange(64)[None, :] < oc[:, None]
+        assert np.array_equal(np.where(mask, fi, 0), np.where(mask, oi, 0)), f"step {step}"
+        scn, si, _, _ = ctx.neighborhood(1)
+        osc, osi = orc.neighborhood(1)
+        assert np.array_equal(scn, osc), f"step {step} solid counts"
+        mask = np.arange(64)[None, :] < osc[:, None]
+        assert np.array_equal(np.where(mask, si, 0), np.where(mask, osi, 0)), f"step {step}"
+        assert np.sum((oc == 64) & (osc > 0)) > 0  # the case under test occurs
