@@ -192,6 +192,10 @@ struct SpatialIndex {
     DBuf<uint32_t> ncells;  // device cell count of the last indexing run
     DBuf<uint4> set;        // static cell set (solids), see CellSet
     uint32_t set_mask = 0;
+    std::vector<uint64_t> set_keys;  // the set's cell keys and scene tags (host copy)
+    std::vector<uint32_t> set_tags;
+    DBuf<uint32_t> set_bits;         // the set's scene-0 keys below 4^set_bits_for as bits
+    int set_bits_for = 0;
     DBuf<uint32_t> orig, rank;
     DBuf<uint2> win;        // [cell][9] slot ranges of the 3x3 neighbour cells
     DBuf<int2> grid;
@@ -199,6 +203,25 @@ struct SpatialIndex {
     uint32_t* sorted_vals = nullptr;
     int key_passes = 0;
     int scene_passes = 0;
+    // Dense cell index (see DENSE_MAX_BITS): tables for up to 4^dense_alloc
+    // bins; the counts are zero between runs.
+    DBuf<uint32_t> dcount, doffset, drank, dscratch;
+    int dense_alloc = 0;
+    uint64_t direct_kmax = 0;  // key maximum of the last run that read the range back
+
+    // Tables for 4^bits bins (allocated outside any graph capture).
+    hipError_t ensure_dense(int bits, uint32_t c, hipStream_t s) {
+        if (bits <= dense_alloc && drank.p) { return hipSuccess; }
+        size_t const bins = (size_t{1} << (2 * bits)) + 2;
+        hipError_t e;
+        if ((e = dcount.alloc(bins)) || (e = doffset.alloc(bins)) || (e = drank.alloc(c)) ||
+            (e = dscratch.alloc(scan_scratch_words(bins) + 64))) {
+            dense_alloc = 0;
+            return e;
+        }
+        dense_alloc = bits;
+        return hipMemsetAsync(dcount.p, 0, bins * sizeof(uint32_t), s);
+    }
 
     // Keeps the arrays when they already hold c points.
     hipError_t ensure(uint32_t c) { return (keys0.p && cap >= c) ? hipSuccess : alloc(c, false); }
@@ -349,6 +372,14 @@ struct Engine {
 
     Pinned* host = nullptr;
     int passes_hint = -1;
+    // Dense cell index of the fluid (one scene): table bits for the next
+    // sub-step (0: radix sort), and a sticky switch-off after a crowded cell.
+    int dense_bits = 0;
+    bool dense_disabled = false;
+    // The indexing choices of the current sub-step attempt (pass count, dense
+    // bits), fixed when it begins: enqueue, graph key and read-back agree.
+    int prep_passes = -1;
+    int prep_dense = 0;
     bool keys_checked = false;
     bool sort_short = false;
     KernelTimer timer;
@@ -671,7 +702,7 @@ hipError_t ensure_geometry(Engine* g) {
 hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
                         uint32_t const* scene_of, uint32_t nscenes, int passes_hint,
                         IndexArgs gathers, bool time_it, SpatialIndex const* other = nullptr,
-                        uint2* owin = nullptr) {
+                        uint2* owin = nullptr, int dense_bits = 0) {
     hipStream_t const s = g->stream;
     auto run = [&](char const* name, auto&& f) -> hipError_t {
         if (time_it) { return timed(g, name, f); }
@@ -689,30 +720,59 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     a.grid = si.grid.p;
     a.keystate = g->keystate.p;
     a.key_partials = g->key_partials.p;
+    bool const dense = dense_bits > 0 && passes_hint >= 0 && nscenes == 1 &&
+                       si.dense_alloc >= dense_bits;
+    if (dense) {
+        a.dense_limit = uint64_t{1} << (2 * dense_bits);
+        a.dcount = si.dcount.p;
+        a.doffset = si.doffset.p;
+        a.drank = si.drank.p;
+    }
     if ((e = run("keys", [&] { launch_keys(a, s); }))) { return e; }
-    if (passes_hint < 0) {
+    if (dense) {
+        // Counting sort over the dense table: bin offsets (the counts are
+        // cleared for the next run), scatter, then the cells stage below puts
+        // each cell in original index order.
+        si.key_passes = 0;
+        si.scene_passes = 0;
+        si.sorted_keys = si.keys1.p;
+        si.sorted_vals = si.vals1.p;
+        a.sorted_keys = si.sorted_keys;
+        a.sorted_vals = si.sorted_vals;
+        uint32_t const bins = static_cast<uint32_t>(a.dense_limit) + 1u;
+        if ((e = run("radix_sort", [&] {
+                 launch_exclusive_scan_clear(si.dcount.p, si.doffset.p, bins, si.dscratch.p,
+                                             si.doffset.p + bins, s);
+                 launch_dense_scatter(a, s);
+             }))) {
+            return e;
+        }
+    } else if (passes_hint < 0) {
         if ((e = hipMemcpyAsync(&g->host->direct, g->keystate.p, sizeof(KeyState),
                                 hipMemcpyDeviceToHost, s))) {
             return e;
         }
         if ((e = hipStreamSynchronize(s))) { return e; }
         si.key_passes = radix_pass_count(g->host->direct.kmin, g->host->direct.kmax);
+        si.direct_kmax = g->host->direct.kmax;
     } else {
         si.key_passes = passes_hint;
     }
-    si.scene_passes = scene_pass_count(nscenes);
-    int which = 0;
-    if ((e = run("radix_sort", [&] {
-             which = launch_radix_sort(si.keys0.p, si.vals0.p, si.keys1.p, si.vals1.p, n,
-                                       si.key_passes, scene_of, si.scene_passes, si.counts.p,
-                                       si.scan_scratch.p, s);
-         }))) {
-        return e;
+    if (!dense) {
+        si.scene_passes = scene_pass_count(nscenes);
+        int which = 0;
+        if ((e = run("radix_sort", [&] {
+                 which = launch_radix_sort(si.keys0.p, si.vals0.p, si.keys1.p, si.vals1.p, n,
+                                           si.key_passes, scene_of, si.scene_passes, si.counts.p,
+                                           si.scan_scratch.p, s);
+             }))) {
+            return e;
+        }
+        si.sorted_keys = which ? si.keys1.p : si.keys0.p;
+        si.sorted_vals = which ? si.vals1.p : si.vals0.p;
     }
-    si.sorted_keys = which ? si.keys1.p : si.keys0.p;
-    si.sorted_vals = which ? si.vals1.p : si.vals0.p;
     a.key_passes = si.key_passes;
-    a.check_keys = passes_hint >= 0 ? 1 : 0;
+    a.check_keys = (passes_hint >= 0 && !dense) ? 1 : 0;
     a.sorted_keys = si.sorted_keys;
     a.sorted_vals = si.sorted_vals;
     a.heads = si.heads.p;
@@ -731,6 +791,11 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
         w.other_set = CellSet{other->set.p, other->set_mask};
     }
     w.max_cells = n;
+    if (dense) {
+        w.dense_limit = a.dense_limit;
+        w.doffset = si.doffset.p;
+        if (other && other->set_bits_for >= dense_bits) { w.other_bits = other->set_bits.p; }
+    }
     w.win = si.win.p;
     w.owin = owin;
     w.report = &g->keystate.p->cells;
@@ -771,6 +836,27 @@ emsph_status build_cell_set(Engine* g, SpatialIndex& si) {
                            g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     si.set_mask = cap - 1;
+    si.set_keys = std::move(keys);
+    si.set_tags = std::move(tags);
+    si.set_bits_for = 0;  // the bitmap (if any) is rebuilt for the new set
+    return EMSPH_OK;
+}
+
+// Bitmap of the set's scene-0 cells over the dense table of 4^bits bins (for
+// the dense windows; outside any graph capture).
+emsph_status build_set_bits(Engine* g, SpatialIndex& si, int bits) {
+    if (si.set_bits_for == bits && si.set_bits.p) { return EMSPH_OK; }
+    uint64_t const bins = uint64_t{1} << (2 * bits);
+    std::vector<uint32_t> words(static_cast<size_t>((bins + 31) / 32), 0u);
+    for (size_t c = 0; c < si.set_keys.size(); ++c) {
+        uint64_t const k = si.set_keys[c];
+        if (si.set_tags[c] == 0u && k < bins) { words[k >> 5] |= 1u << (k & 31u); }
+    }
+    HIP_TRY(si.set_bits.alloc(words.size()));
+    HIP_TRY(hipMemcpyAsync(si.set_bits.p, words.data(), words.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    si.set_bits_for = bits;
     return EMSPH_OK;
 }
 
@@ -893,6 +979,23 @@ int pick_build_cap(KeyState const& k, uint32_t n) {
         if (e_ != hipSuccess) { return e_; } \
     } while (0)
 
+// Table bits B of a dense cell index holding keys up to kmax (4^B > kmax),
+// or 0 when the table would exceed 4^DENSE_MAX_BITS bins or 8 n + 2^16.
+int dense_bits_for(uint64_t kmax, uint32_t n) {
+    int bits = 1;
+    while (bits <= DENSE_MAX_BITS && (kmax >> (2 * bits)) != 0) { ++bits; }
+    if (bits > DENSE_MAX_BITS) { return 0; }
+    uint64_t const bins = uint64_t{1} << (2 * bits);
+    return bins <= 8ull * n + 65536ull ? bits : 0;
+}
+
+// The dense table bits the next indexing run uses (0: radix sort).
+int dense_for(Engine const* g) {
+    bool const ok = g->passes_hint >= 0 && g->M == 1 && !g->dense_disabled &&
+                    g->dense_bits > 0 && g->fidx.dense_alloc >= g->dense_bits;
+    return ok ? g->dense_bits : 0;
+}
+
 // Key-state read-backs into pinned slot k: enqueue, then wait and take it in.
 hipError_t read_async(Engine* g, int k) {
     hipError_t e = hipMemcpyAsync(&g->host->keys[k], g->keystate.p, sizeof(KeyState),
@@ -910,6 +1013,8 @@ hipError_t read_wait(Engine* g, int k) {
         g->keys_checked = true;
         g->sort_short = g->keys.sort_short != 0u;
         g->passes_hint = radix_pass_count(g->keys.kmin, g->keys.kmax);
+        g->dense_bits = dense_bits_for(g->keys.kmax, g->N);
+        if (g->keys.dense_crowded) { g->dense_disabled = true; }
     }
     return hipSuccess;
 }
@@ -992,6 +1097,8 @@ int first_decision(Engine const* g, int kind, int first_check) {
 
 // Host-side state at the start of a sub-step attempt.
 void begin_attempt(Engine* g) {
+    g->prep_passes = g->passes_hint;
+    g->prep_dense = dense_for(g);
     g->keys_checked = g->passes_hint < 0;
     g->sort_short = false;
     g->last_iterations = 0;
@@ -1016,8 +1123,9 @@ hipError_t enqueue_prepare(Engine* g, int build_mode, bool dfsph) {
     ga.step_scenes = g->scenes.p;
     ga.step_stats = g->stats.p;
     ga.step_nscenes = g->M;
-    HIP_PASS(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->passes_hint, ga,
-                          g->timer.enabled, g->NS ? &g->sidx : nullptr, g->swin.p));
+    HIP_PASS(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->prep_passes, ga,
+                          g->timer.enabled, g->NS ? &g->sidx : nullptr, g->swin.p,
+                          g->prep_dense));
     return timed(g, "build", [&] { launch_build(build_mode, g->build_cap, b, g->stream); });
 }
 
@@ -1025,12 +1133,25 @@ hipError_t enqueue_prepare(Engine* g, int build_mode, bool dfsph) {
 // after a graph replay, which runs none of the host code).
 void finish_prepare(Engine* g) {
     SpatialIndex& si = g->fidx;
-    if (g->passes_hint < 0) { g->passes_hint = si.key_passes; }
-    si.key_passes = g->passes_hint;
-    si.scene_passes = scene_pass_count(g->M);
-    bool const odd = ((si.key_passes + si.scene_passes) & 1) != 0;
-    si.sorted_keys = odd ? si.keys1.p : si.keys0.p;
-    si.sorted_vals = odd ? si.vals1.p : si.vals0.p;
+    if (g->prep_passes < 0) {
+        // index_points read the key range back and sorted exactly with the
+        // pass count it left in si.key_passes.
+        if (g->passes_hint < 0) { g->passes_hint = si.key_passes; }
+        g->dense_bits = dense_bits_for(si.direct_kmax, g->N);
+    } else {
+        si.key_passes = g->prep_passes;
+    }
+    if (g->prep_dense) {
+        si.key_passes = 0;
+        si.scene_passes = 0;
+        si.sorted_keys = si.keys1.p;
+        si.sorted_vals = si.vals1.p;
+    } else {
+        si.scene_passes = scene_pass_count(g->M);
+        bool const odd = ((si.key_passes + si.scene_passes) & 1) != 0;
+        si.sorted_keys = odd ? si.keys1.p : si.keys0.p;
+        si.sorted_vals = odd ? si.vals1.p : si.vals0.p;
+    }
     g->have_lists = true;
 }
 
@@ -1039,12 +1160,13 @@ void finish_prepare(Engine* g) {
 // advance (sort pass count known, no kernel timing), else enqueued directly.
 template <class F>
 hipError_t segment(Engine* g, int method, int seg, F&& enqueue) {
-    if (!g->use_graphs || g->timer.enabled || g->passes_hint < 0 || g->user_fn) {
+    if (!g->use_graphs || g->timer.enabled || g->prep_passes < 0 || g->user_fn) {
         return enqueue();
     }
     std::array<uint64_t, 10> const key{
       static_cast<uint64_t>(method), static_cast<uint64_t>(seg),
-      static_cast<uint64_t>(g->passes_hint), static_cast<uint64_t>(g->build_cap), g->layout_gen,
+      static_cast<uint64_t>(g->prep_passes) + 256u * static_cast<uint64_t>(g->prep_dense),
+      static_cast<uint64_t>(g->build_cap), g->layout_gen,
       static_cast<uint64_t>(g->store_geometry), g->N, g->ntiles, static_cast<uint64_t>(g->nb),
       static_cast<uint64_t>(g->xcd_remap)};
     auto it = g->graphs.find(key);
@@ -1361,6 +1483,17 @@ emsph_status engine_sub_step(Engine* g, int method) {
     ST_TRY(check_method(method));
     if (g->N == 0 || g->ntiles == 0) { return EMSPH_OK; }  // N = 0: the reference reads block_indices.back()
     HIP_TRY(hipSetDevice(g->device));
+    // Dense index tables for the bits the last key range asks for (allocated
+    // here, outside any graph capture; a new table is a new layout).
+    if (g->M == 1 && !g->dense_disabled && g->dense_bits > g->fidx.dense_alloc) {
+        if (g->fidx.ensure_dense(g->dense_bits, g->cap, g->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            g->dense_disabled = true;
+        } else if (g->NS && build_set_bits(g, g->sidx, g->fidx.dense_alloc) != EMSPH_OK) {
+            g->dense_disabled = true;
+        }
+        ++g->layout_gen;
+    }
     // A sort run with the previous sub-step's pass count is checked at the
     // first read-back; when it was short the sub-step is redone with the exact
     // count (the second attempt sorts exactly and cannot be short).
@@ -2608,6 +2741,8 @@ emsph_status emsph_pool_create(size_t fluid_capacity, size_t solid_capacity, int
     P->solid_cap = static_cast<uint32_t>(solid_capacity);
     P->max_scenes = static_cast<uint32_t>(max_scenes);
     Engine* g = &P->e;
+    // A pool's memory is its admission budget: no dense index tables.
+    g->dense_disabled = true;
     emsph_status st = engine_init(g);
     if (st) {
         delete P;

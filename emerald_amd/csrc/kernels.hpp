@@ -152,8 +152,18 @@ struct KeyState {
     unsigned int cells;           // cells of the last indexing run
     unsigned int long_lists;      // particles of the last build with a list above 16
     int frame_remaining;          // scene 0's frame quanta left after this sub-step (-1: off)
-    unsigned int pad;
+    unsigned int dense_crowded;   // a cell of the dense index exceeded DENSE_CELL_MAX
 };
+
+// Dense cell index (one scene): cells counted in a table of 4^B bins indexed
+// by Morton key (bin 4^B collects keys outside the table), an exclusive scan
+// gives each cell's first sorted slot, and the particles are scattered there
+// and put in increasing original index within their cell, which is the order
+// of the stable sort by key. A key outside the table or a cell above
+// DENSE_CELL_MAX particles flags the indexing run as short (it is redone by
+// the sort); every slot still receives a valid particle index.
+constexpr int DENSE_MAX_BITS = 12;
+constexpr uint32_t DENSE_CELL_MAX = 256;
 
 // Neighbour-list capacities of the build kernel's LDS list (the full one is
 // the reference's NEIGHBORHOOD_MAX_COUNT).
@@ -285,6 +295,9 @@ size_t scan_scratch_words(size_t n);
 size_t radix_counts_words(size_t n);
 void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32_t* scratch,
                            uint32_t* total, hipStream_t s);
+// Same, then every input element set to zero (in != out).
+void launch_exclusive_scan_clear(uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch,
+                                 uint32_t* total, hipStream_t s);
 int radix_pass_count(uint64_t kmin, uint64_t kmax);
 // Stable LSD sort of (key, val): key_passes 8-bit digits of the key, then
 // scene_passes 8-bit digits of scene_of[val] (scene the most significant).
@@ -324,6 +337,11 @@ struct IndexArgs {
     uint32_t* cell_tag;
     uint32_t* orig;
     uint32_t* rank;
+    // dense cell index (dense_limit = 4^B bins; 0: the radix sort path)
+    uint64_t dense_limit;
+    uint32_t* dcount;       // [dense_limit + 1] per-bin counts (zero between runs)
+    uint32_t* doffset;      // [dense_limit + 2] exclusive scan of dcount, then n
+    uint32_t* drank;        // per point (original order): rank within its bin
     // gathers into sorted order (optional)
     V2 const* vel_in;
     V2* pos_out;
@@ -341,6 +359,13 @@ struct WindowArgs {
     CellList other;          // ranges of the other cells
     CellSet other_set;       // lookup of the other cells; slots == nullptr: none
     uint32_t max_cells;      // launch bound
+    // Dense index of the self cells (dense_limit != 0): a neighbour cell's
+    // range is [doffset[key], doffset[key + 1]) for keys below dense_limit.
+    uint64_t dense_limit;
+    uint32_t const* doffset;
+    // With the dense index: one bit per bin set where the other index (the
+    // solids) has a cell with that key, so most cells skip the set lookups.
+    uint32_t const* other_bits;
     uint2* win;
     uint2* owin;
     unsigned int* report;    // optional: receives the cell count
@@ -348,6 +373,9 @@ struct WindowArgs {
 void launch_cell_windows(WindowArgs const& w, hipStream_t s);
 
 void launch_keys(IndexArgs const& a, hipStream_t s);
+// Dense index: scatter of (key, index) to doffset[bin] + drank into
+// sorted_keys / sorted_vals (bin = min(key, dense_limit)).
+void launch_dense_scatter(IndexArgs const& a, hipStream_t s);
 void launch_cell_heads(IndexArgs const& a, hipStream_t s);
 void launch_cells_finalize(IndexArgs const& a, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);

@@ -209,10 +209,11 @@ __global__ __launch_bounds__(1024) void scan_top(uint32_t* __restrict__ bsum, ui
 // before it (read here, so no separate top-level scan runs; used while nb is
 // small), and the last block writes the grand total to *total (if not null);
 // with nb == 0, bsum already holds the scanned block offsets.
-__global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t const* __restrict__ in,
+__global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t* __restrict__ in,
                                                         uint32_t* __restrict__ out, uint32_t n,
                                                         uint32_t const* __restrict__ bsum,
-                                                        uint32_t nb, uint32_t* __restrict__ total) {
+                                                        uint32_t nb, uint32_t* __restrict__ total,
+                                                        int clear_in) {
     __shared__ uint32_t tile[SC_TILE];
     __shared__ uint32_t block_offset;
     if (nb) {
@@ -235,6 +236,7 @@ __global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t const* __restri
     for (int r = 0; r < SC_ITEMS; ++r) {
         uint32_t const i = base + r * SC_THREADS + threadIdx.x;
         tile[r * SC_THREADS + threadIdx.x] = (i < n) ? in[i] : 0u;
+        if (clear_in && i < n) { in[i] = 0u; }  // each element is read by this thread only
     }
     __syncthreads();
     // Each thread scans SC_ITEMS consecutive elements.
@@ -261,19 +263,31 @@ __global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t const* __restri
 
 size_t scan_scratch_words(size_t n) { return (n + SC_TILE - 1) / SC_TILE + 1; }
 
-void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32_t* scratch,
-                           uint32_t* total, hipStream_t s) {
+namespace {
+void exclusive_scan(uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch, uint32_t* total,
+                    int clear_in, hipStream_t s) {
     if (n == 0) { return; }
     uint32_t const nb = (n + SC_TILE - 1) / SC_TILE;
     hipLaunchKernelGGL(scan_reduce, dim3(nb), dim3(SC_THREADS), 0, s, in, n, scratch);
     if (nb <= SCAN_DIRECT_BLOCKS) {
         hipLaunchKernelGGL(scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, scratch, nb,
-                           total);
+                           total, clear_in);
     } else {
         hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, s, scratch, nb, total);
         hipLaunchKernelGGL(scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, scratch, 0u,
-                           nullptr);
+                           nullptr, clear_in);
     }
+}
+}  // namespace
+
+void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32_t* scratch,
+                           uint32_t* total, hipStream_t s) {
+    exclusive_scan(const_cast<uint32_t*>(in), out, n, scratch, total, 0, s);
+}
+
+void launch_exclusive_scan_clear(uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch,
+                                 uint32_t* total, hipStream_t s) {
+    exclusive_scan(in, out, n, scratch, total, 1, s);
 }
 
 size_t radix_counts_words(size_t n) {

@@ -152,6 +152,29 @@ __global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
         a.keys[i] = key;
         a.vals[i] = i;
         if (a.grid) { a.grid[i] = g; }
+        if (a.dense_limit) {
+            // Dense index: count the point into its bin (keys outside the table
+            // go to the extra bin dense_limit); the count before it is its
+            // rank. Runs of one bin across adjacent lanes (neighbouring points
+            // often share a cell) take one atomic for the whole run.
+            uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
+            uint32_t const lane = __lane_id();
+            uint64_t const prev = __shfl_up(bin, 1);
+            unsigned long long const heads = __ballot(lane == 0 || prev != bin);
+            unsigned long long const active = __ballot(1);
+            unsigned long long const le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+            uint32_t const head = 63u - static_cast<uint32_t>(__clzll(heads & le));
+            unsigned long long const after = heads & ~le;
+            uint32_t const end = after ? static_cast<uint32_t>(__ffsll(after) - 1) : 64u;
+            uint32_t const last = 63u - static_cast<uint32_t>(__clzll(active));
+            uint32_t base = 0;
+            if (lane == head) {
+                uint32_t const run = (end <= last ? end : last + 1u) - head;
+                base = atomicAdd(a.dcount + bin, run);
+            }
+            base = __shfl(base, static_cast<int>(head));
+            a.drank[i] = base + (lane - head);
+        }
         key_min = key < key_min ? key : key_min;
         key_max = key > key_max ? key : key_max;
     }
@@ -283,11 +306,31 @@ __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) 
         a.keystate->kmax = key_max;
         a.keystate->frame_remaining =
           (a.step_stats && a.step_scenes[0].frame.on) ? a.step_scenes[0].frame.next_remaining : -1;
-        a.keystate->sort_short = 0u;
+        // A dense index whose table does not hold every key is short: redone.
+        a.keystate->sort_short = (a.dense_limit && key_max >= a.dense_limit) ? 1u : 0u;
+        a.keystate->dense_crowded = 0u;
         a.keystate->overflow_count = 0u;
         a.keystate->max_count = 0u;
         a.keystate->long_lists = 0u;
     }
+}
+
+// Dense index scatter: point i goes to the first slot of its bin plus its
+// rank (the ranks of one bin are a permutation of 0 .. count - 1, so every
+// slot receives exactly one point).
+__global__ __launch_bounds__(256) void k_dense_scatter(IndexArgs a) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) { return; }
+    uint64_t const key = a.keys[i];
+    uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
+    uint32_t const t = a.doffset[bin] + a.drank[i];
+    const_cast<uint64_t*>(a.sorted_keys)[t] = key;
+    const_cast<uint32_t*>(a.sorted_vals)[t] = i;
+}
+
+void launch_dense_scatter(IndexArgs const& a, hipStream_t s) {
+    if (!a.n) { return; }
+    hipLaunchKernelGGL(k_dense_scatter, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
 
 void launch_keys(IndexArgs const& a, hipStream_t s) {
@@ -325,6 +368,35 @@ __global__ __launch_bounds__(256) void k_cell_heads(IndexArgs a) {
         if (!covered) { a.keystate->sort_short = 1u; }
     }
     if (t >= a.n) { return; }
+    if (a.dense_limit) {
+        // Dense index (one scene): a cell starts at its bin's first slot. Its
+        // head puts the cell's points in increasing original index, the order
+        // of the stable sort (the scatter's order within a bin is the order
+        // of the counting atomics).
+        uint64_t const key = a.sorted_keys[t];
+        uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
+        uint32_t const b = a.doffset[bin];
+        bool const head = (t == b);
+        a.heads[t] = head ? 1u : 0u;
+        if (!head) { return; }
+        uint32_t const e = a.doffset[bin + 1];
+        if (e - b > DENSE_CELL_MAX) {
+            atomicOr(&a.keystate->dense_crowded, 1u);
+            atomicOr(&a.keystate->sort_short, 1u);
+            return;
+        }
+        uint32_t* v = const_cast<uint32_t*>(a.sorted_vals);
+        for (uint32_t k = b + 1; k < e; ++k) {
+            uint32_t const x = v[k];
+            uint32_t j = k;
+            while (j > b && v[j - 1] > x) {
+                v[j] = v[j - 1];
+                --j;
+            }
+            v[j] = x;
+        }
+        return;
+    }
     bool head = (t == 0);
     if (!head) {
         head = a.sorted_keys[t] != a.sorted_keys[t - 1] ||
@@ -616,8 +688,67 @@ __global__ __launch_bounds__(256) void k_cell_windows(WindowArgs w) {
     }
 }
 
+// Windows over the dense index (one scene), one thread per cell: a neighbour
+// cell's range is read from the bin offsets; the solid cells are looked up
+// only where the solid bitmap has the key (or the key is outside the table),
+// and a cell's solid windows are written only when some neighbour cell holds
+// solids (the build reads them only then: WIN_SOLID_NEAR).
+__global__ __launch_bounds__(256) void k_cell_windows_dense(WindowArgs w) {
+    uint32_t const n = *w.self.count;
+    uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0 && w.report) { *w.report = n; }
+    if (c >= n) { return; }
+    uint64_t const key = w.self.key[c];
+    uint32_t const tag = w.self.tag[c];
+    uint32_t const gx0 = compact_bits_2(key);
+    uint32_t const gy0 = compact_bits_2(key >> 1);
+    uint2 r[9];
+    uint64_t nks[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        // neighborhood.h:233-243: j outer, i inner, each from g - 1 to g + 1.
+        uint32_t const gx = gx0 + static_cast<uint32_t>(k % 3) - 1u;
+        uint32_t const gy = gy0 + static_cast<uint32_t>(k / 3) - 1u;
+        uint64_t const nk = morton2(static_cast<int32_t>(gx), static_cast<int32_t>(gy));
+        nks[k] = nk;
+        r[k] = make_uint2(0u, 0u);
+        if (nk < w.dense_limit) {
+            uint32_t const b = w.doffset[nk];
+            uint32_t const e = w.doffset[nk + 1];
+            if (e > b) { r[k] = make_uint2(b, e); }
+        }
+    }
+    if (w.owin) {
+        uint2 o[9];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            uint64_t const nk = nks[k];
+            bool const maybe = w.other_set.slots &&
+                               (nk >= w.dense_limit || !w.other_bits ||
+                                ((w.other_bits[nk >> 5] >> (nk & 31u)) & 1u));
+            o[k] = maybe ? cell_set_range(w.other_set, w.other, nk, tag) : make_uint2(0u, 0u);
+            if (o[k].y > o[k].x) {
+                r[k].y |= WIN_SOLID_NEAR;
+                any = true;
+            }
+        }
+        if (any) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) { w.owin[static_cast<size_t>(c) * 9 + k] = o[k]; }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { w.win[static_cast<size_t>(c) * 9 + k] = r[k]; }
+}
+
 void launch_cell_windows(WindowArgs const& w, hipStream_t s) {
     if (!w.max_cells) { return; }
+    if (w.dense_limit) {
+        hipLaunchKernelGGL(k_cell_windows_dense, dim3((w.max_cells + 255) / 256), dim3(256), 0, s,
+                           w);
+        return;
+    }
     size_t blocks = (static_cast<size_t>(w.max_cells) * 9 + 255) / 256;
     if (blocks > 4096) { blocks = 4096; }
     hipLaunchKernelGGL(k_cell_windows, dim3(blocks), dim3(256), 0, s, w);
