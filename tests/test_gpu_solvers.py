@@ -211,3 +211,66 @@ def test_full_fluid_list_near_solids(gpu, method):
         mask = np.arange(64)[None, :] < osc[:, None]
         assert np.array_equal(np.where(mask, si, 0), np.where(mask, osi, 0)), f"step {step}"
         assert np.sum((oc == 64) & (osc > 0)) > 0  # the case under test occurs
+
+
+def test_dense_index_table_growth_redo(gpu):
+    """A one-scene engine indexes its cells in a dense table sized from the
+    last key range; particles that leave it flag the sub-step, which is redone
+    with the radix sort and an exact range. A block moving right across a
+    power-of-two cell coordinate crosses that boundary: every sub-step equals
+    the oracle bit for bit and at least one sub-step was redone."""
+    method = "iisph"
+    base = scenes.dam_break(12, 16, 0.005, 2e-4, seed=53)
+    pos = base.positions + np.array([0.24, 0.0], np.float32)  # cells 24..30 in x (table 32)
+    vel = np.tile(np.array([[8.0, 0.0]], np.float32), (len(pos), 1))
+    sc = scenes.DamBreak(np.ascontiguousarray(pos), vel, base.solids, base.support, base.spacing,
+                         base.dt, base.mass)
+    cfg = scenes.config_for(sc, method)
+    ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
+    ctx.set_state(sb.State(sc.positions, sc.velocities))
+    orc = oracle_scene(sc, cfg)
+    ctx.frame_init(method)
+    orc.frame_init(method)
+    ctx.iteration_totals(reset=True)
+    for k in range(30):
+        st = ctx.sub_step(method, k * sc.dt, sc.dt)
+        ost = orc.sub_step(method, k * sc.dt, sc.dt)
+        assert st.iterations == ost.iterations, f"step {k}"
+    g = ctx.get_state()
+    p, v = orc.state_f32()
+    assert_bits_equal(g.positions, p, "positions")
+    assert_bits_equal(g.velocities, v, "velocities")
+    assert float(p[:, 0].max()) > 0.32  # the block crossed cell coordinate 32
+    assert ctx.iteration_totals()[3] >= 1  # a sub-step was redone
+
+
+def test_dense_index_crowded_cell(gpu):
+    """300 particles inside one cell (above the dense index's 256 per cell):
+    the dense build flags the sub-step, it is redone with the radix sort and
+    the dense index stays off; lists and states equal the oracle's."""
+    method = "iisph"
+    base = scenes.dam_break(12, 16, 0.005, 2e-4, seed=59)
+    h = base.support
+    rng = np.random.default_rng(59)
+    crowd = (np.array([0.3, 0.3]) + rng.uniform(0.05, 1.95, (300, 2)) * h).astype(np.float32)
+    pos = np.ascontiguousarray(np.concatenate([base.positions, crowd]).astype(np.float32))
+    sc = scenes.DamBreak(pos, np.zeros_like(pos), base.solids, base.support, base.spacing,
+                         base.dt, base.mass)
+    cfg = scenes.config_for(sc, method)
+    ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
+    ctx.set_state(sb.State(sc.positions, sc.velocities))
+    orc = oracle_scene(sc, cfg)
+    ctx.frame_init(method)
+    orc.frame_init(method)
+    for k in range(3):
+        ctx.sub_step(method, 0.0, 1e-6)
+        orc.sub_step(method, 0.0, 1e-6)
+        gc, gi, _, _ = ctx.neighborhood(0)
+        oc, oi = orc.neighborhood(0)
+        assert np.array_equal(gc, oc), f"step {k} counts"
+        mask = np.arange(64)[None, :] < oc[:, None]
+        assert np.array_equal(np.where(mask, gi, 0), np.where(mask, oi, 0)), f"step {k}"
+    g = ctx.get_state()
+    p, v = orc.state_f32()
+    assert_bits_equal(g.positions, p, "positions")
+    assert_bits_equal(g.velocities, v, "velocities")
