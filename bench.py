@@ -346,6 +346,7 @@ def main() -> int:
     ap.add_argument("--served-window", type=float, default=10.0)
     ap.add_argument("--served-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-search", action="store_true")
+    ap.add_argument("--search-only", action="store_true")
     ap.add_argument("--no-small", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -364,6 +365,9 @@ def main() -> int:
         return 0
     if args.served_only:
         print(json.dumps(served_form(args.served_budget_gb, args.served_window)), flush=True)
+        return 0
+    if args.search_only:
+        print(json.dumps(search_form()), flush=True)
         return 0
     log("building scene (1024 x 1024 lattice dam break)")
     scene = scenes.dam_break_1m()
@@ -419,6 +423,11 @@ def main() -> int:
         with open(tpath) as f:
             tj = json.load(f)
         traffic = tj.get(method, {}).get(dom_name)
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "valu.json")
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            valu = json.load(f).get(method, {}).get(dom_name)
     roofline = {
         "bound": "hbm",
         "kernel": dom_name,
@@ -430,6 +439,9 @@ def main() -> int:
         "avg_launch_us": avg_s * 1e6,
         "algorithmic_bytes_per_particle": per,
         "particles_per_launch": n,
+        # Measured share of the SIMDs' cycles taken by VALU issue (rocprof
+        # PMC): the neighbour build is limited by it, not by HBM bytes.
+        "valu_issue_frac": valu,
     }
     kernels = {}
     for k, v in sorted(timing.items(), key=lambda kv: -kv[1][0]):

@@ -2628,19 +2628,42 @@ emsph_status emsph_search_run(emsph_search* s, float cell_size, int store_geomet
         g->N = g->sh[0].n;
         s->cell = cell_size;
     }
+    // After a first run, the next one sorts with that run's key range (the
+    // dense cell index when its table fits) and is redone exactly, inside the
+    // timed region, when the range no longer fits.
+    int passes = -1;
+    int dense = 0;
+    if (s->ran && g->passes_hint >= 0) {
+        passes = g->passes_hint;
+        if (!g->dense_disabled && g->dense_bits > g->fidx.dense_alloc &&
+            g->fidx.ensure_dense(g->dense_bits, g->cap, g->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            g->dense_disabled = true;
+        }
+        if (!g->dense_disabled && g->dense_bits > 0 && g->fidx.dense_alloc >= g->dense_bits) {
+            dense = g->dense_bits;
+        }
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, g->stream));
-    if (g->N) {
+    for (int attempt = 0; attempt < 2 && g->N; ++attempt) {
         IndexArgs ga{};
         ga.pos_out = g->spos.p;
-        HIP_TRY(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, 1, -1, ga, false));
+        HIP_TRY(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, 1,
+                             attempt ? -1 : passes, ga, false, nullptr, nullptr,
+                             attempt ? 0 : dense));
         Batch b = fluid_batch(g);
         b.fl.gw = nullptr;
         launch_build(BUILD_LISTS, g->build_cap, b, g->stream);
         HIP_TRY(hipGetLastError());
         g->have_lists = true;
+        // The key state of this run: list capacity, pass count and dense
+        // table of the next run, and whether this one was short.
+        g->keys_checked = false;
+        HIP_TRY(read_keystate(g));
+        if (!g->sort_short) { break; }
     }
     HIP_TRY(hipEventRecord(e1, g->stream));
     HIP_TRY(hipEventSynchronize(e1));
@@ -2649,7 +2672,6 @@ emsph_status emsph_search_run(emsph_search* s, float cell_size, int store_geomet
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (elapsed_ms) { *elapsed_ms = ms; }
-    if (g->N) { HIP_TRY(read_keystate(g)); }  // list capacity of the next run
     s->ran = true;
     return EMSPH_OK;
 }
