@@ -180,7 +180,14 @@ def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict
         same = same and np.array_equal(a.positions.view(np.uint32), b.positions.view(np.uint32)) \
             and np.array_equal(a.velocities.view(np.uint32), b.velocities.view(np.uint32))
         c.close()
+    # Per-stage GPU time of the batched sub-step, from the engine's HIP events
+    # (timed launches run outside graphs), after the comparison above.
+    batch.set_kernel_timing(True)
+    batch.run_sub_steps(method, 5)
+    stages = batch.kernel_timing()
+    batch.set_kernel_timing(False)
     batch.close()
+    stage_ms = {k: round(v[0] / 5, 4) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])}
     v_batch = n_total * steps / t_batch
     v_plain = n_total * steps / t_plain
     return {
@@ -191,6 +198,7 @@ def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict
         "iterations_min_mean_max": [min(iters), float(np.mean(iters)), max(iters)],
         "dt": "per scene, CFL 0.4 on the device",
         "tuning": {"chosen": chosen, "candidate_ms_per_2_sub_steps": cand_ms},
+        "stage_ms_per_sub_step": stage_ms,
     }
 
 

@@ -731,14 +731,15 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     if ((e = run("keys", [&] { launch_keys(a, s); }))) { return e; }
     if (dense) {
         // Counting sort over the dense table: bin offsets (the counts are
-        // cleared for the next run), scatter, then the cells stage below puts
-        // each cell in original index order.
+        // cleared for the next run), scatter (marking the cell heads), then
+        // the cells stage below puts each cell in original index order.
         si.key_passes = 0;
         si.scene_passes = 0;
         si.sorted_keys = si.keys1.p;
         si.sorted_vals = si.vals1.p;
         a.sorted_keys = si.sorted_keys;
         a.sorted_vals = si.sorted_vals;
+        a.heads = si.heads.p;
         uint32_t const bins = static_cast<uint32_t>(a.dense_limit) + 1u;
         if ((e = run("radix_sort", [&] {
                  launch_exclusive_scan_clear(si.dcount.p, si.doffset.p, bins, si.dscratch.p,
@@ -2435,20 +2436,18 @@ emsph_status emsph_set_cfl(emsph_context* c, float factor) {
     return engine_set_cfl(&c->e, factor);
 }
 
-emsph_status emsph_set_kernel_timing(emsph_context* c, int enable) {
-    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
-    HIP_TRY(timer_flush(&c->e));
-    c->e.timer.enabled = enable != 0;
-    c->e.timer.acc.clear();
+static emsph_status engine_set_kernel_timing(Engine* g, int enable) {
+    HIP_TRY(timer_flush(g));
+    g->timer.enabled = enable != 0;
+    g->timer.acc.clear();
     return EMSPH_OK;
 }
 
-int emsph_get_kernel_timing(emsph_context* c, const char** names, double* total_ms,
-                            int64_t* calls, int max) {
-    if (!c) { return 0; }
-    if (timer_flush(&c->e)) { return -1; }
+static int engine_get_kernel_timing(Engine* g, const char** names, double* total_ms,
+                                    int64_t* calls, int max) {
+    if (timer_flush(g)) { return -1; }
     int k = 0;
-    for (auto const& kv : c->e.timer.acc) {
+    for (auto const& kv : g->timer.acc) {
         if (k >= max) { break; }
         if (names) { names[k] = kv.first.c_str(); }
         if (total_ms) { total_ms[k] = kv.second.first; }
@@ -2456,6 +2455,28 @@ int emsph_get_kernel_timing(emsph_context* c, const char** names, double* total_
         ++k;
     }
     return k;
+}
+
+emsph_status emsph_set_kernel_timing(emsph_context* c, int enable) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    return engine_set_kernel_timing(&c->e, enable);
+}
+
+int emsph_get_kernel_timing(emsph_context* c, const char** names, double* total_ms,
+                            int64_t* calls, int max) {
+    if (!c) { return 0; }
+    return engine_get_kernel_timing(&c->e, names, total_ms, calls, max);
+}
+
+emsph_status emsph_batch_set_kernel_timing(emsph_batch* b, int enable) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    return engine_set_kernel_timing(&b->e, enable);
+}
+
+int emsph_batch_get_kernel_timing(emsph_batch* b, const char** names, double* total_ms,
+                                  int64_t* calls, int max) {
+    if (!b) { return 0; }
+    return engine_get_kernel_timing(&b->e, names, total_ms, calls, max);
 }
 
 //------------------------------------------------------------------------------

@@ -137,72 +137,6 @@ __device__ __forceinline__ void particle_energy(StepConsts const& c, V2 x, V2 v,
 
 //------------------------------------------------------------------------------
 // Keys: sph_common/common.cpp:43-76 (grid coordinate, z key) and
-// neighborhood.cpp:10-16 (index pairs), with the key range reduced on the fly.
-// Each workgroup stores its points' key range in its own partial slot (no
-// atomics: one address taken by every wave serialises at the device-coherent
-// level); k_key_range folds the partials.
-__global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
-    unsigned long long key_min = ~0ull;
-    unsigned long long key_max = 0ull;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
-         i += gridDim.x * blockDim.x) {
-        float const cell = a.scenes[a.scene_of[i]].c.cell;
-        int2 const g = grid_coord(a.pos[i], cell);
-        uint64_t const key = morton2(g.x, g.y);
-        a.keys[i] = key;
-        a.vals[i] = i;
-        if (a.grid) { a.grid[i] = g; }
-        if (a.dense_limit) {
-            // Dense index: count the point into its bin (keys outside the table
-            // go to the extra bin dense_limit); the count before it is its
-            // rank. Runs of one bin across adjacent lanes (neighbouring points
-            // often share a cell) take one atomic for the whole run.
-            uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
-            uint32_t const lane = __lane_id();
-            uint64_t const prev = __shfl_up(bin, 1);
-            unsigned long long const heads = __ballot(lane == 0 || prev != bin);
-            unsigned long long const active = __ballot(1);
-            unsigned long long const le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-            uint32_t const head = 63u - static_cast<uint32_t>(__clzll(heads & le));
-            unsigned long long const after = heads & ~le;
-            uint32_t const end = after ? static_cast<uint32_t>(__ffsll(after) - 1) : 64u;
-            uint32_t const last = 63u - static_cast<uint32_t>(__clzll(active));
-            uint32_t base = 0;
-            if (lane == head) {
-                uint32_t const run = (end <= last ? end : last + 1u) - head;
-                base = atomicAdd(a.dcount + bin, run);
-            }
-            base = __shfl(base, static_cast<int>(head));
-            a.drank[i] = base + (lane - head);
-        }
-        key_min = key < key_min ? key : key_min;
-        key_max = key > key_max ? key : key_max;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long const lo = __shfl_xor(key_min, off);
-        unsigned long long const hi = __shfl_xor(key_max, off);
-        key_min = lo < key_min ? lo : key_min;
-        key_max = hi > key_max ? hi : key_max;
-    }
-    __shared__ unsigned long long smin[4], smax[4];
-    if (__lane_id() == 0) {
-        smin[threadIdx.x / 64] = key_min;
-        smax[threadIdx.x / 64] = key_max;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < 4; ++w) {
-            key_min = smin[w] < key_min ? smin[w] : key_min;
-            key_max = smax[w] > key_max ? smax[w] : key_max;
-        }
-        a.key_partials[2 * blockIdx.x] = key_min;
-        a.key_partials[2 * blockIdx.x + 1] = key_max;
-    }
-}
-
-// One workgroup: the key range over the partials; clears the short-sort flag
-// and the neighbour-build counters of this indexing run.
 // Float-to-integer conversions as the x86-64 host code of the frame loop
 // performs them (cvttsd2si): truncation, and the "integer indefinite" value
 // (the most negative integer) for NaN and out-of-range inputs, which the host
@@ -267,10 +201,12 @@ __device__ __forceinline__ void step_begin_scene(SceneDev* sd, SceneStats* stats
     *stats = st;
 }
 
-// Key range of the index from the per-workgroup partials; with
-// IndexArgs::step_stats also the start of the sub-step for every scene (no
-// kernel reads a scene's dt or statistics before the sort is done).
-__global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) {
+// Key range of the index from the per-workgroup partials (one workgroup of
+// 256 threads), clearing the short-sort flag and the neighbour-build counters
+// of this indexing run; with IndexArgs::step_stats also the start of the
+// sub-step for every scene (no kernel reads a scene's dt or statistics before
+// the sort is done).
+__device__ __forceinline__ void fold_key_range(IndexArgs const& a, uint32_t parts) {
     if (a.step_stats) {
         for (uint32_t s = threadIdx.x; s < a.step_nscenes; s += blockDim.x) {
             step_begin_scene(a.step_scenes + s, a.step_stats + s);
@@ -315,17 +251,90 @@ __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) 
     }
 }
 
+// neighborhood.cpp:10-16 (index pairs), with the key range reduced on the fly.
+// Each workgroup stores its points' key range in its own partial slot (no
+// atomics: one address taken by every wave serialises at the device-coherent
+// level); k_key_range folds the partials.
+__global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
+    unsigned long long key_min = ~0ull;
+    unsigned long long key_max = 0ull;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += gridDim.x * blockDim.x) {
+        float const cell = a.scenes[a.scene_of[i]].c.cell;
+        int2 const g = grid_coord(a.pos[i], cell);
+        uint64_t const key = morton2(g.x, g.y);
+        a.keys[i] = key;
+        a.vals[i] = i;
+        if (a.grid) { a.grid[i] = g; }
+        if (a.dense_limit) {
+            // Dense index: count the point into its bin (keys outside the table
+            // go to the extra bin dense_limit); the count before it is its
+            // rank. Runs of one bin across adjacent lanes (neighbouring points
+            // often share a cell) take one atomic for the whole run.
+            uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
+            uint32_t const lane = __lane_id();
+            uint64_t const prev = __shfl_up(bin, 1);
+            unsigned long long const heads = __ballot(lane == 0 || prev != bin);
+            unsigned long long const active = __ballot(1);
+            unsigned long long const le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+            uint32_t const head = 63u - static_cast<uint32_t>(__clzll(heads & le));
+            unsigned long long const after = heads & ~le;
+            uint32_t const end = after ? static_cast<uint32_t>(__ffsll(after) - 1) : 64u;
+            uint32_t const last = 63u - static_cast<uint32_t>(__clzll(active));
+            uint32_t base = 0;
+            if (lane == head) {
+                uint32_t const run = (end <= last ? end : last + 1u) - head;
+                base = atomicAdd(a.dcount + bin, run);
+            }
+            base = __shfl(base, static_cast<int>(head));
+            a.drank[i] = base + (lane - head);
+        }
+        key_min = key < key_min ? key : key_min;
+        key_max = key > key_max ? key : key_max;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long const lo = __shfl_xor(key_min, off);
+        unsigned long long const hi = __shfl_xor(key_max, off);
+        key_min = lo < key_min ? lo : key_min;
+        key_max = hi > key_max ? hi : key_max;
+    }
+    __shared__ unsigned long long smin[4], smax[4];
+    if (__lane_id() == 0) {
+        smin[threadIdx.x / 64] = key_min;
+        smax[threadIdx.x / 64] = key_max;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            key_min = smin[w] < key_min ? smin[w] : key_min;
+            key_max = smax[w] > key_max ? smax[w] : key_max;
+        }
+        a.key_partials[2 * blockIdx.x] = key_min;
+        a.key_partials[2 * blockIdx.x + 1] = key_max;
+    }
+}
+
+// The key range fold as its own one-workgroup launch. (Folding in the last
+// workgroup of k_keys to finish needs a device-scope release per workgroup,
+// which writes back the XCD's L2 each time: 20 -> 65 us at 1M points.)
+__global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) {
+    fold_key_range(a, parts);
+}
+
 // Dense index scatter: point i goes to the first slot of its bin plus its
 // rank (the ranks of one bin are a permutation of 0 .. count - 1, so every
-// slot receives exactly one point).
+// slot receives exactly one point); the slot of rank 0 heads its cell.
 __global__ __launch_bounds__(256) void k_dense_scatter(IndexArgs a) {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) { return; }
     uint64_t const key = a.keys[i];
     uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
-    uint32_t const t = a.doffset[bin] + a.drank[i];
+    uint32_t const r = a.drank[i];
+    uint32_t const t = a.doffset[bin] + r;
     const_cast<uint64_t*>(a.sorted_keys)[t] = key;
     const_cast<uint32_t*>(a.sorted_vals)[t] = i;
+    a.heads[t] = r == 0u ? 1u : 0u;
 }
 
 void launch_dense_scatter(IndexArgs const& a, hipStream_t s) {
@@ -368,35 +377,6 @@ __global__ __launch_bounds__(256) void k_cell_heads(IndexArgs a) {
         if (!covered) { a.keystate->sort_short = 1u; }
     }
     if (t >= a.n) { return; }
-    if (a.dense_limit) {
-        // Dense index (one scene): a cell starts at its bin's first slot. Its
-        // head puts the cell's points in increasing original index, the order
-        // of the stable sort (the scatter's order within a bin is the order
-        // of the counting atomics).
-        uint64_t const key = a.sorted_keys[t];
-        uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
-        uint32_t const b = a.doffset[bin];
-        bool const head = (t == b);
-        a.heads[t] = head ? 1u : 0u;
-        if (!head) { return; }
-        uint32_t const e = a.doffset[bin + 1];
-        if (e - b > DENSE_CELL_MAX) {
-            atomicOr(&a.keystate->dense_crowded, 1u);
-            atomicOr(&a.keystate->sort_short, 1u);
-            return;
-        }
-        uint32_t* v = const_cast<uint32_t*>(a.sorted_vals);
-        for (uint32_t k = b + 1; k < e; ++k) {
-            uint32_t const x = v[k];
-            uint32_t j = k;
-            while (j > b && v[j - 1] > x) {
-                v[j] = v[j - 1];
-                --j;
-            }
-            v[j] = x;
-        }
-        return;
-    }
     bool head = (t == 0);
     if (!head) {
         head = a.sorted_keys[t] != a.sorted_keys[t - 1] ||
@@ -430,10 +410,56 @@ __global__ __launch_bounds__(256) void k_cells_finalize(IndexArgs a) {
     if (a.kd_out) { a.kd_out[t] = a.kd_in[o]; }
 }
 
+// Dense index (one scene): the scatter left each cell's points in the order
+// of the counting atomics; point t of a cell goes to the cell's first slot
+// plus the number of its points with a smaller original index, which is the
+// order of the stable sort by key, and the cell's data and the gathers are
+// written there. A cell above DENSE_CELL_MAX points flags the indexing run
+// as short (redone by the sort) and keeps the scatter's order, still one
+// point per slot.
+__global__ __launch_bounds__(256) void k_cells_finalize_dense(IndexArgs a) {
+    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) { return; }
+    uint64_t const key = a.sorted_keys[t];
+    uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
+    uint32_t const b = a.doffset[bin];
+    uint32_t const e = a.doffset[bin + 1];
+    uint32_t const* __restrict__ v = a.sorted_vals;
+    uint32_t const o = v[t];
+    uint32_t d = t;
+    if (e - b <= DENSE_CELL_MAX) {
+        uint32_t r = 0;
+        for (uint32_t k = b; k < e; ++k) { r += v[k] < o ? 1u : 0u; }
+        d = b + r;
+    } else if (t == b) {
+        atomicOr(&a.keystate->dense_crowded, 1u);
+        atomicOr(&a.keystate->sort_short, 1u);
+    }
+    uint32_t const cb = a.head_scan[t] + a.heads[t] - 1u;
+    a.block_index[d] = cb;
+    if (t == b) {
+        a.cell_begin[cb] = b;
+        a.cell_end[cb] = e;
+        a.cell_key[cb] = key;
+        a.cell_tag[cb] = a.scene_of[o];
+    }
+    a.orig[d] = o;
+    a.rank[o] = d;
+    if (a.pos_out) { a.pos_out[d] = a.pos[o]; }
+    if (a.vel_out) { a.vel_out[d] = a.vel_in[o]; }
+    if (a.kv_out) { a.kv_out[d] = a.kv_in[o]; }
+    if (a.kd_out) { a.kd_out[d] = a.kd_in[o]; }
+}
+
 void launch_cell_heads(IndexArgs const& a, hipStream_t s) {
+    if (a.dense_limit) { return; }  // the dense scatter marked the heads
     hipLaunchKernelGGL(k_cell_heads, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_cells_finalize(IndexArgs const& a, hipStream_t s) {
+    if (a.dense_limit) {
+        hipLaunchKernelGGL(k_cells_finalize_dense, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+        return;
+    }
     hipLaunchKernelGGL(k_cells_finalize, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
 
@@ -1316,7 +1342,9 @@ void launch_build_mode(int cap, Batch const& a, hipStream_t s) {
                            dim3(a.ntiles * (TILE / BUILD_THREADS)), dim3(BUILD_THREADS), 0, s, a);
         return;
     }
-    uint32_t const dblocks = a.ntiles * 4 < 1024 ? a.ntiles * 4 : 1024;
+    // Deferred lists are rare (under 1/64 of the particles while the 16-entry
+    // build is chosen): one workgroup per CU at most, most of them empty.
+    uint32_t const dblocks = a.ntiles < 256 ? a.ntiles : 256;
     hipLaunchKernelGGL(k_build_deferred<MODE>, dim3(dblocks), dim3(BUILD_THREADS), 0, s, a);
 }
 
@@ -2442,16 +2470,21 @@ __global__ __launch_bounds__(TILE) void k_pci_finalize(Batch a) {
 }
 
 // Per-scene totals from the scene's tile partials: max |v|^2 and, when the
-// final kernel computed them, the FP64 energies. One workgroup per scene;
-// thread i takes tiles i, i + 256, ... of the scene in order, then a fixed
-// tree. The order depends only on the scene's own tiles, so the result is the
-// same alone and in any batch.
-__global__ __launch_bounds__(256) void k_scene_totals(Batch a, int energy) {
+// final kernel computed them, the FP64 energies. One workgroup of
+// TOTALS_THREADS per scene; thread i takes tiles i, i + TOTALS_THREADS, ...
+// of the scene in order, then a fixed tree over the waves. The order depends
+// only on the scene's own tiles, so the result is the same alone and in any
+// batch.
+constexpr int TOTALS_THREADS = 1024;
+__global__ __launch_bounds__(TOTALS_THREADS) void k_scene_totals(Batch a, int energy) {
+    constexpr int WAVES = TOTALS_THREADS / 64;
+    __shared__ double ske[WAVES], spe[WAVES];
+    __shared__ unsigned int svm[WAVES];
     for (uint32_t s = blockIdx.x; s < a.nscenes; s += gridDim.x) {
         SceneDev const& sd = a.scenes[s];
         double ke = 0.0, pe = 0.0;
         unsigned int vm = 0;
-        for (uint32_t k = threadIdx.x; k < sd.ntiles; k += blockDim.x) {
+        for (uint32_t k = threadIdx.x; k < sd.ntiles; k += TOTALS_THREADS) {
             unsigned int const m = a.tile_vmax[sd.first_tile + k];
             vm = m > vm ? m : vm;
             if (energy) {
@@ -2467,8 +2500,6 @@ __global__ __launch_bounds__(256) void k_scene_totals(Batch a, int energy) {
             unsigned int const o = __shfl_xor(vm, off);
             vm = o > vm ? o : vm;
         }
-        __shared__ double ske[4], spe[4];
-        __shared__ unsigned int svm[4];
         if (__lane_id() == 0) {
             ske[threadIdx.x / 64] = ke;
             spe[threadIdx.x / 64] = pe;
@@ -2477,11 +2508,22 @@ __global__ __launch_bounds__(256) void k_scene_totals(Batch a, int energy) {
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned int m = svm[0];
-            for (int w = 1; w < 4; ++w) { m = svm[w] > m ? svm[w] : m; }
+            for (int w = 1; w < WAVES; ++w) { m = svm[w] > m ? svm[w] : m; }
             a.stats[s].vmax2_bits = m;
             if (energy) {
-                a.stats[s].kinetic_energy = (ske[0] + ske[1]) + (ske[2] + ske[3]);
-                a.stats[s].potential_energy = (spe[0] + spe[1]) + (spe[2] + spe[3]);
+                double k2[WAVES / 2], p2[WAVES / 2];
+                for (int w = 0; w < WAVES / 2; ++w) {
+                    k2[w] = ske[2 * w] + ske[2 * w + 1];
+                    p2[w] = spe[2 * w] + spe[2 * w + 1];
+                }
+                for (int h = WAVES / 4; h >= 1; h >>= 1) {
+                    for (int w = 0; w < h; ++w) {
+                        k2[w] = k2[2 * w] + k2[2 * w + 1];
+                        p2[w] = p2[2 * w] + p2[2 * w + 1];
+                    }
+                }
+                a.stats[s].kinetic_energy = k2[0];
+                a.stats[s].potential_energy = p2[0];
                 if (sd.frame.on) {
                     // End of the sub-step: commit the frame loop's progress.
                     FrameDev& fr = const_cast<SceneDev&>(sd).frame;
@@ -2704,7 +2746,8 @@ void launch_repack(uint3 const* seg, uint32_t nseg, uint32_t total, V2 const* po
 
 void launch_scene_totals(Batch const& a, int energy, hipStream_t s) {
     uint32_t const blocks = a.nscenes < 4096 ? a.nscenes : 4096;
-    hipLaunchKernelGGL(k_scene_totals, dim3(blocks ? blocks : 1), dim3(256), 0, s, a, energy);
+    hipLaunchKernelGGL(k_scene_totals, dim3(blocks ? blocks : 1), dim3(TOTALS_THREADS), 0, s, a,
+                       energy);
 }
 void launch_vmax2(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_vmax2, tile_grid(a), dim3(TILE), 0, s, a);

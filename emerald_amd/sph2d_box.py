@@ -499,12 +499,19 @@ class Context:
         check(lib().emsph_set_kernel_timing(self._h, 1 if enable else 0), "set_kernel_timing")
 
     def kernel_timing(self) -> dict:
-        m = 64
-        names = (ctypes.c_char_p * m)()
-        ms = (ctypes.c_double * m)()
-        calls = (ctypes.c_int64 * m)()
-        k = lib().emsph_get_kernel_timing(self._h, names, ms, calls, m)
-        return {names[i].decode(): (ms[i], calls[i]) for i in range(k)}
+        return _kernel_timing(lib().emsph_get_kernel_timing, self._h)
+
+
+def _kernel_timing(fn, h) -> dict:
+    """{kernel class: (total ms, launches)} from the engine's HIP events."""
+    m = 64
+    names = (ctypes.c_char_p * m)()
+    ms = (ctypes.c_double * m)()
+    calls = (ctypes.c_int64 * m)()
+    k = fn(h, names, ms, calls, m)
+    if k < 0:
+        raise _capi.EmsphError(-1, "kernel_timing", "reading the timing events failed")
+    return {names[i].decode(): (ms[i], calls[i]) for i in range(k)}
 
 
 def _simulation_step(method: str, global_time: int, config: Simulation_config, state: State,
@@ -639,6 +646,13 @@ class Batch:
 
     def set_graphs(self, enable: bool) -> None:
         check(lib().emsph_batch_set_graphs(self._h, 1 if enable else 0), "batch_set_graphs")
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        check(lib().emsph_batch_set_kernel_timing(self._h, 1 if enable else 0),
+              "batch_set_kernel_timing")
+
+    def kernel_timing(self) -> dict:
+        return _kernel_timing(lib().emsph_batch_get_kernel_timing, self._h)
 
     def set_tuning(self, persistent_loop: bool = False, graphs: bool = True,
                    neighbor_batch: int = 4, xcd_remap: bool = True) -> None:

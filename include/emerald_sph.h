@@ -347,6 +347,11 @@ emsph_status emsph_batch_get_state(emsph_batch* b, int scene, float* pos_xy, flo
 emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* out);
 emsph_status emsph_batch_set_cfl(emsph_batch* b, float factor);  /* as emsph_set_cfl, per scene */
 emsph_status emsph_batch_set_graphs(emsph_batch* b, int enable);  /* as emsph_set_graphs */
+/* As emsph_set_kernel_timing / emsph_get_kernel_timing, over the batch's
+ * launches (one launch per stage covers every scene). Measurement only. */
+emsph_status emsph_batch_set_kernel_timing(emsph_batch* b, int enable);
+int emsph_batch_get_kernel_timing(emsph_batch* b, const char** names, double* total_ms,
+                                  int64_t* calls, int max);
 emsph_status emsph_batch_set_tuning(emsph_batch* b, const emsph_tuning* tuning);
 /* Per-scene device force model (as emsph_set_force_model; host callbacks are
  * for contexts only). */
