@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # Per sub-step the tables sum to IISPH 160 + 68k (the first iteration's
 # sum_j d_ij p_j is known to be zero: pressures start at zero), DFSPH_P
 # 172 + 68k_v + 72k_rho, PCISPH 116 + 108k.
-NB = {"keys": 16, "radix_sort": 16, "cells": 8, "step_begin": 0, "loop": 0}
+NB = {"keys": 16, "radix_sort": 16, "dense_sort": 16, "cells": 8, "step_begin": 0, "loop": 0}
 ALGO_BYTES = {
     "iisph": {**NB,
               "build": 48,            # rho 12 + (v += g) 16 + d_ii 20

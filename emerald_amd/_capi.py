@@ -204,6 +204,7 @@ SIGNATURES = {
     "emsph_batch_set_graphs": (_ST, [_P, _I]),
     "emsph_batch_set_kernel_timing": (_ST, [_P, _I]),
     "emsph_batch_get_kernel_timing": (_I, [_P, _P, _P, _P, _I]),
+    "emsph_batch_get_iteration_totals": (_ST, [_P, _I, _P]),
     "emsph_batch_set_cfl": (_ST, [_P, _F]),
     "emsph_set_kernel_timing": (_ST, [_P, _I]),
     "emsph_get_kernel_timing": (_I, [_P, _P, _P, _P, _I]),

@@ -194,8 +194,8 @@ struct SpatialIndex {
     uint32_t set_mask = 0;
     std::vector<uint64_t> set_keys;  // the set's cell keys and scene tags (host copy)
     std::vector<uint32_t> set_tags;
-    DBuf<uint32_t> set_bits;         // the set's scene-0 keys below 4^set_bits_for as bits
-    int set_bits_for = 0;
+    DBuf<uint32_t> set_bits;         // the set's cells as bits over a dense layout's bins
+    uint64_t set_bits_for = 0;       // that layout's generation (0: none)
     DBuf<uint32_t> orig, rank;
     DBuf<uint2> win;        // [cell][9] slot ranges of the 3x3 neighbour cells
     DBuf<int2> grid;
@@ -203,24 +203,58 @@ struct SpatialIndex {
     uint32_t* sorted_vals = nullptr;
     int key_passes = 0;
     int scene_passes = 0;
-    // Dense cell index (see DENSE_MAX_BITS): tables for up to 4^dense_alloc
-    // bins; the counts are zero between runs.
+    // Dense cell index (see DenseScene): one table of dense_bins bins in which
+    // scene s owns 4^dense_layout[s] bins (dense_host); the counts are zero
+    // between runs. dense_gen numbers the layouts.
     DBuf<uint32_t> dcount, doffset, drank, dscratch;
-    int dense_alloc = 0;
+    DBuf<DenseScene> dscene;
+    std::vector<int> dense_layout;
+    std::vector<DenseScene> dense_host;
+    uint64_t dense_bins = 0;
+    size_t dense_cap = 0;  // bins the tables hold
+    uint64_t dense_gen = 0;
     uint64_t direct_kmax = 0;  // key maximum of the last run that read the range back
+    DBuf<uint64_t> scene_kmax;  // each scene's largest key of the last run
 
-    // Tables for 4^bits bins (allocated outside any graph capture).
-    hipError_t ensure_dense(int bits, uint32_t c, hipStream_t s) {
-        if (bits <= dense_alloc && drank.p) { return hipSuccess; }
-        size_t const bins = (size_t{1} << (2 * bits)) + 2;
+    // Tables for per-scene bits (allocated outside any graph capture).
+    // *changed: a new layout (graphs captured with the old one are stale).
+    hipError_t set_dense(std::vector<int> const& bits, uint32_t c, hipStream_t s,
+                         bool* changed) {
+        *changed = false;
+        if (dense_bins && bits == dense_layout) { return hipSuccess; }
+        std::vector<DenseScene> ds(bits.size());
+        uint64_t base = 0;
+        for (size_t k = 0; k < bits.size(); ++k) {
+            uint64_t const limit = bits[k] > 0 ? (uint64_t{1} << (2 * bits[k])) : 0;
+            ds[k] = DenseScene{limit, static_cast<uint32_t>(base), 0u};
+            base += limit;
+        }
+        dense_bins = 0;
+        dense_layout.clear();
+        if (base == 0 || base + 2 > (uint64_t{1} << 32)) { return hipSuccess; }
         hipError_t e;
-        if ((e = dcount.alloc(bins)) || (e = doffset.alloc(bins)) || (e = drank.alloc(c)) ||
-            (e = dscratch.alloc(scan_scratch_words(bins) + 64))) {
-            dense_alloc = 0;
+        size_t const need = static_cast<size_t>(base) + 2;
+        if (need > dense_cap || !drank.p || drank.n < c) {
+            dense_cap = 0;
+            if ((e = dcount.alloc(need)) || (e = doffset.alloc(need)) || (e = drank.alloc(c)) ||
+                (e = dscratch.alloc(scan_scratch_words(need) + 64))) {
+                return e;
+            }
+            if ((e = hipMemsetAsync(dcount.p, 0, need * sizeof(uint32_t), s))) { return e; }
+            dense_cap = need;
+        }
+        if (dscene.n < ds.size() && (e = dscene.alloc(ds.size()))) { return e; }
+        if ((e = hipMemcpyAsync(dscene.p, ds.data(), ds.size() * sizeof(DenseScene),
+                                hipMemcpyHostToDevice, s))) {
             return e;
         }
-        dense_alloc = bits;
-        return hipMemsetAsync(dcount.p, 0, bins * sizeof(uint32_t), s);
+        if ((e = hipStreamSynchronize(s))) { return e; }
+        dense_host = std::move(ds);
+        dense_layout = bits;
+        dense_bins = base;
+        ++dense_gen;
+        *changed = true;
+        return hipSuccess;
     }
 
     // Keeps the arrays when they already hold c points.
@@ -372,9 +406,15 @@ struct Engine {
 
     Pinned* host = nullptr;
     int passes_hint = -1;
-    // Dense cell index of the fluid (one scene): table bits for the next
-    // sub-step (0: radix sort), and a sticky switch-off after a crowded cell.
+    // Dense cell index of the fluid: one scene's table bits for the next
+    // sub-step (0: radix sort); a batch's per-scene bits, taken after each
+    // exact run (dense_stale); whether the current sub-step's layout is ready
+    // (dense_on); a sticky switch-off after a crowded cell.
     int dense_bits = 0;
+    std::vector<int> dense_scene_bits;
+    bool dense_scene_ok = false;
+    bool dense_stale = true;
+    bool dense_on = false;
     bool dense_disabled = false;
     // The indexing choices of the current sub-step attempt (pass count, dense
     // bits), fixed when it begins: enqueue, graph key and read-back agree.
@@ -702,7 +742,7 @@ hipError_t ensure_geometry(Engine* g) {
 hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
                         uint32_t const* scene_of, uint32_t nscenes, int passes_hint,
                         IndexArgs gathers, bool time_it, SpatialIndex const* other = nullptr,
-                        uint2* owin = nullptr, int dense_bits = 0) {
+                        uint2* owin = nullptr, bool use_dense = false) {
     hipStream_t const s = g->stream;
     auto run = [&](char const* name, auto&& f) -> hipError_t {
         if (time_it) { return timed(g, name, f); }
@@ -720,10 +760,12 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     a.grid = si.grid.p;
     a.keystate = g->keystate.p;
     a.key_partials = g->key_partials.p;
-    bool const dense = dense_bits > 0 && passes_hint >= 0 && nscenes == 1 &&
-                       si.dense_alloc >= dense_bits;
+    a.scene_kmax = si.scene_kmax.n >= nscenes ? si.scene_kmax.p : nullptr;
+    bool const dense = use_dense && passes_hint >= 0 && si.dense_bins > 0 &&
+                       si.dense_layout.size() == nscenes;
     if (dense) {
-        a.dense_limit = uint64_t{1} << (2 * dense_bits);
+        a.dense_bins = si.dense_bins;
+        a.dscene = si.dscene.p;
         a.dcount = si.dcount.p;
         a.doffset = si.doffset.p;
         a.drank = si.drank.p;
@@ -740,8 +782,8 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
         a.sorted_keys = si.sorted_keys;
         a.sorted_vals = si.sorted_vals;
         a.heads = si.heads.p;
-        uint32_t const bins = static_cast<uint32_t>(a.dense_limit) + 1u;
-        if ((e = run("radix_sort", [&] {
+        uint32_t const bins = static_cast<uint32_t>(a.dense_bins) + 1u;
+        if ((e = run("dense_sort", [&] {
                  launch_exclusive_scan_clear(si.dcount.p, si.doffset.p, bins, si.dscratch.p,
                                              si.doffset.p + bins, s);
                  launch_dense_scatter(a, s);
@@ -793,9 +835,10 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     }
     w.max_cells = n;
     if (dense) {
-        w.dense_limit = a.dense_limit;
+        w.dense_bins = a.dense_bins;
+        w.dscene = a.dscene;
         w.doffset = si.doffset.p;
-        if (other && other->set_bits_for >= dense_bits) { w.other_bits = other->set_bits.p; }
+        if (other && other->set_bits_for == si.dense_gen) { w.other_bits = other->set_bits.p; }
     }
     w.win = si.win.p;
     w.owin = owin;
@@ -843,21 +886,22 @@ emsph_status build_cell_set(Engine* g, SpatialIndex& si) {
     return EMSPH_OK;
 }
 
-// Bitmap of the set's scene-0 cells over the dense table of 4^bits bins (for
+// Bitmap of the set's cells over the bins of the dense layout of index d (for
 // the dense windows; outside any graph capture).
-emsph_status build_set_bits(Engine* g, SpatialIndex& si, int bits) {
-    if (si.set_bits_for == bits && si.set_bits.p) { return EMSPH_OK; }
-    uint64_t const bins = uint64_t{1} << (2 * bits);
-    std::vector<uint32_t> words(static_cast<size_t>((bins + 31) / 32), 0u);
+emsph_status build_set_bits(Engine* g, SpatialIndex& si, SpatialIndex const& d) {
+    if (si.set_bits_for == d.dense_gen && si.set_bits.p) { return EMSPH_OK; }
+    std::vector<uint32_t> words(static_cast<size_t>((d.dense_bins + 31) / 32), 0u);
     for (size_t c = 0; c < si.set_keys.size(); ++c) {
-        uint64_t const k = si.set_keys[c];
-        if (si.set_tags[c] == 0u && k < bins) { words[k >> 5] |= 1u << (k & 31u); }
+        uint32_t const tag = si.set_tags[c];
+        if (tag >= d.dense_host.size()) { continue; }
+        uint64_t const bin = dense_bin(d.dense_host[tag], si.set_keys[c], d.dense_bins);
+        if (bin < d.dense_bins) { words[bin >> 5] |= 1u << (bin & 31u); }
     }
     HIP_TRY(si.set_bits.alloc(words.size()));
     HIP_TRY(hipMemcpyAsync(si.set_bits.p, words.data(), words.size() * sizeof(uint32_t),
                            hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
-    si.set_bits_for = bits;
+    si.set_bits_for = d.dense_gen;
     return EMSPH_OK;
 }
 
@@ -990,11 +1034,108 @@ int dense_bits_for(uint64_t kmax, uint32_t n) {
     return bins <= 8ull * n + 65536ull ? bits : 0;
 }
 
-// The dense table bits the next indexing run uses (0: radix sort).
+// Whether the next indexing run uses the dense cell index (1) or the radix
+// sort (0); dense_prepare readied its layout at the start of the sub-step.
 int dense_for(Engine const* g) {
-    bool const ok = g->passes_hint >= 0 && g->M == 1 && !g->dense_disabled &&
-                    g->dense_bits > 0 && g->fidx.dense_alloc >= g->dense_bits;
-    return ok ? g->dense_bits : 0;
+    bool const ok = g->passes_hint >= 0 && !g->dense_disabled && g->dense_on &&
+                    g->fidx.dense_bins > 0 && g->fidx.dense_layout.size() == g->M;
+    return ok ? 1 : 0;
+}
+
+// Table bits of each scene's dense index. One scene follows the key range of
+// its last run. In a batch each scene's table covers its largest key of the
+// last exact run and the key of its solids' largest cell coordinates (the
+// walls bound the fluid; a Morton key grows with either coordinate), taken
+// again only after an exact run. False: no dense index (some scene's table
+// would be too large).
+bool dense_want(Engine* g, std::vector<int>& bits) {
+    if (g->M == 1) {
+        if (g->dense_bits <= 0) { return false; }
+        bits.assign(1, g->dense_bits);
+        return true;
+    }
+    if (g->dense_stale) {
+        std::vector<uint64_t> km(g->M, 0);
+        if (hipMemcpyAsync(km.data(), g->fidx.scene_kmax.p, g->M * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+            hipStreamSynchronize(g->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            g->dense_disabled = true;
+            return false;
+        }
+        // The solids' bound: the key of their largest non-negative cell
+        // coordinates (cells at negative coordinates, outside every table,
+        // have keys above it and are looked up in the solid cell set).
+        SpatialIndex const& si = g->sidx;
+        std::vector<uint32_t> mx(g->M, 0), my(g->M, 0);
+        std::vector<bool> any(g->M, false);
+        for (size_t c = 0; c < si.set_keys.size(); ++c) {
+            uint32_t const tag = si.set_tags[c];
+            uint32_t const gx = compact_bits_2(si.set_keys[c]);
+            uint32_t const gy = compact_bits_2(si.set_keys[c] >> 1);
+            if (tag >= g->M || gx >= 0x80000000u || gy >= 0x80000000u) { continue; }
+            mx[tag] = std::max(mx[tag], gx);
+            my[tag] = std::max(my[tag], gy);
+            any[tag] = true;
+        }
+        for (uint32_t s = 0; s < g->M; ++s) {
+            if (any[s]) {
+                uint64_t const bound =
+                  morton2(static_cast<int32_t>(mx[s]), static_cast<int32_t>(my[s]));
+                km[s] = std::max(km[s], bound);
+            }
+        }
+        g->dense_scene_bits.assign(g->M, 0);
+        g->dense_scene_ok = true;
+        for (uint32_t s = 0; s < g->M; ++s) {
+            if (g->sh[s].n == 0) { continue; }  // no points: no table
+            int const b = dense_bits_for(km[s], g->sh[s].n);
+            if (!b) { g->dense_scene_ok = false; }
+            g->dense_scene_bits[s] = b;
+        }
+        g->dense_stale = false;
+    }
+    if (!g->dense_scene_ok) { return false; }
+    bits = g->dense_scene_bits;
+    return true;
+}
+
+// The dense layout of the next sub-step, readied outside any graph capture
+// (a scene's table only grows); sets dense_on.
+void dense_prepare(Engine* g) {
+    g->dense_on = false;
+    if (g->dense_disabled || g->N == 0) { return; }
+    SpatialIndex& f = g->fidx;
+    if (g->M > 1 && f.scene_kmax.n < g->M) {
+        if (f.scene_kmax.alloc(g->M) != hipSuccess) {
+            (void)hipGetLastError();
+            g->dense_disabled = true;
+            return;
+        }
+        g->dense_stale = true;
+    }
+    if (g->passes_hint < 0) { return; }  // an exact run: radix sort
+    std::vector<int> bits;
+    if (!dense_want(g, bits)) { return; }
+    if (f.dense_layout.size() == bits.size()) {
+        for (size_t k = 0; k < bits.size(); ++k) { bits[k] = std::max(bits[k], f.dense_layout[k]); }
+    }
+    bool changed = false;
+    if (f.set_dense(bits, g->cap, g->stream, &changed) != hipSuccess) {
+        (void)hipGetLastError();
+        g->dense_disabled = true;
+        return;
+    }
+    if (!f.dense_bins) { return; }
+    if (changed) {
+        ++g->layout_gen;
+        if (g->NS && build_set_bits(g, g->sidx, f) != EMSPH_OK) {
+            (void)hipGetLastError();
+            g->dense_disabled = true;
+            return;
+        }
+    }
+    g->dense_on = true;
 }
 
 // Key-state read-backs into pinned slot k: enqueue, then wait and take it in.
@@ -1139,6 +1280,7 @@ void finish_prepare(Engine* g) {
         // pass count it left in si.key_passes.
         if (g->passes_hint < 0) { g->passes_hint = si.key_passes; }
         g->dense_bits = dense_bits_for(si.direct_kmax, g->N);
+        g->dense_stale = true;  // a batch takes its scenes' key maxima again
     } else {
         si.key_passes = g->prep_passes;
     }
@@ -1486,15 +1628,7 @@ emsph_status engine_sub_step(Engine* g, int method) {
     HIP_TRY(hipSetDevice(g->device));
     // Dense index tables for the bits the last key range asks for (allocated
     // here, outside any graph capture; a new table is a new layout).
-    if (g->M == 1 && !g->dense_disabled && g->dense_bits > g->fidx.dense_alloc) {
-        if (g->fidx.ensure_dense(g->dense_bits, g->cap, g->stream) != hipSuccess) {
-            (void)hipGetLastError();
-            g->dense_disabled = true;
-        } else if (g->NS && build_set_bits(g, g->sidx, g->fidx.dense_alloc) != EMSPH_OK) {
-            g->dense_disabled = true;
-        }
-        ++g->layout_gen;
-    }
+    dense_prepare(g);
     // A sort run with the previous sub-step's pass count is checked at the
     // first read-back; when it was short the sub-step is redone with the exact
     // count (the second attempt sorts exactly and cannot be short).
@@ -2231,14 +2365,24 @@ emsph_status emsph_run_sub_steps(emsph_context* c, int method, float dt, int n_s
     return EMSPH_OK;
 }
 
-emsph_status emsph_get_iteration_totals(emsph_context* c, int reset, int64_t* out) {
-    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+static void engine_iteration_totals(Engine* g, int reset, int64_t* out) {
     if (out) {
-        for (int k = 0; k < 4; ++k) { out[k] = c->e.totals[k]; }
+        for (int k = 0; k < 4; ++k) { out[k] = g->totals[k]; }
     }
     if (reset) {
-        for (int k = 0; k < 4; ++k) { c->e.totals[k] = 0; }
+        for (int k = 0; k < 4; ++k) { g->totals[k] = 0; }
     }
+}
+
+emsph_status emsph_get_iteration_totals(emsph_context* c, int reset, int64_t* out) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    engine_iteration_totals(&c->e, reset, out);
+    return EMSPH_OK;
+}
+
+emsph_status emsph_batch_get_iteration_totals(emsph_batch* b, int reset, int64_t* out) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    engine_iteration_totals(&b->e, reset, out);
     return EMSPH_OK;
 }
 
@@ -2656,14 +2800,8 @@ emsph_status emsph_search_run(emsph_search* s, float cell_size, int store_geomet
     int dense = 0;
     if (s->ran && g->passes_hint >= 0) {
         passes = g->passes_hint;
-        if (!g->dense_disabled && g->dense_bits > g->fidx.dense_alloc &&
-            g->fidx.ensure_dense(g->dense_bits, g->cap, g->stream) != hipSuccess) {
-            (void)hipGetLastError();
-            g->dense_disabled = true;
-        }
-        if (!g->dense_disabled && g->dense_bits > 0 && g->fidx.dense_alloc >= g->dense_bits) {
-            dense = g->dense_bits;
-        }
+        dense_prepare(g);
+        dense = dense_for(g);
     }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));

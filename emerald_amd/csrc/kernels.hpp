@@ -153,17 +153,32 @@ struct KeyState {
     unsigned int long_lists;      // particles of the last build with a list above 16
     int frame_remaining;          // scene 0's frame quanta left after this sub-step (-1: off)
     unsigned int dense_crowded;   // a cell of the dense index exceeded DENSE_CELL_MAX
+    unsigned int dense_out;       // a key fell outside its scene's dense table (k_keys)
 };
 
-// Dense cell index (one scene): cells counted in a table of 4^B bins indexed
-// by Morton key (bin 4^B collects keys outside the table), an exclusive scan
+// Dense cell index: cells counted in a table of bins, scene s owning the
+// 4^B_s bins [base_s, base_s + 4^B_s) indexed by Morton key (one extra bin,
+// the last, collects keys outside their scene's table); an exclusive scan
 // gives each cell's first sorted slot, and the particles are scattered there
 // and put in increasing original index within their cell, which is the order
-// of the stable sort by key. A key outside the table or a cell above
+// of the stable sort by (scene, key). A key outside its table or a cell above
 // DENSE_CELL_MAX particles flags the indexing run as short (it is redone by
 // the sort); every slot still receives a valid particle index.
 constexpr int DENSE_MAX_BITS = 12;
 constexpr uint32_t DENSE_CELL_MAX = 256;
+
+struct DenseScene {
+    uint64_t limit;  // 4^B_s (0: the scene has no table)
+    uint32_t base;   // first bin of the scene's table
+    uint32_t pad;
+};
+
+// Bin of a key of scene d's table; bins (the table's total, i.e. the index of
+// the extra bin) for a key outside it.
+__host__ __device__ __forceinline__ uint64_t dense_bin(DenseScene const& d, uint64_t key,
+                                                       uint64_t bins) {
+    return key < d.limit ? d.base + key : bins;
+}
 
 // Neighbour-list capacities of the build kernel's LDS list (the full one is
 // the reference's NEIGHBORHOOD_MAX_COUNT).
@@ -337,11 +352,14 @@ struct IndexArgs {
     uint32_t* cell_tag;
     uint32_t* orig;
     uint32_t* rank;
-    // dense cell index (dense_limit = 4^B bins; 0: the radix sort path)
-    uint64_t dense_limit;
-    uint32_t* dcount;       // [dense_limit + 1] per-bin counts (zero between runs)
-    uint32_t* doffset;      // [dense_limit + 2] exclusive scan of dcount, then n
+    // dense cell index (dense_bins = the scenes' table bins; 0: the radix path)
+    uint64_t dense_bins;
+    DenseScene const* dscene;  // [scenes] table of each scene
+    uint32_t* dcount;       // [dense_bins + 1] per-bin counts (zero between runs)
+    uint32_t* doffset;      // [dense_bins + 2] exclusive scan of dcount, then n
     uint32_t* drank;        // per point (original order): rank within its bin
+    // optional: each scene's largest key (written at its last sorted slot)
+    uint64_t* scene_kmax;
     // gathers into sorted order (optional)
     V2 const* vel_in;
     V2* pos_out;
@@ -359,9 +377,11 @@ struct WindowArgs {
     CellList other;          // ranges of the other cells
     CellSet other_set;       // lookup of the other cells; slots == nullptr: none
     uint32_t max_cells;      // launch bound
-    // Dense index of the self cells (dense_limit != 0): a neighbour cell's
-    // range is [doffset[key], doffset[key + 1]) for keys below dense_limit.
-    uint64_t dense_limit;
+    // Dense index of the self cells (dense_bins != 0): a neighbour cell's
+    // range is [doffset[bin], doffset[bin + 1]) for keys inside its scene's
+    // table (bin = dense_bin).
+    uint64_t dense_bins;
+    DenseScene const* dscene;
     uint32_t const* doffset;
     // With the dense index: one bit per bin set where the other index (the
     // solids) has a cell with that key, so most cells skip the set lookups.
@@ -374,7 +394,7 @@ void launch_cell_windows(WindowArgs const& w, hipStream_t s);
 
 void launch_keys(IndexArgs const& a, hipStream_t s);
 // Dense index: scatter of (key, index) to doffset[bin] + drank into
-// sorted_keys / sorted_vals (bin = min(key, dense_limit)).
+// sorted_keys / sorted_vals (bin = dense_bin of the point's scene).
 void launch_dense_scatter(IndexArgs const& a, hipStream_t s);
 void launch_cell_heads(IndexArgs const& a, hipStream_t s);
 void launch_cells_finalize(IndexArgs const& a, hipStream_t s);

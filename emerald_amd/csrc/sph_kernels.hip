@@ -242,8 +242,9 @@ __device__ __forceinline__ void fold_key_range(IndexArgs const& a, uint32_t part
         a.keystate->kmax = key_max;
         a.keystate->frame_remaining =
           (a.step_stats && a.step_scenes[0].frame.on) ? a.step_scenes[0].frame.next_remaining : -1;
-        // A dense index whose table does not hold every key is short: redone.
-        a.keystate->sort_short = (a.dense_limit && key_max >= a.dense_limit) ? 1u : 0u;
+        // A dense index whose tables do not hold every key is short: redone.
+        a.keystate->sort_short = (a.dense_bins && a.keystate->dense_out) ? 1u : 0u;
+        a.keystate->dense_out = 0u;
         a.keystate->dense_crowded = 0u;
         a.keystate->overflow_count = 0u;
         a.keystate->max_count = 0u;
@@ -266,12 +267,16 @@ __global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
         a.keys[i] = key;
         a.vals[i] = i;
         if (a.grid) { a.grid[i] = g; }
-        if (a.dense_limit) {
-            // Dense index: count the point into its bin (keys outside the table
-            // go to the extra bin dense_limit); the count before it is its
-            // rank. Runs of one bin across adjacent lanes (neighbouring points
-            // often share a cell) take one atomic for the whole run.
-            uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
+        if (a.dense_bins) {
+            // Dense index: count the point into its bin (keys outside their
+            // scene's table go to the extra bin dense_bins, and the run is
+            // short); the count before it is its rank. Runs of one bin across
+            // adjacent lanes (neighbouring points often share a cell) take one
+            // atomic for the whole run.
+            uint64_t const bin = dense_bin(a.dscene[a.scene_of[i]], key, a.dense_bins);
+            if (__ballot(bin == a.dense_bins) && __lane_id() == 0) {
+                atomicOr(&a.keystate->dense_out, 1u);
+            }
             uint32_t const lane = __lane_id();
             uint64_t const prev = __shfl_up(bin, 1);
             unsigned long long const heads = __ballot(lane == 0 || prev != bin);
@@ -329,7 +334,7 @@ __global__ __launch_bounds__(256) void k_dense_scatter(IndexArgs a) {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) { return; }
     uint64_t const key = a.keys[i];
-    uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
+    uint64_t const bin = dense_bin(a.dscene[a.scene_of[i]], key, a.dense_bins);
     uint32_t const r = a.drank[i];
     uint32_t const t = a.doffset[bin] + r;
     const_cast<uint64_t*>(a.sorted_keys)[t] = key;
@@ -402,6 +407,9 @@ __global__ __launch_bounds__(256) void k_cells_finalize(IndexArgs a) {
     bool const tail = (t + 1 == a.n) || a.sorted_keys[t + 1] != key ||
                       a.scene_of[a.sorted_vals[t + 1]] != tag;
     if (tail) { a.cell_end[b] = t + 1; }
+    if (a.scene_kmax && (t + 1 == a.n || a.scene_of[a.sorted_vals[t + 1]] != tag)) {
+        a.scene_kmax[tag] = key;  // sorted by (scene, key): the scene's largest
+    }
     a.orig[t] = o;
     a.rank[o] = t;
     if (a.pos_out) { a.pos_out[t] = a.pos[o]; }
@@ -410,7 +418,7 @@ __global__ __launch_bounds__(256) void k_cells_finalize(IndexArgs a) {
     if (a.kd_out) { a.kd_out[t] = a.kd_in[o]; }
 }
 
-// Dense index (one scene): the scatter left each cell's points in the order
+// Dense index: the scatter left each cell's points in the order
 // of the counting atomics; point t of a cell goes to the cell's first slot
 // plus the number of its points with a smaller original index, which is the
 // order of the stable sort by key, and the cell's data and the gathers are
@@ -421,11 +429,12 @@ __global__ __launch_bounds__(256) void k_cells_finalize_dense(IndexArgs a) {
     uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) { return; }
     uint64_t const key = a.sorted_keys[t];
-    uint64_t const bin = key < a.dense_limit ? key : a.dense_limit;
-    uint32_t const b = a.doffset[bin];
-    uint32_t const e = a.doffset[bin + 1];
     uint32_t const* __restrict__ v = a.sorted_vals;
     uint32_t const o = v[t];
+    uint32_t const tag = a.scene_of[o];
+    uint64_t const bin = dense_bin(a.dscene[tag], key, a.dense_bins);
+    uint32_t const b = a.doffset[bin];
+    uint32_t const e = a.doffset[bin + 1];
     uint32_t d = t;
     if (e - b <= DENSE_CELL_MAX) {
         uint32_t r = 0;
@@ -441,7 +450,12 @@ __global__ __launch_bounds__(256) void k_cells_finalize_dense(IndexArgs a) {
         a.cell_begin[cb] = b;
         a.cell_end[cb] = e;
         a.cell_key[cb] = key;
-        a.cell_tag[cb] = a.scene_of[o];
+        a.cell_tag[cb] = tag;
+    }
+    // The scene's last cell holds its largest key (cells never span scenes
+    // inside the tables; the extra bin's cell only exists in a short run).
+    if (a.scene_kmax && t + 1 == e && (e == a.n || a.scene_of[v[e]] != tag)) {
+        a.scene_kmax[tag] = key;
     }
     a.orig[d] = o;
     a.rank[o] = d;
@@ -452,11 +466,11 @@ __global__ __launch_bounds__(256) void k_cells_finalize_dense(IndexArgs a) {
 }
 
 void launch_cell_heads(IndexArgs const& a, hipStream_t s) {
-    if (a.dense_limit) { return; }  // the dense scatter marked the heads
+    if (a.dense_bins) { return; }  // the dense scatter marked the heads
     hipLaunchKernelGGL(k_cell_heads, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_cells_finalize(IndexArgs const& a, hipStream_t s) {
-    if (a.dense_limit) {
+    if (a.dense_bins) {
         hipLaunchKernelGGL(k_cells_finalize_dense, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
         return;
     }
@@ -714,11 +728,12 @@ __global__ __launch_bounds__(256) void k_cell_windows(WindowArgs w) {
     }
 }
 
-// Windows over the dense index (one scene), one thread per cell: a neighbour
-// cell's range is read from the bin offsets; the solid cells are looked up
-// only where the solid bitmap has the key (or the key is outside the table),
-// and a cell's solid windows are written only when some neighbour cell holds
-// solids (the build reads them only then: WIN_SOLID_NEAR).
+// Windows over the dense index, one thread per cell: a neighbour cell's
+// range is read from the bin offsets of the cell's scene table; the solid
+// cells are looked up only where the solid bitmap (over the same bins) has
+// the bin, or the key is outside the table, and a cell's solid windows are
+// written only when some neighbour cell holds solids (the build reads them
+// only then: WIN_SOLID_NEAR).
 __global__ __launch_bounds__(256) void k_cell_windows_dense(WindowArgs w) {
     uint32_t const n = *w.self.count;
     uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -726,21 +741,25 @@ __global__ __launch_bounds__(256) void k_cell_windows_dense(WindowArgs w) {
     if (c >= n) { return; }
     uint64_t const key = w.self.key[c];
     uint32_t const tag = w.self.tag[c];
+    DenseScene const ds = w.dscene[tag];
     uint32_t const gx0 = compact_bits_2(key);
     uint32_t const gy0 = compact_bits_2(key >> 1);
     uint2 r[9];
     uint64_t nks[9];
+    uint64_t bins[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         // neighborhood.h:233-243: j outer, i inner, each from g - 1 to g + 1.
         uint32_t const gx = gx0 + static_cast<uint32_t>(k % 3) - 1u;
         uint32_t const gy = gy0 + static_cast<uint32_t>(k / 3) - 1u;
         uint64_t const nk = morton2(static_cast<int32_t>(gx), static_cast<int32_t>(gy));
+        uint64_t const bin = dense_bin(ds, nk, w.dense_bins);
         nks[k] = nk;
+        bins[k] = bin;
         r[k] = make_uint2(0u, 0u);
-        if (nk < w.dense_limit) {
-            uint32_t const b = w.doffset[nk];
-            uint32_t const e = w.doffset[nk + 1];
+        if (bin < w.dense_bins) {
+            uint32_t const b = w.doffset[bin];
+            uint32_t const e = w.doffset[bin + 1];
             if (e > b) { r[k] = make_uint2(b, e); }
         }
     }
@@ -750,9 +769,10 @@ __global__ __launch_bounds__(256) void k_cell_windows_dense(WindowArgs w) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             uint64_t const nk = nks[k];
+            uint64_t const bin = bins[k];
             bool const maybe = w.other_set.slots &&
-                               (nk >= w.dense_limit || !w.other_bits ||
-                                ((w.other_bits[nk >> 5] >> (nk & 31u)) & 1u));
+                               (bin >= w.dense_bins || !w.other_bits ||
+                                ((w.other_bits[bin >> 5] >> (bin & 31u)) & 1u));
             o[k] = maybe ? cell_set_range(w.other_set, w.other, nk, tag) : make_uint2(0u, 0u);
             if (o[k].y > o[k].x) {
                 r[k].y |= WIN_SOLID_NEAR;
@@ -770,7 +790,7 @@ __global__ __launch_bounds__(256) void k_cell_windows_dense(WindowArgs w) {
 
 void launch_cell_windows(WindowArgs const& w, hipStream_t s) {
     if (!w.max_cells) { return; }
-    if (w.dense_limit) {
+    if (w.dense_bins) {
         hipLaunchKernelGGL(k_cell_windows_dense, dim3((w.max_cells + 255) / 256), dim3(256), 0, s,
                            w);
         return;

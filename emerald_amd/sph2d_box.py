@@ -654,6 +654,13 @@ class Batch:
     def kernel_timing(self) -> dict:
         return _kernel_timing(lib().emsph_batch_get_kernel_timing, self._h)
 
+    def iteration_totals(self, reset: bool = False) -> tuple:
+        """As Context.iteration_totals, over the batch's sub-steps."""
+        out = (ctypes.c_int64 * 4)()
+        check(lib().emsph_batch_get_iteration_totals(self._h, 1 if reset else 0, out),
+              "batch_get_iteration_totals")
+        return tuple(int(x) for x in out)
+
     def set_tuning(self, persistent_loop: bool = False, graphs: bool = True,
                    neighbor_batch: int = 4, xcd_remap: bool = True) -> None:
         t = _capi.Tuning(int(persistent_loop), int(graphs), int(neighbor_batch), int(xcd_remap))

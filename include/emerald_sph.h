@@ -352,6 +352,9 @@ emsph_status emsph_batch_set_graphs(emsph_batch* b, int enable);  /* as emsph_se
 emsph_status emsph_batch_set_kernel_timing(emsph_batch* b, int enable);
 int emsph_batch_get_kernel_timing(emsph_batch* b, const char** names, double* total_ms,
                                   int64_t* calls, int max);
+/* As emsph_get_iteration_totals, summed over the batch's sub-steps (out[1],
+ * out[2]: iterations launched, the maximum over the scenes of each sub-step). */
+emsph_status emsph_batch_get_iteration_totals(emsph_batch* b, int reset, int64_t* out);
 emsph_status emsph_batch_set_tuning(emsph_batch* b, const emsph_tuning* tuning);
 /* Per-scene device force model (as emsph_set_force_model; host callbacks are
  * for contexts only). */

@@ -249,3 +249,68 @@ def test_tuning_shapes_equal(gpu, method):
             for i, (a, b) in enumerate(zip(got[1], ref[1])):
                 assert_bits_equal(a, b, f"nb {nb} xcd {xcd} scene {i}")
 
+
+
+def test_batch_dense_index_used(gpu):
+    """After its first (exact) sub-step a batch indexes its cells with the
+    dense index, each scene in its own table (no radix passes), and stays
+    bit-identical to the scenes run alone."""
+    method = "iisph"
+    scs = _mixed_scenes()
+    cfgs = [scenes.config_for(sc, method) for sc in scs]
+    batch = sb.Batch(cfgs, [sb.Solid_state(s.solids) for s in scs],
+                     [sb.State(s.positions, s.velocities) for s in scs])
+    batch.frame_init(method)
+    batch.run_sub_steps(method, 3)
+    batch.set_kernel_timing(True)
+    batch.run_sub_steps(method, 2)
+    stages = batch.kernel_timing()
+    batch.set_kernel_timing(False)
+    print(f"[batch dense] stages {sorted(stages)}", flush=True)
+    assert "dense_sort" in stages and "radix_sort" not in stages
+    for i, (sc, cfg) in enumerate(zip(scs, cfgs)):
+        ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(sc.positions))
+        ctx.set_state(sb.State(sc.positions, sc.velocities))
+        ctx.frame_init(method)
+        for k in range(5):
+            ctx.sub_step(method, k * sc.dt, sc.dt)
+        assert_bits_equal(batch.get_state(i).positions, ctx.get_state().positions, f"scene {i}")
+        ctx.close()
+    batch.close()
+
+
+def test_batch_dense_table_growth_redo(gpu):
+    """A scene whose block, right of its walls (cells up to 101), moves right
+    across cell coordinate 128 leaves the 128 x 128-cell table its keys and
+    walls asked for: the batch's sub-step is flagged and redone with the radix
+    sort, the scene's table grows, and every scene equals the oracle bit for
+    bit."""
+    method = "iisph"
+    base = scenes.dam_break(12, 16, 0.005, 2e-4, seed=53)
+    pos = base.positions + np.array([1.2, 0.0], np.float32)  # cells 120..126 in x
+    vel = np.tile(np.array([[8.0, 0.0]], np.float32), (len(pos), 1))
+    moving = scenes.DamBreak(np.ascontiguousarray(pos), vel, base.solids, base.support,
+                             base.spacing, base.dt, base.mass)
+    still = scenes.dam_break(20, 30, 0.005, 2e-4, seed=54)
+    scs = [moving, still]
+    cfgs = [scenes.config_for(sc, method) for sc in scs]
+    batch = sb.Batch(cfgs, [sb.Solid_state(s.solids) for s in scs],
+                     [sb.State(s.positions, s.velocities) for s in scs])
+    orcs = [oracle_scene(sc, cfg) for sc, cfg in zip(scs, cfgs)]
+    batch.frame_init(method)
+    for o in orcs:
+        o.frame_init(method)
+    batch.iteration_totals(reset=True)
+    for k in range(30):
+        bst = batch.sub_step(method)
+        for i, (sc, o) in enumerate(zip(scs, orcs)):
+            ost = o.sub_step(method, k * sc.dt, sc.dt)
+            assert bst[i].iterations == ost.iterations, (k, i)
+    for i, o in enumerate(orcs):
+        p, v = o.state_f32()
+        st = batch.get_state(i)
+        assert_bits_equal(st.positions, p, f"scene {i} positions")
+        assert_bits_equal(st.velocities, v, f"scene {i} velocities")
+    assert float(orcs[0].state_f32()[0][:, 0].max()) > 1.28  # crossed cell coordinate 128
+    assert batch.iteration_totals()[3] >= 1  # a sub-step was redone
+    batch.close()
