@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_keys(IndexArgs a) {
         int2 const g = grid_coord(a.pos[i], cell);
         uint64_t const key = morton2(g.x, g.y);
         a.keys[i] = key;
-        a.vals[i] = i;
+        if (!a.dense_bins) { a.vals[i] = i; }  // the dense scatter writes the indices itself
         if (a.grid) { a.grid[i] = g; }
         if (a.dense_bins) {
             // Dense index: count the point into its bin (keys outside their
@@ -1435,13 +1435,16 @@ __device__ __forceinline__ void load_batch(Lists const& l, uint32_t stride, uint
 // iteration (:261-425 beta, new pressure and residual). The first iteration's
 // inputs are all known here: the pressures start at +0 (iisph.cpp:83), so
 // every sum_j d_ij p_j is exactly (+0, +0) (each term is (kf * +0) grad W /
-// rho_j^2 = +-0, or an unsafe divide that is skipped, and +0 + +-0 = +0), and
-// beta needs only rho_j and d_jj of the neighbours and the particle's own
-// a_ii and rho*. One pass over the list computes both, with the same IEEE
-// operations in the same order per accumulator (d_ij is the same value in
-// both), and skips the first sum sweep. Workgroup 0 also starts the loop.
-// Writes a_ii, rho*, {sum = (+0, +0), d_ii} and the first pressures
-// {p', rho} into pr_out.
+// rho_j^2 = +-0, or an unsafe divide that is skipped, and +0 + +-0 = +0).
+// beta is then exactly +0 as well: each of its terms is
+// m dot((+0 - 0 d_jj) - (+0 - 0 d_ji), grad W), and d_jj, d_ji and grad W
+// are finite (quotients that is_safe_div admits are below 2^126 in
+// magnitude, the others are 0), so 0 d = +-0, +0 - +-0 = +0, the dot product
+// is +-0 and +0 + +-0 = +0 for every term, fluid or solid. So beta is not
+// summed (no d_jj gather), and rho0 - rho* - beta is rho0 - rho* (x - +0 = x).
+// One pass over the list computes a_ii and rho*, and the first sum sweep is
+// skipped. Workgroup 0 also starts the loop. Writes a_ii, rho*,
+// {sum = (+0, +0), d_ii} and the first pressures {p', rho} into pr_out.
 template <int NB>
 __global__ __launch_bounds__(TILE) void k_iisph_aii_beta0(Batch a) {
     if (blockIdx.x == 0) { loop_begin_block<TILE>(a); }
@@ -1465,7 +1468,6 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii_beta0(Batch a) {
         a.sd[t] = SdDii{si, d};
         float aii;
         float rs;
-        float beta = 0.0f;
         if (!fc) {
             rs = ri;
             aii = 0.0f;
@@ -1476,17 +1478,15 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii_beta0(Batch a) {
                 NbrBatch<NB> nb;
                 load_batch<NB>(a.fl, a.stride, t, k0, fc, nb);
                 float rj[NB];
-                V2 vj[NB], djj[NB];
+                V2 vj[NB];
 #pragma unroll
                 for (int u = 0; u < NB; ++u) {
                     rj[u] = 1.0f;
                     vj[u] = V2{0.0f, 0.0f};
-                    djj[u] = V2{0.0f, 0.0f};
                     if (k0 + u < fc) {
                         uint32_t const j = nb.j[u];
                         rj[u] = a.rho[j];
                         vj[u] = a.vel[j];
-                        djj[u] = a.dii[j];
                     }
                 }
 #pragma unroll
@@ -1500,9 +1500,6 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii_beta0(Batch a) {
                         V2 const dji = neg(dij);
                         if (safe) { acc += mf * dot(sub(d, dji), gw); }  // a_ii
                         dd += dtm * dot(sub(vi, vj[u]), gw);             // rho*
-                        V2 const sjk = sub(V2{0.0f, 0.0f}, scale(pi, dji));  // beta
-                        V2 const djj_pj = scale(0.0f, djj[u]);
-                        beta += mf * dot(sub(sub(si, djj_pj), sjk), gw);
                     }
                 }
             }
@@ -1513,7 +1510,6 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii_beta0(Batch a) {
         if (sc) {
             float acc = 0.0f;
             float dd = 0.0f;
-            float bs = 0.0f;
             for (int k = 0; k < sc; ++k) {
                 size_t const o = static_cast<size_t>(k) * a.stride + t;
                 uint32_t const b = a.sl.idx[o];
@@ -1521,18 +1517,16 @@ __global__ __launch_bounds__(TILE) void k_iisph_aii_beta0(Batch a) {
                 float const ms = a.solids.vol[b] * c.rho0;
                 acc += ms * dot(d, gw);
                 dd += c.dt * ms * dot(sub(vi, a.solids.vel[b]), gw);
-                bs += ms * dot(si, gw);
             }
             aii += acc;
             rs += dd;
-            beta += bs;
         }
         a.aii[t] = aii;
         a.rho_star[t] = rs;
         // iisph_compute_new_pressures for p = +0
         float pn = pi;
         if (aii != 0.0f) {
-            float const numer = c.rho0 - rs - beta;
+            float const numer = c.rho0 - rs;  // - beta, beta = +0
             if (is_safe_div(numer, aii)) {
                 float err = -(numer - (aii * pi)) / c.rho0;
                 err = (0.0f < err) ? err : 0.0f;  // std::max(0.0f, err)
