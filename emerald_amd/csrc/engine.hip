@@ -206,7 +206,7 @@ struct SpatialIndex {
     // Dense cell index (see DenseScene): one table of dense_bins bins in which
     // scene s owns 4^dense_layout[s] bins (dense_host); the counts are zero
     // between runs. dense_gen numbers the layouts.
-    DBuf<uint32_t> dcount, doffset, drank, dscratch;
+    DBuf<uint32_t> dcount, doffset, dcell, drank, dscratch;
     DBuf<DenseScene> dscene;
     std::vector<int> dense_layout;
     std::vector<DenseScene> dense_host;
@@ -236,8 +236,8 @@ struct SpatialIndex {
         size_t const need = static_cast<size_t>(base) + 2;
         if (need > dense_cap || !drank.p || drank.n < c) {
             dense_cap = 0;
-            if ((e = dcount.alloc(need)) || (e = doffset.alloc(need)) || (e = drank.alloc(c)) ||
-                (e = dscratch.alloc(scan_scratch_words(need) + 64))) {
+            if ((e = dcount.alloc(need)) || (e = doffset.alloc(need)) || (e = dcell.alloc(need)) ||
+                (e = drank.alloc(c)) || (e = dscratch.alloc(2 * scan_scratch_words(need) + 64))) {
                 return e;
             }
             if ((e = hipMemsetAsync(dcount.p, 0, need * sizeof(uint32_t), s))) { return e; }
@@ -767,13 +767,15 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
         a.dense_bins = si.dense_bins;
         a.dscene = si.dscene.p;
         a.dcount = si.dcount.p;
+        a.dcell = si.dcell.p;
         a.doffset = si.doffset.p;
         a.drank = si.drank.p;
     }
     if ((e = run("keys", [&] { launch_keys(a, s); }))) { return e; }
     if (dense) {
-        // Counting sort over the dense table: bin offsets (the counts are
-        // cleared for the next run), scatter (marking the cell heads), then
+        // Counting sort over the dense table: each bin's first slot and cell
+        // ordinal (the counts are cleared for the next run; the cell count
+        // goes where the head scan of the radix path puts it), scatter, then
         // the cells stage below puts each cell in original index order.
         si.key_passes = 0;
         si.scene_passes = 0;
@@ -781,11 +783,10 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
         si.sorted_vals = si.vals1.p;
         a.sorted_keys = si.sorted_keys;
         a.sorted_vals = si.sorted_vals;
-        a.heads = si.heads.p;
         uint32_t const bins = static_cast<uint32_t>(a.dense_bins) + 1u;
         if ((e = run("dense_sort", [&] {
-                 launch_exclusive_scan_clear(si.dcount.p, si.doffset.p, bins, si.dscratch.p,
-                                             si.doffset.p + bins, s);
+                 launch_cell_scan_clear(si.dcount.p, si.doffset.p, si.dcell.p, bins,
+                                        si.dscratch.p, si.doffset.p + bins, si.ncells.p, s);
                  launch_dense_scatter(a, s);
              }))) {
             return e;
@@ -844,8 +845,11 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     w.owin = owin;
     w.report = &g->keystate.p->cells;
     return run("cells", [&] {
-        launch_cell_heads(a, s);
-        launch_exclusive_scan(si.heads.p, si.head_scan.p, n, si.scan_scratch.p, si.ncells.p, s);
+        if (!dense) {  // the dense path's cell scan gave the ordinals and the count
+            launch_cell_heads(a, s);
+            launch_exclusive_scan(si.heads.p, si.head_scan.p, n, si.scan_scratch.p, si.ncells.p,
+                                  s);
+        }
         launch_cells_finalize(a, s);
         launch_cell_windows(w, s);
     });

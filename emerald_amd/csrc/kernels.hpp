@@ -313,6 +313,12 @@ void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32
 // Same, then every input element set to zero (in != out).
 void launch_exclusive_scan_clear(uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch,
                                  uint32_t* total, hipStream_t s);
+// Exclusive scan of n bin counts giving each bin's first slot (first) and the
+// number of nonempty bins before it (cells), clearing the counts; *total and
+// *cell_total receive the sums. scratch holds 2 * scan_scratch_words(n) words.
+void launch_cell_scan_clear(uint32_t* counts, uint32_t* first, uint32_t* cells, uint32_t n,
+                            uint32_t* scratch, uint32_t* total, uint32_t* cell_total,
+                            hipStream_t s);
 int radix_pass_count(uint64_t kmin, uint64_t kmax);
 // Stable LSD sort of (key, val): key_passes 8-bit digits of the key, then
 // scene_passes 8-bit digits of scene_of[val] (scene the most significant).
@@ -357,6 +363,7 @@ struct IndexArgs {
     DenseScene const* dscene;  // [scenes] table of each scene
     uint32_t* dcount;       // [dense_bins + 1] per-bin counts (zero between runs)
     uint32_t* doffset;      // [dense_bins + 2] exclusive scan of dcount, then n
+    uint32_t* dcell;        // [dense_bins + 1] cells (nonempty bins) before each bin
     uint32_t* drank;        // per point (original order): rank within its bin
     // optional: each scene's largest key (written at its last sorted slot)
     uint64_t* scene_kmax;

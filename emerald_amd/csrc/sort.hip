@@ -130,29 +130,30 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
 
 //------------------------------------------------------------------------------
 // Exclusive scan of uint32 (reduce -> top scan -> downsweep).
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan(T x) {
     uint32_t const lane = __lane_id();
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        uint32_t const y = __shfl_up(x, off);
+        T const y = __shfl_up(x, off);
         if (lane >= static_cast<uint32_t>(off)) { x += y; }
     }
     return x;
 }
 
 // Block-wide exclusive scan of one value per thread; returns the block total.
-template <int THREADS>
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t& total) {
-    __shared__ uint32_t wsum[THREADS / 64];
+template <int THREADS, class T>
+__device__ __forceinline__ T block_exclusive_scan(T x, T& total) {
+    __shared__ T wsum[THREADS / 64];
     uint32_t const lane = __lane_id();
     uint32_t const wave = threadIdx.x / 64;
-    uint32_t const inc = wave_inclusive_scan(x);
+    T const inc = wave_inclusive_scan(x);
     if (lane == 63) { wsum[wave] = inc; }
     __syncthreads();
-    uint32_t prefix = 0;
-    uint32_t tot = 0;
+    T prefix = 0;
+    T tot = 0;
     for (int w = 0; w < THREADS / 64; ++w) {
-        uint32_t const s = wsum[w];
+        T const s = wsum[w];
         if (static_cast<uint32_t>(w) < wave) { prefix += s; }
         tot += s;
     }
@@ -161,73 +162,115 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t& t
     return prefix + inc - x;
 }
 
+// Scan element of a count: the count itself (uint32), or for the cell scan
+// the count in the low CELL_SHIFT bits and 1 above them when it is nonzero,
+// so one uint64 scan gives both the first slot of every bin and the number of
+// nonempty bins (cells) before it (counts sum to below 2^CELL_SHIFT).
+constexpr int CELL_SHIFT = 33;  // counts and cells are at most 2^30
+template <class T>
+__device__ __forceinline__ T scan_item(uint32_t c) {
+    if constexpr (sizeof(T) == 8) {
+        return static_cast<T>(c) | (c ? (T{1} << CELL_SHIFT) : T{0});
+    } else {
+        return c;
+    }
+}
+
+template <class T>
 __global__ __launch_bounds__(SC_THREADS) void scan_reduce(uint32_t const* __restrict__ in,
-                                                          uint32_t n,
-                                                          uint32_t* __restrict__ bsum) {
+                                                          uint32_t n, T* __restrict__ bsum) {
     uint32_t const base = blockIdx.x * SC_TILE;
-    uint32_t s = 0;
+    T s = 0;
     for (int r = 0; r < SC_ITEMS; ++r) {
         uint32_t const i = base + r * SC_THREADS + threadIdx.x;
-        if (i < n) { s += in[i]; }
+        if (i < n) { s += scan_item<T>(in[i]); }
     }
     for (int off = 32; off > 0; off >>= 1) { s += __shfl_xor(s, off); }
-    __shared__ uint32_t ws[SC_THREADS / 64];
+    __shared__ T ws[SC_THREADS / 64];
     if (__lane_id() == 0) { ws[threadIdx.x / 64] = s; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t t = 0;
+        T t = 0;
         for (int w = 0; w < SC_THREADS / 64; ++w) { t += ws[w]; }
         bsum[blockIdx.x] = t;
     }
 }
 
+// The scan's outputs: element prefixes into out (uint32; for the cell scan
+// the slot part) and, for the cell scan, cell prefixes into cells; the grand
+// total into *total and, for the cell scan, the cell total into *cell_total
+// (each when not null).
+template <class T>
+__device__ __forceinline__ void scan_store(T v, uint32_t* out, uint32_t* cells, uint32_t i) {
+    if constexpr (sizeof(T) == 8) {
+        out[i] = static_cast<uint32_t>(v & ((T{1} << CELL_SHIFT) - 1));
+        cells[i] = static_cast<uint32_t>(v >> CELL_SHIFT);
+    } else {
+        out[i] = v;
+    }
+}
+template <class T>
+__device__ __forceinline__ void scan_total(T v, uint32_t* total, uint32_t* cell_total) {
+    if constexpr (sizeof(T) == 8) {
+        if (total) { *total = static_cast<uint32_t>(v & ((T{1} << CELL_SHIFT) - 1)); }
+        if (cell_total) { *cell_total = static_cast<uint32_t>(v >> CELL_SHIFT); }
+    } else {
+        if (total) { *total = v; }
+    }
+}
+
 // Single workgroup: exclusive scan of nb block sums in place; writes the grand
-// total to *total when total is not null.
-__global__ __launch_bounds__(1024) void scan_top(uint32_t* __restrict__ bsum, uint32_t nb,
-                                                 uint32_t* __restrict__ total) {
+// total(s) (scan_total).
+template <class T>
+__global__ __launch_bounds__(1024) void scan_top(T* __restrict__ bsum, uint32_t nb,
+                                                 uint32_t* __restrict__ total,
+                                                 uint32_t* __restrict__ cell_total) {
     uint32_t const per = (nb + 1023) / 1024;
     uint32_t const lo = threadIdx.x * per;
-    uint32_t s = 0;
+    T s = 0;
     for (uint32_t i = 0; i < per; ++i) {
         uint32_t const j = lo + i;
         if (j < nb) { s += bsum[j]; }
     }
-    uint32_t tot = 0;
-    uint32_t run = block_exclusive_scan<1024>(s, tot);
+    T tot = 0;
+    T run = block_exclusive_scan<1024>(s, tot);
     for (uint32_t i = 0; i < per; ++i) {
         uint32_t const j = lo + i;
         if (j < nb) {
-            uint32_t const v = bsum[j];
+            T const v = bsum[j];
             bsum[j] = run;
             run += v;
         }
     }
-    if (total && threadIdx.x == 0) { *total = tot; }
+    if (threadIdx.x == 0) { scan_total(tot, total, cell_total); }
 }
 
 // Downsweep. With nb > 0 the block's offset is the sum of the block sums
 // before it (read here, so no separate top-level scan runs; used while nb is
-// small), and the last block writes the grand total to *total (if not null);
-// with nb == 0, bsum already holds the scanned block offsets.
+// small), and the last block writes the grand total(s) (scan_total); with
+// nb == 0, bsum already holds the scanned block offsets.
+template <class T>
 __global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t* __restrict__ in,
-                                                        uint32_t* __restrict__ out, uint32_t n,
-                                                        uint32_t const* __restrict__ bsum,
-                                                        uint32_t nb, uint32_t* __restrict__ total,
+                                                        uint32_t* __restrict__ out,
+                                                        uint32_t* __restrict__ cells, uint32_t n,
+                                                        T const* __restrict__ bsum, uint32_t nb,
+                                                        uint32_t* __restrict__ total,
+                                                        uint32_t* __restrict__ cell_total,
                                                         int clear_in) {
-    __shared__ uint32_t tile[SC_TILE];
-    __shared__ uint32_t block_offset;
+    __shared__ T tile[SC_TILE];
+    __shared__ T block_offset;
     if (nb) {
-        uint32_t part = 0;
+        T part = 0;
         for (uint32_t i = threadIdx.x; i < blockIdx.x; i += SC_THREADS) { part += bsum[i]; }
         for (int off = 32; off > 0; off >>= 1) { part += __shfl_xor(part, off); }
-        __shared__ uint32_t ws[SC_THREADS / 64];
+        __shared__ T ws[SC_THREADS / 64];
         if (__lane_id() == 0) { ws[threadIdx.x / 64] = part; }
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint32_t o = 0;
+            T o = 0;
             for (int w = 0; w < SC_THREADS / 64; ++w) { o += ws[w]; }
             block_offset = o;
-            if (total && blockIdx.x == nb - 1) { *total = o + bsum[blockIdx.x]; }
+            if (blockIdx.x == nb - 1) { scan_total<T>(o + bsum[blockIdx.x], total, cell_total); }
         }
     } else if (threadIdx.x == 0) {
         block_offset = bsum[blockIdx.x];
@@ -235,26 +278,26 @@ __global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t* __restrict__ i
     uint32_t const base = blockIdx.x * SC_TILE;
     for (int r = 0; r < SC_ITEMS; ++r) {
         uint32_t const i = base + r * SC_THREADS + threadIdx.x;
-        tile[r * SC_THREADS + threadIdx.x] = (i < n) ? in[i] : 0u;
+        tile[r * SC_THREADS + threadIdx.x] = (i < n) ? scan_item<T>(in[i]) : T{0};
         if (clear_in && i < n) { in[i] = 0u; }  // each element is read by this thread only
     }
     __syncthreads();
     // Each thread scans SC_ITEMS consecutive elements.
     uint32_t const t0 = threadIdx.x * SC_ITEMS;
-    uint32_t s = 0;
+    T s = 0;
     for (int r = 0; r < SC_ITEMS; ++r) { s += tile[t0 + r]; }
-    uint32_t tot = 0;
+    T tot = 0;
     __syncthreads();  // block_offset
-    uint32_t run = block_exclusive_scan<SC_THREADS>(s, tot) + block_offset;
+    T run = block_exclusive_scan<SC_THREADS>(s, tot) + block_offset;
     for (int r = 0; r < SC_ITEMS; ++r) {
-        uint32_t const v = tile[t0 + r];
+        T const v = tile[t0 + r];
         tile[t0 + r] = run;
         run += v;
     }
     __syncthreads();
     for (int r = 0; r < SC_ITEMS; ++r) {
         uint32_t const i = base + r * SC_THREADS + threadIdx.x;
-        if (i < n) { out[i] = tile[r * SC_THREADS + threadIdx.x]; }
+        if (i < n) { scan_store<T>(tile[r * SC_THREADS + threadIdx.x], out, cells, i); }
     }
 }
 
@@ -264,30 +307,41 @@ __global__ __launch_bounds__(SC_THREADS) void scan_down(uint32_t* __restrict__ i
 size_t scan_scratch_words(size_t n) { return (n + SC_TILE - 1) / SC_TILE + 1; }
 
 namespace {
-void exclusive_scan(uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch, uint32_t* total,
-                    int clear_in, hipStream_t s) {
+template <class T>
+void exclusive_scan(uint32_t* in, uint32_t* out, uint32_t* cells, uint32_t n, T* scratch,
+                    uint32_t* total, uint32_t* cell_total, int clear_in, hipStream_t s) {
     if (n == 0) { return; }
     uint32_t const nb = (n + SC_TILE - 1) / SC_TILE;
-    hipLaunchKernelGGL(scan_reduce, dim3(nb), dim3(SC_THREADS), 0, s, in, n, scratch);
+    hipLaunchKernelGGL(scan_reduce<T>, dim3(nb), dim3(SC_THREADS), 0, s, in, n, scratch);
     if (nb <= SCAN_DIRECT_BLOCKS) {
-        hipLaunchKernelGGL(scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, scratch, nb,
-                           total, clear_in);
+        hipLaunchKernelGGL(scan_down<T>, dim3(nb), dim3(SC_THREADS), 0, s, in, out, cells, n,
+                           scratch, nb, total, cell_total, clear_in);
     } else {
-        hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-        hipLaunchKernelGGL(scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, scratch, 0u,
-                           nullptr, clear_in);
+        hipLaunchKernelGGL(scan_top<T>, dim3(1), dim3(1024), 0, s, scratch, nb, total,
+                           cell_total);
+        hipLaunchKernelGGL(scan_down<T>, dim3(nb), dim3(SC_THREADS), 0, s, in, out, cells, n,
+                           scratch, 0u, nullptr, nullptr, clear_in);
     }
 }
 }  // namespace
 
 void launch_exclusive_scan(uint32_t const* in, uint32_t* out, uint32_t n, uint32_t* scratch,
                            uint32_t* total, hipStream_t s) {
-    exclusive_scan(const_cast<uint32_t*>(in), out, n, scratch, total, 0, s);
+    exclusive_scan<uint32_t>(const_cast<uint32_t*>(in), out, nullptr, n, scratch, total, nullptr,
+                             0, s);
 }
 
 void launch_exclusive_scan_clear(uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch,
                                  uint32_t* total, hipStream_t s) {
-    exclusive_scan(in, out, n, scratch, total, 1, s);
+    exclusive_scan<uint32_t>(in, out, nullptr, n, scratch, total, nullptr, 1, s);
+}
+
+void launch_cell_scan_clear(uint32_t* counts, uint32_t* first, uint32_t* cells, uint32_t n,
+                            uint32_t* scratch, uint32_t* total, uint32_t* cell_total,
+                            hipStream_t s) {
+    exclusive_scan<unsigned long long>(counts, first, cells, n,
+                                       reinterpret_cast<unsigned long long*>(scratch), total,
+                                       cell_total, 1, s);
 }
 
 size_t radix_counts_words(size_t n) {

@@ -329,17 +329,15 @@ __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) 
 
 // Dense index scatter: point i goes to the first slot of its bin plus its
 // rank (the ranks of one bin are a permutation of 0 .. count - 1, so every
-// slot receives exactly one point); the slot of rank 0 heads its cell.
+// slot receives exactly one point).
 __global__ __launch_bounds__(256) void k_dense_scatter(IndexArgs a) {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) { return; }
     uint64_t const key = a.keys[i];
     uint64_t const bin = dense_bin(a.dscene[a.scene_of[i]], key, a.dense_bins);
-    uint32_t const r = a.drank[i];
-    uint32_t const t = a.doffset[bin] + r;
+    uint32_t const t = a.doffset[bin] + a.drank[i];
     const_cast<uint64_t*>(a.sorted_keys)[t] = key;
     const_cast<uint32_t*>(a.sorted_vals)[t] = i;
-    a.heads[t] = r == 0u ? 1u : 0u;
 }
 
 void launch_dense_scatter(IndexArgs const& a, hipStream_t s) {
@@ -444,7 +442,7 @@ __global__ __launch_bounds__(256) void k_cells_finalize_dense(IndexArgs a) {
         atomicOr(&a.keystate->dense_crowded, 1u);
         atomicOr(&a.keystate->sort_short, 1u);
     }
-    uint32_t const cb = a.head_scan[t] + a.heads[t] - 1u;
+    uint32_t const cb = a.dcell[bin];  // the cell scan's ordinal of the bin
     a.block_index[d] = cb;
     if (t == b) {
         a.cell_begin[cb] = b;
@@ -466,7 +464,6 @@ __global__ __launch_bounds__(256) void k_cells_finalize_dense(IndexArgs a) {
 }
 
 void launch_cell_heads(IndexArgs const& a, hipStream_t s) {
-    if (a.dense_bins) { return; }  // the dense scatter marked the heads
     hipLaunchKernelGGL(k_cell_heads, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_cells_finalize(IndexArgs const& a, hipStream_t s) {
