@@ -562,6 +562,37 @@ hipError_t timed(Engine* g, char const* name, F&& launch) {
     return hipSuccess;
 }
 
+// The build stage under kernel timing: two records, "build" and
+// "build_deferred", whose events the two dispatches stamp themselves
+// (hipExtLaunchKernelGGL), so the intervals are the kernels' own durations.
+template <class F>
+hipError_t timed_build(Engine* g, F&& launch) {
+    if (!g->timer.enabled) {
+        launch(nullptr);
+        return hipGetLastError();
+    }
+    KernelTimer& tm = g->timer;
+    if (tm.used + 2 > KernelTimer::POOL) {
+        hipError_t const e = timer_flush(g);
+        if (e) { return e; }
+    }
+    size_t const k = tm.used;
+    while (tm.ev.size() < 2 * (k + 2)) {
+        hipEvent_t e = nullptr;
+        hipError_t const r = hipEventCreate(&e);
+        if (r) { return r; }
+        tm.ev.push_back(e);
+    }
+    if (tm.names.size() < k + 2) { tm.names.resize(k + 2); }
+    tm.names[k] = "build";
+    tm.names[k + 1] = "build_deferred";
+    launch(&tm.ev[2 * k]);
+    hipError_t const e = hipGetLastError();
+    if (e) { return e; }
+    tm.used = k + 2;
+    return hipSuccess;
+}
+
 emsph_status engine_init(Engine* g) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -1272,7 +1303,9 @@ hipError_t enqueue_prepare(Engine* g, int build_mode, bool dfsph) {
     HIP_PASS(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->prep_passes, ga,
                           g->timer.enabled, g->NS ? &g->sidx : nullptr, g->swin.p,
                           g->prep_dense));
-    return timed(g, "build", [&] { launch_build(build_mode, g->build_cap, b, g->stream); });
+    return timed_build(g, [&](hipEvent_t const* ev) {
+        launch_build(build_mode, g->build_cap, b, g->stream, ev);
+    });
 }
 
 // Host-side fields of the fluid index that a sub-step's sort leaves (also
@@ -2586,8 +2619,19 @@ emsph_status emsph_set_cfl(emsph_context* c, float factor) {
 
 static emsph_status engine_set_kernel_timing(Engine* g, int enable) {
     HIP_TRY(timer_flush(g));
-    g->timer.enabled = enable != 0;
-    g->timer.acc.clear();
+    KernelTimer& tm = g->timer;
+    if (enable) {
+        // The whole event pool up front: creating events between launches
+        // delays the host, the device drains, and the wait lands inside the
+        // next stage's interval.
+        while (tm.ev.size() < 2 * KernelTimer::POOL) {
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreate(&e));
+            tm.ev.push_back(e);
+        }
+    }
+    tm.enabled = enable != 0;
+    tm.acc.clear();
     return EMSPH_OK;
 }
 

@@ -427,7 +427,10 @@ void launch_loop_decide(Batch const& b, int kind, int it, int slot, hipStream_t 
 
 // cap: LDS list capacity (one of BUILD_CAPS). Below 64 a particle whose list
 // would exceed it is deferred and rebuilt by a full-capacity pass.
-void launch_build(int mode, int cap, Batch const& b, hipStream_t s);
+// ev (optional, kernel timing): start/stop events of the build kernel and of
+// its deferred pass, [4].
+void launch_build(int mode, int cap, Batch const& b, hipStream_t s,
+                  hipEvent_t const* ev = nullptr);
 // a_ii, rho* and the pressure loop's first iteration (pressures +0) in one
 // sweep, into pr_out.
 void launch_iisph_aii_beta0(Batch const& b, hipStream_t s);

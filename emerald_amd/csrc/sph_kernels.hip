@@ -14,6 +14,7 @@
 // the fluid pass and then the solid pass of its particle without changing any
 // result.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 
@@ -1346,32 +1347,49 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build_deferred(Batch a) {
     }
 }
 
+// ev (timing only, else null): start/stop events of the build launch and of
+// the deferred pass, stamped by the dispatches themselves.
 template <int MODE>
-void launch_build_mode(int cap, Batch const& a, hipStream_t s) {
+void launch_build_mode(int cap, Batch const& a, hipStream_t s, hipEvent_t const* ev) {
+    hipEvent_t const e0 = ev ? ev[0] : nullptr;
+    hipEvent_t const e1 = ev ? ev[1] : nullptr;
     uint32_t const blocks = a.ntiles * (TILE / BUILD_WG);
     if (cap <= 16) {
-        hipLaunchKernelGGL((k_build<MODE, 16, BUILD_WG>), dim3(blocks), dim3(BUILD_WG), 0, s, a);
+        hipExtLaunchKernelGGL((k_build<MODE, 16, BUILD_WG>), dim3(blocks), dim3(BUILD_WG), 0, s,
+                              e0, e1, 0, a);
     } else if (cap <= 32) {
-        hipLaunchKernelGGL((k_build<MODE, 32, BUILD_WG>), dim3(blocks), dim3(BUILD_WG), 0, s, a);
+        hipExtLaunchKernelGGL((k_build<MODE, 32, BUILD_WG>), dim3(blocks), dim3(BUILD_WG), 0, s,
+                              e0, e1, 0, a);
     } else {
         // 64 entries of 64 lanes: 32 KB of LDS per workgroup.
-        hipLaunchKernelGGL((k_build<MODE, MAXN, BUILD_THREADS>),
-                           dim3(a.ntiles * (TILE / BUILD_THREADS)), dim3(BUILD_THREADS), 0, s, a);
+        hipExtLaunchKernelGGL((k_build<MODE, MAXN, BUILD_THREADS>),
+                              dim3(a.ntiles * (TILE / BUILD_THREADS)), dim3(BUILD_THREADS), 0, s,
+                              e0, e1, 0, a);
+        if (ev) {  // no deferred pass: an empty interval
+            (void)hipEventRecord(ev[2], s);
+            (void)hipEventRecord(ev[3], s);
+        }
         return;
     }
     // Deferred lists are rare (under 1/64 of the particles while the 16-entry
     // build is chosen): one workgroup per CU at most, most of them empty.
     uint32_t const dblocks = a.ntiles < 256 ? a.ntiles : 256;
-    hipLaunchKernelGGL(k_build_deferred<MODE>, dim3(dblocks), dim3(BUILD_THREADS), 0, s, a);
+    hipExtLaunchKernelGGL(k_build_deferred<MODE>, dim3(dblocks), dim3(BUILD_THREADS), 0, s,
+                          ev ? ev[2] : nullptr, ev ? ev[3] : nullptr, 0, a);
 }
 
-void launch_build(int mode, int cap, Batch const& a, hipStream_t s) {
-    if (!a.ntiles) { return; }
+void launch_build(int mode, int cap, Batch const& a, hipStream_t s, hipEvent_t const* ev) {
+    if (!a.ntiles) {
+        if (ev) {
+            for (int k = 0; k < 4; ++k) { (void)hipEventRecord(ev[k], s); }
+        }
+        return;
+    }
     switch (mode) {
-    case BUILD_IISPH: launch_build_mode<BUILD_IISPH>(cap, a, s); break;
-    case BUILD_DFSPH: launch_build_mode<BUILD_DFSPH>(cap, a, s); break;
-    case BUILD_AP: launch_build_mode<BUILD_AP>(cap, a, s); break;
-    default: launch_build_mode<BUILD_LISTS>(cap, a, s); break;
+    case BUILD_IISPH: launch_build_mode<BUILD_IISPH>(cap, a, s, ev); break;
+    case BUILD_DFSPH: launch_build_mode<BUILD_DFSPH>(cap, a, s, ev); break;
+    case BUILD_AP: launch_build_mode<BUILD_AP>(cap, a, s, ev); break;
+    default: launch_build_mode<BUILD_LISTS>(cap, a, s, ev); break;
     }
 }
 
