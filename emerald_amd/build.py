@@ -68,24 +68,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     os.replace(tmp, LIB)
     build_headless(verbose)
     build_api_check(verbose)
-    build_probe(verbose)
     return LIB
-
-
-PROBE = os.path.join(HERE, "bandwidth_probe")
-
-
-def build_probe(verbose: bool = False) -> str:
-    """HBM copy probe used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE."""
-    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "bandwidth_probe.hip"), "-o", PROBE + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
-    if p.returncode != 0:
-        sys.stderr.write(p.stdout.decode(errors="replace"))
-        raise RuntimeError("hipcc failed on bandwidth_probe.hip")
-    os.replace(PROBE + ".tmp", PROBE)
-    return PROBE
 
 
 HEADLESS = os.path.join(HERE, "sph2d_box_headless")

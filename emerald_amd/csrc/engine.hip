@@ -464,6 +464,9 @@ struct Engine {
     } fhost;
 
     ~Engine() {
+        // Work still queued may copy into the pinned block or run the graphs:
+        // drain the stream before anything it uses is released.
+        if (stream) { (void)hipStreamSynchronize(stream); }
         if (host) { (void)hipHostFree(host); }
         for (hipEvent_t e : read_ev) {
             if (e) { (void)hipEventDestroy(e); }
@@ -1897,6 +1900,9 @@ emsph_status engine_set_dt(Engine* g, uint32_t s, float dt) {
         SceneDev const sd = scene_dev(g->sh[s], first_tile);
         HIP_TRY(hipMemcpyAsync(g->scenes.p + s, &sd, sizeof(SceneDev), hipMemcpyHostToDevice,
                                g->stream));
+        // sd lives on this stack frame: the copy must have read it before the
+        // frame ends.
+        HIP_TRY(hipStreamSynchronize(g->stream));
     }
     return EMSPH_OK;
 }

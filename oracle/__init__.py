@@ -23,8 +23,16 @@ Parity status of each oracle function (what pins it):
     test pins them; pinned by invariants (uniform lattice density, pairwise
     antisymmetry, v=0 -> div=0, p=0 -> s=0 and beta=0) and by an independent
     FP64 brute-force restatement on small scenes (tests/test_oracle_solvers.py).
-  * Imath Vec2::length() tiny-vector branch (|v|^2 < 2*FLT_MIN): parity
-    unpinned (no reference test and Imath is absent from the image).
+  * Imath Vec2::length() (IlmBase 2.5 ImathVec.h, absent from the image),
+    including the lengthTiny branch for |v|^2 < 2*FLT_MIN: the IEEE root of
+    |v|^2 above the threshold; below it, within 2 float ulps of the exact
+    length where x*x underflows, exact on axis vectors and under power-of-two
+    scaling, checks a plain sqrtf(x*x + y*y) fails (tests/test_oracle_kats.py).
+  * scene inputs: world_walls_initial_solid_state with its random obstacles
+    (solids.cpp:139-206) and dam_break_initial_state (initial_state.cpp:14-97):
+    the C++ standard's mt19937_64 known answer, a pure-Python restatement of
+    the float draws and the boxes, and an all-pairs kill test
+    (tests/test_oracle_scene.py).
 """
 
 from __future__ import annotations
@@ -158,6 +166,10 @@ def lib():
             "orc_simulation_config": (None, [_F, _F, _I64, _I, _P]),
             "orc_estimate_solid_box_emission_count": (_SZ, [_F, _F, _F, _F, _F]),
             "orc_emit_solid_box": (_SZ, [_F, _F, _F, _F, _F, _SZ, _P]),
+            "orc_world_walls_positions": (_SZ, [_F, _F, _I, _P, _SZ]),
+            "orc_search_seconds": (_D, [_SZ, _P, _F, _P]),
+            "orc_dam_break_positions": (_SZ, [_F, _F, _SZ, _P, _P, _SZ]),
+            "orc_mt19937_64_uniform_float": (None, [_U64, _F, _F, _SZ, _P]),
             "orc_scene_create": (_P, [_P, _SZ, _P, _SZ, _P, _P, _I]),
             "orc_scene_destroy": (None, [_P]),
             "orc_set_threads": (None, [_I]),
@@ -257,6 +269,15 @@ def neighborhoods(self_xy, other_xy, same_array: bool, max_distance: float):
     return counts, idx, dist, vec
 
 
+def search_seconds(xy: np.ndarray, cell: float):
+    """(seconds, neighbour pairs) of one neighbour search of the points: the
+    reference's timed region of test_neighborhood.cpp:178-231."""
+    xy = np.ascontiguousarray(xy, np.float32)
+    pairs = ctypes.c_uint64()
+    sec = lib().orc_search_seconds(len(xy), _ptr(xy), cell, ctypes.byref(pairs))
+    return float(sec), int(pairs.value)
+
+
 def simulation_config(support: float, rho0: float, time_per_step: int,
                       sub_steps: int) -> dict:
     out = np.empty(4, np.float32)
@@ -271,6 +292,32 @@ def emit_solid_box(box, R: float) -> np.ndarray:
     out = np.empty((max(n, 1), 2), np.float32)
     m = lib().orc_emit_solid_box(x0, y0, x1, y1, R, n, _ptr(out))
     return out[:m].copy()
+
+
+def world_walls_positions(support: float, length: float = 1.0,
+                          random_obstacles: bool = True) -> np.ndarray:
+    """world_walls_initial_solid_state positions (solids.cpp:139-206)."""
+    n = lib().orc_world_walls_positions(support, length, int(random_obstacles), None, 0)
+    out = np.empty((max(n, 1), 2), np.float32)
+    lib().orc_world_walls_positions(support, length, int(random_obstacles), _ptr(out), n)
+    return out[:n].copy()
+
+
+def dam_break_positions(support: float, solid_xy, length: float = 1.0) -> np.ndarray:
+    """dam_break_initial_state positions (initial_state.cpp:14-97)."""
+    s = np.ascontiguousarray(solid_xy, np.float32).reshape(-1, 2)
+    n = lib().orc_dam_break_positions(support, length, len(s), _ptr(s), None, 0)
+    out = np.empty((max(n, 1), 2), np.float32)
+    lib().orc_dam_break_positions(support, length, len(s), _ptr(s), _ptr(out), n)
+    return out[:n].copy()
+
+
+def uniform_float_draws(seed: int, a: float, b: float, n: int) -> np.ndarray:
+    """n draws of std::uniform_real_distribution<float>{a, b} over
+    std::mt19937_64{seed} (the host C++ library's)."""
+    out = np.empty(n, np.float32)
+    lib().orc_mt19937_64_uniform_float(seed, a, b, n, _ptr(out))
+    return out
 
 
 class Scene:

@@ -25,12 +25,14 @@
 //
 // Imath 2.5 (Vec2 length/dot/operators) is absent from this image; its
 // semantics are restated in struct V2 below. Vec2::length() and lengthTiny()
-// follow Imath's published source; that restatement is parity-unpinned for
-// |v|^2 < 2*FLT_MIN (coincident particles) because no reference test covers it.
+// follow Imath's published algorithm; the tiny branch (|v|^2 < 2*FLT_MIN,
+// coincident particles) is checked against the exact length and its scaling
+// properties by the tests, since no reference test covers it.
 //
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off -fopenmp, no fast-math).
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -42,6 +44,7 @@
 #include <vector>
 
 #include <omp.h>
+#include <parallel/algorithm>
 
 namespace orc {
 
@@ -194,6 +197,13 @@ struct Neighborhood {
         kernels.resize(n * MAXN);
         kernel_gradients.resize(n * MAXN);
     }
+    // Only the arrays create_neighborhoods writes.
+    void resize_lists(size_t n) {
+        counts.assign(n, 0);
+        indices.resize(n * MAXN);
+        distances.resize(n * MAXN);
+        vectors_to.resize(n * MAXN);
+    }
 };
 
 // The spatial index of one particle array: grid coords, z keys, sorted
@@ -232,21 +242,44 @@ void build_spatial_index(size_t n, V2<T> const* pos, T cell, SpatialIndex<T>& si
                              static_cast<uint32_t>(si.grid[2 * i + 1]));
         si.pairs[i] = {si.keys[i], static_cast<uint32_t>(i)};  // neighborhood.cpp:10-16
     }
-    // neighborhood.cpp:18-26: lexicographic sort of unique (key, index) pairs.
-    std::sort(si.pairs.begin(), si.pairs.end());
+    // neighborhood.cpp:18-26: lexicographic sort of unique (key, index) pairs
+    // (tbb::parallel_sort there; a parallel sort here: unique pairs have one
+    // sorted order).
+    __gnu_parallel::sort(si.pairs.begin(), si.pairs.end());
     si.blocks.clear();
     si.block_map.clear();
     if (n == 0) { return; }
-    // neighborhood.cpp:33-74: block ordinal per sorted slot.
+    // neighborhood.cpp:33-74: block ordinal per sorted slot (tbb::parallel_scan
+    // there): key changes counted per chunk, chunk offsets, then the ordinals.
     si.block_indices[0] = 0;
-    uint32_t sum = 0;
-    for (size_t i = 1; i < n; ++i) {
-        if (si.pairs[i - 1].first != si.pairs[i].first) { ++sum; }
-        si.block_indices[i] = sum;
+    {
+        int const nt = std::max(1, omp_get_max_threads());
+        std::vector<uint32_t> part(static_cast<size_t>(nt) + 1, 0);
+        size_t const per = (n + nt - 1) / nt;
+#pragma omp parallel for schedule(static, 1)
+        for (int t = 0; t < nt; ++t) {
+            size_t const lo = std::max<size_t>(1, t * per);
+            size_t const hi = std::min(n, (t + 1) * per);
+            uint32_t c = 0;
+            for (size_t i = lo; i < hi; ++i) { c += si.pairs[i - 1].first != si.pairs[i].first; }
+            part[t + 1] = c;
+        }
+        for (int t = 0; t < nt; ++t) { part[t + 1] += part[t]; }
+#pragma omp parallel for schedule(static, 1)
+        for (int t = 0; t < nt; ++t) {
+            size_t const lo = std::max<size_t>(1, t * per);
+            size_t const hi = std::min(n, (t + 1) * per);
+            uint32_t sum = part[t];
+            for (size_t i = lo; i < hi; ++i) {
+                sum += si.pairs[i - 1].first != si.pairs[i].first;
+                si.block_indices[i] = sum;
+            }
+        }
     }
     size_t const block_count = static_cast<size_t>(si.block_indices[n - 1]) + 1;
     si.blocks.resize(block_count);
     // neighborhood.cpp:76-106: each run head scans to its run end.
+#pragma omp parallel for schedule(static, 512)
     for (size_t i = 0; i < n; ++i) {
         uint32_t const b = si.block_indices[i];
         if (i == 0 || si.block_indices[i - 1] != b) {
@@ -1683,6 +1716,121 @@ inline void simulation_config(float support, float rho0, int64_t time_per_step,
     *pcisph_denom = beta * (gsum.dot(gsum) + gdot);
 }
 
+// solids.cpp:139-206 world_walls_initial_solid_state, the positions: four
+// walls 3H thick, then (random_obstacles) the 16 "tiny_block" boxes whose
+// offsets come from std::mt19937_64{1919} through
+// std::uniform_real_distribution<float>{-0.06f, 0.06f}. The reference passes
+// three draws as the arguments of one call (solids.cpp:172); its clang build
+// evaluates them left to right (x offset, y offset, half-size offset), which is
+// written out here because g++ would evaluate the call's arguments in another
+// order. Boxes are emitted in order into one array of 100 + the estimated
+// counts (solids.cpp:183-199).
+inline std::vector<float> world_walls_positions(float support, float length,
+                                                bool random_obstacles) {
+    float const H = support;
+    float const R = 0.99f * (H * 0.5f);
+    float const L = length;
+    float const border = H * 3.0f;
+    struct Box {
+        float x0, y0, x1, y1;
+    };
+    std::vector<Box> boxes;
+    boxes.push_back({-border, -border, 0.0f, L + border});
+    boxes.push_back({L, -border, L + border, L + border});
+    boxes.push_back({-0.5f * border, -border, L + 0.5f * border, 0.0f});
+    boxes.push_back({-0.5f * border, L, L + 0.5f * border, L + border});
+    if (random_obstacles) {
+        std::mt19937_64 gen{1919};
+        std::uniform_real_distribution<float> dist{-0.06f, 0.06f};
+        int const N = 4;
+        for (int by = 0; by < N; ++by) {
+            float const byf = float(by + 1) / (N + 1);
+            for (int bx = 0; bx < N; ++bx) {
+                float const bxf = float(bx + 1) / (N + 1);
+                V2<float> unit;
+                unit.x = bxf + dist(gen);
+                unit.y = byf + dist(gen);
+                float const roff = dist(gen);
+                // solids.cpp:160-163 tiny_block
+                float const r = 0.04f + std::abs(roff);
+                V2<float> const lo = L * (unit - V2<float>{r, r});
+                V2<float> const hi = L * (unit + V2<float>{r, r});
+                boxes.push_back({lo.x, lo.y, hi.x, hi.y});
+            }
+        }
+    }
+    size_t total = 100;
+    for (auto const& b : boxes) {
+        total += estimate_solid_box_emission_count(b.x0, b.y0, b.x1, b.y1, R);
+    }
+    std::vector<float> xy(2 * total);
+    size_t remaining = total;
+    size_t at = 0;
+    for (auto const& b : boxes) {
+        size_t const k = emit_solid_box(b.x0, b.y0, b.x1, b.y1, R, remaining, xy.data() + 2 * at);
+        remaining -= k;
+        at += k;
+    }
+    xy.resize(2 * at);
+    return xy;
+}
+
+// initial_state.cpp:14-97 dam_break_initial_state, the positions: a hex
+// block [0, L/2] x [0, 3.25 L / 4] emitted with radius R = 0.99 H / 2, then
+// every particle with a solid closer than 2R removed. The solids are looked up
+// through their own spatial index (cell 2H, the 3x3 cells around the
+// particle's cell, each cell's sorted slots in order) as the reference does
+// with the solid state's block map (initial_state.cpp:36-72).
+inline std::vector<float> dam_break_positions(float support, float length, size_t ns,
+                                              float const* solid_xy) {
+    float const H = support;
+    float const R = 0.99f * (H * 0.5f);
+    float const L = length;
+    float const x1 = L / 2.0f;
+    float const y1 = 3.25f * L / 4.0f;
+    size_t const max_count = 100 + estimate_solid_box_emission_count(0.0f, 0.0f, x1, y1, R);
+    std::vector<float> xy(2 * max_count);
+    size_t const n = emit_solid_box(0.0f, 0.0f, x1, y1, R, max_count, xy.data());
+    std::vector<V2<float>> spos(ns);
+    for (size_t b = 0; b < ns; ++b) { spos[b] = V2<float>{solid_xy[2 * b], solid_xy[2 * b + 1]}; }
+    float const cell = 2.0f * H;
+    SpatialIndex<float> si;
+    build_spatial_index(ns, spos.data(), cell, si);
+    std::vector<uint8_t> kill(n, 0);
+#pragma omp parallel for schedule(static, 512)
+    for (size_t pi = 0; pi < n; ++pi) {
+        V2<float> const pos{xy[2 * pi], xy[2 * pi + 1]};
+        int32_t g[2];
+        grid_coord(pos, cell, g);
+        bool k = false;
+        for (int32_t j = g[1] - 1; !k && j <= g[1] + 1; ++j) {
+            for (int32_t i = g[0] - 1; !k && i <= g[0] + 1; ++i) {
+                auto const it = si.block_map.find(
+                  z_index(static_cast<uint32_t>(i), static_cast<uint32_t>(j)));
+                if (it == si.block_map.end()) { continue; }
+                for (uint32_t sip = it->second.first; sip != it->second.second; ++sip) {
+                    uint32_t const oi = si.pairs[sip].second;
+                    float const distance = (spos[oi] - pos).length();
+                    if (distance < 2 * R) {
+                        k = true;
+                        break;
+                    }
+                }
+            }
+        }
+        kill[pi] = k ? 1 : 0;
+    }
+    std::vector<float> out;
+    out.reserve(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        if (!kill[i]) {
+            out.push_back(xy[2 * i]);
+            out.push_back(xy[2 * i + 1]);
+        }
+    }
+    return out;
+}
+
 }  // namespace orc
 
 //==============================================================================
@@ -1762,6 +1910,26 @@ void orc_neighborhoods(size_t n_self, float const* self_xy, size_t n_other,
     if (vectors) { std::memcpy(vectors, nb.vectors_to.data(), n_self * MAXN * 2 * sizeof(float)); }
 }
 
+// Wall seconds of one neighbour search of n points, the region the
+// reference times in sph_common/tests/test_neighborhood.cpp:178-231: grid
+// coords, keys, sorted pairs, block indices, blocks and block map, the list
+// arrays' allocation and create_regular_neighborhoods. *pairs receives the sum
+// of the neighbour counts (a check that the work was done).
+double orc_search_seconds(size_t n, float const* xy, float cell, uint64_t* pairs) {
+    auto const t0 = std::chrono::steady_clock::now();
+    auto const* p = reinterpret_cast<V2<float> const*>(xy);
+    SpatialIndex<float> si;
+    build_spatial_index(n, p, cell, si);
+    Neighborhood<float> nb;
+    nb.resize_lists(n);
+    create_neighborhoods(n, cell, nb, p, si.grid.data(), p, si, true);
+    auto const t1 = std::chrono::steady_clock::now();
+    uint64_t total = 0;
+    for (size_t i = 0; i < n; ++i) { total += nb.counts[i]; }
+    if (pairs) { *pairs = total; }
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
 void orc_simulation_config(float support, float rho0, int64_t time_per_step, int sub_steps,
                            float* out4) {
     simulation_config(support, rho0, time_per_step, sub_steps, &out4[0], &out4[1], &out4[2],
@@ -1774,6 +1942,28 @@ size_t orc_estimate_solid_box_emission_count(float minx, float miny, float maxx,
 size_t orc_emit_solid_box(float minx, float miny, float maxx, float maxy, float R,
                           size_t max_count, float* out_xy) {
     return emit_solid_box(minx, miny, maxx, maxy, R, max_count, out_xy);
+}
+// Counts are returned; out_xy (may be NULL) receives up to capacity points.
+size_t orc_world_walls_positions(float support, float length, int random_obstacles,
+                                 float* out_xy, size_t capacity) {
+    std::vector<float> const xy = world_walls_positions(support, length, random_obstacles != 0);
+    size_t const n = xy.size() / 2;
+    if (out_xy) { std::memcpy(out_xy, xy.data(), 2 * std::min(n, capacity) * sizeof(float)); }
+    return n;
+}
+size_t orc_dam_break_positions(float support, float length, size_t ns, float const* solid_xy,
+                               float* out_xy, size_t capacity) {
+    std::vector<float> const xy = dam_break_positions(support, length, ns, solid_xy);
+    size_t const n = xy.size() / 2;
+    if (out_xy) { std::memcpy(out_xy, xy.data(), 2 * std::min(n, capacity) * sizeof(float)); }
+    return n;
+}
+// The float draws of std::uniform_real_distribution<float>{a, b} over
+// std::mt19937_64{seed} (the random-obstacle offsets' source).
+void orc_mt19937_64_uniform_float(uint64_t seed, float a, float b, size_t n, float* out) {
+    std::mt19937_64 gen{seed};
+    std::uniform_real_distribution<float> dist{a, b};
+    for (size_t i = 0; i < n; ++i) { out[i] = dist(gen); }
 }
 
 // Scene configuration block passed from Python (all floats stored as double so

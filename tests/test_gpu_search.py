@@ -103,6 +103,22 @@ def test_lattice_distance_ties(gpu):
     _compare(xy, 0.02)
 
 
+def test_near_coincident_tiny_lengths(gpu):
+    # Points near the origin at separations far below sqrt(2 FLT_MIN): their
+    # float |v|^2 underflows, so every distance among them is Imath's
+    # lengthTiny (neighborhood.h:263); multiples of 2^-100, of the smallest
+    # subnormal, and offsets around the branch threshold (|v| ~ 1.5e-19).
+    rng = np.random.default_rng(21)
+    i, j = np.meshgrid(np.arange(5), np.arange(4), indexing="ij")
+    grid = np.stack([np.ldexp(1.0 + i.ravel(), -100), np.ldexp(1.0 + j.ravel(), -100)], 1)
+    sub = np.stack([np.arange(1, 9) * 1.4e-45, (np.arange(8) % 3) * 1.4e-45], 1)
+    thr = rng.random((12, 2)) * 3e-19
+    bg = rng.random((3000, 2))
+    xy = np.concatenate([bg, grid, sub, thr]).astype(np.float32)
+    g = _compare(xy, 1.0 / 64.0)
+    assert g["counts"][3000:].min() >= 39  # the 40 points near the origin see each other
+
+
 def test_search_points_sample(gpu):
     xy = scenes.search_points(200_000, seed=3)
     _compare(xy, 1.0 / 2048.0, geometry=False)

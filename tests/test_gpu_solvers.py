@@ -74,6 +74,38 @@ def test_single_sub_step_fields(gpu, method):
         assert np.array_equal(np.where(mask, gi, 0), np.where(mask, oi, 0))
 
 
+def near_coincident_dam_break():
+    """The small dam break plus 12 particles near the origin whose separations
+    are far below sqrt(2 FLT_MIN) (multiples of 2^-100, and of the smallest
+    subnormal): every distance and kernel gradient among them goes through
+    Imath's lengthTiny branch (their float |v|^2 underflows)."""
+    sc = small_dam_break()
+    k = np.arange(6)
+    tiny = np.stack([np.ldexp(1.0 + k, -100), np.ldexp(2.0 + (k % 3), -100)], 1)
+    sub = np.stack([(1 + k) * 1.4e-45, (k % 2) * 1.4e-45 + 2.8e-45], 1)
+    sc.positions = np.concatenate([sc.positions, tiny, sub]).astype(np.float32)
+    sc.velocities = np.zeros_like(sc.positions)
+    return sc
+
+
+@pytest.mark.parametrize("method", ALL)
+def test_near_coincident_particles(gpu, method):
+    scene = near_coincident_dam_break()
+    cfg = scenes.config_for(scene, method)
+    ctx = sb.Context(cfg, sb.Solid_state(scene.solids), len(scene.positions))
+    ctx.set_state(sb.State(scene.positions, scene.velocities))
+    orc = oracle_scene(scene, cfg)
+    ctx.frame_init(method)
+    orc.frame_init(method)
+    print(f"[solver parity] {method} near-coincident n={len(scene.positions)}", flush=True)
+    for k in range(3):
+        st = ctx.sub_step(method, k * scene.dt, scene.dt)
+        ost = orc.sub_step(method, k * scene.dt, scene.dt)
+        assert (st.iterations, st.divergence_iterations) == \
+            (ost.iterations, ost.divergence_iterations), f"step {k}"
+    _check_step(method, ctx, orc, 3)
+
+
 @pytest.mark.parametrize("method", ALL)
 def test_many_sub_steps_bit_exact(gpu, method):
     scene, cfg, ctx, orc = _make(method, nx=30, ny=50, seed=9)

@@ -120,6 +120,91 @@ def test_length_matches_ieee_sqrt():
     assert np.array_equal(got, ref)
 
 
+# --- Imath Vec2<float>::length() below |v|^2 = 2 FLT_MIN --------------------
+# Imath (IlmBase 2.5, ImathVec.h; absent from the image) returns lengthTiny()
+# there: the components divided by the larger magnitude, the root of their
+# squares, times that magnitude. The mathematics pins it: the result is the
+# Euclidean length to within 2 float ulps even where x*x underflows, axis
+# vectors come out exact, and scaling by a power of two scales the result
+# exactly. A plain sqrtf(x*x + y*y) fails every one of these checks.
+_TINY2 = np.float32(2.0) * np.finfo(np.float32).tiny
+
+
+def _orc_len(v):
+    return np.array([L().orc_length(float(a), float(b)) for a, b in v], np.float32)
+
+
+def _plain_len(v):
+    v = v.astype(np.float32)
+    return np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1])
+
+
+def _ulps(a, exact):
+    """|a - exact| in units of the float32 spacing at exact (subnormal spacing
+    2^-149 below FLT_MIN)."""
+    a = a.astype(np.float64)
+    m, e = np.frexp(exact)
+    ulp = np.ldexp(1.0, np.maximum(e - 24, -149))
+    return np.abs(a - exact) / ulp
+
+
+def _tiny_vectors(n, seed):
+    rng = np.random.default_rng(seed)
+    ex = rng.integers(-149, -64, size=(n, 2))
+    mant = rng.random((n, 2)) + 0.5
+    sign = np.where(rng.random((n, 2)) < 0.5, -1.0, 1.0)
+    v = (sign * np.ldexp(mant, ex)).astype(np.float32)
+    v[rng.random(n) < 0.1, 1] = 0.0
+    return v
+
+
+def test_length_tiny_zero_and_axes():
+    assert L().orc_length(0.0, 0.0) == 0.0
+    assert L().orc_length(-0.0, 0.0) == 0.0
+    for d in [1e-45, 3e-45, 1.2e-40, 5.877e-39, 1e-30, 1e-20]:
+        d = float(np.float32(d))
+        for v in [(d, 0.0), (0.0, d), (-d, 0.0), (0.0, -d)]:
+            assert L().orc_length(*v) == np.float32(d), v
+    # the plain formula loses these: d*d underflows to zero or a subnormal
+    assert _plain_len(np.array([[1e-30, 0.0]], np.float32))[0] == 0.0
+
+
+def test_length_tiny_against_exact_length():
+    v = _tiny_vectors(4000, 11)
+    exact = np.hypot(v[:, 0].astype(np.float64), v[:, 1].astype(np.float64))
+    assert np.all(v[:, 0].astype(np.float64) ** 2 + v[:, 1].astype(np.float64) ** 2 < _TINY2)
+    got = _orc_len(v)
+    assert np.max(_ulps(got, exact)) <= 2.0
+    plain = _plain_len(v)
+    assert np.mean(_ulps(plain, exact) > 2.0) > 0.5
+
+
+def test_length_tiny_power_of_two_scaling():
+    # vectors in the tiny branch whose lengths are normal floats
+    rng = np.random.default_rng(12)
+    v = ((rng.random((500, 2)) - 0.5) * 2.0 ** -66).astype(np.float32)
+    v = v[np.abs(v).max(axis=1) > 2.0 ** -70]
+    base = _orc_len(v)
+    for k in (1, 3, 7):
+        s = np.float32(2.0 ** -k)
+        assert np.array_equal(_orc_len(v * s), base * s)
+    assert not np.array_equal(_plain_len(v * np.float32(2.0 ** -7)),
+                              _plain_len(v) * np.float32(2.0 ** -7))
+
+
+def test_length_branch_threshold():
+    # At |v|^2 >= 2 FLT_MIN the result is the IEEE root of the float |v|^2.
+    rng = np.random.default_rng(13)
+    v = ((rng.random((2000, 2)) + 0.25) * 2.0 ** -63).astype(np.float32)
+    l2 = v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]
+    big = l2 >= _TINY2
+    assert big.any() and (~big).any()
+    got = _orc_len(v)
+    assert np.array_equal(got[big], np.sqrt(l2[big]))
+    exact = np.hypot(v[:, 0].astype(np.float64), v[:, 1].astype(np.float64))
+    assert np.max(_ulps(got, exact)) <= 2.0
+
+
 # --- sort / blocks: neighborhood.cpp:10-124 closed forms --------------------
 def test_sort_and_blocks_closed_form():
     xy = oracle.fixture_points(17272, 7927, -100.0, 100.0)
