@@ -241,6 +241,7 @@ SIGNATURES = {
     "emsph_search_set_points": (_ST, [_P, _P, _SZ]),
     "emsph_search_run": (_ST, [_P, _F, _I, _P]),
     "emsph_search_get": (_ST, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "emsph_search_lists_noalloc": (_ST, [_P, _SZ, _F, _P, _P, _P]),
 }
 
 _lib = None

@@ -29,6 +29,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3151,6 +3152,233 @@ emsph_status emsph_pool_save_snapshot(emsph_pool* P, int id, const char* path,
     int const k = pool_index(P, id);
     if (k < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no such live scene"); }
     return save_state(&P->e, static_cast<uint32_t>(k), path, time_flicks, P->method, false);
+}
+
+}  // extern "C"
+
+//------------------------------------------------------------------------------
+// Neighbour lists without allocation: the stand-alone search
+// (emsph_search_run + emsph_search_get's lists) for callers that cannot
+// allocate device memory, on caller-owned device buffers. Every intermediate
+// buffer of the search (keys, sort passes, cell tables, cell windows, sorted
+// positions, slot-major lists, scene and tile tables, key state) lives in one
+// file-scope device array, carved at fixed offsets for up to
+// NOALLOC_MAX_POINTS points. Each GPU holds its own copy of the array until the
+// process ends; its address on a device is taken once, with that device
+// current, by hipGetSymbolAddress, and only that address is passed to kernels
+// (the array's name is not a device address in host code). Calls on one GPU
+// share the array: calls that overlap on a device race.
+namespace {
+
+constexpr uint32_t NOALLOC_MAX_POINTS = 65536;
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t{32} << 20;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Scratch address of the current device (taken on the first call there).
+emsph_status noalloc_scratch(int device, unsigned char** out) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "device index above the supported range");
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            return fail(EMSPH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(e));
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[device]);
+    return EMSPH_OK;
+}
+
+// Fixed carving of the scratch for n points: each buffer 256-byte aligned and
+// followed by DBuf's tail padding (the build reads up to 3 positions past a
+// cell's range).
+struct NoallocLayout {
+    size_t at = 0;
+    unsigned char* base = nullptr;
+    template <class T>
+    T* take(size_t count) {
+        at = (at + 255) & ~size_t{255};
+        T* p = reinterpret_cast<T*>(base ? base + at : nullptr);
+        at += (count + DBuf<T>::TAIL_PAD) * sizeof(T);
+        return p;
+    }
+};
+
+struct NoallocBuffers {
+    SceneDev* scenes;
+    Tile* tiles;
+    uint32_t* scene_of;
+    KeyState* keystate;
+    unsigned long long* key_partials;
+    uint64_t *keys0, *keys1, *cell_key;
+    uint32_t *vals0, *vals1, *counts, *scan, *heads, *head_scan, *block_index, *cell_begin,
+      *cell_end, *cell_tag, *ncells, *orig, *rank, *fl_idx;
+    uint2* win;
+    uint2* overflow;
+    V2* spos;
+    uint8_t* fcount;
+};
+
+size_t noalloc_carve(unsigned char* base, uint32_t n, uint32_t stride, NoallocBuffers* b) {
+    NoallocLayout L;
+    L.base = base;
+    uint32_t const ntiles = (n + TILE - 1) / TILE;
+    b->scenes = L.take<SceneDev>(1);
+    b->tiles = L.take<Tile>(ntiles ? ntiles : 1);
+    b->scene_of = L.take<uint32_t>(n);
+    b->keystate = L.take<KeyState>(1);
+    b->key_partials = L.take<unsigned long long>(2 * KEY_PARTS);
+    b->keys0 = L.take<uint64_t>(n);
+    b->keys1 = L.take<uint64_t>(n);
+    b->vals0 = L.take<uint32_t>(n);
+    b->vals1 = L.take<uint32_t>(n);
+    b->counts = L.take<uint32_t>(radix_counts_words(n) + 256);
+    b->scan = L.take<uint32_t>(scan_scratch_words(radix_counts_words(n) + n) + 64);
+    b->heads = L.take<uint32_t>(n);
+    b->head_scan = L.take<uint32_t>(n);
+    b->block_index = L.take<uint32_t>(n);
+    b->cell_begin = L.take<uint32_t>(n);
+    b->cell_end = L.take<uint32_t>(n);
+    b->cell_tag = L.take<uint32_t>(n);
+    b->cell_key = L.take<uint64_t>(n);
+    b->ncells = L.take<uint32_t>(1);
+    b->orig = L.take<uint32_t>(n);
+    b->rank = L.take<uint32_t>(n);
+    b->win = L.take<uint2>(static_cast<size_t>(n) * 9);
+    b->overflow = L.take<uint2>(n);
+    b->spos = L.take<V2>(n);
+    b->fcount = L.take<uint8_t>(stride);
+    b->fl_idx = L.take<uint32_t>(static_cast<size_t>(MAXN) * stride);
+    return L.at;
+}
+
+// The scene entry and the tile table of one scene of n points, written on the
+// device from kernel arguments (no host copy).
+__global__ __launch_bounds__(256) void k_noalloc_setup(SceneDev sd, SceneDev* scenes, Tile* tiles,
+                                                     uint32_t n, uint32_t ntiles) {
+    uint32_t const k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) { scenes[0] = sd; }
+    if (k < ntiles) {
+        uint32_t const first = k * TILE;
+        tiles[k] = Tile{0u, first, std::min<uint32_t>(TILE, n - first), 0u};
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+emsph_status emsph_search_lists_noalloc(const float* xy_dev, size_t count, float cell_size,
+                                        uint8_t* counts_dev, uint32_t* indices_dev,
+                                        void* stream) {
+    if (count > NOALLOC_MAX_POINTS) {
+        return fail(EMSPH_ERR_CAPACITY, "emsph_search_lists_noalloc: " + std::to_string(count) +
+                                          " points above the maximum of " +
+                                          std::to_string(NOALLOC_MAX_POINTS));
+    }
+    if (!(cell_size > 0.0f)) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "cell size <= 0"); }
+    if (count == 0) { return EMSPH_OK; }
+    if (!xy_dev || !counts_dev || !indices_dev) {
+        return fail(EMSPH_ERR_INVALID_ARGUMENT, "null buffer");
+    }
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    unsigned char* base = nullptr;
+    ST_TRY(noalloc_scratch(device, &base));
+    auto const n = static_cast<uint32_t>(count);
+    uint32_t const stride = round_up(n, 256);
+    uint32_t const ntiles = (n + TILE - 1) / TILE;
+    NoallocBuffers B{};
+    if (noalloc_carve(nullptr, NOALLOC_MAX_POINTS, round_up(NOALLOC_MAX_POINTS, 256), &B) >
+        NOALLOC_SCRATCH_BYTES) {
+        return fail(EMSPH_ERR_CAPACITY, "scratch layout exceeds the scratch array");
+    }
+    noalloc_carve(base, n, stride, &B);
+    auto const s = static_cast<hipStream_t>(stream);
+
+    // The scene: one point set, cut-off and cell = cell_size (2h), as
+    // emsph_search_run configures it.
+    SceneHost h;
+    emsph_default_parameters(&h.cfg.params);
+    h.cfg.mass_per_particle = 1.0f;
+    h.cfg.params.support = cell_size * 0.5f;
+    h.cap = n;
+    h.n = n;
+    SceneDev const sd = scene_dev(h, 0);
+    hipLaunchKernelGGL(k_noalloc_setup, dim3((ntiles + 255) / 256), dim3(256), 0, s, sd, B.scenes,
+                       B.tiles, n, ntiles);
+    launch_scene_of(B.tiles, ntiles, B.scene_of, s);
+
+    // Keys, all eight 8-bit digits of the sort (no key range read back),
+    // cells and cell windows (neighborhoods_from_state.cpp:14-40).
+    IndexArgs a{};
+    a.pos = reinterpret_cast<V2 const*>(xy_dev);
+    a.n = n;
+    a.scenes = B.scenes;
+    a.scene_of = B.scene_of;
+    a.keys = B.keys0;
+    a.vals = B.vals0;
+    a.keystate = B.keystate;
+    a.key_partials = B.key_partials;
+    a.pos_out = B.spos;
+    launch_keys(a, s);
+    int const which = launch_radix_sort(B.keys0, B.vals0, B.keys1, B.vals1, n, 8, B.scene_of, 0,
+                                        B.counts, B.scan, s);
+    a.key_passes = 8;
+    a.check_keys = 0;
+    a.sorted_keys = which ? B.keys1 : B.keys0;
+    a.sorted_vals = which ? B.vals1 : B.vals0;
+    a.heads = B.heads;
+    a.head_scan = B.head_scan;
+    a.block_index = B.block_index;
+    a.cell_begin = B.cell_begin;
+    a.cell_end = B.cell_end;
+    a.cell_key = B.cell_key;
+    a.cell_tag = B.cell_tag;
+    a.orig = B.orig;
+    a.rank = B.rank;
+    launch_cell_heads(a, s);
+    launch_exclusive_scan(B.heads, B.head_scan, n, B.scan, B.ncells, s);
+    launch_cells_finalize(a, s);
+    WindowArgs w{};
+    w.self = CellList{B.cell_key, B.cell_tag, B.cell_begin, B.cell_end, B.ncells};
+    w.max_cells = n;
+    w.win = B.win;
+    w.report = &B.keystate->cells;
+    launch_cell_windows(w, s);
+
+    // 64-capped lists in reference order (neighborhood.h:198-282), then in
+    // original order into the caller's buffers.
+    Batch b{};
+    b.scenes = B.scenes;
+    b.tiles = B.tiles;
+    b.ntiles = ntiles;
+    b.nscenes = 1;
+    b.stride = stride;
+    b.keystate = B.keystate;
+    b.pos = B.spos;
+    b.orig = B.orig;
+    b.rank = B.rank;
+    b.cell_of = B.block_index;
+    b.win = B.win;
+    b.overflow = B.overflow;
+    b.fl = Lists{B.fcount, B.fl_idx, nullptr, nullptr, nullptr};
+    b.nb = 4;
+    b.xcd_remap = 1;
+    launch_build(BUILD_LISTS, MAXN, b, s);
+    launch_export_lists(b.fl, n, stride, B.orig, B.orig, B.scene_of, B.scenes, 0, counts_dev,
+                        indices_dev, nullptr, nullptr, s);
+    HIP_TRY(hipGetLastError());
+    return EMSPH_OK;
 }
 
 }  // extern "C"

@@ -461,6 +461,21 @@ emsph_status emsph_search_get(emsph_search* s, int32_t* grid_xy, uint64_t* keys,
                               uint32_t* block_end, size_t* block_count, uint8_t* counts,
                               uint32_t* indices, float* distances, float* vectors_xy);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The neighbour lists of emsph_search_run +
+ * emsph_search_get (counts and indices, original order, [count][64], entries
+ * past a count zero) for count <= 65536 points already on the device, written
+ * into caller-owned device buffers; every intermediate buffer lives in a
+ * 32 MiB device array of the library that each GPU holds until the process
+ * ends. No hipMalloc, hipFree or other allocation is made. Work is enqueued on
+ * `stream` (a hipStream_t; NULL: the null stream) and is not waited for. Calls
+ * that overlap on one GPU share that array and race: serialise them. count
+ * above 65536 fails with EMSPH_ERR_CAPACITY. */
+emsph_status emsph_search_lists_noalloc(const float* xy_dev, size_t count, float cell_size,
+                                        uint8_t* counts_dev, uint32_t* indices_dev,
+                                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif
