@@ -6,30 +6,43 @@ read-back, pressure application and integration). Inputs are resident on the
 device before the timed region; the timed region is exactly K sub-steps
 bracketed by device synchronisation.
 
+Two windows of the same run are timed:
+  * collapse (the headline): K sub-steps from the state after ADVANCE sub-steps
+    of the dam break (8,000 sub-steps = 0.125 s of simulated time by default:
+    the column is collapsing and the front runs along the floor);
+  * rest: K sub-steps right after the warm-up, the lattice still near rest
+    (the easiest phase: the solver's minimum of 3 iterations).
+
 Prints one JSON line with the metric, a roofline entry for the dominant kernel
 (algorithmic bytes per launch over its average launch time, measured with HIP
-events on the context's stream) and the CPU baseline (the oracle, a C++
-restatement of the reference's CPU path, timed on this host's cores on a
-bounded sample of the same workload).
+events on the context's stream) and the CPU baseline: the oracle, a C++
+restatement of the reference's CPU path, timed on this host's cores from the
+same collapse state (one warm-up sub-step, the median of 3 runs, and one
+sub-step on one thread), its state after those sub-steps compared bit for bit
+with the GPU's.
 
 The same line carries the other workloads as objects beside the headline:
-"batched" (256 ragged scenes, one launch per stage, against per-scene
+"solvers" (PCISPH and DFSPH_P at 1M, GPU and CPU from their own collapse
+states), "batched" (256 ragged scenes, one launch per stage, against per-scene
 contexts), "served" (a Poisson request stream over HBM-budgeted pools, p50/p99
-step latency), "search" (4,194,304 points, keys, sort, cells and lists alone)
-and "small" (the 12,800-particle dam break, 1000 sub-steps, GPU against the
-oracle on the host cores, final states compared bit for bit).
+step latency, FP64 error of sampled results), "search" (4,194,304 points, keys,
+sort, cells and lists alone, GPU and CPU) and "small" (the 12,800-particle dam
+break, 1000 sub-steps, GPU against the oracle on the host cores, final states
+compared bit for bit).
 
-  python bench.py [--steps K] [--warmup W]
+  python bench.py [--steps K] [--warmup W] [--advance A]
                   [--method iisph|iisph_ap|iisph_pseudo_ap|dfsph_p|pcisph]
-                  [--no-cpu-baseline] [--no-batched] [--no-served] [--no-search]
-                  [--no-small]
+                  [--no-cpu-baseline] [--no-solvers] [--no-batched] [--no-served]
+                  [--no-search] [--no-small]
 """
 
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import statistics
 import sys
 import time
 
@@ -96,48 +109,260 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, cfg, method: str, gpu_state_after, budget_s: float = 12.0) -> dict:
-    """Time the oracle (C++ restatement of the reference CPU path, OpenMP static
-    chunks of 512 over host cores) on the first sub-steps of the same scene."""
+def host_cpu() -> dict:
+    """The host's CPU: model, the CPUs this process may run on (affinity, the
+    cgroup quota) and the threads the CPU legs use: all of them, or the
+    OMP_NUM_THREADS share when the environment sets one."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = affinity
+    if quota:
+        threads = min(threads, max(1, int(math.floor(quota))))
+    if env and env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    return {"model": model, "affinity_cpus": affinity, "cgroup_cpus": quota,
+            "omp_num_threads_env": env, "threads": threads}
+
+
+def _bits_equal(a_pos, a_vel, b_pos, b_vel) -> bool:
+    return bool(np.array_equal(a_pos.view(np.uint32), b_pos.view(np.uint32)) and
+                np.array_equal(a_vel.view(np.uint32), b_vel.view(np.uint32)))
+
+
+def gpu_states_after(ctx, method, dt, state, counts):
+    """The GPU state after each of counts sub-steps from state (a fresh frame)."""
+    from emerald_amd import sph2d_box as sb
+    ctx.set_state(sb.State(state.positions, state.velocities))
+    ctx.frame_init(method)
+    out = {}
+    done = 0
+    for k in sorted(counts):
+        ctx.run_sub_steps(method, dt, k - done)
+        done = k
+        st = ctx.get_state()
+        out[k] = (st.positions, st.velocities)
+    return out
+
+
+def cpu_leg(scene, cfg, method, state, gpu_after, cpu, per_run: int = 7, runs: int = 3) -> dict:
+    """The oracle from the given state: one warm-up sub-step, then runs x
+    per_run sub-steps on cpu["threads"] threads (median of the runs), then one
+    sub-step on one thread. The oracle's state after 1 + runs x per_run and
+    after one more sub-step is compared bit for bit with the GPU's."""
     import oracle
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from helpers import oracle_scene
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    from helpers import oracle_cfg
+    threads = cpu["threads"]
     oracle.set_threads(threads)
-    o = oracle_scene(scene, cfg)
+    o = oracle.Scene(oracle_cfg(cfg), scene.solids, state.positions, state.velocities)
     o.frame_init(method)
-    steps = 0
+    n = len(state.positions)
+    dt = scene.dt
     t0 = time.perf_counter()
-    while True:
-        o.sub_step(method, steps * scene.dt, scene.dt)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 20:
-            break
-    n = len(scene.positions)
+    o.sub_step(method, 0.0, dt)
+    warm = time.perf_counter() - t0
+    times = []
+    done = 1
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for _ in range(per_run):
+            o.sub_step(method, done * dt, dt)
+            done += 1
+        times.append(time.perf_counter() - t0)
+    pos, vel = o.state_f32()
     parity = None
-    if gpu_state_after is not None and steps in gpu_state_after:
+    if done in gpu_after:
+        parity = _bits_equal(pos, vel, *gpu_after[done])
+    oracle.set_threads(1)
+    t0 = time.perf_counter()
+    o.sub_step(method, done * dt, dt)
+    one = time.perf_counter() - t0
+    done += 1
+    oracle.set_threads(threads)
+    if parity is not None and done in gpu_after:
         pos, vel = o.state_f32()
-        gp, gv = gpu_state_after[steps]
-        parity = bool(np.array_equal(pos.view(np.uint32), gp.view(np.uint32)) and
-                      np.array_equal(vel.view(np.uint32), gv.view(np.uint32)))
+        parity = parity and _bits_equal(pos, vel, *gpu_after[done])
+    med = statistics.median(times)
+    del o
     return {
-        "value": n * steps / el,
+        "value": n * per_run / med,
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{method} {n}-particle dam break, first {steps} sub-steps from the "
-                  f"initial state ({el:.1f} s); GPU state after the same sub-steps "
-                  f"bit-identical: {parity}",
+        "one_thread_value": n / one,
+        "runs_s": [round(t, 4) for t in times],
+        "warmup_s": round(warm, 4),
+        "sample": f"{method} {n}-particle dam break from the GPU's state at the start of the "
+                  f"timed window: 1 warm-up sub-step, median of {runs} runs of {per_run} "
+                  f"sub-steps on {threads} threads, then 1 sub-step on 1 thread; oracle "
+                  f"state after {done - 1} and {done} sub-steps bit-identical to the GPU's: "
+                  f"{parity}",
+        "bit_identical_to_gpu": parity,
+        "host": cpu,
     }
+
+
+def solver_window(method: str, steps: int, warmup: int, advance: int, cpu: dict | None,
+                  autotune: bool, kernel_timing: bool, rest_window: bool) -> dict:
+    """One solver at 1M: warm-up, optionally the rest window, the advance to the
+    collapse state, the timed collapse window, a kernel-timing pass, and the
+    CPU leg from the same collapse state."""
+    from emerald_amd import scenes
+    from emerald_amd import sph2d_box as sb
+    log(f"{method}: building the 1024 x 1024 lattice dam break")
+    scene = scenes.dam_break_1m()
+    cfg = scenes.config_for(scene, method)
+    n = len(scene.positions)
+    ctx = sb.Context(cfg, sb.Solid_state(scene.solids), n)
+    ctx.set_state(sb.State(scene.positions, scene.velocities))
+    ctx.frame_init(method)
+    out = {"method": method, "particles": n, "solids": int(len(scene.solids)), "dt": scene.dt}
+    log(f"{method}: warm-up {warmup} sub-steps")
+    ctx.run_sub_steps(method, scene.dt, warmup)
+    done = warmup
+    if autotune:
+        log(f"{method}: autotune")
+        chosen, cand_ms = ctx.autotune(method, scene.dt, 5)
+        out["tuning"] = {"chosen": chosen, "candidate_ms_per_5_sub_steps": cand_ms,
+                         "candidates": "neighbour batch 2/4/8 x XCD remap on/off with graphs, "
+                                       "then the best without graphs"}
+    if rest_window:
+        ctx.iteration_totals(reset=True)
+        t0 = time.perf_counter()
+        dev_ms, _ = ctx.run_sub_steps(method, scene.dt, steps)
+        wall = time.perf_counter() - t0
+        _, it, dit, _ = ctx.iteration_totals(reset=True)
+        out["rest"] = {"sub_steps": [done, done + steps], "value": n * steps / wall,
+                       "ms_per_step": wall * 1e3 / steps, "device_ms_per_step": dev_ms / steps,
+                       "mean_iterations": it / steps, "mean_divergence_iterations": dit / steps}
+        done += steps
+    log(f"{method}: advancing to sub-step {advance}")
+    if advance > done:
+        ctx.run_sub_steps(method, scene.dt, advance - done)
+        done = advance
+    start = ctx.get_state()
+    ctx.frame_init(method)
+    log(f"{method}: timing {steps} sub-steps from sub-step {done}")
+    ctx.iteration_totals(reset=True)
+    t0 = time.perf_counter()
+    dev_ms, st = ctx.run_sub_steps(method, scene.dt, steps)
+    wall = time.perf_counter() - t0
+    _, it, dit, redone = ctx.iteration_totals(reset=True)
+    k_mean = it / steps
+    kdiv_mean = dit / steps
+    value = n * steps / wall
+    v2 = np.max(np.sum(start.velocities.astype(np.float64) ** 2, axis=1))
+    out["collapse"] = {
+        "sub_steps": [done, done + steps], "time_s": done * scene.dt,
+        "max_speed_at_start": float(math.sqrt(v2)),
+        "value": value, "ms_per_step": wall * 1e3 / steps, "device_ms_per_step": dev_ms / steps,
+        "mean_iterations": k_mean, "mean_divergence_iterations": kdiv_mean,
+        "sub_steps_redone": int(redone),
+        "last_iterations": int(st.iterations), "sort_passes": int(st.sort_passes),
+        "cells": int(st.cell_count), "max_neighbors": int(st.max_neighbors),
+        "algorithmic_bytes_per_particle_step": step_bytes(method, k_mean, kdiv_mean),
+    }
+    out["collapse"]["achieved_GBps"] = out["collapse"]["algorithmic_bytes_per_particle_step"] * \
+        value / 1e9
+    if kernel_timing:
+        log(f"{method}: kernel timing pass, {steps} sub-steps")
+        ctx.set_kernel_timing(True)
+        ctx.run_sub_steps(method, scene.dt, steps)
+        out["_timing"] = ctx.kernel_timing()
+        out["_timing_steps"] = steps
+        ctx.set_kernel_timing(False)
+    if cpu is not None:
+        per_run, runs = 7, 3
+        total = 1 + per_run * runs
+        log(f"{method}: GPU states for the CPU parity check")
+        gpu_after = gpu_states_after(ctx, method, scene.dt, start, [total, total + 1])
+        ctx.close()
+        log(f"{method}: CPU leg (oracle on {cpu['threads']} threads, then 1 thread)")
+        out["cpu"] = cpu_leg(scene, cfg, method, start, gpu_after, cpu, per_run, runs)
+        out["speedup_vs_cpu"] = value / out["cpu"]["value"]
+    else:
+        ctx.close()
+    return out
+
+
+# The reference's DFSPH_P at this fixed dt ejects a particle at ~1e5 m/s near
+# sub-step 2,700 of this dam break (t = 0.042 s) and the run diverges (its
+# density and divergence solves then hit their 15-iteration caps); the GPU
+# follows it bit for bit. Its window is taken before that.
+DFSPH_ADVANCE_LIMIT = 2000
+
+
+def advance_for(method: str, advance: int) -> int:
+    return min(advance, DFSPH_ADVANCE_LIMIT) if method == "dfsph_p" else advance
+
+
+def roofline_of(method: str, timing: dict, steps: int, n: int) -> tuple:
+    """The dominant kernel class by device time: its algorithmic GB/s over its
+    average launch, the rocprof HBM bytes per launch and VALU issue share
+    recorded in profiles/, and the per-kernel table."""
+    table = ALGO_BYTES[method]
+    dom_name, (dom_ms, dom_calls) = max(timing.items(), key=lambda kv: kv[1][0])
+    avg_s = dom_ms / dom_calls / 1e3
+    per = table.get(dom_name)
+    achieved = (per * n / avg_s / 1e9) if per else None
+    traffic = valu = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(method, {}).get(dom_name)
+    vpath = os.path.join(ROOT, "profiles", "valu.json")
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            valu = json.load(f).get(method, {}).get(dom_name)
+    # A kernel whose measured VALU issue takes most of the SIMD cycles is
+    # bound by instruction issue; the HBM fraction is reported beside it.
+    bound = "valu" if (valu is not None and valu >= 0.6) else "hbm"
+    roofline = {
+        "bound": bound,
+        "kernel": dom_name,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+        "traffic": traffic,
+        "avg_launch_us": avg_s * 1e6,
+        "algorithmic_bytes_per_particle": per,
+        "particles_per_launch": n,
+        "valu_issue_frac": valu,
+    }
+    kernels = {}
+    for k, v in sorted(timing.items(), key=lambda kv: -kv[1][0]):
+        e = {"ms_per_step": v[0] / steps, "calls_per_step": v[1] / steps,
+             "avg_launch_us": v[0] / v[1] * 1e3}
+        if table.get(k):
+            e["algorithmic_GBps"] = table[k] * n / (v[0] / v[1] / 1e3) / 1e9
+        kernels[k] = e
+    return roofline, kernels
 
 
 def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict:
     """256 dam breaks of uneven size (log-uniform 2k-200k particles, aspect
     0.25-2, per-scene CFL 0.4 dt) stepped as one batch (one launch per sub-step
-    stage for all scenes) and as 256 plain per-scene contexts, over the same
-    sub-steps from the same states; every scene's state must come out
-    bit-identical in both forms."""
+    stage for all scenes) and as 256 plain per-scene contexts (one alive at a
+    time, each timed over the same sub-steps from the same state); every
+    scene's state must come out bit-identical in both forms."""
     from emerald_amd import scenes
     from emerald_amd import sph2d_box as sb
     log(f"batched form: generating {count} scenes")
@@ -159,34 +384,32 @@ def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict
     t_batch = time.perf_counter() - t0
     last = batch.sub_step(method)  # one more, for the per-scene statistics
     iters = [st.iterations for st in last]
-
-    log(f"batched form: {count} plain per-scene contexts")
-    ctxs = []
-    for sc, cfg, sol, st in zip(scs, cfgs, solids, states):
-        c = sb.Context(cfg, sol, len(sc.positions))
-        c.set_cfl(0.4)
-        c.set_state(st)
-        c.frame_init(method)
-        c.run_sub_steps(method, cfg.seconds_per_sub_step, warmup)
-        ctxs.append(c)
-    t0 = time.perf_counter()
-    for c, cfg in zip(ctxs, cfgs):
-        c.run_sub_steps(method, cfg.seconds_per_sub_step, steps)
-    t_plain = time.perf_counter() - t0
-    same = True
-    for i, (c, cfg) in enumerate(zip(ctxs, cfgs)):
-        c.run_sub_steps(method, cfg.seconds_per_sub_step, 1)
-        a, b = c.get_state(), batch.get_state(i)
-        same = same and np.array_equal(a.positions.view(np.uint32), b.positions.view(np.uint32)) \
-            and np.array_equal(a.velocities.view(np.uint32), b.velocities.view(np.uint32))
-        c.close()
+    finals = [batch.get_state(i) for i in range(count)]
     # Per-stage GPU time of the batched sub-step, from the engine's HIP events
-    # (timed launches run outside graphs), after the comparison above.
+    # (timed launches run outside graphs).
     batch.set_kernel_timing(True)
     batch.run_sub_steps(method, 5)
     stages = batch.kernel_timing()
     batch.set_kernel_timing(False)
     batch.close()
+
+    log(f"batched form: {count} plain per-scene contexts, one at a time")
+    t_plain = 0.0
+    same = True
+    for i, (sc, cfg, sol, st) in enumerate(zip(scs, cfgs, solids, states)):
+        c = sb.Context(cfg, sol, len(sc.positions))
+        c.set_cfl(0.4)
+        c.set_state(st)
+        c.frame_init(method)
+        c.run_sub_steps(method, cfg.seconds_per_sub_step, warmup)
+        t0 = time.perf_counter()
+        c.run_sub_steps(method, cfg.seconds_per_sub_step, steps)
+        t_plain += time.perf_counter() - t0
+        c.run_sub_steps(method, cfg.seconds_per_sub_step, 1)
+        a = c.get_state()
+        same = same and _bits_equal(a.positions, a.velocities, finals[i].positions,
+                                    finals[i].velocities)
+        c.close()
     stage_ms = {k: round(v[0] / 5, 4) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])}
     v_batch = n_total * steps / t_batch
     v_plain = n_total * steps / t_plain
@@ -202,12 +425,15 @@ def batched_form(method: str, steps: int, warmup: int, count: int = 256) -> dict
     }
 
 
-def search_form(runs: int = 5, n: int = 4_194_304) -> dict:
+def search_form(cpu: dict | None, runs: int = 5, n: int = 4_194_304) -> dict:
     """Z-index keys, sort, cells and 64-capped neighbour lists alone (no
     dynamics) on 4,194,304 points, half uniform and half from 8 Gaussian
     clusters (sigma 0.02), cell 1/2048 (h = 1/4096). Algorithmic bytes per point
     40 + 1 + 4 n (key build 40, count 1, u32 indices; the lists are the
-    product)."""
+    product). The CPU leg is the oracle's search over the region the reference
+    times (test_neighborhood.cpp:178-231): one warm-up, the median of 3, and one
+    run on one thread."""
+    import oracle
     from emerald_amd import scenes
     from emerald_amd.search import NeighborSearch
     log(f"search form: {n} points")
@@ -222,7 +448,7 @@ def search_form(runs: int = 5, n: int = 4_194_304) -> dict:
     nbar = float(counts.mean())
     best = min(ms)
     per_point = 40.0 + 1.0 + 4.0 * nbar
-    return {
+    out = {
         "workload": "4,194,304 points (half U[0,1)^2, half 8 Gaussian clusters sigma 0.02), "
                     "cell 1/2048, 64-capped lists",
         "points": n,
@@ -235,9 +461,24 @@ def search_form(runs: int = 5, n: int = 4_194_304) -> dict:
         "achieved_GBps": per_point * n / (best / 1e3) / 1e9,
         "cells": int(len(r["block_begin"])) if "block_begin" in r else None,
     }
+    if cpu is not None:
+        log(f"search form: CPU leg on {cpu['threads']} threads, then 1 thread")
+        oracle.set_threads(cpu["threads"])
+        _, pairs = oracle.search_seconds(xy, 1.0 / 2048.0)
+        times = [oracle.search_seconds(xy, 1.0 / 2048.0)[0] for _ in range(3)]
+        oracle.set_threads(1)
+        one, _ = oracle.search_seconds(xy, 1.0 / 2048.0)
+        oracle.set_threads(cpu["threads"])
+        med = statistics.median(times)
+        out["cpu"] = {"ms_per_search": med * 1e3, "points_per_s": n / med, "cores": cpu["threads"],
+                      "one_thread_ms": one * 1e3, "runs_s": [round(t, 4) for t in times],
+                      "neighbour_pairs_equal_gpu": bool(pairs == int(counts.sum())),
+                      "kind": "port"}
+        out["speedup_vs_cpu"] = (med * 1e3) / float(np.median(ms))
+    return out
 
 
-def small_form(steps: int = 1000, cpu_budget: float = 20.0) -> dict:
+def small_form(cpu: dict, steps: int = 1000, cpu_budget: float = 20.0) -> dict:
     """The reference's own CPU-runnable case: the 12,800-particle IISPH dam
     break (80 x 160 lattice, spacing 0.005, dt 2e-4), 1000 sub-steps on the GPU
     and the same sub-steps in the oracle on the host cores, with the final
@@ -261,7 +502,7 @@ def small_form(steps: int = 1000, cpu_budget: float = 20.0) -> dict:
     _, iters, _, _ = ctx.iteration_totals(reset=True)
     gst = ctx.get_state()
     ctx.close()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = cpu["threads"]
     oracle.set_threads(threads)
     o = oracle_scene(scene, cfg)
     o.frame_init("iisph")
@@ -276,8 +517,7 @@ def small_form(steps: int = 1000, cpu_budget: float = 20.0) -> dict:
     same = None
     if done == steps:
         pos, vel = o.state_f32()
-        same = bool(np.array_equal(pos.view(np.uint32), gst.positions.view(np.uint32)) and
-                    np.array_equal(vel.view(np.uint32), gst.velocities.view(np.uint32)))
+        same = _bits_equal(pos, vel, gst.positions, gst.velocities)
     return {
         "workload": "iisph dam break, 80x160 lattice (12,800 particles), spacing 0.005, "
                     "dt 2e-4, hex walls",
@@ -295,19 +535,59 @@ def small_form(steps: int = 1000, cpu_budget: float = 20.0) -> dict:
     }
 
 
-def served_form(budget_gb: float, window: float, rate: float = 200.0,
+def fp64_errors(req, state, cpu: dict) -> dict:
+    """A completed served request (FP32 on the GPU) against the oracle's FP64
+    build of the same scene over the same number of sub-steps, each with its
+    own CFL 0.4 dt (cfl.cpp:12-21 on its own velocities): error per state
+    field, and the kinetic energies."""
+    import oracle
+    from emerald_amd import serve
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import oracle_scene
+    sc, cfg = serve.request_scene(req)
+    oracle.set_threads(cpu["threads"])
+    o = oracle_scene(sc, cfg, fp64=True)
+    o.frame_init(req.method)
+    h = float(np.float32(sc.support))
+    dt_max = float(np.float32(cfg.seconds_per_sub_step))
+    vmax2 = 0.0
+    t = 0.0
+    for _ in range(req.steps):
+        dt = dt_max if vmax2 <= 0 else min(dt_max, float(np.float32(0.4)) * h / math.sqrt(vmax2))
+        o.sub_step(req.method, t, dt)
+        t += dt
+        _, vel = o.state_f64()
+        vmax2 = float(np.max(np.sum(vel * vel, axis=1)))
+    pos64, vel64 = o.state_f64()
+    m = float(np.float32(cfg.mass_per_particle))
+    p32 = state.positions.astype(np.float64)
+    v32 = state.velocities.astype(np.float64)
+    dp = np.hypot(*(p32 - pos64).T)
+    dv = np.hypot(*(v32 - vel64).T)
+    ke32 = 0.5 * m * np.sum(v32 ** 2)
+    ke64 = 0.5 * m * np.sum(vel64 ** 2)
+    return {"request": req.rid, "method": req.method, "particles": len(sc.positions),
+            "steps": req.steps,
+            "positions_max": float(dp.max()), "positions_rms": float(np.sqrt(np.mean(dp ** 2))),
+            "velocities_max": float(dv.max()), "velocities_rms": float(np.sqrt(np.mean(dv ** 2))),
+            "kinetic_energy_rel": float(abs(ke32 - ke64) / ke64) if ke64 > 0 else None}
+
+
+def served_form(budget_gb: float, window: float, cpu: dict | None, rate: float = 200.0,
                 plain_steps: int = 2) -> dict:
     """Poisson stream of dam-break requests (200/s, 5k-500k particles, IISPH or
     DFSPH_P, 100-2000 steps) served from two device-resident pools (one per
     solver) with continuous batching under an HBM budget. After the window,
     the scenes still live are also stepped as plain per-scene contexts (one at
-    a time) for the throughput comparison."""
+    a time) for the throughput comparison, and the two smallest completed
+    requests are compared with the oracle's FP64 build."""
     from emerald_amd import serve
     from emerald_amd import sph2d_box as sb
     reqs = serve.poisson_requests(rate, window + 5.0)
-    res = serve.serve(reqs, budget_gb * 1e9, window, log=log)
+    # Final states are kept for the requests small enough for the FP64 check.
+    small = lambda r: r.n * r.steps <= 2e7  # noqa: E731
+    res = serve.serve(reqs, budget_gb * 1e9, window, log=log, keep_states=small)
     loops = res["_loops"]
-    # Plain per-scene contexts on the pools' final live sets.
     log("served form: plain per-scene comparison on the final live sets")
     n_steps = 0
     t_plain = 0.0
@@ -329,11 +609,17 @@ def served_form(budget_gb: float, window: float, rate: float = 200.0,
             n_steps += len(sc.positions) * plain_steps
             scenes_done += 1
             c.close()
+    finals = {rid: st for lp in loops.values() for rid, st in lp.finals.items()}
     serve.close(res)
     res["plain_value_final_live_set"] = n_steps / t_plain if t_plain > 0 else None
     res["plain_scenes"] = scenes_done
     res["speedup_vs_plain"] = (res["value"] / res["plain_value_final_live_set"]
                                if res["plain_value_final_live_set"] else None)
+    if cpu is not None and finals:
+        by_id = {r.rid: r for r in reqs}
+        picks = sorted(finals, key=lambda rid: by_id[rid].n * by_id[rid].steps)[:2]
+        log(f"served form: FP64 oracle on requests {picks}")
+        res["fp64_error"] = [fp64_errors(by_id[rid], finals[rid], cpu) for rid in picks]
     return res
 
 
@@ -342,10 +628,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--advance", type=int, default=8000,
+                    help="sub-steps from the initial state to the timed collapse window")
     ap.add_argument("--method", default="iisph",
                     choices=["iisph", "iisph_ap", "iisph_pseudo_ap", "dfsph_p", "pcisph"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-solvers", action="store_true")
+    ap.add_argument("--solver-steps", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-only", action="store_true")
@@ -362,142 +651,77 @@ def main() -> int:
                                    "not supported", "n_gpus": args.gpus}))
         return 2
 
-    from emerald_amd import lib, scenes
-    from emerald_amd import sph2d_box as sb
+    from emerald_amd import lib
     if lib().emsph_device_count() < 1:
         raise SystemExit("bench: no HIP device")
+    cpu = None if args.no_cpu_baseline else host_cpu()
 
     method = args.method
     if args.batched_only:
         print(json.dumps(batched_form(method, args.batched_steps, 2)), flush=True)
         return 0
     if args.served_only:
-        print(json.dumps(served_form(args.served_budget_gb, args.served_window)), flush=True)
+        print(json.dumps(served_form(args.served_budget_gb, args.served_window, cpu)), flush=True)
         return 0
     if args.search_only:
-        print(json.dumps(search_form()), flush=True)
+        print(json.dumps(search_form(cpu)), flush=True)
         return 0
-    log("building scene (1024 x 1024 lattice dam break)")
-    scene = scenes.dam_break_1m()
-    cfg = scenes.config_for(scene, method)
-    n = len(scene.positions)
-    ctx = sb.Context(cfg, sb.Solid_state(scene.solids), n)
-    ctx.set_state(sb.State(scene.positions, scene.velocities))
-    ctx.frame_init(method)
 
-    # Parity sample: the GPU state after the first k sub-steps, for the CPU leg.
-    gpu_after = {}
-    if not args.no_cpu_baseline:
-        for k in range(1, 21):
-            ctx.sub_step(method, (k - 1) * scene.dt, scene.dt)
-            st = ctx.get_state()
-            gpu_after[k] = (st.positions, st.velocities)
-        ctx.set_state(sb.State(scene.positions, scene.velocities))
-        ctx.frame_init(method)
-
-    log(f"warmup {args.warmup} sub-steps")
-    ctx.run_sub_steps(method, scene.dt, args.warmup)
-    log("autotune")
-    chosen, cand_ms = ctx.autotune(method, scene.dt, 5)
-
-    log(f"timing {args.steps} sub-steps")
-    ctx.iteration_totals(reset=True)
-    t0 = time.perf_counter()
-    dev_ms, st = ctx.run_sub_steps(method, scene.dt, args.steps)
-    wall = time.perf_counter() - t0
-    value = n * args.steps / wall
-    _, iters, div_iters, _ = ctx.iteration_totals(reset=True)
-    k_mean = iters / args.steps
-    kdiv_mean = div_iters / args.steps
-    bytes_step = step_bytes(method, k_mean, kdiv_mean)
-
-    # Per-kernel timing pass over the same number of sub-steps: HIP events
-    # recorded around each launch on the context's stream, resolved afterwards.
-    log(f"kernel timing pass, {args.steps} sub-steps")
-    ctx.set_kernel_timing(True)
-    prof_steps = args.steps
-    ctx.run_sub_steps(method, scene.dt, prof_steps)
-    timing = ctx.kernel_timing()
-    ctx.set_kernel_timing(False)
-    table = ALGO_BYTES[method]
-    dom = max(timing.items(), key=lambda kv: kv[1][0])
-    dom_name, (dom_ms, dom_calls) = dom
-    avg_s = dom_ms / dom_calls / 1e3
-    per = table.get(dom_name)
-    achieved = (per * n / avg_s / 1e9) if per else None
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        traffic = tj.get(method, {}).get(dom_name)
-    valu = None
-    vpath = os.path.join(ROOT, "profiles", "valu.json")
-    if os.path.exists(vpath):
-        with open(vpath) as f:
-            valu = json.load(f).get(method, {}).get(dom_name)
-    roofline = {
-        "bound": "hbm",
-        "kernel": dom_name,
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-        "traffic": traffic,
-        "avg_launch_us": avg_s * 1e6,
-        "algorithmic_bytes_per_particle": per,
-        "particles_per_launch": n,
-        # Measured share of the SIMDs' cycles taken by VALU issue (rocprof
-        # PMC): the neighbour build is limited by it, not by HBM bytes.
-        "valu_issue_frac": valu,
-    }
-    kernels = {}
-    for k, v in sorted(timing.items(), key=lambda kv: -kv[1][0]):
-        e = {"ms_per_step": v[0] / prof_steps, "calls_per_step": v[1] / prof_steps,
-             "avg_launch_us": v[0] / v[1] * 1e3}
-        if table.get(k):
-            e["algorithmic_GBps"] = table[k] * n / (v[0] / v[1] / 1e3) / 1e9
-        kernels[k] = e
+    head = solver_window(method, args.steps, args.warmup, advance_for(method, args.advance), cpu,
+                         autotune=True, kernel_timing=True, rest_window=True)
+    n = head["particles"]
+    roofline, kernels = roofline_of(method, head.pop("_timing"), head.pop("_timing_steps"), n)
+    col = head["collapse"]
     whole_step = {
-        "algorithmic_bytes_per_particle_step": bytes_step,
-        "mean_iterations": k_mean,
-        "mean_divergence_iterations": kdiv_mean,
-        "achieved_GBps": bytes_step * value / 1e9,
-        "frac_of_hbm_peak": bytes_step * value / 1e9 / HBM_PEAK_GBS,
+        "algorithmic_bytes_per_particle_step": col["algorithmic_bytes_per_particle_step"],
+        "mean_iterations": col["mean_iterations"],
+        "mean_divergence_iterations": col["mean_divergence_iterations"],
+        "achieved_GBps": col["achieved_GBps"],
+        "frac_of_hbm_peak": col["achieved_GBps"] / HBM_PEAK_GBS,
     }
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle on host cores)")
-        cpu = cpu_baseline(scene, cfg, method, gpu_after, args.cpu_budget)
-
+    solvers = None
+    if not args.no_solvers:
+        solvers = {}
+        for m in ("iisph", "pcisph", "dfsph_p"):
+            if m == method:
+                continue
+            r = solver_window(m, args.solver_steps, 5, advance_for(m, args.advance), cpu,
+                              autotune=False, kernel_timing=False, rest_window=False)
+            solvers[m] = r
     batched = None
     if not args.no_batched:
-        ctx.close()
         batched = batched_form(method, args.batched_steps, 2)
     served = None
     if not args.no_served:
-        ctx.close()
-        served = served_form(args.served_budget_gb, args.served_window)
+        served = served_form(args.served_budget_gb, args.served_window, cpu)
+        if batched:
+            served["served_over_batched"] = served["value"] / batched["value"]
     search = None
     if not args.no_search:
-        ctx.close()
-        search = search_form()
+        search = search_form(cpu)
     small = None
-    if not args.no_small:
-        ctx.close()
-        small = small_form()
+    if not args.no_small and cpu is not None:
+        small = small_form(cpu)
 
+    cpu_baseline = None
+    if cpu is not None:
+        c = head["cpu"]
+        cpu_baseline = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample",
+                                          "one_thread_value", "runs_s", "bit_identical_to_gpu")}
+        cpu_baseline["model"] = cpu["model"]
+        cpu_baseline["host_cpus"] = {k: cpu[k] for k in ("affinity_cpus", "cgroup_cpus",
+                                                         "omp_num_threads_env")}
     out = {
         "metric": "IISPH particle-steps/s @1M (1 GPU)" if method == "iisph"
         else f"{method.upper()} particle-steps/s @1M (1 GPU)",
-        "value": value,
+        "value": col["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "device_ms_per_step": dev_ms / args.steps,
+        "ms_per_step": col["ms_per_step"],
+        "device_ms_per_step": col["device_ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -505,21 +729,25 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{method} dam break, 1024x1024 lattice (1,048,576 fluid particles), "
-                        f"spacing 1/2560, h = spacing, dt 1.5625e-5, hex walls",
+                        f"spacing 1/2560, h = spacing, dt 1.5625e-5, hex walls; timed window "
+                        f"sub-steps {col['sub_steps'][0]}-{col['sub_steps'][1]} "
+                        f"(t = {col['time_s']:.3f} s, the collapse)",
             "particles": n,
-            "solids": int(len(scene.solids)),
-            "dt": scene.dt,
-            "last_iterations": int(st.iterations),
-            "sort_passes": int(st.sort_passes),
-            "cells": int(st.cell_count),
-            "max_neighbors": int(st.max_neighbors),
+            "solids": head["solids"],
+            "dt": head["dt"],
+            "mean_iterations": col["mean_iterations"],
+            "last_iterations": col["last_iterations"],
+            "sort_passes": col["sort_passes"],
+            "cells": col["cells"],
+            "max_neighbors": col["max_neighbors"],
         },
+        "windows": {"collapse": col, "rest": head.get("rest")},
         "roofline": roofline,
         "whole_step": whole_step,
-        "tuning": {"chosen": chosen, "candidate_ms_per_5_sub_steps": cand_ms,
-                   "candidates": "neighbour batch 2/4/8 x XCD remap on/off with graphs, "
-                                 "then the best without graphs"},
-        "cpu_baseline": cpu,
+        "tuning": head.get("tuning"),
+        "cpu_baseline": cpu_baseline,
+        "speedup_vs_cpu": head.get("speedup_vs_cpu"),
+        "solvers": solvers,
         "batched": batched,
         "served": served,
         "search": search,
@@ -527,7 +755,6 @@ def main() -> int:
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
     return 0
 
 

@@ -102,7 +102,7 @@ class _PoolLoop:
         self.commit_s = 0.0                    # wall time in commits
         self.step_s = 0.0                      # wall time in sub-steps
 
-    def run(self, t0: float, end: float, keep_states: bool) -> None:
+    def run(self, t0: float, end: float, keep_states) -> None:
         pool = self.pool
         last_commit = -1e9
         staged = False
@@ -164,7 +164,7 @@ class _PoolLoop:
                 v[1] += 1
                 n_live += v[3]
                 if v[1] >= v[0].steps:
-                    if keep_states:
+                    if keep_states is True or (keep_states and keep_states(v[0])):
                         self.finals[v[0].rid] = pool.get_state(sid)
                     pool.remove(sid)
                     self.completed.append(v[0].rid)
@@ -176,9 +176,10 @@ class _PoolLoop:
 
 def serve(requests: list, budget_bytes: float, window: float, cfl: float = 0.4,
           methods=("iisph", "dfsph_p"), max_scenes: int = 4096, solid_ratio: float = 0.25,
-          keep_states: bool = False, commit_interval: float = 0.2, log=None) -> dict:
+          keep_states=False, commit_interval: float = 0.2, log=None) -> dict:
     """Runs the request stream for window seconds; returns the served metrics
-    (and the pools' final states of completed requests when keep_states)."""
+    (and the pools' final states of completed requests when keep_states is
+    True, or of those for which keep_states(request) is true)."""
     # Device bytes per fluid particle (with solid_ratio solids per fluid
     # particle), measured on a small pool.
     probe = sb.Pool(1 << 20, int(solid_ratio * (1 << 20)), 16, methods[0])

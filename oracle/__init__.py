@@ -181,6 +181,7 @@ def lib():
             "orc_scene_solid_volumes": (None, [_P, _P]),
             "orc_scene_get_field": (_I, [_P, ctypes.c_char_p, _P]),
             "orc_scene_get_state_f32": (None, [_P, _P, _P]),
+            "orc_scene_get_state_f64": (None, [_P, _P, _P]),
             "orc_scene_get_neighborhood": (None, [_P, _I, _P, _P]),
             "orc_fixture_points_mt19937_64": (None, [_U64, _SZ, _F, _F, _P]),
             "orc_anti_coupling_angle": (_F, [_U64]),
@@ -393,6 +394,13 @@ class Scene:
         pos = np.empty((self.n, 2), np.float32)
         vel = np.empty((self.n, 2), np.float32)
         lib().orc_scene_get_state_f32(self._h, _ptr(pos), _ptr(vel))
+        return pos, vel
+
+    def state_f64(self):
+        """The state in the scene's own precision, widened to float64."""
+        pos = np.empty((self.n, 2), np.float64)
+        vel = np.empty((self.n, 2), np.float64)
+        lib().orc_scene_get_state_f64(self._h, _ptr(pos), _ptr(vel))
         return pos, vel
 
     def neighborhood(self, which: int = 0):

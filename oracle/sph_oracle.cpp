@@ -2171,6 +2171,23 @@ int orc_scene_get_colors(void* p, float* rgba, uint8_t* rgba8) {
 }
 void orc_convert_colors(size_t n, float const* rgba, uint8_t* out) { convert_colors(n, out, rgba); }
 
+void orc_scene_get_state_f64(void* p, double* pos, double* vel) {
+    auto* s = static_cast<OrcScene*>(p);
+    size_t const n = s->fp64 ? s->d.pos.size() : s->f.pos.size();
+    for (size_t i = 0; i < n; ++i) {
+        if (s->fp64) {
+            pos[2 * i] = s->d.pos[i].x;
+            pos[2 * i + 1] = s->d.pos[i].y;
+            vel[2 * i] = s->d.vel[i].x;
+            vel[2 * i + 1] = s->d.vel[i].y;
+        } else {
+            pos[2 * i] = s->f.pos[i].x;
+            pos[2 * i + 1] = s->f.pos[i].y;
+            vel[2 * i] = s->f.vel[i].x;
+            vel[2 * i + 1] = s->f.vel[i].y;
+        }
+    }
+}
 void orc_scene_get_state_f32(void* p, float* pos, float* vel) {
     auto* s = static_cast<OrcScene*>(p);
     size_t const n = s->fp64 ? s->d.pos.size() : s->f.pos.size();
