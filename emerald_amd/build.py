@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["engine.hip", "sph_kernels.hip", "sort.hip", "host_scene.cpp", "snapshot.cpp"]
-HEADERS = ["device_math.hpp", "kernels.hpp"]
+HEADERS = ["device_math.hpp", "kernels.hpp", "sort_networks.hpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -378,7 +378,7 @@ struct Engine {
     DBuf<V2> pos, vel;
     DBuf<float> kv, kd;
     SpatialIndex fidx;
-    DBuf<V2> spos, svel, dii, vext, xstar, vstar0, vstar1, force;
+    DBuf<V2> spos, svel, dii, vext, xstar, xstar1, vstar0, vstar1, force;
     DBuf<V2> pr0, pr1;  // IISPH {p, rho} ping-pong
     DBuf<V2> pacc;      // IISPH_AP / IISPH_PSEUDO_AP pressure accelerations
     DBuf<SdDii> sd;     // IISPH {sum_j d_ij p_j, d_ii}
@@ -735,7 +735,8 @@ hipError_t alloc_fluid(Engine* g, uint32_t cap) {
     if ((e = g->pos.alloc(cap)) || (e = g->vel.alloc(cap)) || (e = g->kv.alloc(cap)) ||
         (e = g->kd.alloc(cap)) || (e = g->fidx.alloc(cap, false)) || (e = g->spos.alloc(cap)) ||
         (e = g->svel.alloc(cap)) || (e = g->dii.alloc(cap)) ||
-        (e = g->vext.alloc(cap)) || (e = g->xstar.alloc(cap)) || (e = g->vstar0.alloc(cap)) ||
+        (e = g->vext.alloc(cap)) || (e = g->xstar.alloc(cap)) || (e = g->xstar1.alloc(cap)) ||
+        (e = g->vstar0.alloc(cap)) ||
         (e = g->vstar1.alloc(cap)) || (e = g->force.alloc(cap)) || (e = g->rho.alloc(cap)) ||
         (e = g->rho_star.alloc(cap)) || (e = g->aii.alloc(cap)) || (e = g->p0.alloc(cap)) ||
         (e = g->p1.alloc(cap)) || (e = g->alpha.alloc(cap)) || (e = g->kvs.alloc(cap)) ||
@@ -1611,14 +1612,14 @@ emsph_status dfsph_sub_step(Engine* g) {
 
 // pcisph.cpp:36-164 for every scene of the batch.
 emsph_status pcisph_sub_step(Engine* g) {
-    HIP_TRY(ensure_geometry(g));
     begin_attempt(g);
     Batch b = fluid_batch(g, nullptr, g->p0.p);
-    b.fl.dist = g->fl_dist.p;
-    b.fl.vec = g->fl_vec.p;
     V2* vin = g->vstar0.p;
     V2* vout = g->vstar1.p;
+    V2* xa = g->xstar.p;
+    V2* xb = g->xstar1.p;
     b.vstar_out = vin;
+    b.xstar = xa;
     HIP_TRY(segment(g, EMSPH_PCISPH, SEG_PREFIX, [&]() -> hipError_t {
         HIP_PASS(enqueue_prepare(g, BUILD_LISTS, false));
         HIP_PASS(enqueue_forces(g, b, 0, false));
@@ -1629,6 +1630,10 @@ emsph_status pcisph_sub_step(Engine* g) {
     bool short_sort = false;
     int const pci_from = first_decision(g, 3, 2);
     HIP_TRY(run_loop(g, 3, 6, 2, [&](int it) -> hipError_t {
+        // x* ping-pongs like v*: density and force read the iteration's x*,
+        // force writes the next one.
+        b.xstar_in = (it & 1) ? xb : xa;
+        b.xstar = (it & 1) ? xa : xb;
         HIP_PASS(timed(g, "pci_density", [&] { launch_pci_density(b, g->stream); }));
         b.vstar_in = (it & 1) ? vout : vin;
         b.vstar_out = (it & 1) ? vin : vout;
@@ -1645,6 +1650,8 @@ emsph_status pcisph_sub_step(Engine* g) {
     b.vstar_in = g->vstar0.p;
     b.vstar_out = g->vstar1.p;
     b.vstar_alt = g->vstar1.p;
+    b.xstar_in = xa;
+    b.xstar_alt = xb;
     HIP_TRY(segment(g, EMSPH_PCISPH, SEG_SUFFIX, [&]() -> hipError_t {
         HIP_PASS(timed(g, "pci_finalize", [&] { launch_pci_finalize(b, g->stream); }));
         launch_scene_totals(b, 1, g->stream);

@@ -283,7 +283,9 @@ struct Batch {
     float* kds;
     // PCISPH
     V2* vext;
-    V2* xstar;
+    V2* xstar;               // PCISPH: the predicted positions written (init, force)
+    V2 const* xstar_in;      // PCISPH: the predicted positions of the iteration
+    V2 const* xstar_alt;     // PCISPH: the other x* buffer (the final kernel)
     V2 const* vstar_in;
     V2* vstar_out;
     V2 const* vstar_alt;

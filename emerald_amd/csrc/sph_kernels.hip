@@ -20,6 +20,7 @@
 
 #include "device_math.hpp"
 #include "kernels.hpp"
+#include "sort_networks.hpp"
 
 namespace emsph {
 
@@ -817,40 +818,13 @@ __device__ __forceinline__ bool entry_after(uint64_t a, uint64_t b,
     return oorig[static_cast<uint32_t>(a)] > oorig[static_cast<uint32_t>(b)];
 }
 
-// Batcher's odd-even merge sorting network on N inputs (N a power of two),
-// as a compile-time comparator list (63 comparators for N = 16).
-struct Comparator {
-    int a;
-    int b;
-};
 template <int N>
-struct OddEvenMergeNet {
-    Comparator ce[N * N];
-    int count = 0;
-    constexpr OddEvenMergeNet() : ce{} {
-        for (int p = 1; p < N; p <<= 1) {
-            for (int k = p; k >= 1; k >>= 1) {
-                for (int j = k % p; j <= N - 1 - k; j += 2 * k) {
-                    int const last = (k - 1 < N - j - k - 1) ? k - 1 : N - j - k - 1;
-                    for (int i = 0; i <= last; ++i) {
-                        if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
-                            ce[count].a = i + j;
-                            ce[count].b = i + j + k;
-                            ++count;
-                        }
-                    }
-                }
-            }
-        }
-    }
-};
-constexpr OddEvenMergeNet<16> NET16{};
-constexpr OddEvenMergeNet<32> NET32{};
-constexpr OddEvenMergeNet<64> NET64{};
-
-template <int N>
-__device__ constexpr OddEvenMergeNet<N> const& net_of() {
-    if constexpr (N == 16) {
+__device__ constexpr auto const& net_of() {
+    if constexpr (N == 8) {
+        return NET8;
+    } else if constexpr (N == 12) {
+        return NET12;
+    } else if constexpr (N == 16) {
         return NET16;
     } else if constexpr (N == 32) {
         return NET32;
@@ -859,12 +833,14 @@ __device__ constexpr OddEvenMergeNet<N> const& net_of() {
     }
 }
 
-// Lists of every capacity are collected unsorted and sorted once by a network
-// (no per-candidate insertion chains through LDS: those run as long as the
-// longest lane's shift, for every accepted candidate of any lane).
-template <int CAP>
-constexpr bool net_sorted() {
-    return true;
+// Largest value of v over the lanes of the wave.
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int const o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    return v;
 }
 
 // Sorts one lane's cnt <= N collected entries into the reference order (by
@@ -876,25 +852,37 @@ constexpr bool net_sorted() {
 // insertion pass with the exact comparator puts in original index order. A
 // |v|^2 below 2 FLT_MIN, where imath_length scales instead, sends the list
 // to that pass with every distance taken again from the positions.
+//
+// The entries are compared as doubles: an entry's sign bit is clear and its
+// top 11 bits (the double's exponent field) come from a finite float's
+// exponent, so it is never a NaN or infinity, and non-negative doubles order
+// exactly like their bit patterns. A compare-exchange is then one v_min_f64
+// and one v_max_f64 (the kernels keep f64 denormals, so an entry with a tiny
+// |v|^2 keeps its bits). Unused inputs hold the largest finite double, which
+// sorts after every entry.
 template <int N, int W>
 __device__ __forceinline__ void net_sort(uint64_t (*ent)[W], uint32_t lane, int cnt,
                                          uint32_t const* __restrict__ oorig,
                                          V2 const* __restrict__ opos, V2 p) {
-    constexpr OddEvenMergeNet<N> const& net = net_of<N>();
-    uint64_t e[N];
+    constexpr auto const& net = net_of<N>();
+    static_assert(net.inputs == N, "network size");
+    constexpr uint64_t PAD = 0x7FEFFFFFFFFFFFFFull;
+    double d64[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         uint64_t const v = ent[k][lane];
-        e[k] = (k < cnt) ? v : ~0ull;
+        d64[k] = __builtin_bit_cast(double, (k < cnt) ? v : PAD);
     }
 #pragma unroll
     for (int c = 0; c < net.count; ++c) {
-        uint64_t const x = e[net.ce[c].a];
-        uint64_t const y = e[net.ce[c].b];
-        bool const swap = y < x;
-        e[net.ce[c].a] = swap ? y : x;
-        e[net.ce[c].b] = swap ? x : y;
+        double const x = d64[net.ce[c].a];
+        double const y = d64[net.ce[c].b];
+        d64[net.ce[c].a] = __builtin_fmin(x, y);
+        d64[net.ce[c].b] = __builtin_fmax(x, y);
     }
+    uint64_t e[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) { e[k] = __builtin_bit_cast(uint64_t, d64[k]); }
     bool tiny = false;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
@@ -955,113 +943,92 @@ __device__ __forceinline__ void load_chunk(V2 (&q)[C], V2 const* __restrict__ op
     }
 }
 
-// Acceptance of one loaded chunk, in scan order: appended for the 16-entry
-// build (sorted afterwards), inserted in order otherwise. Returns false when
-// the scan stops: a capped list overflowed (cnt = -1) or a full-capacity list
-// is full.
-template <int CAP, int C, int W>
+// Acceptance of one loaded chunk, in scan order: appended unsorted (sorted
+// afterwards by net_sort). Returns false when the scan stops: a capped list
+// overflowed (cnt = -1) or a full-capacity list is full. CUT2: acceptance as
+// |v|^2 < cut2 (the scene's cut2 > 0, always for a normal cut-off), else as
+// Imath length < maxd; the choice is made once per particle, not per
+// candidate.
+//
+template <int CAP, int C, int W, bool CUT2>
 __device__ __forceinline__ bool take_chunk(uint64_t (*ent)[W], uint32_t lane, V2 p,
                                            uint32_t self, bool same, V2 const (&q)[C],
-                                           uint32_t s0, uint32_t e,
-                                           uint32_t const* __restrict__ oorig, float maxd,
-                                           float cut2, int& cnt) {
+                                           uint32_t s0, uint32_t e, float maxd, float cut2,
+                                           int& cnt) {
     bool in[C];
     float l2[C];
 #pragma unroll
     for (int u = 0; u < C; ++u) {
         V2 const v = sub(q[u], p);
         l2[u] = v.x * v.x + v.y * v.y;  // as in imath_length
-        in[u] = (cut2 > 0.0f) ? (l2[u] < cut2) : (imath_length(v) < maxd);
-    }
-    if (net_sorted<CAP>()) {
-        // Unsorted append of (|v|^2 bits << 32 | slot); the distances are
-        // taken after the sort (net_sort), once per accepted entry. A capped
-        // list (CAP < 64) that overflows is deferred (cnt = -1); a full
-        // 64-entry list keeps the first 64 accepted in scan order
-        // (neighborhood.h:233-250) and ends the scan.
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            uint32_t const s = s0 + u;
-            bool const acc = s < e && !(same && s == self) && in[u];
-            if (acc && cnt < CAP) {
-                ent[cnt][lane] = (static_cast<uint64_t>(__float_as_uint(l2[u])) << 32) | s;
-            }
-            if (CAP == MAXN) {
-                cnt += (acc && cnt < CAP) ? 1 : 0;
-            } else {
-                cnt += acc ? 1 : 0;
-            }
+        if constexpr (CUT2) {
+            in[u] = l2[u] < cut2;
+        } else {
+            in[u] = imath_length(v) < maxd;
         }
-        if (CAP < MAXN && cnt > CAP) {
-            cnt = -1;
-            return false;
-        }
-        return !(CAP == MAXN && cnt == CAP);
     }
+    // Unsorted append of (|v|^2 bits << 32 | slot); the distances are taken
+    // after the sort (net_sort), once per accepted entry. A capped list
+    // (CAP < 64) that overflows is deferred (cnt = -1); a full 64-entry list
+    // keeps the first 64 accepted in scan order (neighborhood.h:233-250) and
+    // ends the scan.
 #pragma unroll
     for (int u = 0; u < C; ++u) {
         uint32_t const s = s0 + u;
-        if (s < e && !(same && s == self) && in[u]) {
-            if (CAP < MAXN && cnt == CAP) {
-                cnt = -1;
-                return false;
-            }
-            float const d = imath_length(sub(q[u], p));
-            uint64_t const key = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | s;
-            int kk = cnt;
-            while (kk > 0) {
-                uint64_t const prev = ent[kk - 1][lane];
-                if (!entry_after(prev, key, oorig)) { break; }
-                ent[kk][lane] = prev;
-                --kk;
-            }
-            ent[kk][lane] = key;
-            ++cnt;
-            if (CAP == MAXN && cnt == MAXN) { return false; }
+        bool const acc = s < e && !(same && s == self) && in[u];
+        uint64_t const entry = (static_cast<uint64_t>(__float_as_uint(l2[u])) << 32) | s;
+        if (acc && cnt < CAP) { ent[cnt][lane] = entry; }
+        if constexpr (CAP < MAXN) {
+            cnt += acc ? 1 : 0;
+        } else {
+            cnt += (acc && cnt < CAP) ? 1 : 0;
         }
     }
-    return true;
+    if (CAP < MAXN && cnt > CAP) {
+        cnt = -1;
+        return false;
+    }
+    return !(CAP == MAXN && cnt == CAP);
 }
 
-// Accepted candidates of one particle over its 3x3 cell window. Returns the
-// count, or -1 when CAP < 64 and more than CAP candidates are accepted.
-// A candidate is accepted when |v| < maxd; with cut2 > 0 (StepConsts::cut2)
-// that is decided as |v|^2 < cut2, the same answer without the square root,
-// which is then taken for accepted candidates only. (Loading each range's
-// first chunk while the previous range is scanned, with the nine ranges
-// unrolled, measured slower: 217 vs 174 us at 1M.)
-// Window ranges carry WIN_SOLID_NEAR in their end's top bit when the same
-// neighbour cell holds solids (k_cell_windows); *near receives the OR.
-template <int CAP, int W>
-__device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
-                                              V2 p, uint32_t self, bool same,
-                                              uint2 const* __restrict__ win,
-                                              V2 const* __restrict__ opos,
-                                              uint32_t const* __restrict__ oorig, float maxd,
-                                              float cut2, uint32_t* near = nullptr) {
+// The scan of one particle's 3x3 cell window (nine slot ranges in scan order)
+// in chunks of C candidates. Returns the count (-1: a capped list overflowed)
+// and ORs the ranges' end words (solid flags) into flags; a full 64-entry list
+// stops early and the remaining ranges' flags are taken from the window.
+template <int CAP, int W, bool CUT2>
+__device__ __forceinline__ int scan_window(uint64_t (*ent)[W], uint32_t lane, V2 p,
+                                           uint32_t self, bool same,
+                                           uint2 const* __restrict__ win,
+                                           V2 const* __restrict__ opos, float maxd, float cut2,
+                                           uint32_t& flags) {
     constexpr int C = 4;
     int cnt = 0;
-    uint32_t flags = 0;
     uint2 cur = win[0];
     if constexpr (CAP <= 16) {
         // The 16-entry build (the common case): a list either fits or is
-        // deferred, and every lane sorts with the 16-input network.
+        // deferred. The first chunk of each range is loaded while the range
+        // before it is scanned (ranges of a few candidates are the rule).
+        V2 qn[C];
+        load_chunk<C>(qn, opos, cur.x);
+        uint2 nxt = win[1];
         for (int k = 0; k < 9; ++k) {
-            uint2 const nxt = win[k < 8 ? k + 1 : 8];  // next range in flight
+            V2 q[C];
+#pragma unroll
+            for (int u = 0; u < C; ++u) { q[u] = qn[u]; }
+            uint2 const nn = win[k < 7 ? k + 2 : 8];
+            if (k < 8) { load_chunk<C>(qn, opos, nxt.x); }
             flags |= cur.y;
             uint32_t const e = cur.y & ~WIN_SOLID_NEAR;
             for (uint32_t s0 = cur.x; s0 < e; s0 += C) {
-                V2 q[C];
-                load_chunk<C>(q, opos, s0);
-                if (!take_chunk<CAP, C, W>(ent, lane, p, self, same, q, s0, e, oorig, maxd,
-                                           cut2, cnt)) {
+                if (s0 != cur.x) { load_chunk<C>(q, opos, s0); }
+                if (!take_chunk<CAP, C, W, CUT2>(ent, lane, p, self, same, q, s0, e, maxd, cut2,
+                                                 cnt)) {
                     return cnt;  // overflow (-1)
                 }
             }
             cur = nxt;
+            nxt = nn;
         }
-        if (near) { *near = flags & WIN_SOLID_NEAR; }
-        if (cnt > 0) { net_sort<16, W>(ent, lane, cnt, oorig, opos, p); }
         return cnt;
     }
     bool go = true;
@@ -1072,28 +1039,50 @@ __device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
         for (uint32_t s0 = cur.x; s0 < e; s0 += C) {
             V2 q[C];
             load_chunk<C>(q, opos, s0);
-            if (!take_chunk<CAP, C, W>(ent, lane, p, self, same, q, s0, e, oorig, maxd, cut2,
-                                       cnt)) {
+            if (!take_chunk<CAP, C, W, CUT2>(ent, lane, p, self, same, q, s0, e, maxd, cut2,
+                                             cnt)) {
                 go = false;  // overflow (-1) or a full 64-entry list
                 break;
             }
         }
         cur = nxt;
     }
-    if (cnt < 0) { return cnt; }
     // A full list stops before the last ranges: their solid flags are taken
     // from the window directly.
     for (int k = 0; k < 9 && !go; ++k) { flags |= win[k].y; }
+    return cnt;
+}
+
+// Accepted candidates of one particle over its 3x3 cell window. Returns the
+// count, or -1 when CAP < 64 and more than CAP candidates are accepted.
+// A candidate is accepted when |v| < maxd; with cut2 > 0 (StepConsts::cut2)
+// that is decided as |v|^2 < cut2, the same answer without the square root,
+// which is then taken for accepted candidates only.
+// Window ranges carry WIN_SOLID_NEAR in their end's top bit when the same
+// neighbour cell holds solids (k_cell_windows); *near receives the OR.
+template <int CAP, int W>
+__device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
+                                              V2 p, uint32_t self, bool same,
+                                              uint2 const* __restrict__ win,
+                                              V2 const* __restrict__ opos,
+                                              uint32_t const* __restrict__ oorig, float maxd,
+                                              float cut2, uint32_t* near = nullptr) {
+    uint32_t flags = 0;
+    int const cnt = (cut2 > 0.0f)
+                      ? scan_window<CAP, W, true>(ent, lane, p, self, same, win, opos, maxd, cut2,
+                                                  flags)
+                      : scan_window<CAP, W, false>(ent, lane, p, self, same, win, opos, maxd,
+                                                   cut2, flags);
     if (near) { *near = flags & WIN_SOLID_NEAR; }
-    // The smallest network that covers the wave's longest list.
-    int wmax = cnt;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        int const o = __shfl_xor(wmax, off);
-        wmax = o > wmax ? o : wmax;
-    }
+    // The smallest network that covers the wave's longest list (a deferred,
+    // overflowed list counts as empty here).
+    int const wmax = wave_max(cnt);
     if (wmax > 0) {  // also turns a single entry's |v|^2 into its distance
-        if (wmax <= 16) {
+        if (wmax <= 8) {
+            net_sort<8, W>(ent, lane, cnt, oorig, opos, p);
+        } else if (wmax <= 12) {
+            net_sort<12, W>(ent, lane, cnt, oorig, opos, p);
+        } else if (wmax <= 16) {
             net_sort<16, W>(ent, lane, cnt, oorig, opos, p);
         } else if (CAP <= 32 || wmax <= 32) {
             if constexpr (CAP >= 32) { net_sort<32, W>(ent, lane, cnt, oorig, opos, p); }
@@ -2321,7 +2310,10 @@ __global__ __launch_bounds__(TILE) void k_pci_init(Batch a) {
 
 // :90-115 geometry at x* (neighborhoods_from_state.cpp:103-128), densities
 // (pcisph_ops.cpp:18-63), pressure update (:65-76) and the max density error
-// (common.cpp:78-85). PCISPH densities are kept in rho.
+// (common.cpp:78-85). PCISPH densities are kept in rho. Of the recomputed
+// pair geometry only the kernel gradients are stored (the force sweep takes
+// the vectors and distances again from x*, the same arithmetic); the vectors
+// and distances are written as well when they are kept for a read-back.
 // Neighbour entries are taken NB at a time (their loads in flight together)
 // and processed in list order.
 template <int NB>
@@ -2332,7 +2324,7 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
     if (r.valid) {
         uint32_t const t = r.t;
         StepConsts const c = a.scenes[r.scene].c;
-        V2 const xs = a.xstar[t];
+        V2 const xs = a.xstar_in[t];
         int const fc = a.fl.count[t];
         float w = c.kc.w0;
         for (int k0 = 0; k0 < fc; k0 += NB) {
@@ -2346,7 +2338,7 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
                 xj[u] = xs;
-                if (k0 + u < fc) { xj[u] = a.xstar[j[u]]; }
+                if (k0 + u < fc) { xj[u] = a.xstar_in[j[u]]; }
             }
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
@@ -2354,8 +2346,10 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
                     size_t const o = static_cast<size_t>(k0 + u) * a.stride + t;
                     V2 const vec = sub(xj[u], xs);
                     float const d = imath_length(vec);
-                    a.fl.vec[o] = vec;
-                    a.fl.dist[o] = d;
+                    if (a.store_geometry) {
+                        a.fl.vec[o] = vec;
+                        a.fl.dist[o] = d;
+                    }
                     float wk;
                     V2 gw;
                     kernel_W_GradW_r(neg(vec), d, c.kc, wk, gw);
@@ -2411,23 +2405,20 @@ __global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
     int const fc = a.fl.count[t];
     if (fc) {
         V2 const v = a.vstar_in[t];
+        V2 const xs = a.xstar_in[t];
         V2 f{0.0f, 0.0f};
         for (int k0 = 0; k0 < fc; k0 += NB) {
             uint32_t j[NB];
-            V2 rv[NB], gw[NB], vj[NB];
-            float od[NB], pj[NB], dist[NB];
+            V2 gw[NB], vj[NB], xj[NB];
+            float od[NB], pj[NB];
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
                 j[u] = 0u;
-                rv[u] = V2{0.0f, 0.0f};
                 gw[u] = V2{0.0f, 0.0f};
-                dist[u] = 1.0f;
                 if (k0 + u < fc) {
                     size_t const o = static_cast<size_t>(k0 + u) * a.stride + t;
                     j[u] = a.fl.idx[o];
-                    rv[u] = a.fl.vec[o];
                     gw[u] = a.fl.gw[o];
-                    dist[u] = a.fl.dist[o];
                 }
             }
 #pragma unroll
@@ -2435,10 +2426,12 @@ __global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
                 od[u] = 1.0f;
                 pj[u] = 0.0f;
                 vj[u] = v;
+                xj[u] = xs;
                 if (k0 + u < fc) {
                     od[u] = rho[j[u]];
                     pj[u] = p[j[u]];
                     vj[u] = a.vstar_in[j[u]];
+                    xj[u] = a.xstar_in[j[u]];
                 }
             }
 #pragma unroll
@@ -2446,9 +2439,13 @@ __global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
                 if (k0 + u < fc) {
                     f = add(f, scale(por + (pj[u] / (od[u] * od[u])), gw[u]));
                     V2 const dv = sub(vj[u], v);
-                    float const dvr = dot(rv[u], dv);
+                    // The pair's vector and distance at x*, as the density
+                    // sweep took them (neighborhoods_from_state.cpp:103-128).
+                    V2 const rv = sub(xj[u], xs);
+                    float const dvr = dot(rv, dv);
                     if (dvr < 0.0f) {
-                        float const mu = H * dvr / (dist[u] + N2);
+                        float const dist = imath_length(rv);
+                        float const mu = H * dvr / (dist + N2);
                         float const den_avg = (density + od[u]) / 2.0f;
                         float const pi_ab =
                           ((-1.0f * c.viscosity * mu) + (2.0f * (mu * mu))) / den_avg;
@@ -2477,17 +2474,17 @@ __global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
 }
 
 // :151-163 final update: v = v_ext + (dt/m) F; x += dt v. Both equal the last
-// predictor written by k_pci_force (in vstar_alt after an odd number of
-// iterations).
+// predictor written by k_pci_force (in vstar_alt and xstar_alt after an odd
+// number of iterations).
 __global__ __launch_bounds__(TILE) void k_pci_finalize(Batch a) {
     TileRef const r = tile_ref(a, TILE);
     unsigned int vm = 0;
     double ke = 0.0, pe = 0.0;
     if (r.valid) {
         uint32_t const t = r.t;
-        V2 const* vs = (a.stats[r.scene].iterations & 1) ? a.vstar_alt : a.vstar_in;
-        V2 const v = vs[t];
-        V2 const x = a.xstar[t];
+        bool const odd = (a.stats[r.scene].iterations & 1) != 0;
+        V2 const v = (odd ? a.vstar_alt : a.vstar_in)[t];
+        V2 const x = (odd ? a.xstar_alt : a.xstar_in)[t];
         uint32_t const o = a.orig[t];
         a.pos_orig[o] = x;
         a.vel_orig[o] = v;
