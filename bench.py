@@ -574,12 +574,13 @@ def fp64_errors(req, state, cpu: dict) -> dict:
 
 
 def served_form(budget_gb: float, window: float, cpu: dict | None, rate: float = 200.0,
-                plain_steps: int = 2) -> dict:
+                plain_steps: int = 2, batch_steps: int = 5) -> dict:
     """Poisson stream of dam-break requests (200/s, 5k-500k particles, IISPH or
     DFSPH_P, 100-2000 steps) served from two device-resident pools (one per
     solver) with continuous batching under an HBM budget. After the window,
-    the scenes still live are also stepped as plain per-scene contexts (one at
-    a time) for the throughput comparison, and the two smallest completed
+    the scenes still live are stepped from their final states as one batch
+    per solver and as plain per-scene contexts (one at a time), for the
+    throughput comparisons at the same states, and the two smallest completed
     requests are compared with the oracle's FP64 build."""
     from emerald_amd import serve
     from emerald_amd import sph2d_box as sb
@@ -588,33 +589,58 @@ def served_form(budget_gb: float, window: float, cpu: dict | None, rate: float =
     small = lambda r: r.n * r.steps <= 2e7  # noqa: E731
     res = serve.serve(reqs, budget_gb * 1e9, window, log=log, keep_states=small)
     loops = res["_loops"]
-    log("served form: plain per-scene comparison on the final live sets")
+    log("served form: the final live sets as batches and as plain per-scene contexts")
+    live_sets = {}
+    for m, lp in loops.items():
+        live = list(lp.live.items())
+        live_sets[m] = [(v[0], lp.pool.get_state(sid)) for sid, v in live]
+        lp.pool.close()
+    finals = {rid: st for lp in loops.values() for rid, st in lp.finals.items()}
+    serve.close(res)
+    b_steps = 0
+    t_batch = 0.0
+    b_iters = []
     n_steps = 0
     t_plain = 0.0
     scenes_done = 0
-    for m, lp in loops.items():
-        live = list(lp.live.items())
-        states = {sid: lp.pool.get_state(sid) for sid, _ in live}
-        lp.pool.close()
-        for sid, v in live:
-            req = v[0]
-            sc, cfg = serve.request_scene(req)
+    for m, items in live_sets.items():
+        if not items:
+            continue
+        built = [serve.request_scene(req) for req, _ in items]
+        batch = sb.Batch([cfg for _, cfg in built], [sb.Solid_state(sc.solids) for sc, _ in built],
+                         [st for _, st in items])
+        batch.set_cfl(0.4)
+        batch.frame_init(m)
+        batch.run_sub_steps(m, 2)
+        batch.iteration_totals(reset=True)
+        t0 = time.perf_counter()
+        batch.run_sub_steps(m, batch_steps)
+        t_batch += time.perf_counter() - t0
+        steps, it, dit, _ = batch.iteration_totals(reset=True)
+        b_iters.append({"method": m, "iterations_per_sub_step": it / steps,
+                        "divergence_iterations_per_sub_step": dit / steps})
+        b_steps += sum(len(sc.positions) for sc, _ in built) * batch_steps
+        batch.close()
+        for (req, st), (sc, cfg) in zip(items, built):
             c = sb.Context(cfg, sb.Solid_state(sc.solids), len(sc.positions))
             c.set_cfl(0.4)
-            c.set_state(states[sid])
+            c.set_state(st)
             c.frame_init(m)
+            c.run_sub_steps(m, cfg.seconds_per_sub_step, 1)
             t0 = time.perf_counter()
             c.run_sub_steps(m, cfg.seconds_per_sub_step, plain_steps)
             t_plain += time.perf_counter() - t0
             n_steps += len(sc.positions) * plain_steps
             scenes_done += 1
             c.close()
-    finals = {rid: st for lp in loops.values() for rid, st in lp.finals.items()}
-    serve.close(res)
+    res["batched_value_final_live_set"] = b_steps / t_batch if t_batch > 0 else None
+    res["batched_iterations_final_live_set"] = b_iters
     res["plain_value_final_live_set"] = n_steps / t_plain if t_plain > 0 else None
     res["plain_scenes"] = scenes_done
     res["speedup_vs_plain"] = (res["value"] / res["plain_value_final_live_set"]
                                if res["plain_value_final_live_set"] else None)
+    res["served_over_batched_same_states"] = (res["value"] / res["batched_value_final_live_set"]
+                                              if res["batched_value_final_live_set"] else None)
     if cpu is not None and finals:
         by_id = {r.rid: r for r in reqs}
         picks = sorted(finals, key=lambda rid: by_id[rid].n * by_id[rid].steps)[:2]

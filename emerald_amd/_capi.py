@@ -242,6 +242,7 @@ SIGNATURES = {
     "emsph_search_run": (_ST, [_P, _F, _I, _P]),
     "emsph_search_get": (_ST, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "emsph_search_lists_noalloc": (_ST, [_P, _SZ, _F, _P, _P, _P]),
+    "emsph_pool_get_iteration_totals": (_ST, [_P, _I, _P]),
 }
 
 _lib = None

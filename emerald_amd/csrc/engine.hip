@@ -325,6 +325,7 @@ struct SceneHost {
     float cfl = 0.0f;     // > 0: per-sub-step dt from the scene's CFL number
     bool force_set = false;     // force model set explicitly (else gravity of the params)
     emsph_force_model force{};
+    bool retired = false;       // a pool scene removed, not yet committed
 };
 
 // The scene's force model: the one set explicitly, or the reference's
@@ -685,6 +686,7 @@ SceneDev scene_dev(SceneHost const& h, uint32_t first_tile) {
     sd.force.drag_mult = -fm.drag_magnitude * fm.drag_diameter;
     sd.force.anti_maxd = fm.anti_coupling_max_distance;
     sd.force.anti_mag = fm.anti_coupling_magnitude;
+    sd.retired = h.retired ? 1u : 0u;
     return sd;
 }
 
@@ -2019,6 +2021,12 @@ emsph_status pool_commit(emsph_pool* P) {
     g->sh = std::move(hs);
     g->M = static_cast<uint32_t>(g->sh.size());
     g->N = o;
+    // The dense cell index is laid out again for the new scene list: table
+    // sizes from the next exact run, and a new layout generation, so the
+    // solid-cell bitmap over its bins is rebuilt from the new solids.
+    g->fidx.dense_layout.clear();
+    g->fidx.dense_bins = 0;
+    g->dense_stale = true;
     ST_TRY(upload_layout(g));
     ST_TRY(build_solids(g, sx));
     ST_TRY(upload_layout(g));  // solid offsets are now known
@@ -2428,6 +2436,12 @@ static void engine_iteration_totals(Engine* g, int reset, int64_t* out) {
 emsph_status emsph_get_iteration_totals(emsph_context* c, int reset, int64_t* out) {
     if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
     engine_iteration_totals(&c->e, reset, out);
+    return EMSPH_OK;
+}
+
+emsph_status emsph_pool_get_iteration_totals(emsph_pool* P, int reset, int64_t* out) {
+    if (!P || !out) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null argument"); }
+    engine_iteration_totals(&P->e, reset, out);
     return EMSPH_OK;
 }
 
@@ -2984,8 +2998,6 @@ emsph_status emsph_pool_create(size_t fluid_capacity, size_t solid_capacity, int
     P->solid_cap = static_cast<uint32_t>(solid_capacity);
     P->max_scenes = static_cast<uint32_t>(max_scenes);
     Engine* g = &P->e;
-    // A pool's memory is its admission budget: no dense index tables.
-    g->dense_disabled = true;
     emsph_status st = engine_init(g);
     if (st) {
         delete P;
@@ -3058,9 +3070,22 @@ emsph_status emsph_pool_remove(emsph_pool* P, int id) {
             return EMSPH_OK;
         }
     }
-    if (pool_index(P, id) < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no such scene"); }
+    int const k = pool_index(P, id);
+    if (k < 0) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "no such scene"); }
     if (std::find(P->removes.begin(), P->removes.end(), id) == P->removes.end()) {
         P->removes.push_back(id);
+        // Until the commit the scene still rides along in the sort and the
+        // neighbour build, but no longer in the solver iterations.
+        Engine* g = &P->e;
+        SceneHost& h = g->sh[static_cast<size_t>(k)];
+        h.retired = true;
+        uint32_t first_tile = 0;
+        for (int j = 0; j < k; ++j) { first_tile += (g->sh[static_cast<size_t>(j)].n + TILE - 1) / TILE; }
+        SceneDev const sd = scene_dev(h, first_tile);
+        HIP_TRY(hipSetDevice(g->device));
+        HIP_TRY(hipMemcpyAsync(g->scenes.p + k, &sd, sizeof(SceneDev), hipMemcpyHostToDevice,
+                               g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));  // sd lives on this stack frame
     }
     return EMSPH_OK;
 }

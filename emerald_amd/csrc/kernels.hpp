@@ -96,6 +96,8 @@ struct SceneDev {
     float dt_max;
     ForceDev force;
     FrameDev frame;
+    uint32_t retired;     // a pool scene removed but not yet committed: no solver iterations
+    uint32_t pad_[3];
 };
 
 struct Tile {

@@ -490,12 +490,13 @@ void launch_step_begin(Batch const& b, hipStream_t s) {
     hipLaunchKernelGGL(k_step_begin, dim3(blocks ? blocks : 1), dim3(256), 0, s, b);
 }
 
-// Every scene with particles enters the solver loop (one workgroup).
+// Every scene with particles enters the solver loop (one workgroup), except
+// a pool scene already removed (its results are discarded at the commit).
 template <int THREADS>
 __device__ __forceinline__ void loop_begin_block(Batch const& b) {
     uint32_t cnt = 0;
     for (uint32_t s = threadIdx.x; s < b.nscenes; s += blockDim.x) {
-        unsigned int const act = b.scenes[s].n > 0 ? 1u : 0u;
+        unsigned int const act = (b.scenes[s].n > 0 && !b.scenes[s].retired) ? 1u : 0u;
         b.stats[s].active = act;
         cnt += act;
     }

@@ -246,6 +246,7 @@ def serve(requests: list, budget_bytes: float, window: float, cfl: float = 0.4,
                               "p99": 1e3 * percentile(wait, 99) if wait else None},
         "per_pool": {m: {"ticks": lp.ticks, "admitted": lp.admitted,
                          "completed": len(lp.completed),
+                         "iterations_per_tick": _iterations_per_tick(lp),
                          "commit_s": lp.commit_s, "sub_step_s": lp.step_s,
                          "mean_live_particles": float(np.mean(lp.live_particles))
                          if lp.live_particles else 0.0}
@@ -253,6 +254,15 @@ def serve(requests: list, budget_bytes: float, window: float, cfl: float = 0.4,
     }
     out["_loops"] = loops
     return out
+
+
+def _iterations_per_tick(lp) -> dict:
+    """Solver iterations launched per sub-step of a pool (the maximum over its
+    live scenes), over the window."""
+    steps, it, dit, _ = lp.pool.iteration_totals()
+    if not steps:
+        return {"iterations": None, "divergence_iterations": None}
+    return {"iterations": it / steps, "divergence_iterations": dit / steps}
 
 
 def close(result: dict) -> None:

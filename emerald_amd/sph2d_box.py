@@ -764,6 +764,14 @@ class Pool:
         check(lib().emsph_pool_save_snapshot(self._h, int(sid), path.encode(), int(time_flicks)),
               "pool_save_snapshot")
 
+    def iteration_totals(self, reset: bool = False) -> tuple:
+        """As Context.iteration_totals, over the pool's sub-steps (iterations
+        launched: the maximum over the live scenes of each sub-step)."""
+        out = (ctypes.c_int64 * 4)()
+        check(lib().emsph_pool_get_iteration_totals(self._h, 1 if reset else 0, out),
+              "pool_get_iteration_totals")
+        return tuple(int(x) for x in out)
+
     def get_state(self, sid: int) -> State:
         n = self.counts[sid]
         pos = np.empty((n, 2), np.float32)

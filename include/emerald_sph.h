@@ -396,8 +396,12 @@ emsph_status emsph_pool_commit(emsph_pool* pool);
 int emsph_pool_live(const emsph_pool* pool, int* ids, int max);
 emsph_status emsph_pool_set_cfl(emsph_pool* pool, float factor);
 emsph_status emsph_pool_set_tuning(emsph_pool* pool, const emsph_tuning* tuning);
-/* One sub-step of every live scene; stats: one per live scene (may be NULL). */
+/* One sub-step of every live scene; stats: one per live scene (may be NULL).
+ * A scene removed with emsph_pool_remove rides along until the commit in the
+ * sort and the neighbour build, but takes no solver iterations. */
 emsph_status emsph_pool_sub_step(emsph_pool* pool, emsph_step_stats* stats);
+/* As emsph_batch_get_iteration_totals, over the pool's sub-steps. */
+emsph_status emsph_pool_get_iteration_totals(emsph_pool* pool, int reset, int64_t* out);
 emsph_status emsph_pool_get_state(emsph_pool* pool, int id, float* pos_xy, float* vel_xy);
 /* Device bytes currently held by the library (all contexts, batches, pools). */
 int64_t emsph_device_bytes(void);

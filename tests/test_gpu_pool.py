@@ -97,3 +97,35 @@ def test_pool_capacity(gpu):
         assert pool.sub_step() == []
     finally:
         pool.close()
+
+
+@pytest.mark.parametrize("method", ["iisph", "dfsph_p"])
+def test_removed_scene_takes_no_iterations_until_commit(gpu, method):
+    # A scene removed but not yet committed rides along in the sort and the
+    # neighbour build but takes no solver iterations; the other scenes'
+    # results stay those of each scene alone.
+    scs = _scenes()
+    cfgs = [scenes.config_for(sc, method) for sc in scs]
+    pool = sb.Pool(40_000, 60_000, 8, method)
+    try:
+        ids = [pool.add(cfgs[i], sb.Solid_state(scs[i].solids),
+                        sb.State(scs[i].positions, scs[i].velocities)) for i in range(3)]
+        pool.commit()
+        for _ in range(4):
+            pool.sub_step()
+        pool.remove(ids[1])
+        assert pool.live() == ids  # staged: still live until the commit
+        for _ in range(3):
+            st = pool.sub_step()
+            assert st[1].iterations == 0 and st[1].divergence_iterations == 0
+            assert st[0].iterations > 0 and st[2].iterations > 0
+        pool.commit()
+        assert pool.live() == [ids[0], ids[2]]
+        pool.sub_step()
+        for i in (0, 2):
+            want, _ = _alone(scs[i], cfgs[i], method, 8)
+            got = pool.get_state(ids[i])
+            assert_bits_equal(got.positions, want.positions, f"scene {i} positions")
+            assert_bits_equal(got.velocities, want.velocities, f"scene {i} velocities")
+    finally:
+        pool.close()
