@@ -131,7 +131,8 @@ class _PoolLoop:
                 self.pending = None
                 staged = True
             # Commits (which repack the pool) at most every commit_interval;
-            # retired scenes keep stepping, unaccounted, until then.
+            # until then removed scenes ride along in the sort and the
+            # neighbour build, unaccounted, but take no solver iterations.
             idle = len(self.live) == 0 or not pool.live()
             if (staged or self.dirty) and (idle or now - last_commit >= self.commit_interval):
                 tb = time.perf_counter()
