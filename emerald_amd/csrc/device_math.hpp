@@ -90,6 +90,11 @@ struct KernelConsts {
     float four_h;  // 4 * h
     float w0;      // W(0, h)
     double inv_h, inv_k, inv_hh, inv_hhh, inv_four_h;  // RN64(1 / x)
+    // Upper end of the distances r in [2^-48, r_fast_hi] for which every
+    // quotient of kernel_W_GradW_r is normal or zero (so div_by_recip's fast
+    // path is exact without per-quotient tests): 4h when h lies in
+    // [2^-20, 2^10], else -1 (no fast range). See kernel_W_GradW_r.
+    float r_fast_hi;
 };
 
 // sph_common/kernels.h:16-35 W(r, h). The two pieces' final divides by k are
@@ -140,40 +145,40 @@ __device__ __forceinline__ V2 kernel_GradW_r(V2 dP, float r, KernelConsts const&
     return V2{0.0f, 0.0f};
 }
 
-// div_by_recip's fast path: RN32(RN64(a * inv)), clearing ok where
-// div_by_recip would take the IEEE divide. A zero quotient (a = 0, or an
-// infinite divisor) is exact and stays on the fast path: a / b is then the
-// same signed zero.
-__device__ __forceinline__ float div_recip_checked(float a, double inv, bool& ok) {
-    double const q = static_cast<double>(a) * inv;
-    ok = ok && (__builtin_fabs(q) >= 0x1p-126 || q == 0.0);
-    return static_cast<float>(q);
-}
-
 // W(r) and GradW(dP) of one neighbour, r = |dP| (r == imath_length(dP) bit
 // for bit): the values of kernel_W(r) and kernel_GradW_r(dP, r), bit for bit,
-// with q = r / h computed once, the pieces' operands selected instead of
-// branched on, and one test for the rare IEEE-divide cases of div_by_recip
-// (then both are evaluated by those functions).
+// with q = r / h computed once and the pieces' operands selected instead of
+// branched on. For r in [2^-48, r_fast_hi] (h in [2^-20, 2^10]) every
+// quotient is normal or exactly zero, so div_by_recip's product is exact
+// without its per-quotient test:
+//   r / h >= 2^-48 / 2^10;  wa / k >= 2^-2 / (4 * 2^30) (wa >= 1/4 for q <= 1);
+//   wb / k >= 2^-71 / 2^32 or 0 (t = 2 - q is exact, a multiple of 2^-23 for
+//   q in [1, 4], so t^3 / 4 >= 2^-71 unless t = 0);
+//   3 r / h^2 >= 3 * 2^-48 / 2^20;  3 t^2 / 4h >= 2^-56 or 0;
+//   2.25 r^2 / h^3 >= 2.25 * 2^-96 / 2^30 > 2^-126;
+//   x / k with |x| >= 0.74 r / h^2 for q <= 1 (-3 + 2.25 q <= -0.75), and
+//   x = -3 t^2 / 4h otherwise: >= 2^-100 or 0;
+// and r <= 4h keeps every quotient below 2^84. Other distances take
+// kernel_W and kernel_GradW_r, whose divides are tested one by one.
 __device__ __forceinline__ void kernel_W_GradW_r(V2 dP, float r, KernelConsts const& c,
                                                  float& w, V2& gw) {
-    bool ok = true;
-    float const q = div_recip_checked(r, c.inv_h, ok);  // r / h
-    bool const inner = q <= 1.0f;
-    float const t = 2.0f - q;
-    float const wa = 1.0f + q * q * (q * 0.75f - 1.5f);
-    float const wb = (t * t * t) / 4.0f;
-    float const wk = div_recip_checked(inner ? wa : wb, c.inv_k, ok);  // / k
-    float const num = inner ? -3.0f * r : -3.0f * (t * t);
-    float x = div_recip_checked(num, inner ? c.inv_hh : c.inv_four_h, ok);  // / h^2 or / 4h
-    float const x2 = div_recip_checked(2.25f * r * r, c.inv_hhh, ok);      // 2.25 r^2 / h^3
-    x = inner ? x + x2 : x;
-    float const dwk = div_recip_checked(x, c.inv_k, ok);  // / k
-    if (!ok) {
+    if (!(r >= 0x1p-48f && r <= c.r_fast_hi)) {
         w = kernel_W(r, c);
         gw = kernel_GradW_r(dP, r, c);
         return;
     }
+    float const q = static_cast<float>(static_cast<double>(r) * c.inv_h);  // r / h
+    bool const inner = q <= 1.0f;
+    float const t = 2.0f - q;
+    float const wa = 1.0f + q * q * (q * 0.75f - 1.5f);
+    float const wb = (t * t * t) / 4.0f;
+    float const wk = static_cast<float>(static_cast<double>(inner ? wa : wb) * c.inv_k);  // / k
+    float const num = inner ? -3.0f * r : -3.0f * (t * t);
+    float x = static_cast<float>(static_cast<double>(num) *
+                                 (inner ? c.inv_hh : c.inv_four_h));  // / h^2 or / 4h
+    float const x2 = static_cast<float>(static_cast<double>(2.25f * r * r) * c.inv_hhh);  // 2.25 r^2 / h^3
+    x = inner ? x + x2 : x;
+    float const dwk = static_cast<float>(static_cast<double>(x) * c.inv_k);  // / k
     bool const in_support = q <= 2.0f;
     w = in_support ? wk : 0.0f;
     float const dw = in_support ? dwk : 0.0f;

@@ -92,6 +92,7 @@ KernelConsts make_kernel_consts(float h) {
     c.inv_hh = 1.0 / static_cast<double>(c.hh);
     c.inv_hhh = 1.0 / static_cast<double>(c.hhh);
     c.inv_four_h = 1.0 / static_cast<double>(c.four_h);
+    c.r_fast_hi = (h >= 0x1p-20f && h <= 0x1p10f) ? 4.0f * h : -1.0f;
     return c;
 }
 

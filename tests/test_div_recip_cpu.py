@@ -61,3 +61,80 @@ def test_kernel_constants_dense():
             assert _same_bits(_recip_div(a, b), a / b).all(), float(d)
             w = (rng.standard_normal(1_000_000) * 4.0).astype(np.float32)
             assert _same_bits(_recip_div(w, b), w / b).all(), float(d)
+
+
+def _kernel_consts(h):
+    h = np.float32(h)
+    pi = np.float32(3.14159274101257324218750)
+    hh = h * h
+    hhh = h * h * h
+    k = h * h * h * pi
+    four_h = np.float32(4.0) * h
+    return h, hh, hhh, k, four_h
+
+
+def _fast_and_ieee(r, h):
+    """The five quotients of kernel_W_GradW_r: by the unchecked reciprocal
+    product (the fast path for r in [2^-48, 4h]) and by IEEE divides."""
+    h, hh, hhh, k, four_h = _kernel_consts(h)
+    inv = lambda b: np.float64(1.0) / np.float64(b)
+    f32 = np.float32
+    prod = lambda a, b: a.astype(np.float64) * inv(b)
+    fast = lambda a, b: prod(a, b).astype(np.float32)
+    out = {}
+    for name, rc in (("fast", fast), ("ieee", lambda a, b: a / b)):
+        q = rc(r, h)
+        inner = q <= f32(1.0)
+        t = f32(2.0) - q
+        wa = f32(1.0) + q * q * (q * f32(0.75) - f32(1.5))
+        wb = (t * t * t) / f32(4.0)
+        wsel = np.where(inner, wa, wb).astype(np.float32)
+        wk = rc(wsel, k)
+        num = np.where(inner, f32(-3.0) * r, f32(-3.0) * (t * t)).astype(np.float32)
+        x_in = rc(num, hh)
+        x_out = rc(num, four_h)
+        x2 = rc(f32(2.25) * r * r, hhh)
+        x = np.where(inner, x_in + x2, x_out).astype(np.float32)
+        dwk = rc(x, k)
+        out[name] = (q, wk, x_in, x_out, x2, dwk)
+        if name == "fast":
+            # Every product that the fast path rounds is normal or zero.
+            for a, b in ((r, h), (wsel, k), (num, hh), (num, four_h), (f32(2.25) * r * r, hhh),
+                         (x, k)):
+                p = np.abs(prod(a, b))
+                sel = np.ones(len(r), bool)
+                if b is hh:
+                    sel = inner
+                if b is four_h:
+                    sel = ~inner
+                if b is hhh:
+                    sel = inner
+                assert ((p[sel] >= 2.0 ** -126) | (p[sel] == 0.0)).all()
+                assert (p[sel] < 2.0 ** 84).all()
+    return out
+
+
+def test_kernel_fast_range_needs_no_per_quotient_test():
+    # kernel_W_GradW_r skips div_by_recip's per-quotient test for
+    # r in [2^-48, 4h] when h lies in [2^-20, 2^10]: there every quotient is
+    # normal or zero, and the reciprocal product equals the IEEE divide.
+    rng = np.random.default_rng(11)
+    hs = [2.0 ** -20, 2.0 ** 10, 0.005, 1.0 / 2560.0, 0.8 / 1000.0, 1.0]
+    hs += list(2.0 ** rng.uniform(-20, 10, 10))
+    with np.errstate(all="ignore"):
+        for h in hs:
+            hf = np.float32(h)
+            lo, hi = 2.0 ** -48, float(np.float32(4.0) * hf)
+            r = np.exp(rng.uniform(np.log(lo), np.log(hi), 400_000)).astype(np.float32)
+            # the range's ends, the piece boundaries q = 1 and q = 2, and
+            # q just above 1 and 2 (t = 2 - q at its smallest nonzero values)
+            edges = [lo, hi, float(hf), 2.0 * float(hf)]
+            for e in edges:
+                e32 = np.float32(e)
+                r = np.concatenate([r, np.nextafter(e32, np.float32(0)) * np.ones(1, np.float32),
+                                    e32 * np.ones(1, np.float32),
+                                    np.nextafter(e32, np.float32(np.inf)) * np.ones(1, np.float32)])
+            r = r[(r >= np.float32(lo)) & (r <= np.float32(hi))]
+            res = _fast_and_ieee(r, h)
+            for a, b in zip(res["fast"], res["ieee"]):
+                assert _same_bits(a, b).all(), h
