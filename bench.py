@@ -220,10 +220,15 @@ def cpu_leg(scene, cfg, method, state, gpu_after, cpu, per_run: int = 7, runs: i
 
 
 def solver_window(method: str, steps: int, warmup: int, advance: int, cpu: dict | None,
-                  autotune: bool, kernel_timing: bool, rest_window: bool) -> dict:
+                  autotune: bool, kernel_timing: bool, rest_window: bool,
+                  save_state: str | None = None, from_state: str | None = None) -> dict:
     """One solver at 1M: warm-up, optionally the rest window, the advance to the
     collapse state, the timed collapse window, a kernel-timing pass, and the
-    CPU leg from the same collapse state."""
+    CPU leg from the same collapse state. save_state writes the state at the
+    start of the timed window (an .npz: positions, velocities, sub-step);
+    from_state starts from such a file instead of the initial lattice (the
+    warm-up sub-steps then run from it, and there is no rest window or
+    advance), so that a profiled run can cover the collapse window alone."""
     from emerald_amd import scenes
     from emerald_amd import sph2d_box as sb
     log(f"{method}: building the 1024 x 1024 lattice dam break")
@@ -231,12 +236,23 @@ def solver_window(method: str, steps: int, warmup: int, advance: int, cpu: dict 
     cfg = scenes.config_for(scene, method)
     n = len(scene.positions)
     ctx = sb.Context(cfg, sb.Solid_state(scene.solids), n)
-    ctx.set_state(sb.State(scene.positions, scene.velocities))
+    done = 0
+    if from_state:
+        with np.load(from_state) as f:
+            pos, vel, done = f["positions"], f["velocities"], int(f["sub_step"])
+        if pos.shape != scene.positions.shape or vel.shape != scene.velocities.shape:
+            raise SystemExit(f"bench: {from_state} does not hold a state of this scene")
+        log(f"{method}: starting from {from_state} (sub-step {done})")
+        ctx.set_state(sb.State(pos, vel))
+        rest_window = False
+        advance = 0
+    else:
+        ctx.set_state(sb.State(scene.positions, scene.velocities))
     ctx.frame_init(method)
     out = {"method": method, "particles": n, "solids": int(len(scene.solids)), "dt": scene.dt}
     log(f"{method}: warm-up {warmup} sub-steps")
     ctx.run_sub_steps(method, scene.dt, warmup)
-    done = warmup
+    done += warmup
     if autotune:
         log(f"{method}: autotune")
         chosen, cand_ms = ctx.autotune(method, scene.dt, 5)
@@ -258,6 +274,10 @@ def solver_window(method: str, steps: int, warmup: int, advance: int, cpu: dict 
         ctx.run_sub_steps(method, scene.dt, advance - done)
         done = advance
     start = ctx.get_state()
+    if save_state:
+        np.savez(save_state, positions=start.positions, velocities=start.velocities,
+                 sub_step=np.int64(done))
+        log(f"{method}: state at sub-step {done} saved to {save_state}")
     ctx.frame_init(method)
     log(f"{method}: timing {steps} sub-steps from sub-step {done}")
     ctx.iteration_totals(reset=True)
@@ -671,6 +691,12 @@ def main() -> int:
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--search-only", action="store_true")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--save-state", default=None,
+                    help="write the headline solver's state at the start of its timed window "
+                         "(.npz)")
+    ap.add_argument("--from-state", default=None,
+                    help="start the headline solver from a state written by --save-state "
+                         "(no rest window or advance)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark runs on one GPU only; --gpus above 1 is "
@@ -694,7 +720,8 @@ def main() -> int:
         return 0
 
     head = solver_window(method, args.steps, args.warmup, advance_for(method, args.advance), cpu,
-                         autotune=True, kernel_timing=True, rest_window=True)
+                         autotune=True, kernel_timing=True, rest_window=True,
+                         save_state=args.save_state, from_state=args.from_state)
     n = head["particles"]
     roofline, kernels = roofline_of(method, head.pop("_timing"), head.pop("_timing_steps"), n)
     col = head["collapse"]

@@ -735,64 +735,77 @@ __global__ __launch_bounds__(256) void k_cell_windows(WindowArgs w) {
 // written only when some neighbour cell holds solids (the build reads them
 // only then: WIN_SOLID_NEAR).
 __global__ __launch_bounds__(256) void k_cell_windows_dense(WindowArgs w) {
+    // Grid-stride over blocks of 256 cells; each block's 256 x 9 ranges are
+    // staged in LDS and written out as one contiguous run.
+    __shared__ uint2 sw[256 * 9];
     uint32_t const n = *w.self.count;
-    uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c == 0 && w.report) { *w.report = n; }
-    if (c >= n) { return; }
-    uint64_t const key = w.self.key[c];
-    uint32_t const tag = w.self.tag[c];
-    DenseScene const ds = w.dscene[tag];
-    uint32_t const gx0 = compact_bits_2(key);
-    uint32_t const gy0 = compact_bits_2(key >> 1);
-    uint2 r[9];
-    uint64_t nks[9];
-    uint64_t bins[9];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && w.report) { *w.report = n; }
+    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+        uint32_t const c = base + threadIdx.x;
+        uint2 r[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        // neighborhood.h:233-243: j outer, i inner, each from g - 1 to g + 1.
-        uint32_t const gx = gx0 + static_cast<uint32_t>(k % 3) - 1u;
-        uint32_t const gy = gy0 + static_cast<uint32_t>(k / 3) - 1u;
-        uint64_t const nk = morton2(static_cast<int32_t>(gx), static_cast<int32_t>(gy));
-        uint64_t const bin = dense_bin(ds, nk, w.dense_bins);
-        nks[k] = nk;
-        bins[k] = bin;
-        r[k] = make_uint2(0u, 0u);
-        if (bin < w.dense_bins) {
-            uint32_t const b = w.doffset[bin];
-            uint32_t const e = w.doffset[bin + 1];
-            if (e > b) { r[k] = make_uint2(b, e); }
-        }
-    }
-    if (w.owin) {
-        uint2 o[9];
-        bool any = false;
+        for (int k = 0; k < 9; ++k) { r[k] = make_uint2(0u, 0u); }
+        if (c < n) {
+            uint64_t const key = w.self.key[c];
+            uint32_t const tag = w.self.tag[c];
+            DenseScene const ds = w.dscene[tag];
+            uint32_t const gx0 = compact_bits_2(key);
+            uint32_t const gy0 = compact_bits_2(key >> 1);
+            uint64_t nks[9];
+            uint64_t bins[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            uint64_t const nk = nks[k];
-            uint64_t const bin = bins[k];
-            bool const maybe = w.other_set.slots &&
-                               (bin >= w.dense_bins || !w.other_bits ||
-                                ((w.other_bits[bin >> 5] >> (bin & 31u)) & 1u));
-            o[k] = maybe ? cell_set_range(w.other_set, w.other, nk, tag) : make_uint2(0u, 0u);
-            if (o[k].y > o[k].x) {
-                r[k].y |= WIN_SOLID_NEAR;
-                any = true;
+            for (int k = 0; k < 9; ++k) {
+                // neighborhood.h:233-243: j outer, i inner, each from g - 1 to g + 1.
+                uint32_t const gx = gx0 + static_cast<uint32_t>(k % 3) - 1u;
+                uint32_t const gy = gy0 + static_cast<uint32_t>(k / 3) - 1u;
+                uint64_t const nk = morton2(static_cast<int32_t>(gx), static_cast<int32_t>(gy));
+                uint64_t const bin = dense_bin(ds, nk, w.dense_bins);
+                nks[k] = nk;
+                bins[k] = bin;
+                if (bin < w.dense_bins) {
+                    uint32_t const b = w.doffset[bin];
+                    uint32_t const e = w.doffset[bin + 1];
+                    if (e > b) { r[k] = make_uint2(b, e); }
+                }
+            }
+            if (w.owin) {
+                uint2 o[9];
+                bool any = false;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    uint64_t const nk = nks[k];
+                    uint64_t const bin = bins[k];
+                    bool const maybe = w.other_set.slots &&
+                                       (bin >= w.dense_bins || !w.other_bits ||
+                                        ((w.other_bits[bin >> 5] >> (bin & 31u)) & 1u));
+                    o[k] = maybe ? cell_set_range(w.other_set, w.other, nk, tag) : make_uint2(0u, 0u);
+                    if (o[k].y > o[k].x) {
+                        r[k].y |= WIN_SOLID_NEAR;
+                        any = true;
+                    }
+                }
+                if (any) {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) { w.owin[static_cast<size_t>(c) * 9 + k] = o[k]; }
+                }
             }
         }
-        if (any) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) { w.owin[static_cast<size_t>(c) * 9 + k] = o[k]; }
-        }
+        for (int k = 0; k < 9; ++k) { sw[threadIdx.x * 9 + k] = r[k]; }
+        __syncthreads();
+        uint32_t const cells = (n - base) < 256u ? (n - base) : 256u;
+        uint2* const out = w.win + static_cast<size_t>(base) * 9;
+        for (uint32_t i = threadIdx.x; i < cells * 9u; i += 256u) { out[i] = sw[i]; }
+        __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) { w.win[static_cast<size_t>(c) * 9 + k] = r[k]; }
 }
 
 void launch_cell_windows(WindowArgs const& w, hipStream_t s) {
     if (!w.max_cells) { return; }
     if (w.dense_bins) {
-        hipLaunchKernelGGL(k_cell_windows_dense, dim3((w.max_cells + 255) / 256), dim3(256), 0, s,
-                           w);
+        uint32_t blocks = (w.max_cells + 255) / 256;
+        if (blocks > 1024u) { blocks = 1024u; }
+        hipLaunchKernelGGL(k_cell_windows_dense, dim3(blocks), dim3(256), 0, s, w);
         return;
     }
     size_t blocks = (static_cast<size_t>(w.max_cells) * 9 + 255) / 256;
