@@ -153,3 +153,25 @@ def test_full_size_search_sortedness(gpu, n):
     assert np.array_equal(r["counts"], oc)
     mask = np.arange(64)[None, :] < oc[:, None]
     assert np.array_equal(np.where(mask, r["indices"], 0), np.where(mask, oi, 0))
+
+
+def test_repeatability(gpu):
+    """test_repeatability.cpp:53-112: grid coordinates, keys, sorted pairs,
+    blocks and neighbour indices identical over repeated runs (here 20 runs of
+    one search object, the first with the exact sort and list capacity unknown,
+    the later ones on the dense index and the learned capacity)."""
+    xy = scenes.search_points(100_000, seed=11)
+    s = NeighborSearch(len(xy))
+    s.set_points(xy)
+    first = None
+    for k in range(20):
+        s.run(1.0 / 2048.0)
+        r = s.results()
+        mask = np.arange(64)[None, :] < r["counts"][:, None]
+        r["indices"] = np.where(mask, r["indices"], 0)
+        if first is None:
+            first = r
+            continue
+        for key, v in first.items():
+            assert np.array_equal(np.asarray(r[key]), np.asarray(v)), f"run {k}: {key}"
+    s.close()

@@ -106,6 +106,48 @@ def test_near_coincident_particles(gpu, method):
     _check_step(method, ctx, orc, 3)
 
 
+def sparse_edge_dam_break():
+    """The small dam break plus particles on the reference's edge paths
+    (neighborhood.h:198-282, kernels.h:57-64, iisph_ops.cpp:376-379): four
+    isolated particles (empty lists, so a_ii = 0 and the pressure is kept),
+    two at one position (r = 0: W(0) counts, GradW is 0), two exactly on cell
+    boundaries, and pairs one cell width apart (the acceptance test
+    distance < cut-off at its edge) and just inside it."""
+    sc = small_dam_break()
+    cut = np.float32(0.01)  # 2h for spacing 0.005
+    extra = np.array([
+        [0.62, 0.61], [0.71, 0.47], [0.83, 0.79], [0.55, 0.88],   # isolated
+        [0.4, 0.45], [0.4, 0.45],                                 # coincident
+        [0.5, 0.3], [0.5, 0.31],                                  # on cell boundaries
+        [0.3, 0.7], [0.3 + cut, 0.7],                             # one cut-off apart
+        [0.2, 0.6], [np.nextafter(np.float32(0.2) + cut, np.float32(0)), 0.6],
+    ], np.float32)
+    sc.positions = np.concatenate([sc.positions, extra]).astype(np.float32)
+    sc.velocities = np.zeros_like(sc.positions)
+    return sc
+
+
+@pytest.mark.parametrize("method", ALL)
+def test_isolated_and_edge_particles(gpu, method):
+    scene = sparse_edge_dam_break()
+    cfg = scenes.config_for(scene, method)
+    ctx = sb.Context(cfg, sb.Solid_state(scene.solids), len(scene.positions))
+    ctx.set_state(sb.State(scene.positions, scene.velocities))
+    orc = oracle_scene(scene, cfg)
+    ctx.frame_init(method)
+    orc.frame_init(method)
+    print(f"[solver parity] {method} isolated and edge particles n={len(scene.positions)}",
+          flush=True)
+    for k in range(4):
+        st = ctx.sub_step(method, k * scene.dt, scene.dt)
+        ost = orc.sub_step(method, k * scene.dt, scene.dt)
+        assert (st.iterations, st.divergence_iterations) == \
+            (ost.iterations, ost.divergence_iterations), f"step {k}"
+    _check_step(method, ctx, orc, 4)
+    gc, _, _, _ = ctx.neighborhood(0)
+    assert (gc[-12:-8] == 0).all()  # the isolated particles have no neighbours
+
+
 @pytest.mark.parametrize("method", ALL)
 def test_many_sub_steps_bit_exact(gpu, method):
     scene, cfg, ctx, orc = _make(method, nx=30, ny=50, seed=9)
