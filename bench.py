@@ -34,6 +34,7 @@ compared bit for bit).
                   [--method iisph|iisph_ap|iisph_pseudo_ap|dfsph_p|pcisph]
                   [--no-cpu-baseline] [--no-solvers] [--no-batched] [--no-served]
                   [--no-search] [--no-small]
+                  [--batched-only | --served-only | --search-only | --small-only]
 """
 
 from __future__ import annotations
@@ -498,11 +499,11 @@ def search_form(cpu: dict | None, runs: int = 5, n: int = 4_194_304) -> dict:
     return out
 
 
-def small_form(cpu: dict, steps: int = 1000, cpu_budget: float = 20.0) -> dict:
+def small_form(cpu: dict | None, steps: int = 1000, cpu_budget: float = 20.0) -> dict:
     """The reference's own CPU-runnable case: the 12,800-particle IISPH dam
     break (80 x 160 lattice, spacing 0.005, dt 2e-4), 1000 sub-steps on the GPU
-    and the same sub-steps in the oracle on the host cores, with the final
-    states compared bit for bit."""
+    and the same sub-steps in the oracle on the host cores (cpu None: the GPU
+    alone), with the final states compared bit for bit."""
     import oracle
     from emerald_amd import scenes
     from emerald_amd import sph2d_box as sb
@@ -522,6 +523,18 @@ def small_form(cpu: dict, steps: int = 1000, cpu_budget: float = 20.0) -> dict:
     _, iters, _, _ = ctx.iteration_totals(reset=True)
     gst = ctx.get_state()
     ctx.close()
+    gpu = {
+        "workload": "iisph dam break, 80x160 lattice (12,800 particles), spacing 0.005, "
+                    "dt 2e-4, hex walls",
+        "sub_steps": steps,
+        "value": n * steps / wall,
+        "unit": "particle-steps/s",
+        "ms_per_sub_step": wall * 1e3 / steps,
+        "device_ms_per_sub_step": dev_ms / steps,
+        "mean_iterations": iters / steps,
+    }
+    if cpu is None:
+        return gpu
     threads = cpu["threads"]
     oracle.set_threads(threads)
     o = oracle_scene(scene, cfg)
@@ -539,14 +552,7 @@ def small_form(cpu: dict, steps: int = 1000, cpu_budget: float = 20.0) -> dict:
         pos, vel = o.state_f32()
         same = _bits_equal(pos, vel, gst.positions, gst.velocities)
     return {
-        "workload": "iisph dam break, 80x160 lattice (12,800 particles), spacing 0.005, "
-                    "dt 2e-4, hex walls",
-        "sub_steps": steps,
-        "value": n * steps / wall,
-        "unit": "particle-steps/s",
-        "ms_per_sub_step": wall * 1e3 / steps,
-        "device_ms_per_sub_step": dev_ms / steps,
-        "mean_iterations": iters / steps,
+        **gpu,
         "cpu_value": n * done / cwall,
         "cpu_cores": threads,
         "cpu_sub_steps": done,
@@ -691,6 +697,7 @@ def main() -> int:
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--search-only", action="store_true")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--small-only", action="store_true")
     ap.add_argument("--save-state", default=None,
                     help="write the headline solver's state at the start of its timed window "
                          "(.npz)")
@@ -717,6 +724,9 @@ def main() -> int:
         return 0
     if args.search_only:
         print(json.dumps(search_form(cpu)), flush=True)
+        return 0
+    if args.small_only:
+        print(json.dumps(small_form(cpu)), flush=True)
         return 0
 
     head = solver_window(method, args.steps, args.warmup, advance_for(method, args.advance), cpu,

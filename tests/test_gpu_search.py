@@ -80,6 +80,22 @@ def test_list_capacity_switches(gpu):
         s.close()
 
 
+def test_mostly_long_lists_take_the_full_build(gpu):
+    # Nearly every list is capped at 64 in scan order: the first run defers
+    # them from the 32-entry build to the full-capacity pass, the next runs
+    # build every list at the full 64-entry capacity.
+    rng = np.random.default_rng(13)
+    xy = (0.5 + 0.01 * rng.standard_normal((12000, 2))).astype(np.float32)
+    s = NeighborSearch(len(xy))
+    try:
+        for _ in range(3):
+            g = _compare(xy, 1.0 / 256.0, s=s)
+        c = g["counts"]
+        assert (c > 32).mean() > 0.5 and c.max() == 64
+    finally:
+        s.close()
+
+
 def test_few_long_lists_deferred_at_cap_16(gpu):
     # Sparse points with a small tight cluster: the cluster's lists (above 16
     # entries) are under 1/64 of the particles, so the second run keeps the
