@@ -54,6 +54,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# Skin candidate lists in the 1M solver windows (--skin-lists on|off).
+SKIN_LISTS = False
+
 # Algorithmic HBM bytes per fluid particle for one launch of each timed kernel
 # class, per solver. Conventions: FP32 SoA; every per-particle field read or
 # written once per sweep (neighbour gathers hit cache after the Morton sort);
@@ -237,6 +240,7 @@ def solver_window(method: str, steps: int, warmup: int, advance: int, cpu: dict 
     cfg = scenes.config_for(scene, method)
     n = len(scene.positions)
     ctx = sb.Context(cfg, sb.Solid_state(scene.solids), n)
+    ctx.set_skin_lists(SKIN_LISTS)
     done = 0
     if from_state:
         with np.load(from_state) as f:
@@ -282,9 +286,11 @@ def solver_window(method: str, steps: int, warmup: int, advance: int, cpu: dict 
     ctx.frame_init(method)
     log(f"{method}: timing {steps} sub-steps from sub-step {done}")
     ctx.iteration_totals(reset=True)
+    skin0 = ctx.skin_lists()
     t0 = time.perf_counter()
     dev_ms, st = ctx.run_sub_steps(method, scene.dt, steps)
     wall = time.perf_counter() - t0
+    skin1 = ctx.skin_lists()
     _, it, dit, redone = ctx.iteration_totals(reset=True)
     k_mean = it / steps
     kdiv_mean = dit / steps
@@ -296,6 +302,8 @@ def solver_window(method: str, steps: int, warmup: int, advance: int, cpu: dict 
         "value": value, "ms_per_step": wall * 1e3 / steps, "device_ms_per_step": dev_ms / steps,
         "mean_iterations": k_mean, "mean_divergence_iterations": kdiv_mean,
         "sub_steps_redone": int(redone),
+        "skin_lists": {"on": skin1[0], "rebuilds": skin1[1] - skin0[1],
+                       "reuses": skin1[2] - skin0[2]},
         "last_iterations": int(st.iterations), "sort_passes": int(st.sort_passes),
         "cells": int(st.cell_count), "max_neighbors": int(st.max_neighbors),
         "algorithmic_bytes_per_particle_step": step_bytes(method, k_mean, kdiv_mean),
@@ -698,6 +706,8 @@ def main() -> int:
     ap.add_argument("--search-only", action="store_true")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--small-only", action="store_true")
+    ap.add_argument("--skin-lists", choices=["on", "off"], default="off",
+                    help="skin candidate lists in the 1M solver windows")
     ap.add_argument("--save-state", default=None,
                     help="write the headline solver's state at the start of its timed window "
                          "(.npz)")
@@ -705,6 +715,8 @@ def main() -> int:
                     help="start the headline solver from a state written by --save-state "
                          "(no rest window or advance)")
     args = ap.parse_args()
+    global SKIN_LISTS
+    SKIN_LISTS = args.skin_lists == "on"
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark runs on one GPU only; --gpus above 1 is "
                                    "not supported", "n_gpus": args.gpus}))

@@ -195,6 +195,8 @@ SIGNATURES = {
     "emsph_run_sub_steps": (_ST, [_P, _I, _F, _I, _P, _P]),
     "emsph_set_cfl": (_ST, [_P, _F]),
     "emsph_set_graphs": (_ST, [_P, _I]),
+    "emsph_set_skin_lists": (_ST, [_P, _I]),
+    "emsph_get_skin_lists": (_ST, [_P, _P, _P, _P]),
     "emsph_get_tuning": (_ST, [_P, _P]),
     "emsph_set_tuning": (_ST, [_P, _P]),
     "emsph_autotune": (_ST, [_P, _I, _F, _I, _P, _P, _P]),
@@ -202,6 +204,8 @@ SIGNATURES = {
     "emsph_batch_autotune": (_ST, [_P, _I, _I, _P, _P, _P]),
     "emsph_pool_set_tuning": (_ST, [_P, _P]),
     "emsph_batch_set_graphs": (_ST, [_P, _I]),
+    "emsph_batch_set_skin_lists": (_ST, [_P, _I]),
+    "emsph_batch_get_skin_lists": (_ST, [_P, _P, _P, _P]),
     "emsph_batch_set_kernel_timing": (_ST, [_P, _I]),
     "emsph_batch_get_kernel_timing": (_I, [_P, _P, _P, _P, _I]),
     "emsph_batch_get_iteration_totals": (_ST, [_P, _I, _P]),

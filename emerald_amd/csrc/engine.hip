@@ -121,6 +121,17 @@ StepConsts step_consts(emsph_config const& cfg, float dt) {
     k.omega = p.iisph_omega;
     k.pci_denom = cfg.pressure_correction_denom;
     k.viscosity = p.viscosity;
+    // Skin lists (SkinView): candidates within cell + h, rounded up with a
+    // margin above the float |v|^2 test's rounding (3 roundings of 2^-24);
+    // reuse while |x - x_rebuild|^2 stays below (h/2)^2, rounded down with a
+    // margin for the rounding of the displacement and of this test. Then a
+    // point outside the candidates (exactly farther than cell + h at the
+    // rebuild) stays farther than cell + 2^-16 cell, where the reference's
+    // |v|^2 < cut2 fails.
+    double const rs = static_cast<double>(k.cell) + static_cast<double>(p.support);
+    k.skin_r2 = std::nextafter(static_cast<float>(rs * rs * (1.0 + 0x1p-18)), HUGE_VALF);
+    double const hl = 0.5 * static_cast<double>(p.support);
+    k.skin_lim2 = std::nextafter(static_cast<float>(hl * hl * (1.0 - 0x1p-12)), 0.0f);
     return k;
 }
 
@@ -450,6 +461,24 @@ struct Engine {
     void* user_data = nullptr;
     bool use_fext = false;
     DBuf<V2> fext;
+    // Skin candidate lists (SkinView): enabled by the caller; valid after a
+    // rebuild sub-step built them without overflow; a reuse sub-step takes its
+    // lists from them. skin_rho: sqrt of the largest displacement ratio seen
+    // by the last reuse (0 at a rebuild), skin_rho_prev the one before, from
+    // which the next sub-step's ratio is extrapolated.
+    bool skin_on = false;
+    bool skin_valid = false;
+    bool skin_reuse = false;    // this attempt reuses the candidates
+    bool skin_built = false;    // this attempt builds them
+    bool skin_seen = false;     // this attempt's skin flags were taken in
+    float skin_rho = 0.0f, skin_rho_prev = 0.0f;
+    int64_t skin_counts[2] = {0, 0};  // sub-steps that rebuilt / reused
+    int skin_run = 0;        // reuses since the last candidate build
+    int skin_pause = 0;      // sub-steps left without candidates (motion too fast to pay)
+    DBuf<uint32_t> skin_cf, skin_cs;
+    DBuf<uint8_t> skin_nf, skin_ns;
+    DBuf<V2> skin_xref;
+    DBuf<unsigned int> skin_tmax;
     float step_time = 0.0f;  // time of the current sub-step (for the callback)
     bool frame_on = false;   // a device-side frame loop is running (its time is on the device)
     // State::colors of a context: computed at the end of each frame step (or
@@ -662,6 +691,10 @@ Batch fluid_batch(Engine* g, float const* p_in = nullptr, float* p_out = nullptr
     b.store_geometry = g->store_geometry ? 1 : 0;
     b.nb = g->nb;
     b.xcd_remap = g->xcd_remap ? 1 : 0;
+    if (g->skin_on) {
+        b.skin = SkinView{g->skin_reuse ? 1 : 0, g->skin_cf.p, g->skin_nf.p, g->skin_cs.p,
+                          g->skin_ns.p, g->skin_xref.p, g->skin_tmax.p};
+    }
     return b;
 }
 
@@ -781,7 +814,8 @@ hipError_t ensure_geometry(Engine* g) {
 hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
                         uint32_t const* scene_of, uint32_t nscenes, int passes_hint,
                         IndexArgs gathers, bool time_it, SpatialIndex const* other = nullptr,
-                        uint2* owin = nullptr, bool use_dense = false) {
+                        uint2* owin = nullptr, bool use_dense = false,
+                        WindowArgs* wout = nullptr) {
     hipStream_t const s = g->stream;
     auto run = [&](char const* name, auto&& f) -> hipError_t {
         if (time_it) { return timed(g, name, f); }
@@ -883,6 +917,7 @@ hipError_t index_points(Engine* g, SpatialIndex& si, V2 const* pos, uint32_t n,
     w.win = si.win.p;
     w.owin = owin;
     w.report = &g->keystate.p->cells;
+    if (wout) { *wout = w; }
     return run("cells", [&] {
         if (!dense) {  // the dense path's cell scan gave the ordinals and the count
             launch_cell_heads(a, s);
@@ -951,6 +986,7 @@ emsph_status build_set_bits(Engine* g, SpatialIndex& si, SpatialIndex const& d) 
 // Solids of every scene: index, solid-solid lists with geometry, volumes
 // (solids.cpp:76-137).
 emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
+    g->skin_valid = false;
     g->NS = 0;
     for (uint32_t s = 0; s < g->M; ++s) {
         g->sh[s].soffset = g->NS;
@@ -1025,6 +1061,7 @@ emsph_status build_solids(Engine* g, std::vector<float const*> const& sxy) {
 // Configure the engine with M scenes; fluid capacity per scene, solids per scene.
 emsph_status engine_configure(Engine* g, std::vector<SceneHost> scenes,
                               std::vector<float const*> const& solid_xy) {
+    g->skin_valid = false;
     g->sh = std::move(scenes);
     g->M = static_cast<uint32_t>(g->sh.size());
     uint32_t off = 0;
@@ -1284,7 +1321,13 @@ int first_decision(Engine const* g, int kind, int first_check) {
 void begin_attempt(Engine* g) {
     g->prep_passes = g->passes_hint;
     g->prep_dense = dense_for(g);
-    g->keys_checked = g->passes_hint < 0;
+    // A reuse of the skin candidates has its short flag (a particle moved too
+    // far) checked like a sort's; a rebuild builds them when this attempt's
+    // index is the dense one (the same test as index_points').
+    SpatialIndex const& si = g->fidx;
+    g->skin_built = g->skin_on && g->skin_pause == 0 && !g->skin_reuse && g->prep_dense &&
+                    g->prep_passes >= 0 && si.dense_bins > 0 && si.dense_layout.size() == g->M;
+    g->keys_checked = g->passes_hint < 0 && !g->skin_reuse;
     g->sort_short = false;
     g->last_iterations = 0;
     g->last_div_iterations = 0;
@@ -1294,6 +1337,24 @@ void begin_attempt(Engine* g) {
 // work (it can be captured into a graph); otherwise the key range is read back
 // first.
 hipError_t enqueue_prepare(Engine* g, int build_mode, bool dfsph) {
+    if (g->skin_reuse) {
+        // The lists from the skin candidates: no keys, sort or cells; the
+        // state gathered into the last rebuild's order.
+        Batch b = fluid_batch(g);
+        auto run = [&](char const* name, auto&& f) -> hipError_t {
+            if (g->timer.enabled) { return timed(g, name, f); }
+            f();
+            return hipGetLastError();
+        };
+        HIP_PASS(run("skin_gather", [&] {
+            launch_skin_begin(b, g->stream);
+            launch_skin_gather(b, dfsph ? g->kv.p : nullptr, dfsph ? g->kd.p : nullptr,
+                               g->stream);
+        }));
+        return timed_build(g, [&](hipEvent_t const* ev) {
+            launch_build(build_mode, g->build_cap, b, g->stream, ev);
+        });
+    }
     IndexArgs ga{};
     ga.vel_in = g->vel.p;
     ga.pos_out = g->spos.p;
@@ -1308,17 +1369,31 @@ hipError_t enqueue_prepare(Engine* g, int build_mode, bool dfsph) {
     ga.step_scenes = g->scenes.p;
     ga.step_stats = g->stats.p;
     ga.step_nscenes = g->M;
+    WindowArgs w{};
     HIP_PASS(index_points(g, g->fidx, g->pos.p, g->N, g->scene_of.p, g->M, g->prep_passes, ga,
                           g->timer.enabled, g->NS ? &g->sidx : nullptr, g->swin.p,
-                          g->prep_dense));
-    return timed_build(g, [&](hipEvent_t const* ev) {
+                          g->prep_dense, &w));
+    HIP_PASS(timed_build(g, [&](hipEvent_t const* ev) {
         launch_build(build_mode, g->build_cap, b, g->stream, ev);
-    });
+    }));
+    if (g->skin_built && w.dense_bins && w.dscene) {
+        if (g->timer.enabled) {
+            return timed(g, "skin_candidates", [&] { launch_skin_candidates(b, w, g->stream); });
+        }
+        launch_skin_candidates(b, w, g->stream);
+        return hipGetLastError();
+    }
+    g->skin_built = false;  // (begin_attempt's test and index_points' agree)
+    return hipSuccess;
 }
 
 // Host-side fields of the fluid index that a sub-step's sort leaves (also
 // after a graph replay, which runs none of the host code).
 void finish_prepare(Engine* g) {
+    if (g->skin_reuse) {  // no indexing run: the last rebuild's index stays
+        g->have_lists = true;
+        return;
+    }
     SpatialIndex& si = g->fidx;
     if (g->prep_passes < 0) {
         // index_points read the key range back and sorted exactly with the
@@ -1353,7 +1428,8 @@ hipError_t segment(Engine* g, int method, int seg, F&& enqueue) {
     }
     std::array<uint64_t, 10> const key{
       static_cast<uint64_t>(method), static_cast<uint64_t>(seg),
-      static_cast<uint64_t>(g->prep_passes) + 256u * static_cast<uint64_t>(g->prep_dense),
+      static_cast<uint64_t>(g->prep_passes) + 256u * static_cast<uint64_t>(g->prep_dense) +
+        65536u * static_cast<uint64_t>(g->skin_reuse ? 2 : (g->skin_built ? 1 : 0)),
       static_cast<uint64_t>(g->build_cap), g->layout_gen,
       static_cast<uint64_t>(g->store_geometry), g->N, g->ntiles, static_cast<uint64_t>(g->nb),
       static_cast<uint64_t>(g->xcd_remap)};
@@ -1673,6 +1749,75 @@ emsph_status check_method(int method) {
                 "method " + std::to_string(method) + " is not implemented on the GPU path");
 }
 
+// Skin candidate lists before a sub-step's attempt: the buffers for the
+// engine's capacity (allocated here, outside any graph capture; new buffers
+// are a new layout and no candidates) and the choice: a
+// first attempt reuses valid candidates unless the displacement ratio,
+// extrapolated from the last two sub-steps, would reach 0.9 of the limit; a
+// redo always rebuilds.
+hipError_t skin_prepare(Engine* g, int attempt) {
+    g->skin_reuse = false;
+    g->skin_built = false;
+    g->skin_seen = false;
+    if (!g->skin_on) { return hipSuccess; }
+    if (g->skin_pause > 0) {
+        if (attempt == 0) { --g->skin_pause; }
+        g->skin_valid = false;
+        return hipSuccess;
+    }
+    void const* before[3] = {g->skin_cf.p, g->skin_cs.p, g->skin_tmax.p};
+    size_t const stride = g->stride;
+    HIP_PASS(g->skin_cf.ensure(static_cast<size_t>(SKIN_FLUID) * stride));
+    HIP_PASS(g->skin_cs.ensure(static_cast<size_t>(SKIN_SOLID) * stride));
+    HIP_PASS(g->skin_nf.ensure(stride));
+    HIP_PASS(g->skin_ns.ensure(stride));
+    HIP_PASS(g->skin_xref.ensure(g->cap));
+    HIP_PASS(g->skin_tmax.ensure(std::max<uint32_t>(g->ntiles, 1)));
+    if (before[0] != g->skin_cf.p || before[1] != g->skin_cs.p || before[2] != g->skin_tmax.p) {
+        ++g->layout_gen;
+        g->skin_valid = false;
+    }
+    float const next = 2.0f * g->skin_rho - g->skin_rho_prev;
+    g->skin_reuse = attempt == 0 && g->skin_valid && next < 0.9f;
+    if (attempt == 0 && !g->skin_reuse && g->skin_valid) {
+        // The candidates of this interval are spent: an interval of fewer than
+        // SKIN_MIN_RUN reuses does not repay the candidate build (one build
+        // costs about as much as SKIN_MIN_RUN reuses save), so the next
+        // SKIN_PAUSE sub-steps take the full path alone.
+        if (g->skin_run < SKIN_MIN_RUN) { g->skin_pause = SKIN_PAUSE; }
+        g->skin_valid = false;
+    }
+    return hipSuccess;
+}
+
+// After a sub-step that was not short: the candidates just built are valid
+// unless one overflowed (the engine then stops using them: its lists are too
+// crowded), a reuse records its displacement ratio, and a sub-step without
+// either (the radix path) leaves none.
+void skin_after(Engine* g) {
+    if (!g->skin_on) { return; }
+    if (g->skin_pause > 0 && !g->skin_reuse) {
+        g->skin_valid = false;
+        return;
+    }
+    if (g->skin_reuse) {
+        ++g->skin_counts[1];
+        ++g->skin_run;
+        g->skin_rho_prev = g->skin_rho;
+        g->skin_rho = std::sqrt(std::max(0.0f, __builtin_bit_cast(float, g->keys.skin_disp)));
+    } else if (g->skin_built) {
+        ++g->skin_counts[0];
+        bool const over = (g->keys.skin_flags & SKIN_OVERFLOW) != 0u;
+        g->skin_valid = !over;
+        if (over) { g->skin_on = false; }
+        g->skin_rho = 0.0f;
+        g->skin_rho_prev = 0.0f;
+        g->skin_run = 0;
+    } else {
+        g->skin_valid = false;
+    }
+}
+
 emsph_status engine_sub_step(Engine* g, int method) {
     ST_TRY(check_method(method));
     if (g->N == 0 || g->ntiles == 0) { return EMSPH_OK; }  // N = 0: the reference reads block_indices.back()
@@ -1685,9 +1830,15 @@ emsph_status engine_sub_step(Engine* g, int method) {
     // count (the second attempt sorts exactly and cannot be short).
     for (int attempt = 0; attempt < 2; ++attempt) {
         if (attempt == 1) {
+            // A reuse of the skin candidates whose list would have needed the
+            // reference's truncation at 64 entries turns them off; one whose
+            // particles moved too far is redone like a short sort, by the
+            // exact full path.
+            if (g->skin_reuse && (g->keys.skin_flags & SKIN_FULL)) { g->skin_on = false; }
             g->passes_hint = -1;
             g->totals[3] += 1;
         }
+        HIP_TRY(skin_prepare(g, attempt));
         switch (method) {
         case EMSPH_IISPH: ST_TRY(iisph_sub_step(g)); break;
         case EMSPH_IISPH_AP: ST_TRY(iisph_ap_sub_step(g, false)); break;
@@ -1697,6 +1848,7 @@ emsph_status engine_sub_step(Engine* g, int method) {
         }
         if (!g->sort_short) { break; }
     }
+    skin_after(g);
     g->last_method = method;
     g->totals[0] += 1;
     g->totals[1] += g->last_iterations;
@@ -1838,6 +1990,7 @@ emsph_status engine_set_state(Engine* g, uint32_t s, float const* pos_xy, float 
                               size_t count) {
     SceneHost& h = g->sh[s];
     if (count > h.cap) { return fail(EMSPH_ERR_CAPACITY, "fluid count above capacity"); }
+    g->skin_valid = false;  // new positions: the candidates are rebuilt
     if (count) {
         HIP_TRY(hipMemcpyAsync(g->pos.p + h.offset, pos_xy, count * sizeof(V2),
                                hipMemcpyHostToDevice, g->stream));
@@ -1944,6 +2097,7 @@ emsph_status pool_commit(emsph_pool* P) {
     Engine* g = &P->e;
     if (P->adds.empty() && P->removes.empty()) { return EMSPH_OK; }
     HIP_TRY(hipSetDevice(g->device));
+    g->skin_valid = false;
     hipStream_t const st = g->stream;
     std::vector<PoolScene> next;
     std::vector<uint3> seg;  // {src, dst, n} of survivors that move
@@ -2086,6 +2240,7 @@ hipError_t snapshot_take(Engine* g, Snapshot& sn) {
 }
 
 hipError_t snapshot_restore(Engine* g, Snapshot const& sn) {
+    g->skin_valid = false;
     if (g->N) {
         HIP_PASS(hipMemcpyAsync(g->pos.p, sn.pos.p, g->N * sizeof(V2), hipMemcpyDeviceToDevice,
                                 g->stream));
@@ -2215,6 +2370,21 @@ const char* emsph_build_info(void) {
 
 //------------------------------------------------------------------------------
 // Single scene: a batch of one.
+static void engine_set_skin(Engine* g, int enable) {
+    // Skin lists need the |v|^2 < cut2 form of the acceptance test (every
+    // scene's cut2 > 0, i.e. a cut-off above 1e-18).
+    bool ok = enable != 0;
+    for (auto const& h : g->sh) { ok = ok && step_consts(h.cfg, h.dt).cut2 > 0.0f; }
+    g->skin_on = ok;
+    g->skin_valid = false;
+}
+
+static void engine_get_skin(Engine const* g, int* enabled, int64_t* rebuilds, int64_t* reuses) {
+    if (enabled) { *enabled = g->skin_on ? 1 : 0; }
+    if (rebuilds) { *rebuilds = g->skin_counts[0]; }
+    if (reuses) { *reuses = g->skin_counts[1]; }
+}
+
 emsph_status emsph_create(const emsph_config* config, const float* solid_xy, size_t solid_count,
                           size_t fluid_capacity, emsph_context** out) {
     if (!config || !out || (solid_count && !solid_xy)) {
@@ -2241,6 +2411,7 @@ emsph_status emsph_create(const emsph_config* config, const float* solid_xy, siz
         delete c;
         return s;
     }
+    engine_set_skin(&c->e, 1);  // skin candidate lists on by default
     *out = c;
     return EMSPH_OK;
 }
@@ -2635,6 +2806,32 @@ emsph_status emsph_set_graphs(emsph_context* c, int enable) {
     return EMSPH_OK;
 }
 
+emsph_status emsph_set_skin_lists(emsph_context* c, int enable) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    engine_set_skin(&c->e, enable);
+    return EMSPH_OK;
+}
+
+emsph_status emsph_get_skin_lists(emsph_context* c, int* enabled, int64_t* rebuilds,
+                                  int64_t* reuses) {
+    if (!c) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null context"); }
+    engine_get_skin(&c->e, enabled, rebuilds, reuses);
+    return EMSPH_OK;
+}
+
+emsph_status emsph_batch_set_skin_lists(emsph_batch* b, int enable) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    engine_set_skin(&b->e, enable);
+    return EMSPH_OK;
+}
+
+emsph_status emsph_batch_get_skin_lists(emsph_batch* b, int* enabled, int64_t* rebuilds,
+                                        int64_t* reuses) {
+    if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
+    engine_get_skin(&b->e, enabled, rebuilds, reuses);
+    return EMSPH_OK;
+}
+
 emsph_status emsph_batch_set_graphs(emsph_batch* b, int enable) {
     if (!b) { return fail(EMSPH_ERR_INVALID_ARGUMENT, "null batch"); }
     b->e.use_graphs = enable != 0;
@@ -2736,6 +2933,7 @@ emsph_status emsph_batch_create(int n_scenes, const emsph_config* configs,
         delete b;
         return st;
     }
+    engine_set_skin(&b->e, 1);  // skin candidate lists on by default
     *out = b;
     return EMSPH_OK;
 }

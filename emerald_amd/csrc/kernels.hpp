@@ -37,6 +37,12 @@ struct StepConsts {
     float omega;         // IISPH relaxation
     float pci_denom;     // PCISPH pressure correction denominator
     float viscosity;
+    // Skin candidate lists (SkinView): a candidate is recorded at a rebuild
+    // when |v|^2 < skin_r2 (at least (cell + h)^2), and the lists serve while
+    // every particle's |x - x_rebuild|^2 < skin_lim2 (at most (h/2)^2 less a
+    // margin for the rounding of both tests; both set on the host).
+    float skin_r2;
+    float skin_lim2;
 };
 
 // Solver-loop exit thresholds of one scene (sph2d_box/parameters.h:22-33).
@@ -153,6 +159,8 @@ struct KeyState {
     unsigned int max_count;       // largest fluid or solid list of the last build
     unsigned int cells;           // cells of the last indexing run
     unsigned int long_lists;      // particles of the last build with a list above 16
+    unsigned int skin_flags;      // SKIN_* of the last candidate build or reuse
+    unsigned int skin_disp;       // float bits: max |x - x_rebuild|^2 / skin_lim2 of the last reuse
     int frame_remaining;          // scene 0's frame quanta left after this sub-step (-1: off)
     unsigned int dense_crowded;   // a cell of the dense index exceeded DENSE_CELL_MAX
     unsigned int dense_out;       // a key fell outside its scene's dense table (k_keys)
@@ -216,6 +224,39 @@ struct SolidView {
     float const* vol;
     uint32_t const* rank;   // original solid index -> sorted slot
     uint32_t const* orig;   // sorted slot -> original solid index
+};
+
+// Skin candidate lists (Verlet lists made exact): at a rebuild sub-step each
+// fluid particle records, from the 5x5 cells around its own, the fluid and
+// solid particles closer than its scene's cut-off plus h (StepConsts::skin_r2).
+// A later sub-step whose particles have all moved less than h/2 since
+// (skin_lim2) finds every particle the reference would accept among those
+// candidates: one that was farther than cut-off + h cannot have come closer
+// than the cut-off. The build then applies the reference's own test to each
+// candidate — the candidate's cell within one of the particle's cell in both
+// axes (the 3x3 window, from the cells of this sub-step's positions) and
+// |v|^2 < cut2 — so the lists are the reference's; their order comes from the
+// sort by (distance, index) as before (the window test is made only for pairs
+// whose separation is within the rounding of the cell assignment of a whole
+// cell in some axis; all others are in neighbouring cells). Lists that would
+// reach 64 entries
+// (truncated in scan order by the reference) flag the sub-step for a redo by
+// the full path, as do particles that moved too far.
+constexpr int SKIN_FLUID = 48;  // fluid candidates per particle
+constexpr int SKIN_SOLID = 32;  // solid candidates per particle
+constexpr unsigned int SKIN_OVERFLOW = 1u;  // a candidate list did not fit: no reuse
+constexpr unsigned int SKIN_STALE = 2u;     // a particle moved too far: redo by the full path
+constexpr unsigned int SKIN_FULL = 4u;      // a reused list reached 64 entries: redo likewise
+constexpr int SKIN_MIN_RUN = 6;    // reuses a candidate build must serve to pay for itself
+constexpr int SKIN_PAUSE = 64;     // sub-steps on the full path alone after a shorter run
+struct SkinView {
+    int reuse;              // 1: this sub-step's lists come from the candidates
+    uint32_t* cf;           // [SKIN_FLUID][stride] fluid candidates (sorted slots)
+    uint8_t* nf;            // per slot: fluid candidate count
+    uint32_t* cs;           // [SKIN_SOLID][stride] solid candidates (solid sorted slots)
+    uint8_t* ns;            // per slot: solid candidate count
+    V2* xref;               // per slot: position at the rebuild
+    unsigned int* tmax;     // per tile: largest displacement ratio (float bits) of the gather
 };
 
 // Neighbour lists in slot-major layout: entry k of slot t at [k * stride + t].
@@ -301,6 +342,7 @@ struct Batch {
     // the sweeps (2, 4 or 8) and the XCD-aware workgroup -> tile remap.
     int nb;
     int xcd_remap;
+    SkinView skin;
 };
 
 // BUILD_AP: density and the external-force velocity update (IISPH_AP and
@@ -404,6 +446,21 @@ struct WindowArgs {
 void launch_cell_windows(WindowArgs const& w, hipStream_t s);
 
 void launch_keys(IndexArgs const& a, hipStream_t s);
+
+// Skin candidate lists at a rebuild (after the index and the build of the
+// sub-step, dense index only): every fluid slot's fluid and solid candidates
+// into b.skin (cf, nf, cs, ns, xref). w: the sub-step's windows arguments
+// (its dense tables, and the solid cells with their bitmap and set).
+void launch_skin_candidates(Batch const& b, WindowArgs const& w, hipStream_t s);
+// Start of a sub-step that reuses the candidates: each scene's sub-step start
+// (as the key-range kernel does) and the build counters cleared; the key
+// range of the last indexing run stays.
+void launch_skin_begin(Batch const& b, hipStream_t s);
+// Positions and velocities (and the DFSPH stiffnesses when kv_in is set) into
+// the last rebuild's sorted order, and the displacement test against
+// b.skin.xref (its largest ratio per tile, then folded into
+// KeyState::skin_disp).
+void launch_skin_gather(Batch const& b, float const* kv_in, float const* kd_in, hipStream_t s);
 // Dense index: scatter of (key, index) to doffset[bin] + drank into
 // sorted_keys / sorted_vals (bin = dense_bin of the point's scene).
 void launch_dense_scatter(IndexArgs const& a, hipStream_t s);

@@ -251,6 +251,8 @@ __device__ __forceinline__ void fold_key_range(IndexArgs const& a, uint32_t part
         a.keystate->overflow_count = 0u;
         a.keystate->max_count = 0u;
         a.keystate->long_lists = 0u;
+        a.keystate->skin_flags = 0u;
+        a.keystate->skin_disp = 0u;
     }
 }
 
@@ -1067,6 +1069,29 @@ __device__ __forceinline__ int scan_window(uint64_t (*ent)[W], uint32_t lane, V2
     return cnt;
 }
 
+// Sorts each lane's collected entries (net_sort) with the smallest network that
+// covers the wave's longest list (a deferred, overflowed list counts as empty
+// here); this also turns a single entry's |v|^2 into its distance.
+template <int CAP, int W>
+__device__ __forceinline__ void sort_collected(uint64_t (*ent)[W], uint32_t lane, int cnt,
+                                               uint32_t const* __restrict__ oorig,
+                                               V2 const* __restrict__ opos, V2 p) {
+    int const wmax = wave_max(cnt);
+    if (wmax > 0) {
+        if (wmax <= 8) {
+            net_sort<8, W>(ent, lane, cnt, oorig, opos, p);
+        } else if (wmax <= 12) {
+            net_sort<12, W>(ent, lane, cnt, oorig, opos, p);
+        } else if (wmax <= 16) {
+            net_sort<16, W>(ent, lane, cnt, oorig, opos, p);
+        } else if (CAP <= 32 || wmax <= 32) {
+            if constexpr (CAP >= 32) { net_sort<32, W>(ent, lane, cnt, oorig, opos, p); }
+        } else {
+            if constexpr (CAP >= 64) { net_sort<64, W>(ent, lane, cnt, oorig, opos, p); }
+        }
+    }
+}
+
 // Accepted candidates of one particle over its 3x3 cell window. Returns the
 // count, or -1 when CAP < 64 and more than CAP candidates are accepted.
 // A candidate is accepted when |v| < maxd; with cut2 > 0 (StepConsts::cut2)
@@ -1088,40 +1113,115 @@ __device__ __forceinline__ int collect_window(uint64_t (*ent)[W], uint32_t lane,
                       : scan_window<CAP, W, false>(ent, lane, p, self, same, win, opos, maxd,
                                                    cut2, flags);
     if (near) { *near = flags & WIN_SOLID_NEAR; }
-    // The smallest network that covers the wave's longest list (a deferred,
-    // overflowed list counts as empty here).
-    int const wmax = wave_max(cnt);
-    if (wmax > 0) {  // also turns a single entry's |v|^2 into its distance
-        if (wmax <= 8) {
-            net_sort<8, W>(ent, lane, cnt, oorig, opos, p);
-        } else if (wmax <= 12) {
-            net_sort<12, W>(ent, lane, cnt, oorig, opos, p);
-        } else if (wmax <= 16) {
-            net_sort<16, W>(ent, lane, cnt, oorig, opos, p);
-        } else if (CAP <= 32 || wmax <= 32) {
-            if constexpr (CAP >= 32) { net_sort<32, W>(ent, lane, cnt, oorig, opos, p); }
-        } else {
-            if constexpr (CAP >= 64) { net_sort<64, W>(ent, lane, cnt, oorig, opos, p); }
+    sort_collected<CAP, W>(ent, lane, cnt, oorig, opos, p);
+    return cnt;
+}
+
+// Skin reuse (SkinView): the accepted candidates of slot t, appended unsorted
+// in candidate order as in take_chunk, then sorted. A candidate is accepted
+// by the reference's test: its cell (this sub-step's, in pg) within one of
+// the particle's in both axes — the 3x3 window of neighborhood.h:233-243,
+// in the wrapping 32-bit cell arithmetic of its keys — and |v|^2 < cut2.
+// Returns the count, or -1 when CAP < 64 and the list does not fit; a list
+// above 64 entries, which the reference truncates in its scan order, sets
+// SKIN_FULL and a short sub-step (redone by the full path).
+template <int CAP, int W, int ROWS>
+__device__ __forceinline__ int collect_skin(uint64_t (*ent)[W], uint32_t lane, uint32_t t,
+                                            uint32_t const* __restrict__ cand, int n,
+                                            uint32_t stride, V2 const* __restrict__ cpos,
+                                            V2 p, float cell, float cut2, uint32_t self,
+                                            uint32_t const* __restrict__ oorig, KeyState* ks) {
+    // The 3x3 window test matters only for a pair whose separation in an axis
+    // comes within the rounding of floor(x / cell) of a whole cell: cells are
+    // assigned from x / cell rounded to float, off by at most 2^-24 |x| / cell
+    // each, so points less than cell - 2^-21 (|x| + |y| + 4 cell) apart in both
+    // axes (x, y: this particle's coordinates; a candidate lies within 2 cells
+    // of them) are in neighbouring or equal cells. Other accepted pairs take
+    // both cells as the reference assigns them.
+    float const band =
+      cell - 0x1p-21f * (__builtin_fabsf(p.x) + __builtin_fabsf(p.y) + 4.0f * cell);
+    int cnt = 0;
+    // Candidate indices are read two chunks ahead of their gathers, without
+    // branches: every row read is inside the list's ROWS rows (clamped), and
+    // an index past the count is replaced by a valid slot (self) only when it
+    // is used, so its load need not have arrived before then. The gathers of
+    // a chunk are issued before the next indices, so waiting for them does
+    // not wait for those.
+    auto row = [&](int k) -> uint32_t {
+        int const r = k < ROWS ? k : ROWS - 1;
+        return cand[static_cast<size_t>(r) * stride + t];
+    };
+    uint32_t ia[4], ib[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        ia[u] = row(u);
+        ib[u] = row(4 + u);
+    }
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        uint32_t s[4];
+        V2 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s[u] = (k0 + u < n) ? ia[u] : self;
+            q[u] = cpos[s[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ia[u] = ib[u];
+            ib[u] = row(k0 + 8 + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            V2 const v = sub(q[u], p);
+            float const l2 = v.x * v.x + v.y * v.y;  // as in imath_length
+            bool acc = k0 + u < n && l2 < cut2;
+            if (acc && !(__builtin_fabsf(v.x) < band && __builtin_fabsf(v.y) < band)) {
+                int2 const gi = grid_coord(p, cell);
+                int2 const gj = grid_coord(q[u], cell);
+                uint32_t const dx = static_cast<uint32_t>(gj.x) - static_cast<uint32_t>(gi.x) + 1u;
+                uint32_t const dy = static_cast<uint32_t>(gj.y) - static_cast<uint32_t>(gi.y) + 1u;
+                acc = dx <= 2u && dy <= 2u;
+            }
+            uint64_t const entry = (static_cast<uint64_t>(__float_as_uint(l2)) << 32) | s[u];
+            if (acc && cnt < CAP) { ent[cnt][lane] = entry; }
+            cnt += acc ? 1 : 0;
+        }
+        if (CAP < MAXN && cnt > CAP) {
+            cnt = -1;
+            break;
         }
     }
+    if (CAP == MAXN && cnt > MAXN) {
+        atomicOr(&ks->skin_flags, SKIN_FULL);
+        atomicOr(&ks->sort_short, 1u);
+        cnt = MAXN;
+    }
+    sort_collected<CAP, W>(ent, lane, cnt, oorig, cpos, p);
     return cnt;
 }
 
 // Lists and the fused per-particle sums of slot t. Returns false (nothing of
 // the particle's state changed) when a list would exceed CAP.
-template <int MODE, int CAP, int W>
+template <int MODE, int CAP, int W, bool SKIN>
 __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
                                           uint32_t lane, uint32_t t, uint32_t scene,
                                           int& max_count) {
     StepConsts const c = a.scenes[scene].c;
     V2 const p = a.pos[t];
-    uint32_t const cell = a.cell_of[t];
+    uint32_t const cell = SKIN ? 0u : a.cell_of[t];  // windows of the full path
     float const mf = c.rho0 * c.fluid_volume;  // rho0 * V_j (uniform fluid volume)
 
-    // Fluid neighbours.
+    // Fluid neighbours: from the cell window, or from the skin candidates.
     uint32_t solid_near = 0;
-    int const fc = collect_window<CAP, W>(ent, lane, p, t, true, a.win + static_cast<size_t>(cell) * 9,
-                                       a.pos, a.orig, c.cell, c.cut2, &solid_near);
+    int fc;
+    if constexpr (SKIN) {
+        fc = collect_skin<CAP, W, SKIN_FLUID>(ent, lane, t, a.skin.cf, a.skin.nf[t], a.stride,
+                                              a.pos, p, c.cell, c.cut2, t, a.orig, a.keystate);
+        solid_near = a.skin.ns[t] != 0 ? 1u : 0u;
+    } else {
+        fc = collect_window<CAP, W>(ent, lane, p, t, true, a.win + static_cast<size_t>(cell) * 9,
+                                    a.pos, a.orig, c.cell, c.cut2, &solid_near);
+    }
     if (fc < 0) {
         max_count = CAP + 1;
         return false;
@@ -1182,7 +1282,15 @@ __device__ __forceinline__ bool build_one(Batch const& a, uint64_t (*ent)[W],
     }
     SolidView const& sv = a.solids;
     int sc = 0;
-    if (a.swin && sv.pos && solid_near) {
+    if (SKIN && sv.pos && solid_near) {
+        sc = collect_skin<CAP, W, SKIN_SOLID>(ent, lane, t, a.skin.cs, a.skin.ns[t], a.stride,
+                                              sv.pos, p, c.cell, c.cut2, 0u, sv.orig,
+                                              a.keystate);
+        if (sc < 0) {
+            max_count = CAP + 1;
+            return false;
+        }
+    } else if (a.swin && sv.pos && solid_near) {
         sc = collect_window<CAP, W>(ent, lane, p, 0xFFFFFFFFu, false,
                                  a.swin + static_cast<size_t>(cell) * 9, sv.pos, sv.orig, c.cell,
                                  c.cut2);
@@ -1298,7 +1406,7 @@ __device__ __forceinline__ void note_max_count(Batch const& a, int m) {
     }
 }
 
-template <int MODE, int CAP, int WG>
+template <int MODE, int CAP, int WG, bool SKIN>
 __global__ __launch_bounds__(WG) void k_build(Batch a) {
     __shared__ uint64_t ent[CAP][WG];
     uint32_t const lane = threadIdx.x;
@@ -1313,7 +1421,7 @@ __global__ __launch_bounds__(WG) void k_build(Batch a) {
     bool const valid = sub_off < tl.count;
     uint32_t const t = tl.start + sub_off;
     int m = 0;
-    bool const ok = valid ? build_one<MODE, CAP, WG>(a, ent, lane, t, tl.scene, m) : true;
+    bool const ok = valid ? build_one<MODE, CAP, WG, SKIN>(a, ent, lane, t, tl.scene, m) : true;
     note_max_count(a, m);
     if (CAP < MAXN) {
         // Deferred particles, appended with one atomic per wave.
@@ -1337,7 +1445,7 @@ __global__ __launch_bounds__(WG) void k_build(Batch a) {
 }
 
 // Full-capacity pass over the particles the capped build deferred.
-template <int MODE>
+template <int MODE, bool SKIN>
 __global__ __launch_bounds__(BUILD_THREADS) void k_build_deferred(Batch a) {
     __shared__ uint64_t ent[MAXN][BUILD_THREADS];
     uint32_t const n = a.keystate->overflow_count;
@@ -1345,27 +1453,27 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_build_deferred(Batch a) {
          i += gridDim.x * BUILD_THREADS) {
         uint2 const e = a.overflow[i];
         int m = 0;
-        (void)build_one<MODE, MAXN, BUILD_THREADS>(a, ent, threadIdx.x, e.x, e.y, m);
+        (void)build_one<MODE, MAXN, BUILD_THREADS, SKIN>(a, ent, threadIdx.x, e.x, e.y, m);
         if (m > 0) { atomicMax(&a.keystate->max_count, static_cast<unsigned int>(m)); }
     }
 }
 
 // ev (timing only, else null): start/stop events of the build launch and of
 // the deferred pass, stamped by the dispatches themselves.
-template <int MODE>
+template <int MODE, bool SKIN>
 void launch_build_mode(int cap, Batch const& a, hipStream_t s, hipEvent_t const* ev) {
     hipEvent_t const e0 = ev ? ev[0] : nullptr;
     hipEvent_t const e1 = ev ? ev[1] : nullptr;
     uint32_t const blocks = a.ntiles * (TILE / BUILD_WG);
     if (cap <= 16) {
-        hipExtLaunchKernelGGL((k_build<MODE, 16, BUILD_WG>), dim3(blocks), dim3(BUILD_WG), 0, s,
-                              e0, e1, 0, a);
+        hipExtLaunchKernelGGL((k_build<MODE, 16, BUILD_WG, SKIN>), dim3(blocks), dim3(BUILD_WG), 0,
+                              s, e0, e1, 0, a);
     } else if (cap <= 32) {
-        hipExtLaunchKernelGGL((k_build<MODE, 32, BUILD_WG>), dim3(blocks), dim3(BUILD_WG), 0, s,
-                              e0, e1, 0, a);
+        hipExtLaunchKernelGGL((k_build<MODE, 32, BUILD_WG, SKIN>), dim3(blocks), dim3(BUILD_WG), 0,
+                              s, e0, e1, 0, a);
     } else {
         // 64 entries of 64 lanes: 32 KB of LDS per workgroup.
-        hipExtLaunchKernelGGL((k_build<MODE, MAXN, BUILD_THREADS>),
+        hipExtLaunchKernelGGL((k_build<MODE, MAXN, BUILD_THREADS, SKIN>),
                               dim3(a.ntiles * (TILE / BUILD_THREADS)), dim3(BUILD_THREADS), 0, s,
                               e0, e1, 0, a);
         if (ev) {  // no deferred pass: an empty interval
@@ -1377,8 +1485,18 @@ void launch_build_mode(int cap, Batch const& a, hipStream_t s, hipEvent_t const*
     // Deferred lists are rare (under 1/64 of the particles while the 16-entry
     // build is chosen): one workgroup per CU at most, most of them empty.
     uint32_t const dblocks = a.ntiles < 256 ? a.ntiles : 256;
-    hipExtLaunchKernelGGL(k_build_deferred<MODE>, dim3(dblocks), dim3(BUILD_THREADS), 0, s,
-                          ev ? ev[2] : nullptr, ev ? ev[3] : nullptr, 0, a);
+    hipExtLaunchKernelGGL((k_build_deferred<MODE, SKIN>), dim3(dblocks), dim3(BUILD_THREADS), 0,
+                          s, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr, 0, a);
+}
+
+// The lists from the cell windows, or (b.skin.reuse) from the skin candidates.
+template <int MODE>
+void launch_build_mode(int cap, Batch const& a, hipStream_t s, hipEvent_t const* ev) {
+    if (a.skin.reuse) {
+        launch_build_mode<MODE, true>(cap, a, s, ev);
+    } else {
+        launch_build_mode<MODE, false>(cap, a, s, ev);
+    }
 }
 
 void launch_build(int mode, int cap, Batch const& a, hipStream_t s, hipEvent_t const* ev) {
@@ -1394,6 +1512,189 @@ void launch_build(int mode, int cap, Batch const& a, hipStream_t s, hipEvent_t c
     case BUILD_AP: launch_build_mode<BUILD_AP>(cap, a, s, ev); break;
     default: launch_build_mode<BUILD_LISTS>(cap, a, s, ev); break;
     }
+}
+
+//------------------------------------------------------------------------------
+// Skin candidate lists (SkinView in kernels.hpp).
+
+// The columns (or rows) d in [-2, 2] around cell g of a point at coordinate x
+// that may hold points within r of it: -1, 0 and 1 always (r >= cell), -2 when
+// x lies within r of the cell's low edge plus a margin, 2 likewise for the high
+// edge. The margin covers the float rounding of floor(x / cell) that assigns
+// points to cells (relative 2^-24 of the coordinate) and of the edges here.
+__device__ __forceinline__ void skin_span(float x, int32_t g, float cell, float r, int& lo,
+                                          int& hi) {
+    double const m = 1e-6 * (fabs(static_cast<double>(x)) + static_cast<double>(r));
+    double const low = static_cast<double>(g) * cell;
+    double const high = (static_cast<double>(g) + 1.0) * cell;
+    lo = (static_cast<double>(x) - low < static_cast<double>(r) + m) ? -2 : -1;
+    hi = (high - static_cast<double>(x) < static_cast<double>(r) + m) ? 2 : 1;
+}
+
+// One thread per fluid slot (tiles): the fluid and solid points with
+// |v|^2 < skin_r2 in the reachable cells of the 5x5 window, in window order.
+__global__ __launch_bounds__(TILE) void k_skin_candidates(Batch a, WindowArgs w) {
+    TileRef const r = tile_ref(a, TILE);
+    if (!r.valid) { return; }
+    uint32_t const t = r.t;
+    StepConsts const c = a.scenes[r.scene].c;
+    DenseScene const ds = w.dscene[r.scene];
+    V2 const p = a.pos[t];
+    int2 const g = grid_coord(p, c.cell);
+    float const rr = __builtin_sqrtf(c.skin_r2);
+    int xlo, xhi, ylo, yhi;
+    skin_span(p.x, g.x, c.cell, rr, xlo, xhi);
+    skin_span(p.y, g.y, c.cell, rr, ylo, yhi);
+    SolidView const& sv = a.solids;
+    bool const solids = sv.pos && w.other_set.slots;
+    int nf = 0;
+    int ns = 0;
+    bool over = false;
+    for (int dy = ylo; dy <= yhi; ++dy) {
+        // The row's cells: bins, then every range bound requested together.
+        uint64_t nk[5];
+        uint64_t bin[5];
+        uint2 rg[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            int const dx = xlo + k;
+            nk[k] = morton2(static_cast<int32_t>(static_cast<uint32_t>(g.x) + dx),
+                            static_cast<int32_t>(static_cast<uint32_t>(g.y) + dy));
+            bin[k] = (dx <= xhi) ? dense_bin(ds, nk[k], w.dense_bins) : w.dense_bins;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            rg[k] = (bin[k] < w.dense_bins) ? make_uint2(w.doffset[bin[k]], w.doffset[bin[k] + 1])
+                                            : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            for (uint32_t s0 = rg[k].x; s0 < rg[k].y; s0 += 4) {
+                V2 q[4];
+                load_chunk<4>(q, a.pos, s0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    uint32_t const s = s0 + u;
+                    V2 const v = sub(q[u], p);
+                    float const l2 = v.x * v.x + v.y * v.y;
+                    if (s < rg[k].y && s != t && l2 < c.skin_r2) {
+                        if (nf < SKIN_FLUID) {
+                            a.skin.cf[static_cast<size_t>(nf) * a.stride + t] = s;
+                            ++nf;
+                        } else {
+                            over = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (solids) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                int const dx = xlo + k;
+                bool const maybe =
+                  dx <= xhi && (bin[k] >= w.dense_bins || !w.other_bits ||
+                                ((w.other_bits[bin[k] >> 5] >> (bin[k] & 31u)) & 1u));
+                if (!maybe) { continue; }
+                uint2 const rs = cell_set_range(w.other_set, w.other, nk[k], r.scene);
+                for (uint32_t s = rs.x; s < rs.y; ++s) {
+                    V2 const v = sub(sv.pos[s], p);
+                    float const l2 = v.x * v.x + v.y * v.y;
+                    if (l2 < c.skin_r2) {
+                        if (ns < SKIN_SOLID) {
+                            a.skin.cs[static_cast<size_t>(ns) * a.stride + t] = s;
+                            ++ns;
+                        } else {
+                            over = true;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    a.skin.nf[t] = static_cast<uint8_t>(nf);
+    a.skin.ns[t] = static_cast<uint8_t>(ns);
+    a.skin.xref[t] = p;
+    if (over) { atomicOr(&a.keystate->skin_flags, SKIN_OVERFLOW); }
+}
+
+void launch_skin_candidates(Batch const& b, WindowArgs const& w, hipStream_t s) {
+    if (!b.ntiles) { return; }
+    hipLaunchKernelGGL(k_skin_candidates, dim3(b.ntiles), dim3(TILE), 0, s, b, w);
+}
+
+// One workgroup: the sub-step start of every scene (as fold_key_range does)
+// and the counters of this sub-step's build and reuse test cleared.
+__global__ __launch_bounds__(256) void k_skin_begin(Batch b) {
+    for (uint32_t s = threadIdx.x; s < b.nscenes; s += blockDim.x) {
+        step_begin_scene(const_cast<SceneDev*>(b.scenes) + s, b.stats + s);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        KeyState* const ks = b.keystate;
+        ks->frame_remaining = b.scenes[0].frame.on ? b.scenes[0].frame.next_remaining : -1;
+        ks->sort_short = 0u;
+        ks->dense_out = 0u;
+        ks->dense_crowded = 0u;
+        ks->overflow_count = 0u;
+        ks->max_count = 0u;
+        ks->long_lists = 0u;
+        ks->skin_flags = 0u;
+        ks->skin_disp = 0u;
+    }
+}
+
+void launch_skin_begin(Batch const& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_skin_begin, dim3(1), dim3(256), 0, s, b);
+}
+
+// One thread per fluid slot (tiles): the state into the rebuild's sorted
+// order (as the cells stage gathers it) and |x - x_rebuild|^2 against
+// skin_lim2: a point at or past it makes the sub-step short (SKIN_STALE). The
+// largest ratio per tile, folded into KeyState::skin_disp, guides the host's
+// choice of the next rebuild.
+__global__ __launch_bounds__(TILE) void k_skin_gather(Batch a, float const* kv_in,
+                                                      float const* kd_in) {
+    TileRef const r = tile_ref(a, TILE);
+    float ratio = 0.0f;
+    bool stale = false;
+    if (r.valid) {
+        uint32_t const t = r.t;
+        StepConsts const c = a.scenes[r.scene].c;
+        uint32_t const o = a.orig[t];
+        V2 const x = a.pos_orig[o];
+        a.pos[t] = x;
+        a.vel[t] = a.vel_orig[o];
+        if (kv_in) {
+            a.kvs[t] = kv_in[o];
+            a.kds[t] = kd_in[o];
+        }
+        V2 const d = sub(x, a.skin.xref[t]);
+        float const d2 = d.x * d.x + d.y * d.y;
+        stale = !(d2 < c.skin_lim2);  // the exact test; the ratio only guides the host
+        ratio = stale ? 1.0f : d2 * (1.0f / c.skin_lim2);
+    }
+    if (__ballot(stale) && __lane_id() == 0) {
+        atomicOr(&a.keystate->skin_flags, SKIN_STALE);
+        atomicOr(&a.keystate->sort_short, 1u);
+    }
+    block_max<TILE>(__float_as_uint(ratio), a.skin.tmax + r.tile);
+}
+
+// One workgroup: the tiles' largest displacement ratios into KeyState::skin_disp.
+__global__ __launch_bounds__(256) void k_skin_fold(Batch b) {
+    unsigned int m = 0;
+    for (uint32_t k = threadIdx.x; k < b.ntiles; k += 256u) {
+        unsigned int const v = b.skin.tmax[k];
+        m = v > m ? v : m;
+    }
+    block_max<256>(m, &b.keystate->skin_disp);
+}
+
+void launch_skin_gather(Batch const& b, float const* kv_in, float const* kd_in, hipStream_t s) {
+    if (!b.ntiles) { return; }
+    hipLaunchKernelGGL(k_skin_gather, dim3(b.ntiles), dim3(TILE), 0, s, b, kv_in, kd_in);
+    hipLaunchKernelGGL(k_skin_fold, dim3(1), dim3(256), 0, s, b);
 }
 
 // solids.cpp:52-74 compute_volumes: 1.7 / (W0 + sum W) over solid-solid lists

@@ -467,6 +467,19 @@ class Context:
         """Replay the fixed segments of each sub-step from captured HIP graphs."""
         check(lib().emsph_set_graphs(self._h, 1 if enable else 0), "set_graphs")
 
+    def set_skin_lists(self, enable: bool) -> None:
+        """Skin candidate lists for the neighbour search (emsph_set_skin_lists):
+        the same lists, found from candidates recorded at a rebuild while no
+        particle has moved h/2 since."""
+        check(lib().emsph_set_skin_lists(self._h, 1 if enable else 0), "set_skin_lists")
+
+    def skin_lists(self) -> tuple:
+        """(on, sub-steps that rebuilt candidates, sub-steps that reused them)."""
+        on, r, u = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().emsph_get_skin_lists(self._h, ctypes.byref(on), ctypes.byref(r),
+                                         ctypes.byref(u)), "get_skin_lists")
+        return bool(on.value), int(r.value), int(u.value)
+
     def tuning(self) -> dict:
         t = _capi.Tuning()
         check(lib().emsph_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
@@ -646,6 +659,17 @@ class Batch:
 
     def set_graphs(self, enable: bool) -> None:
         check(lib().emsph_batch_set_graphs(self._h, 1 if enable else 0), "batch_set_graphs")
+
+    def set_skin_lists(self, enable: bool) -> None:
+        """As Context.set_skin_lists, for every scene of the batch."""
+        check(lib().emsph_batch_set_skin_lists(self._h, 1 if enable else 0),
+              "batch_set_skin_lists")
+
+    def skin_lists(self) -> tuple:
+        on, r, u = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().emsph_batch_get_skin_lists(self._h, ctypes.byref(on), ctypes.byref(r),
+                                               ctypes.byref(u)), "batch_get_skin_lists")
+        return bool(on.value), int(r.value), int(u.value)
 
     def set_kernel_timing(self, enable: bool) -> None:
         check(lib().emsph_batch_set_kernel_timing(self._h, 1 if enable else 0),

@@ -294,6 +294,28 @@ emsph_status emsph_set_cfl(emsph_context* ctx, float factor);
  * once the sort pass count is known and kernel timing is off). Results are
  * identical either way. */
 emsph_status emsph_set_graphs(emsph_context* ctx, int enable);
+/* Skin candidate lists for the neighbour search (on by default for contexts
+ * and batches; pools do not use them). No
+ * counterpart in the reference, whose lists every sub-step builds again from
+ * the 3x3 cell windows (neighborhood.h:198-282); the lists here are the same
+ * (the same neighbours, in the same (distance, index) order), only found
+ * another way: a rebuild sub-step (the full keys, sort, cells and build, plus
+ * one kernel) records for each particle the particles within the cut-off plus
+ * h; the following sub-steps skip the keys, sort and cells and test only those
+ * candidates with the reference's acceptance (3x3 cell window of this
+ * sub-step's cells, |v| < cut-off), until some particle could have moved h/2
+ * since the rebuild (predicted from the last two sub-steps; a sub-step where
+ * one did is redone by the full path). Used with the dense cell index only;
+ * paused for 64 sub-steps after candidates that served fewer than 6 sub-steps
+ * (motion too fast to repay their build); turned off by itself when a
+ * candidate list would exceed 48 fluid or 32 solid candidates, or a reused
+ * list would reach the 64-entry cap (where the reference truncates in scan
+ * order). */
+emsph_status emsph_set_skin_lists(emsph_context* ctx, int enable);
+/* Whether skin lists are on (they turn off by themselves, see above) and the
+ * sub-steps since creation that built candidates and that reused them. */
+emsph_status emsph_get_skin_lists(emsph_context* ctx, int* enabled, int64_t* rebuilds,
+                                  int64_t* reuses);
 
 /* Execution choices that do not change any result. */
 typedef struct emsph_tuning {
@@ -347,6 +369,10 @@ emsph_status emsph_batch_get_state(emsph_batch* b, int scene, float* pos_xy, flo
 emsph_status emsph_batch_get_field(emsph_batch* b, int scene, int field, float* out);
 emsph_status emsph_batch_set_cfl(emsph_batch* b, float factor);  /* as emsph_set_cfl, per scene */
 emsph_status emsph_batch_set_graphs(emsph_batch* b, int enable);  /* as emsph_set_graphs */
+/* As emsph_set_skin_lists / emsph_get_skin_lists, for every scene of the batch. */
+emsph_status emsph_batch_set_skin_lists(emsph_batch* b, int enable);
+emsph_status emsph_batch_get_skin_lists(emsph_batch* b, int* enabled, int64_t* rebuilds,
+                                        int64_t* reuses);
 /* As emsph_set_kernel_timing / emsph_get_kernel_timing, over the batch's
  * launches (one launch per stage covers every scene). Measurement only. */
 emsph_status emsph_batch_set_kernel_timing(emsph_batch* b, int enable);

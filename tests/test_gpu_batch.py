@@ -137,6 +137,7 @@ def test_cfl_sub_step_redone_after_short_sort(gpu):
     a = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
     b = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
     for ctx in (a, b):
+        ctx.set_skin_lists(False)  # the full path's sort, every sub-step
         ctx.set_cfl(factor)
         ctx.set_state(st0)
         ctx.frame_init(method)
@@ -260,6 +261,7 @@ def test_batch_dense_index_used(gpu):
     cfgs = [scenes.config_for(sc, method) for sc in scs]
     batch = sb.Batch(cfgs, [sb.Solid_state(s.solids) for s in scs],
                      [sb.State(s.positions, s.velocities) for s in scs])
+    batch.set_skin_lists(False)  # the index runs every sub-step
     batch.frame_init(method)
     batch.run_sub_steps(method, 3)
     batch.set_kernel_timing(True)
@@ -297,6 +299,7 @@ def test_batch_dense_table_growth_redo(gpu):
     batch = sb.Batch(cfgs, [sb.Solid_state(s.solids) for s in scs],
                      [sb.State(s.positions, s.velocities) for s in scs])
     orcs = [oracle_scene(sc, cfg) for sc, cfg in zip(scs, cfgs)]
+    batch.set_skin_lists(False)  # the index runs every sub-step
     batch.frame_init(method)
     for o in orcs:
         o.frame_init(method)

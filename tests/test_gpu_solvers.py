@@ -301,6 +301,7 @@ def test_dense_index_table_growth_redo(gpu):
                          base.dt, base.mass)
     cfg = scenes.config_for(sc, method)
     ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
+    ctx.set_skin_lists(False)  # the index runs every sub-step
     ctx.set_state(sb.State(sc.positions, sc.velocities))
     orc = oracle_scene(sc, cfg)
     ctx.frame_init(method)
@@ -332,6 +333,7 @@ def test_dense_index_crowded_cell(gpu):
                          base.dt, base.mass)
     cfg = scenes.config_for(sc, method)
     ctx = sb.Context(cfg, sb.Solid_state(sc.solids), len(pos))
+    ctx.set_skin_lists(False)  # the index runs every sub-step
     ctx.set_state(sb.State(sc.positions, sc.velocities))
     orc = oracle_scene(sc, cfg)
     ctx.frame_init(method)
