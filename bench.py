@@ -54,8 +54,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Skin candidate lists in the 1M solver windows (--skin-lists on|off).
-SKIN_LISTS = False
+# Skin candidate lists in the 1M solver windows (--skin-lists on|off; on is
+# the library's default for contexts).
+SKIN_LISTS = True
 
 # Algorithmic HBM bytes per fluid particle for one launch of each timed kernel
 # class, per solver. Conventions: FP32 SoA; every per-particle field read or
@@ -706,8 +707,8 @@ def main() -> int:
     ap.add_argument("--search-only", action="store_true")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--small-only", action="store_true")
-    ap.add_argument("--skin-lists", choices=["on", "off"], default="off",
-                    help="skin candidate lists in the 1M solver windows")
+    ap.add_argument("--skin-lists", choices=["on", "off"], default="on",
+                    help="skin candidate lists in the 1M solver windows (the library default)")
     ap.add_argument("--save-state", default=None,
                     help="write the headline solver's state at the start of its timed window "
                          "(.npz)")

@@ -1142,13 +1142,14 @@ __device__ __forceinline__ int collect_skin(uint64_t (*ent)[W], uint32_t lane, u
       cell - 0x1p-21f * (__builtin_fabsf(p.x) + __builtin_fabsf(p.y) + 4.0f * cell);
     int cnt = 0;
     // Candidate indices are read two chunks ahead of their gathers, without
-    // branches: every row read is inside the list's ROWS rows (clamped), and
-    // an index past the count is replaced by a valid slot (self) only when it
-    // is used, so its load need not have arrived before then. The gathers of
-    // a chunk are issued before the next indices, so waiting for them does
-    // not wait for those.
+    // branches: a row past the count reads the list's last row again (a cache
+    // hit, and inside the ROWS rows), and an index past the count is replaced
+    // by a valid slot (self) only when it is used, so its load need not have
+    // arrived before then. The gathers of a chunk are issued before the next
+    // indices, so waiting for them does not wait for those.
+    int const last = n > 0 ? (n < ROWS ? n - 1 : ROWS - 1) : 0;
     auto row = [&](int k) -> uint32_t {
-        int const r = k < ROWS ? k : ROWS - 1;
+        int const r = k < last ? k : last;
         return cand[static_cast<size_t>(r) * stride + t];
     };
     uint32_t ia[4], ib[4];
