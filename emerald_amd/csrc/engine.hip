@@ -470,7 +470,6 @@ struct Engine {
     bool skin_valid = false;
     bool skin_reuse = false;    // this attempt reuses the candidates
     bool skin_built = false;    // this attempt builds them
-    bool skin_seen = false;     // this attempt's skin flags were taken in
     float skin_rho = 0.0f, skin_rho_prev = 0.0f;
     int64_t skin_counts[2] = {0, 0};  // sub-steps that rebuilt / reused
     int skin_run = 0;        // reuses since the last candidate build
@@ -1758,7 +1757,6 @@ emsph_status check_method(int method) {
 hipError_t skin_prepare(Engine* g, int attempt) {
     g->skin_reuse = false;
     g->skin_built = false;
-    g->skin_seen = false;
     if (!g->skin_on) { return hipSuccess; }
     if (g->skin_pause > 0) {
         if (attempt == 0) { --g->skin_pause; }
