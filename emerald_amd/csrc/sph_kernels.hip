@@ -1519,17 +1519,20 @@ void launch_build(int mode, int cap, Batch const& a, hipStream_t s, hipEvent_t c
 // Skin candidate lists (SkinView in kernels.hpp).
 
 // The columns (or rows) d in [-2, 2] around cell g of a point at coordinate x
-// that may hold points within r of it: -1, 0 and 1 always (r >= cell), -2 when
-// x lies within r of the cell's low edge plus a margin, 2 likewise for the high
-// edge. The margin covers the float rounding of floor(x / cell) that assigns
-// points to cells (relative 2^-24 of the coordinate) and of the edges here.
+// (in cell g) that may hold points within r (cell <= r < 2 cell) of it: -1, 0
+// and 1 always, -2 when x - r falls below the low edge of cell g - 1 plus a
+// margin, 2 when x + r reaches the low edge of cell g + 2 less the margin. The
+// margin covers the float rounding of floor(x / cell) that assigns points to
+// cells (relative 2^-24 of the coordinate) and of the edges here.
 __device__ __forceinline__ void skin_span(float x, int32_t g, float cell, float r, int& lo,
                                           int& hi) {
-    double const m = 1e-6 * (fabs(static_cast<double>(x)) + static_cast<double>(r));
-    double const low = static_cast<double>(g) * cell;
-    double const high = (static_cast<double>(g) + 1.0) * cell;
-    lo = (static_cast<double>(x) - low < static_cast<double>(r) + m) ? -2 : -1;
-    hi = (high - static_cast<double>(x) < static_cast<double>(r) + m) ? 2 : 1;
+    double const dx = static_cast<double>(x);
+    double const dr = static_cast<double>(r);
+    double const m = 1e-6 * (fabs(dx) + dr);
+    double const edge_lo = (static_cast<double>(g) - 1.0) * cell;
+    double const edge_hi = (static_cast<double>(g) + 2.0) * cell;
+    lo = (dx - dr < edge_lo + m) ? -2 : -1;
+    hi = (dx + dr > edge_hi - m) ? 2 : 1;
 }
 
 // One thread per fluid slot (tiles): the fluid and solid points with
