@@ -57,6 +57,9 @@ HBM_PEAK_GBS = 8000.0
 # Skin candidate lists in the 1M solver windows (--skin-lists on|off; on is
 # the library's default for contexts).
 SKIN_LISTS = True
+# Fixed (neighbour batch, XCD remap) of the 1M solver windows (--tuning), or
+# None for the autotuner.
+FIXED_TUNING = None
 
 # Algorithmic HBM bytes per fluid particle for one launch of each timed kernel
 # class, per solver. Conventions: FP32 SoA; every per-particle field read or
@@ -259,7 +262,11 @@ def solver_window(method: str, steps: int, warmup: int, advance: int, cpu: dict 
     log(f"{method}: warm-up {warmup} sub-steps")
     ctx.run_sub_steps(method, scene.dt, warmup)
     done += warmup
-    if autotune:
+    if autotune and FIXED_TUNING:
+        nb, xcd = FIXED_TUNING
+        ctx.set_tuning(graphs=False, neighbor_batch=nb, xcd_remap=bool(xcd))
+        out["tuning"] = {"chosen": ctx.tuning(), "fixed": True}
+    elif autotune:
         log(f"{method}: autotune")
         chosen, cand_ms = ctx.autotune(method, scene.dt, 5)
         out["tuning"] = {"chosen": chosen, "candidate_ms_per_5_sub_steps": cand_ms,
@@ -709,6 +716,9 @@ def main() -> int:
     ap.add_argument("--small-only", action="store_true")
     ap.add_argument("--skin-lists", choices=["on", "off"], default="on",
                     help="skin candidate lists in the 1M solver windows (the library default)")
+    ap.add_argument("--tuning", default=None, metavar="NB,XCD",
+                    help="fixed tuning of the 1M solver windows instead of the autotuner: "
+                         "neighbour batch (2, 4 or 8) and XCD remap (0 or 1), without graphs")
     ap.add_argument("--save-state", default=None,
                     help="write the headline solver's state at the start of its timed window "
                          "(.npz)")
@@ -716,8 +726,13 @@ def main() -> int:
                     help="start the headline solver from a state written by --save-state "
                          "(no rest window or advance)")
     args = ap.parse_args()
-    global SKIN_LISTS
+    global SKIN_LISTS, FIXED_TUNING
     SKIN_LISTS = args.skin_lists == "on"
+    if args.tuning:
+        nb, xcd = (int(v) for v in args.tuning.split(","))
+        if nb not in (2, 4, 8) or xcd not in (0, 1):
+            raise SystemExit("bench: --tuning takes NB,XCD with NB in 2, 4, 8 and XCD in 0, 1")
+        FIXED_TUNING = (nb, xcd)
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark runs on one GPU only; --gpus above 1 is "
                                    "not supported", "n_gpus": args.gpus}))

@@ -334,9 +334,10 @@ emsph_status emsph_set_tuning(emsph_context* ctx, const emsph_tuning* tuning);
 /* Autotuner: times every candidate tuning (neighbour batch depth x XCD remap
  * with graphs, then the best of those without graphs) on trial_sub_steps
  * sub-steps of this context's own state (restored after each trial and at the
- * end, so the simulation does not advance) and keeps the fastest. times_ms (up
- * to 8 entries, may be NULL) receives each candidate's time and n_candidates
- * their count. No counterpart in the reference. */
+ * end, so the simulation does not advance), three trials each in turn, and
+ * keeps the candidate with the fastest best trial. times_ms (up to 8 entries,
+ * may be NULL) receives each candidate's best time and n_candidates their
+ * count. No counterpart in the reference. */
 emsph_status emsph_autotune(emsph_context* ctx, int method, float dt, int trial_sub_steps,
                             emsph_tuning* chosen, double* times_ms, int* n_candidates);
 emsph_status emsph_set_kernel_timing(emsph_context* ctx, int enable);
