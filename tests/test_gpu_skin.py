@@ -118,14 +118,15 @@ def test_skin_lists_set_state_mid_run(gpu):
     _same(full, skin, "set_state")
 
 
-@pytest.mark.parametrize("cfl", [0.0, 0.4])
-def test_skin_lists_batch(gpu, cfl):
+@pytest.mark.parametrize("method,cfl", [("iisph", 0.0), ("iisph", 0.4), ("dfsph_p", 0.0),
+                                        ("pcisph", 0.0)])
+def test_skin_lists_batch(gpu, method, cfl):
     # Fixed dt: the candidates serve many sub-steps once the first splashes
     # of the jittered lattices calm down. CFL 0.4: each scene's dt (at most the
     # fixed one) lets its fastest particle move up to 0.4 h per sub-step, and
-    # rebuilds come sooner. The results are the same either way.
+    # rebuilds come sooner. The results are the same either way. DFSPH_P's
+    # reuses also gather its stiffnesses into the rebuild's order.
     scs = [small_dam_break(36 + 8 * k, 48 + 6 * k, seed=20 + k) for k in range(4)]
-    method = "iisph"
     out = []
     for skin in (False, True):
         cfgs = [scenes.config_for(sc, method) for sc in scs]
@@ -139,10 +140,12 @@ def test_skin_lists_batch(gpu, cfl):
             out.append((its, [b.get_state(i) for i in range(len(scs))], b.skin_lists()))
         finally:
             b.close()
-    print(f"[skin batch] cfl {cfl}: {out[1][2]}", flush=True)
+    print(f"[skin batch] {method} cfl {cfl}: {out[1][2]}", flush=True)
     assert out[0][0] == out[1][0]
     on, rebuilds, reuses = out[1][2]
-    assert on and rebuilds >= 2 and reuses >= 20
+    # DFSPH_P's and PCISPH's splashes end intervals sooner here, and short
+    # intervals pause the candidates for a while (the full path alone).
+    assert on and rebuilds >= 2 and reuses >= (20 if method == "iisph" else 10)
     for i in range(len(scs)):
         assert_bits_equal(out[0][1][i].positions, out[1][1][i].positions, f"scene {i} positions")
         assert_bits_equal(out[0][1][i].velocities, out[1][1][i].velocities,
