@@ -64,6 +64,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
             _wait(procs.pop(0))
     for p in procs:
         _wait(p)
+    _check_build_layout(os.path.join(out, "sph_kernels.hip.o"))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
@@ -73,6 +74,46 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     build_headless(verbose)
     build_api_check(verbose)
     return LIB
+
+
+# The 16-entry IISPH neighbour build and the byte of a 4 KB page its entry
+# should sit at in sph_kernels.hip's code object (k_layout_pad there).
+BUILD_KERNEL = "_ZN5emsph7k_buildILi1ELi16ELi64ELb0EEEvNS_5BatchE"
+BUILD_KERNEL_PAGE_OFFSET = 3840
+
+
+def _check_build_layout(obj: str) -> None:
+    """Warn when the neighbour build's entry is not at BUILD_KERNEL_PAGE_OFFSET
+    of a page (an edit before it in sph_kernels.hip moved it; the size of
+    k_layout_pad there sets it). Diagnostics only: never fails the build."""
+    llvm = os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "lib", "llvm", "bin")
+    if not os.path.isdir(llvm):
+        llvm = "/opt/rocm/lib/llvm/bin"
+    tmp = obj + ".layout"
+    try:
+        subprocess.run([os.path.join(llvm, "llvm-objcopy"), f"--dump-section=.hip_fatbin={tmp}.fb",
+                        obj, os.devnull], check=True, capture_output=True)
+        subprocess.run([os.path.join(llvm, "clang-offload-bundler"), "--unbundle", "--type=o",
+                        f"--input={tmp}.fb", f"--targets=hipv4-amdgcn-amd-amdhsa--{ARCH}",
+                        f"--output={tmp}.co"], check=True, capture_output=True)
+        syms = subprocess.run([os.path.join(llvm, "llvm-readelf"), "-s", "--wide", f"{tmp}.co"],
+                              check=True, capture_output=True, text=True).stdout
+    except (OSError, subprocess.CalledProcessError):
+        return
+    finally:
+        for ext in (".fb", ".co"):
+            if os.path.exists(tmp + ext):
+                os.remove(tmp + ext)
+    for line in syms.splitlines():
+        f = line.split()
+        if len(f) >= 8 and f[7] == BUILD_KERNEL and f[3] == "FUNC":
+            off = int(f[1], 16) % 4096
+            if off != BUILD_KERNEL_PAGE_OFFSET:
+                print(f"build: the 16-entry IISPH neighbour build starts at byte {off} of a "
+                      f"4 KB page, not {BUILD_KERNEL_PAGE_OFFSET} (resize k_layout_pad in "
+                      f"sph_kernels.hip; the build's time depends on it)", file=sys.stderr,
+                      flush=True)
+            return
 
 
 HEADLESS = os.path.join(HERE, "sph2d_box_headless")

@@ -445,6 +445,9 @@ struct WindowArgs {
 };
 void launch_cell_windows(WindowArgs const& w, hipStream_t s);
 
+// Keys, grid coordinates and the key range (KeyState::kmin/kmax, with the
+// sub-step start of every scene); with the dense index (a.dense_bins set) the
+// range is folded by launch_dense_scatter, which must follow.
 void launch_keys(IndexArgs const& a, hipStream_t s);
 
 // Skin candidate lists at a rebuild (after the index and the build of the
@@ -462,7 +465,8 @@ void launch_skin_begin(Batch const& b, hipStream_t s);
 // KeyState::skin_disp).
 void launch_skin_gather(Batch const& b, float const* kv_in, float const* kd_in, hipStream_t s);
 // Dense index: scatter of (key, index) to doffset[bin] + drank into
-// sorted_keys / sorted_vals (bin = dense_bin of the point's scene).
+// sorted_keys / sorted_vals (bin = dense_bin of the point's scene); its first
+// workgroup also folds the key range that launch_keys left as partials.
 void launch_dense_scatter(IndexArgs const& a, hipStream_t s);
 void launch_cell_heads(IndexArgs const& a, hipStream_t s);
 void launch_cells_finalize(IndexArgs const& a, hipStream_t s);

@@ -24,6 +24,15 @@
 
 namespace emsph {
 
+// Never launched: its size places the neighbour build that follows it in this
+// file's code object (k_build<BUILD_IISPH, 16, 64, false>) at byte 3840 of a
+// 4 KB page. With the same instructions, that kernel's time at 1M particles
+// depends on its offset in the page (104-121 us over the offsets measured;
+// its waves wait on instruction fetch a fifth of their cycles, and where the
+// page boundaries fall in its loops changes that), and 3840 was the fastest.
+// The build step checks the offset and warns when an edit has moved it.
+__global__ void k_layout_pad() { asm volatile(".rept 768\n s_nop 0\n.endr"); }
+
 constexpr int BUILD_THREADS = 64;  // the full-capacity build and deferred pass
 constexpr int BUILD_WG = 64;        // the capped builds (16 or 32 entries)
 
@@ -331,10 +340,21 @@ __global__ __launch_bounds__(256) void k_key_range(IndexArgs a, uint32_t parts) 
     fold_key_range(a, parts);
 }
 
+// Workgroups of k_keys over n points, each leaving one key-range partial.
+static uint32_t key_parts(uint32_t n) {
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > KEY_PARTS) { blocks = KEY_PARTS; }
+    return blocks ? blocks : 1u;
+}
+
 // Dense index scatter: point i goes to the first slot of its bin plus its
 // rank (the ranks of one bin are a permutation of 0 .. count - 1, so every
-// slot receives exactly one point).
-__global__ __launch_bounds__(256) void k_dense_scatter(IndexArgs a) {
+// slot receives exactly one point). Workgroup 0 first folds the key range
+// and starts the sub-step (fold_key_range, for k_keys' `parts` partials):
+// the dense path has no k_key_range launch, and nothing before this kernel
+// reads what the fold writes.
+__global__ __launch_bounds__(256) void k_dense_scatter(IndexArgs a, uint32_t parts) {
+    if (blockIdx.x == 0) { fold_key_range(a, parts); }
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) { return; }
     uint64_t const key = a.keys[i];
@@ -345,16 +365,16 @@ __global__ __launch_bounds__(256) void k_dense_scatter(IndexArgs a) {
 }
 
 void launch_dense_scatter(IndexArgs const& a, hipStream_t s) {
-    if (!a.n) { return; }
-    hipLaunchKernelGGL(k_dense_scatter, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+    uint32_t const blocks = a.n ? (a.n + 255) / 256 : 1u;
+    hipLaunchKernelGGL(k_dense_scatter, dim3(blocks), dim3(256), 0, s, a, key_parts(a.n));
 }
 
+// Keys and their range; with the dense index (a.dense_bins) the range is
+// folded by the dense scatter's first workgroup instead of a launch of its own.
 void launch_keys(IndexArgs const& a, hipStream_t s) {
-    uint32_t blocks = (a.n + 255) / 256;
-    if (blocks > KEY_PARTS) { blocks = KEY_PARTS; }
-    if (!blocks) { blocks = 1; }
+    uint32_t const blocks = key_parts(a.n);
     hipLaunchKernelGGL(k_keys, dim3(blocks), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_key_range, dim3(1), dim3(256), 0, s, a, blocks);
+    if (!a.dense_bins) { hipLaunchKernelGGL(k_key_range, dim3(1), dim3(256), 0, s, a, blocks); }
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
