@@ -31,8 +31,7 @@ FLAGS = [
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # Loop heads on 128-byte boundaries: the neighbour build's time depends on
-    # its code layout (instruction fetch; 21 % of its wave cycles wait on it),
-    # and this layout measured 3-4 % faster than the default one.
+    # its code layout, and this layout measured 3-4 % faster than the default.
     "-falign-loops=128",
     "-Wall",
     "-Wno-unused-result",

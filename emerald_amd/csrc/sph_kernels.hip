@@ -27,10 +27,9 @@ namespace emsph {
 // Never launched: its size places the neighbour build that follows it in this
 // file's code object (k_build<BUILD_IISPH, 16, 64, false>) at byte 3840 of a
 // 4 KB page. With the same instructions, that kernel's time at 1M particles
-// depends on its offset in the page (104-121 us over the offsets measured;
-// its waves wait on instruction fetch a fifth of their cycles, and where the
-// page boundaries fall in its loops changes that), and 3840 was the fastest.
-// The build step checks the offset and warns when an edit has moved it.
+// depends on its offset in the page (104-121 us over the offsets measured,
+// repeatably; the hardware mechanism is not established), and 3840 was the
+// fastest. The build step checks the offset and warns when an edit moved it.
 __global__ void k_layout_pad() { asm volatile(".rept 768\n s_nop 0\n.endr"); }
 
 constexpr int BUILD_THREADS = 64;  // the full-capacity build and deferred pass
