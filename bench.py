@@ -782,7 +782,7 @@ def main() -> int:
             if m == method:
                 continue
             r = solver_window(m, args.solver_steps, 5, advance_for(m, args.advance), cpu,
-                              autotune=False, kernel_timing=False, rest_window=False)
+                              autotune=True, kernel_timing=False, rest_window=False)
             solvers[m] = r
     batched = None
     if not args.no_batched:
