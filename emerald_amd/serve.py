@@ -82,11 +82,13 @@ def percentile(xs, q):
 
 class _PoolLoop:
     def __init__(self, method: str, fluid_cap: int, solid_cap: int, max_scenes: int, cfl: float,
-                 commit_interval: float):
+                 commit_interval: float, tuning: dict | None = None):
         self.method = method
         self.commit_interval = commit_interval
         self.pool = sb.Pool(fluid_cap, solid_cap, max_scenes, method)
         self.pool.set_cfl(cfl)
+        if tuning:
+            self.pool.set_tuning(**tuning)
         self.ready = queue.Queue(maxsize=256)  # (Request, DamBreak, config) in arrival order
         self.pending = None                    # head that did not fit yet
         self.live = {}                         # id -> [request, steps done, last time, n]
@@ -177,10 +179,13 @@ class _PoolLoop:
 
 def serve(requests: list, budget_bytes: float, window: float, cfl: float = 0.4,
           methods=("iisph", "dfsph_p"), max_scenes: int = 4096, solid_ratio: float = 0.25,
-          keep_states=False, commit_interval: float = 0.2, log=None) -> dict:
+          keep_states=False, commit_interval: float = 0.2, log=None,
+          tuning: dict | None = None) -> dict:
     """Runs the request stream for window seconds; returns the served metrics
     (and the pools' final states of completed requests when keep_states is
-    True, or of those for which keep_states(request) is true)."""
+    True, or of those for which keep_states(request) is true). tuning: keyword
+    arguments of Pool.set_tuning for every pool (None: the library's
+    defaults)."""
     # Device bytes per fluid particle (with solid_ratio solids per fluid
     # particle), measured on a small pool.
     probe = sb.Pool(1 << 20, int(solid_ratio * (1 << 20)), 16, methods[0])
@@ -189,7 +194,7 @@ def serve(requests: list, budget_bytes: float, window: float, cfl: float = 0.4,
     share = budget_bytes / len(methods)
     fluid_cap = int(min(share / per_particle, (1 << 30) - 1))
     loops = {m: _PoolLoop(m, fluid_cap, int(solid_ratio * fluid_cap), max_scenes, cfl,
-                          commit_interval)
+                          commit_interval, tuning)
              for m in methods}
     if log:
         log(f"served: {len(methods)} pools x {fluid_cap} particles "
