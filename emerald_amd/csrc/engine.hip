@@ -448,6 +448,7 @@ struct Engine {
     hipEvent_t read_ev[2] = {nullptr, nullptr};
     KeyState keys{};  // the last key state read back
     int loop_expect[4] = {64, 64, 64, 64};  // iterations each solver loop needed last time
+    bool loop_known = false;  // keys holds a read-back of the running solver loop
     // Captured graphs of the fixed segments of a sub-step, by what their launch
     // arguments depend on; layout_gen changes whenever buffers or sizes do.
     bool use_graphs = true;
@@ -1261,12 +1262,14 @@ hipError_t run_loop(Engine* g, int kind, int max_it, int first_check, F&& launch
                     bool* short_sort) {
     *done = max_it;
     *short_sort = false;
+    g->loop_known = false;
     int const expect = g->loop_expect[kind];
     int pending = -1;  // iteration whose read-back is in flight
     for (int it = 0; it < max_it; ++it) {
         if (pending >= 0 && it >= expect) {
             hipError_t e = read_wait(g, pending & 1);
             if (e) { return e; }
+            g->loop_known = true;
             if (g->sort_short || g->keys.active_count == 0) {
                 *short_sort = g->sort_short;
                 *done = pending + 1;
@@ -1281,6 +1284,7 @@ hipError_t run_loop(Engine* g, int kind, int max_it, int first_check, F&& launch
         if (check && (e = read_async(g, it & 1))) { return e; }
         if (pending >= 0) {
             if ((e = read_wait(g, pending & 1))) { return e; }
+            g->loop_known = true;
             if (g->sort_short || g->keys.active_count == 0) {
                 if (check && (e = read_wait(g, it & 1))) { return e; }  // drain the slot
                 *short_sort = g->sort_short;
@@ -1298,6 +1302,36 @@ hipError_t run_loop(Engine* g, int kind, int max_it, int first_check, F&& launch
     }
     g->loop_expect[kind] = *done;
     return hipSuccess;
+}
+
+// The tiles of the scenes still iterating by the latest read-back of this
+// loop (g->keys.active_bits, a superset of those iterating now), when that
+// read-back exists, the engine holds several scenes and at most
+// ACTIVE_TILES_MAX of them are left; else n = 0 (every tile).
+ActiveTiles active_tiles(Engine const* g) {
+    ActiveTiles at{};
+    if (!g->loop_known || g->M < 2 || g->M > ACTIVE_BIT_SCENES ||
+        g->keys.active_count > static_cast<unsigned int>(ACTIVE_TILES_MAX)) {
+        return at;
+    }
+    uint32_t first = 0;
+    uint32_t total = 0;
+    uint32_t n = 0;
+    for (uint32_t s = 0; s < g->M; ++s) {
+        uint32_t const nt = (g->sh[s].n + TILE - 1) / TILE;
+        if ((g->keys.active_bits[s >> 5] >> (s & 31u)) & 1u) {
+            if (n == static_cast<uint32_t>(ACTIVE_TILES_MAX)) { return ActiveTiles{}; }
+            at.first[n] = first;
+            at.start[n] = total;
+            total += nt;
+            ++n;
+        }
+        first += nt;
+    }
+    if (n == 0 || total == 0) { return ActiveTiles{}; }
+    at.n = n;
+    at.start[n] = total;
+    return at;
 }
 
 // The first iteration of a solver loop whose exit test can stop some scene:
@@ -1559,8 +1593,9 @@ emsph_status iisph_sub_step(Engine* g) {
         // iisph.cpp:85-119: sum_j d_ij p_j, then beta and the new pressures
         // (iteration 0 ran in the a_ii sweep).
         if (it > 0) {
-            HIP_PASS(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, g->stream); }));
-            HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, g->stream); }));
+            ActiveTiles const at = active_tiles(g);
+            HIP_PASS(timed(g, "iisph_sdp", [&] { launch_iisph_sdp(b, at, g->stream); }));
+            HIP_PASS(timed(g, "iisph_beta_p", [&] { launch_iisph_beta_p(b, at, g->stream); }));
         }
         if (it < decide_from) { return hipSuccess; }
         return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); });

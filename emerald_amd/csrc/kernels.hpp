@@ -164,6 +164,25 @@ struct KeyState {
     int frame_remaining;          // scene 0's frame quanta left after this sub-step (-1: off)
     unsigned int dense_crowded;   // a cell of the dense index exceeded DENSE_CELL_MAX
     unsigned int dense_out;       // a key fell outside its scene's dense table (k_keys)
+    // Scenes still iterating in the solver loop, one bit per scene for the
+    // first ACTIVE_BIT_SCENES scenes (set at the loop's start, cleared by the
+    // exit test): the host's read-back sizes later sweeps to them.
+    unsigned int active_bits[128];
+};
+constexpr uint32_t ACTIVE_BIT_SCENES = 128 * 32;
+
+// The tiles a sweep launch covers when few scenes are still iterating: the
+// launch's workgroup k (after the XCD remap) takes tile first[j] + k - start[j]
+// for the j with start[j] <= k < start[j + 1] (start[n] = total, the grid).
+// n = 0: every tile, in order. Passed by value (kernel arguments); built on
+// the host from an earlier iteration's active bits, a superset of the scenes
+// still iterating (a scene never re-enters the loop), so the kernels keep
+// their per-scene active test.
+constexpr int ACTIVE_TILES_MAX = 64;
+struct ActiveTiles {
+    uint32_t n;
+    uint32_t first[ACTIVE_TILES_MAX];
+    uint32_t start[ACTIVE_TILES_MAX + 1];
 };
 
 // Dense cell index: cells counted in a table of bins, scene s owning the
@@ -499,8 +518,8 @@ void launch_build(int mode, int cap, Batch const& b, hipStream_t s,
 // a_ii, rho* and the pressure loop's first iteration (pressures +0) in one
 // sweep, into pr_out.
 void launch_iisph_aii_beta0(Batch const& b, hipStream_t s);
-void launch_iisph_sdp(Batch const& b, hipStream_t s);
-void launch_iisph_beta_p(Batch const& b, hipStream_t s);
+void launch_iisph_sdp(Batch const& b, ActiveTiles const& at, hipStream_t s);
+void launch_iisph_beta_p(Batch const& b, ActiveTiles const& at, hipStream_t s);
 // IISPH_AP (pseudo = 0) and IISPH_PSEUDO_AP (pseudo = 1): diagonals and
 // density stars with the loop start and the first iteration (pressures +0);
 // pressure accelerations of active scenes; one in-place Jacobi iteration of

@@ -30,7 +30,7 @@ namespace emsph {
 // depends on its offset in the page (104-121 us over the offsets measured,
 // repeatably; the hardware mechanism is not established), and 3840 was the
 // fastest. The build step checks the offset and warns when an edit moved it.
-__global__ void k_layout_pad() { asm volatile(".rept 768\n s_nop 0\n.endr"); }
+__global__ void k_layout_pad() { asm volatile(".rept 832\n s_nop 0\n.endr"); }
 
 constexpr int BUILD_THREADS = 64;  // the full-capacity build and deferred pass
 constexpr int BUILD_WG = 64;        // the capped builds (16 or 32 entries)
@@ -47,6 +47,16 @@ struct TileRef {
 // tiles (Batch::xcd_remap, a tuning choice), or in launch order.
 __device__ __forceinline__ uint32_t tile_index(Batch const& b) {
     return b.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+}
+
+// This workgroup's tile in a launch over ActiveTiles (uniform per workgroup,
+// so the search runs on scalar registers).
+__device__ __forceinline__ uint32_t active_tile(Batch const& b, ActiveTiles const& at) {
+    uint32_t const k = tile_index(b);
+    if (at.n == 0) { return k; }
+    uint32_t j = 0;
+    for (uint32_t i = 1; i < at.n; ++i) { j = (at.start[i] <= k) ? i : j; }
+    return at.first[j] + (k - at.start[j]);
 }
 
 __device__ __forceinline__ TileRef tile_ref(Batch const& b, uint32_t threads) {
@@ -521,6 +531,15 @@ __device__ __forceinline__ void loop_begin_block(Batch const& b) {
         b.stats[s].active = act;
         cnt += act;
     }
+    uint32_t const words = (b.nscenes < ACTIVE_BIT_SCENES ? b.nscenes : ACTIVE_BIT_SCENES) + 31u;
+    for (uint32_t w = threadIdx.x; w < words / 32u; w += blockDim.x) {
+        unsigned int bits = 0u;
+        for (uint32_t i = 0; i < 32u && 32u * w + i < b.nscenes; ++i) {
+            SceneDev const& sc = b.scenes[32u * w + i];
+            bits |= (sc.n > 0 && !sc.retired) ? (1u << i) : 0u;
+        }
+        b.keystate->active_bits[w] = bits;
+    }
     for (int off = 32; off > 0; off >>= 1) { cnt += __shfl_xor(cnt, off); }
     __shared__ uint32_t ws[THREADS / 64];
     if (__lane_id() == 0) { ws[threadIdx.x / 64] = cnt; }
@@ -597,6 +616,7 @@ __device__ __forceinline__ void loop_decide_scene(Batch const& b, int kind, int 
             st.iterations = it + 1;
         }
         atomicSub(&b.keystate->active_count, 1u);
+        if (s < ACTIVE_BIT_SCENES) { atomicAnd(&b.keystate->active_bits[s >> 5], ~(1u << (s & 31u))); }
     }
 }
 
@@ -1916,8 +1936,8 @@ __device__ __forceinline__ void iisph_sdp_tile(Batch const& a, Tile const& tl) {
 
 // Converged scenes skip the sweep.
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a) {
-    Tile const tl = a.tiles[tile_index(a)];
+__global__ __launch_bounds__(TILE) void k_iisph_sdp(Batch a, ActiveTiles at) {
+    Tile const tl = a.tiles[active_tile(a, at)];
     if (!a.stats[tl.scene].active) { return; }
     iisph_sdp_tile<NB>(a, tl);
 }
@@ -2009,8 +2029,8 @@ __device__ __forceinline__ void iisph_beta_p_tile(Batch const& a, Tile const& tl
 // A converged scene skips the sweep; its result stays in the buffer of its
 // last iteration.
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a) {
-    uint32_t const tile = tile_index(a);
+__global__ __launch_bounds__(TILE) void k_iisph_beta_p(Batch a, ActiveTiles at) {
+    uint32_t const tile = active_tile(a, at);
     Tile const tl = a.tiles[tile];
     if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
     iisph_beta_p_tile<NB>(a, tl, tile);
@@ -2968,11 +2988,15 @@ static inline dim3 tile_grid(Batch const& b) { return dim3(b.ntiles ? b.ntiles :
 void launch_iisph_aii_beta0(Batch const& a, hipStream_t s) {
     NB_DISPATCH(a.nb, k_iisph_aii_beta0, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_iisph_sdp(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_iisph_sdp, tile_grid(a), dim3(TILE), 0, s, a);
+// Over the tiles of `at` (its total) when at.n > 0, else every tile.
+static inline dim3 active_grid(Batch const& b, ActiveTiles const& at) {
+    return at.n ? dim3(at.start[at.n] ? at.start[at.n] : 1) : tile_grid(b);
 }
-void launch_iisph_beta_p(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_iisph_beta_p, tile_grid(a), dim3(TILE), 0, s, a);
+void launch_iisph_sdp(Batch const& a, ActiveTiles const& at, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_iisph_sdp, active_grid(a, at), dim3(TILE), 0, s, a, at);
+}
+void launch_iisph_beta_p(Batch const& a, ActiveTiles const& at, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_iisph_beta_p, active_grid(a, at), dim3(TILE), 0, s, a, at);
 }
 
 #define AP_DISPATCH(KERNEL, PSEUDO)                                                    \
