@@ -1643,8 +1643,11 @@ emsph_status iisph_ap_sub_step(Engine* g, bool pseudo) {
     int const decide_from = first_decision(g, 0, 2);
     HIP_TRY(run_loop(g, 0, max_it, 2, [&](int it) -> hipError_t {
         if (it > 0) {
-            HIP_PASS(timed(g, "ap_accel", [&] { launch_ap_accel(b, g->stream); }));
-            HIP_PASS(timed(g, "ap_iterate", [&] { launch_ap_iterate(b, pseudo ? 1 : 0, g->stream); }));
+            ActiveTiles const at = active_tiles(g);
+            HIP_PASS(timed(g, "ap_accel", [&] { launch_ap_accel(b, at, g->stream); }));
+            HIP_PASS(timed(g, "ap_iterate", [&] {
+                launch_ap_iterate(b, pseudo ? 1 : 0, at, g->stream);
+            }));
         }
         if (it < decide_from) { return hipSuccess; }
         return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_IISPH, it, it, g->stream); });
@@ -1683,8 +1686,9 @@ emsph_status dfsph_sub_step(Engine* g) {
     bool short_sort = false;
     int const div_from = first_decision(g, 1, 1);
     HIP_TRY(run_loop(g, 1, max_it, 1, [&](int it) -> hipError_t {
-        HIP_PASS(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, g->stream); }));
-        HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
+        ActiveTiles const at = active_tiles(g);
+        HIP_PASS(timed(g, "dfsph_div", [&] { launch_dfsph_div(b, at, g->stream); }));
+        HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream, at); }));
         if (it < div_from) { return hipSuccess; }
         return timed(g, "loop", [&] {
             launch_loop_decide(b, LOOP_DIVERGENCE, it, it, g->stream);
@@ -1702,8 +1706,9 @@ emsph_status dfsph_sub_step(Engine* g) {
     }));
     int const dens_from = first_decision(g, 2, 2);
     HIP_TRY(run_loop(g, 2, max_it, 2, [&](int it) -> hipError_t {
-        HIP_PASS(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, g->stream); }));
-        HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream); }));
+        ActiveTiles const at = active_tiles(g);
+        HIP_PASS(timed(g, "dfsph_dens", [&] { launch_dfsph_dens(b, at, g->stream); }));
+        HIP_PASS(timed(g, "apply", [&] { launch_apply(b, 0, 0, 0, g->stream, at); }));
         if (it < dens_from) { return hipSuccess; }
         return timed(g, "loop", [&] {
             launch_loop_decide(b, LOOP_DENSITY, it, base + it, g->stream);
@@ -1747,10 +1752,11 @@ emsph_status pcisph_sub_step(Engine* g) {
         // force writes the next one.
         b.xstar_in = (it & 1) ? xb : xa;
         b.xstar = (it & 1) ? xa : xb;
-        HIP_PASS(timed(g, "pci_density", [&] { launch_pci_density(b, g->stream); }));
+        ActiveTiles const at = active_tiles(g);
+        HIP_PASS(timed(g, "pci_density", [&] { launch_pci_density(b, at, g->stream); }));
         b.vstar_in = (it & 1) ? vout : vin;
         b.vstar_out = (it & 1) ? vin : vout;
-        HIP_PASS(timed(g, "pci_force", [&] { launch_pci_force(b, g->stream); }));
+        HIP_PASS(timed(g, "pci_force", [&] { launch_pci_force(b, at, g->stream); }));
         if (it < pci_from) { return hipSuccess; }
         return timed(g, "loop", [&] { launch_loop_decide(b, LOOP_PCISPH, it, it, g->stream); });
     }, &g->last_iterations, &short_sort));

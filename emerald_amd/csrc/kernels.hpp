@@ -526,17 +526,20 @@ void launch_iisph_beta_p(Batch const& b, ActiveTiles const& at, hipStream_t s);
 // active scenes; and the final accelerations with the integration (every
 // scene), writing the state back in original order.
 void launch_ap_diag0(Batch const& b, int pseudo, hipStream_t s);
-void launch_ap_accel(Batch const& b, hipStream_t s);
-void launch_ap_iterate(Batch const& b, int pseudo, hipStream_t s);
+void launch_ap_accel(Batch const& b, ActiveTiles const& at, hipStream_t s);
+void launch_ap_iterate(Batch const& b, int pseudo, ActiveTiles const& at, hipStream_t s);
 void launch_ap_final(Batch const& b, int pseudo, hipStream_t s);
 // Apply pressure accelerations (pressure.cpp:9-80) with p_in as the pressure,
 // for active scenes (or all scenes when all_scenes != 0).
 // with_gravity: first integrate the external gravity force (dynamics.cpp:24-37).
 // finalize: then x += dt v and scatter state to original order (dynamics.cpp:46-51).
-void launch_apply(Batch const& b, int with_gravity, int finalize, int all_scenes, hipStream_t s);
+// all_scenes = 0: scenes still iterating only, over the tiles of `at` (every
+// tile when at.n = 0); all_scenes = 1: every tile.
+void launch_apply(Batch const& b, int with_gravity, int finalize, int all_scenes, hipStream_t s,
+                  ActiveTiles const& at = ActiveTiles{});
 void launch_finalize(Batch const& b, int dfsph, hipStream_t s);
-void launch_dfsph_div(Batch const& b, hipStream_t s);
-void launch_dfsph_dens(Batch const& b, hipStream_t s);
+void launch_dfsph_div(Batch const& b, ActiveTiles const& at, hipStream_t s);
+void launch_dfsph_dens(Batch const& b, ActiveTiles const& at, hipStream_t s);
 void launch_pci_init(Batch const& b, hipStream_t s);
 // colors.cpp:9-20 target_density_colors from the densities of the last
 // sub-step, RGBA per particle in original order.
@@ -548,8 +551,8 @@ void launch_convert_colors(float4 const* in, uchar4* out, uint32_t n, hipStream_
 // scene's force models. apply != 0 (IISPH, IISPH_AP, IISPH_PSEUDO_AP): instead
 // of only storing, also integrate them into the velocities (dynamics.cpp:24-37).
 void launch_external_forces(Batch const& b, int apply, int host_forces, hipStream_t s);
-void launch_pci_density(Batch const& b, hipStream_t s);
-void launch_pci_force(Batch const& b, hipStream_t s);
+void launch_pci_density(Batch const& b, ActiveTiles const& at, hipStream_t s);
+void launch_pci_force(Batch const& b, ActiveTiles const& at, hipStream_t s);
 void launch_pci_finalize(Batch const& b, hipStream_t s);
 void launch_vmax2(Batch const& b, hipStream_t s);
 void launch_scatter_f32(float const* src_sorted, uint32_t const* orig, float* dst, uint32_t n,

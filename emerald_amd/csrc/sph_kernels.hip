@@ -30,7 +30,7 @@ namespace emsph {
 // depends on its offset in the page (104-121 us over the offsets measured,
 // repeatably; the hardware mechanism is not established), and 3840 was the
 // fastest. The build step checks the offset and warns when an edit moved it.
-__global__ void k_layout_pad() { asm volatile(".rept 832\n s_nop 0\n.endr"); }
+__global__ void k_layout_pad() { asm volatile(".rept 960\n s_nop 0\n.endr"); }
 
 constexpr int BUILD_THREADS = 64;  // the full-capacity build and deferred pass
 constexpr int BUILD_WG = 64;        // the capped builds (16 or 32 entries)
@@ -59,16 +59,19 @@ __device__ __forceinline__ uint32_t active_tile(Batch const& b, ActiveTiles cons
     return at.first[j] + (k - at.start[j]);
 }
 
-__device__ __forceinline__ TileRef tile_ref(Batch const& b, uint32_t threads) {
-    uint32_t const k = tile_index(b);
+__device__ __forceinline__ TileRef tile_ref_of(Batch const& b, uint32_t k) {
     Tile const tl = b.tiles[k];
-    (void)threads;
     TileRef r;
     r.scene = tl.scene;
     r.tile = k;
     r.t = tl.start + threadIdx.x;
     r.valid = threadIdx.x < tl.count;
     return r;
+}
+
+__device__ __forceinline__ TileRef tile_ref(Batch const& b, uint32_t threads) {
+    (void)threads;
+    return tile_ref_of(b, tile_index(b));
 }
 
 // Workgroup partials, written to the workgroup's own tile slot (no atomics:
@@ -2186,8 +2189,8 @@ __device__ __forceinline__ V2 ap_acceleration(Batch const& a, StepConsts const& 
 
 // Accelerations from the current pressures, for the scenes still iterating.
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_ap_accel(Batch a) {
-    TileRef const r = tile_ref(a, TILE);
+__global__ __launch_bounds__(TILE) void k_ap_accel(Batch a, ActiveTiles at) {
+    TileRef const r = tile_ref_of(a, active_tile(a, at));
     if (!a.stats[r.scene].active) { return; }  // the whole workgroup
     if (!r.valid) { return; }
     StepConsts const c = a.scenes[r.scene].c;
@@ -2197,8 +2200,8 @@ __global__ __launch_bounds__(TILE) void k_ap_accel(Batch a) {
 // iisph_ap_ops.cpp:108-194 / iisph_pseudo_ap_ops.cpp:227-305: A p from the
 // accelerations, the in-place pressure update and its residual.
 template <int NB, int PSEUDO>
-__global__ __launch_bounds__(TILE) void k_ap_iterate(Batch a) {
-    uint32_t const tile = tile_index(a);
+__global__ __launch_bounds__(TILE) void k_ap_iterate(Batch a, ActiveTiles at) {
+    uint32_t const tile = active_tile(a, at);
     Tile const tl = a.tiles[tile];
     if (!a.stats[tl.scene].active) { return; }  // the whole workgroup
     unsigned long long qsum = 0;
@@ -2280,8 +2283,8 @@ __global__ __launch_bounds__(TILE) void k_ap_final(Batch a) {
 //------------------------------------------------------------------------------
 // pressure.cpp:9-80 apply_pressures_in_place (fluid pass, then solid pass).
 template <int GRAVITY, int FINAL, int NB>
-__global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes) {
-    TileRef const r = tile_ref(a, TILE);
+__global__ __launch_bounds__(TILE) void k_apply(Batch a, int all_scenes, ActiveTiles at) {
+    TileRef const r = tile_ref_of(a, active_tile(a, at));
     if (!all_scenes && !a.stats[r.scene].active) { return; }  // whole workgroup
     unsigned int vm = 0;
     double ke = 0.0, pe = 0.0;
@@ -2440,8 +2443,8 @@ __device__ __forceinline__ float divergence_of(Batch const& a, StepConsts const&
 // One divergence-solve iteration: divergence, kappa_v, residual of
 // max(0, dt * div / rho0) (dfsph_p.cpp:157-197).
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a) {
-    TileRef const r = tile_ref(a, TILE);
+__global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a, ActiveTiles at) {
+    TileRef const r = tile_ref_of(a, active_tile(a, at));
     if (!a.stats[r.scene].active) { return; }
     unsigned long long qsum = 0;
     unsigned int emax = 0;
@@ -2466,8 +2469,8 @@ __global__ __launch_bounds__(TILE) void k_dfsph_div(Batch a) {
 // One density-solve iteration: density star (density.cpp:111-145), kappa_rho,
 // residual of max(0, (rho* - rho0) / rho0) (dfsph_p.cpp:82-122).
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a) {
-    TileRef const r = tile_ref(a, TILE);
+__global__ __launch_bounds__(TILE) void k_dfsph_dens(Batch a, ActiveTiles at) {
+    TileRef const r = tile_ref_of(a, active_tile(a, at));
     if (!a.stats[r.scene].active) { return; }
     unsigned long long qsum = 0;
     unsigned int emax = 0;
@@ -2675,8 +2678,8 @@ __global__ __launch_bounds__(TILE) void k_pci_init(Batch a) {
 // Neighbour entries are taken NB at a time (their loads in flight together)
 // and processed in list order.
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
-    TileRef const r = tile_ref(a, TILE);
+__global__ __launch_bounds__(TILE) void k_pci_density(Batch a, ActiveTiles at) {
+    TileRef const r = tile_ref_of(a, active_tile(a, at));
     if (!a.stats[r.scene].active) { return; }
     unsigned int emax = 0;
     if (r.valid) {
@@ -2746,8 +2749,8 @@ __global__ __launch_bounds__(TILE) void k_pci_density(Batch a) {
 // predictor (:73-88) from the new force. A converged scene carries its last
 // predictor forward. Neighbour entries are taken NB at a time.
 template <int NB>
-__global__ __launch_bounds__(TILE) void k_pci_force(Batch a) {
-    TileRef const r = tile_ref(a, TILE);
+__global__ __launch_bounds__(TILE) void k_pci_force(Batch a, ActiveTiles at) {
+    TileRef const r = tile_ref_of(a, active_tile(a, at));
     if (!r.valid) { return; }
     uint32_t const t = r.t;
     if (!a.stats[r.scene].active) { return; }  // last predictor stays in place
@@ -2972,6 +2975,10 @@ __global__ __launch_bounds__(256) void k_export_lists(Lists l, uint32_t n, uint3
 
 //------------------------------------------------------------------------------
 static inline dim3 tile_grid(Batch const& b) { return dim3(b.ntiles ? b.ntiles : 1); }
+// Over the tiles of `at` (its total) when at.n > 0, else every tile.
+static inline dim3 active_grid(Batch const& b, ActiveTiles const& at) {
+    return at.n ? dim3(at.start[at.n] ? at.start[at.n] : 1) : tile_grid(b);
+}
 
 // Kernel instantiation for the batch depth Batch::nb (2, 4 or 8).
 #define NB_DISPATCH(nb, KERNEL, ...)                                                  \
@@ -2987,10 +2994,6 @@ static inline dim3 tile_grid(Batch const& b) { return dim3(b.ntiles ? b.ntiles :
 
 void launch_iisph_aii_beta0(Batch const& a, hipStream_t s) {
     NB_DISPATCH(a.nb, k_iisph_aii_beta0, tile_grid(a), dim3(TILE), 0, s, a);
-}
-// Over the tiles of `at` (its total) when at.n > 0, else every tile.
-static inline dim3 active_grid(Batch const& b, ActiveTiles const& at) {
-    return at.n ? dim3(at.start[at.n] ? at.start[at.n] : 1) : tile_grid(b);
 }
 void launch_iisph_sdp(Batch const& a, ActiveTiles const& at, hipStream_t s) {
     NB_DISPATCH(a.nb, k_iisph_sdp, active_grid(a, at), dim3(TILE), 0, s, a, at);
@@ -3010,6 +3013,18 @@ void launch_iisph_beta_p(Batch const& a, ActiveTiles const& at, hipStream_t s) {
         }                                                                             \
     } while (0)
 
+// As AP_DISPATCH, over `grid` with the ActiveTiles `at` as the second argument.
+#define AP_DISPATCH_AT(KERNEL, PSEUDO)                                                 \
+    do {                                                                              \
+        if (a.nb <= 2) {                                                              \
+            hipLaunchKernelGGL((KERNEL<2, PSEUDO>), grid, dim3(TILE), 0, s, a, at);       \
+        } else if (a.nb <= 4) {                                                       \
+            hipLaunchKernelGGL((KERNEL<4, PSEUDO>), grid, dim3(TILE), 0, s, a, at);       \
+        } else {                                                                      \
+            hipLaunchKernelGGL((KERNEL<8, PSEUDO>), grid, dim3(TILE), 0, s, a, at);       \
+        }                                                                             \
+    } while (0)
+
 void launch_ap_diag0(Batch const& a, int pseudo, hipStream_t s) {
     if (pseudo) {
         AP_DISPATCH(k_ap_diag0, 1);
@@ -3017,14 +3032,15 @@ void launch_ap_diag0(Batch const& a, int pseudo, hipStream_t s) {
         AP_DISPATCH(k_ap_diag0, 0);
     }
 }
-void launch_ap_accel(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_ap_accel, tile_grid(a), dim3(TILE), 0, s, a);
+void launch_ap_accel(Batch const& a, ActiveTiles const& at, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_ap_accel, active_grid(a, at), dim3(TILE), 0, s, a, at);
 }
-void launch_ap_iterate(Batch const& a, int pseudo, hipStream_t s) {
+void launch_ap_iterate(Batch const& a, int pseudo, ActiveTiles const& at, hipStream_t s) {
+    dim3 const grid = active_grid(a, at);
     if (pseudo) {
-        AP_DISPATCH(k_ap_iterate, 1);
+        AP_DISPATCH_AT(k_ap_iterate, 1);
     } else {
-        AP_DISPATCH(k_ap_iterate, 0);
+        AP_DISPATCH_AT(k_ap_iterate, 0);
     }
 }
 void launch_ap_final(Batch const& a, int pseudo, hipStream_t s) {
@@ -3036,26 +3052,29 @@ void launch_ap_final(Batch const& a, int pseudo, hipStream_t s) {
 }
 
 template <int G, int F>
-void launch_apply_nb(Batch const& a, int all_scenes, hipStream_t s) {
+void launch_apply_nb(Batch const& a, int all_scenes, ActiveTiles const& at, hipStream_t s) {
+    dim3 const grid = active_grid(a, at);
     if (a.nb <= 2) {
-        hipLaunchKernelGGL((k_apply<G, F, 2>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+        hipLaunchKernelGGL((k_apply<G, F, 2>), grid, dim3(TILE), 0, s, a, all_scenes, at);
     } else if (a.nb <= 4) {
-        hipLaunchKernelGGL((k_apply<G, F, 4>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+        hipLaunchKernelGGL((k_apply<G, F, 4>), grid, dim3(TILE), 0, s, a, all_scenes, at);
     } else {
-        hipLaunchKernelGGL((k_apply<G, F, 8>), tile_grid(a), dim3(TILE), 0, s, a, all_scenes);
+        hipLaunchKernelGGL((k_apply<G, F, 8>), grid, dim3(TILE), 0, s, a, all_scenes, at);
     }
 }
 
 void launch_apply(Batch const& a, int with_gravity, int finalize, int all_scenes,
-                  hipStream_t s) {
+                  hipStream_t s, ActiveTiles const& at) {
+    ActiveTiles const every{};
+    ActiveTiles const& use = all_scenes ? every : at;  // all scenes: every tile
     if (with_gravity && finalize) {
-        launch_apply_nb<1, 1>(a, all_scenes, s);
+        launch_apply_nb<1, 1>(a, all_scenes, use, s);
     } else if (with_gravity) {
-        launch_apply_nb<1, 0>(a, all_scenes, s);
+        launch_apply_nb<1, 0>(a, all_scenes, use, s);
     } else if (finalize) {
-        launch_apply_nb<0, 1>(a, all_scenes, s);
+        launch_apply_nb<0, 1>(a, all_scenes, use, s);
     } else {
-        launch_apply_nb<0, 0>(a, all_scenes, s);
+        launch_apply_nb<0, 0>(a, all_scenes, use, s);
     }
 }
 void launch_finalize(Batch const& a, int dfsph, hipStream_t s) {
@@ -3065,20 +3084,20 @@ void launch_finalize(Batch const& a, int dfsph, hipStream_t s) {
         hipLaunchKernelGGL(k_finalize<0>, tile_grid(a), dim3(TILE), 0, s, a);
     }
 }
-void launch_dfsph_div(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_dfsph_div, tile_grid(a), dim3(TILE), 0, s, a);
+void launch_dfsph_div(Batch const& a, ActiveTiles const& at, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_dfsph_div, active_grid(a, at), dim3(TILE), 0, s, a, at);
 }
-void launch_dfsph_dens(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_dfsph_dens, tile_grid(a), dim3(TILE), 0, s, a);
+void launch_dfsph_dens(Batch const& a, ActiveTiles const& at, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_dfsph_dens, active_grid(a, at), dim3(TILE), 0, s, a, at);
 }
 void launch_pci_init(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_init, tile_grid(a), dim3(TILE), 0, s, a);
 }
-void launch_pci_density(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_pci_density, tile_grid(a), dim3(TILE), 0, s, a);
+void launch_pci_density(Batch const& a, ActiveTiles const& at, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_pci_density, active_grid(a, at), dim3(TILE), 0, s, a, at);
 }
-void launch_pci_force(Batch const& a, hipStream_t s) {
-    NB_DISPATCH(a.nb, k_pci_force, tile_grid(a), dim3(TILE), 0, s, a);
+void launch_pci_force(Batch const& a, ActiveTiles const& at, hipStream_t s) {
+    NB_DISPATCH(a.nb, k_pci_force, active_grid(a, at), dim3(TILE), 0, s, a, at);
 }
 void launch_pci_finalize(Batch const& a, hipStream_t s) {
     hipLaunchKernelGGL(k_pci_finalize, tile_grid(a), dim3(TILE), 0, s, a);
